@@ -1,0 +1,136 @@
+/* pmx -- MI355X (gfx950) nonbonded force engine for ProtoMol.
+ *
+ * C ABI of the drop-in boundary.  Plain pointers and sizes only; nothing
+ * throws across it; every call returns PMX_OK (0) or a negative pmx_status.
+ *
+ * It replaces, for one ProtoMol topology on one GPU, the work done by
+ *   SystemForce::evaluate(const GenericTopology*, const Vector3DBlock*,
+ *                         Vector3DBlock*, ScalarStructure*)
+ *     (protomol/src/protomol/force/system/SystemForce.h:32-35)
+ * for the force lines
+ *   force LennardJones [Coulomb] -algorithm NonbondedCutoff -switchingFunction ...
+ *     (protomol/src/protomol/force/nonbonded/NonbondedCutoffSystemForce.h:45-55,
+ *      registered in protomol/src/protomol/module/NonbondedCutoffForceModule.cpp:40-201)
+ *   force Coulomb -algorithm PMEwald -real -reciprocal -correction -interpolation BSpline ...
+ *     (mdl/interface/force/nonbonded/NonbondedPMEwaldSystemForce.h:245-319)
+ * When both lines are present they share one device system and one pair pass.
+ *
+ * Ownership: the caller keeps the host arrays; the library owns device
+ * buffers (positions, cell/cluster lists, PME grids) which stay resident in
+ * HBM between calls.  A pmx_ctx (and every pmx_sys made from it) is used by
+ * one host thread at a time.
+ */
+#ifndef PMX_H
+#define PMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pmx_ctx pmx_ctx;
+typedef struct pmx_sys pmx_sys;
+
+typedef enum {
+  PMX_OK = 0,
+  PMX_ERR_ARG = -1,         /* invalid argument or descriptor */
+  PMX_ERR_OOM = -2,         /* device allocation failed */
+  PMX_ERR_HIP = -3,         /* HIP runtime error */
+  PMX_ERR_UNSUPPORTED = -4, /* valid ProtoMol input outside this engine (e.g. non-orthogonal box) */
+  PMX_ERR_NODEV = -5,       /* no GPU or kernels missing for this device */
+  PMX_ERR_OVERFLOW = -6     /* internal list capacity exceeded (retry is automatic; reported if persistent) */
+} pmx_status;
+
+typedef enum { PMX_FP64 = 0, PMX_MIXED = 1 } pmx_precision;
+/* Switching functions: CutoffSwitchingFunction, C1SwitchingFunction,
+ * C2SwitchingFunction (protomol/src/protomol/switch/). */
+typedef enum { PMX_SW_NONE = 0, PMX_SW_CUTOFF = 1, PMX_SW_C1 = 2, PMX_SW_C2 = 3 } pmx_switch;
+typedef enum { PMX_COUL_NONE = 0, PMX_COUL_CUTOFF = 1, PMX_COUL_PME = 2 } pmx_coulomb;
+/* PME term selection (-real -reciprocal -correction). */
+enum { PMX_PME_REAL = 1, PMX_PME_RECIP = 2, PMX_PME_CORR = 4 };
+/* Energy slots written by pmx_sys_evaluate: the first two entries of
+ * ScalarStructure (protomol/src/protomol/type/ScalarStructure.h:26-29). */
+enum { PMX_E_COULOMB = 0, PMX_E_LENNARDJONES = 1, PMX_E_COUNT = 2 };
+/* evaluate flags */
+enum {
+  PMX_EVAL_PAIRSTATS = 1u,  /* also count in-cutoff pairs and hash the pair set */
+  PMX_EVAL_TIMING = 2u      /* record per-stage device times (pmx_sys_stage_times) */
+};
+
+typedef struct {
+  int n;                  /* atoms */
+  int ntypes;             /* LJ types */
+  int nexcl;              /* exclusion entries */
+  const double *charge;   /* [n] scaled charges, q * SQRTCOULOMBCONSTANT */
+  const int *type;        /* [n] LJ type */
+  const int *molecule;    /* [n] molecule id (unused for pairs: exclusions are explicit) */
+  const double *lj;       /* [ntypes*ntypes*4] {A, B, A14, B14} (LennardJonesParameterTable) */
+  const int *excl;        /* [3*nexcl] a1 a2 class, class 1 = FULL, 2 = MODIFIED (ExclusionTable) */
+  double box[3];          /* orthogonal cell basis lengths (cellBasisVector1..3) */
+  double origin[3];       /* cellorigin */
+  double csf;             /* coulombScalingFactor */
+  double cellsize;        /* cellsize of the reference cell manager (membership queries) */
+  int lj_switch;          /* pmx_switch; PMX_SW_NONE = no LennardJones line */
+  double lj_on, lj_rc;    /* -switchon, -cutoff */
+  int coul_kind;          /* pmx_coulomb */
+  int coul_switch;        /* cutoff Coulomb: CUTOFF/C1/C2; PME real: CUTOFF or C1 */
+  double coul_on, coul_rc;
+  int pme_grid[3];        /* -gridsize nx ny nz */
+  int pme_order;          /* -order (even, >= 2) */
+  double pme_accuracy;    /* -accuracy */
+  double pme_alpha;       /* -alpha (<= 0: chosen from accuracy and cutoff) */
+  int pme_terms;          /* PMX_PME_* mask */
+  int precision;          /* pmx_precision */
+} pmx_sys_desc;
+
+/* Context: one GPU, one HIP stream. */
+int pmx_ctx_create(int device, pmx_ctx **out);
+int pmx_ctx_destroy(pmx_ctx *ctx);
+/* Last error message for this context ("" if none). Never NULL. */
+const char *pmx_last_error(const pmx_ctx *ctx);
+
+/* Builds the resident device system (doMake + initialize, including the
+ * PME alpha choice and influence tables;
+ * NonbondedPMEwaldSystemForce.h:460-595). */
+int pmx_sys_create(pmx_ctx *ctx, const pmx_sys_desc *desc, pmx_sys **out);
+int pmx_sys_destroy(pmx_sys *sys);
+/* Drops cached tables so the next evaluate rebuilds them (Force::uncache). */
+int pmx_sys_uncache(pmx_sys *sys);
+
+/* SystemForce::evaluate: x_aos [3n] positions; f_aos [3n] is ACCUMULATED
+ * (+=); energies[PMX_E_COUNT] are accumulated into the two slots.
+ * Synchronous. */
+int pmx_sys_evaluate(pmx_sys *sys, const double *x_aos, double *f_aos, double *energies, unsigned flags);
+
+/* Device-resident form (positions already in HBM): positions are uploaded
+ * once with pmx_sys_set_positions; pmx_sys_compute runs the force pass on
+ * the device without host transfers; pmx_sys_get_forces fetches them. */
+int pmx_sys_set_positions(pmx_sys *sys, const double *x_aos);
+int pmx_sys_compute(pmx_sys *sys, unsigned flags);
+int pmx_sys_get_forces(pmx_sys *sys, double *f_aos, double *energies);
+int pmx_sys_sync(pmx_sys *sys);
+
+/* PME splitting parameter in use (after pmx_sys_create). */
+int pmx_sys_alpha(const pmx_sys *sys, double *alpha);
+
+/* Pair statistics of the last evaluate run with PMX_EVAL_PAIRSTATS:
+ * number of non-excluded pairs with minimal-image d^2 <= max cutoff^2
+ * (each unordered pair once) and a 64-bit order-independent hash of the
+ * set (sum over pairs of pmx_pair_hash(min(i,j), max(i,j)), wrapping). */
+int pmx_sys_pair_stats(const pmx_sys *sys, long long *npairs, unsigned long long *hash);
+/* The hash used by pmx_sys_pair_stats (so callers can hash a reference set). */
+unsigned long long pmx_pair_hash(int i, int j);
+
+/* Reference cell of every atom for the CubicCellManager with the descriptor's
+ * cellsize (Topology::updateCellLists binning, Topology.h:44-81), computed
+ * on the device from the last positions; index (x*ny + y)*nz + z. */
+int pmx_sys_cells(pmx_sys *sys, int *cell_of_atom, int *dims);
+
+/* Device milliseconds of the last evaluate with PMX_EVAL_TIMING, in the
+ * order: sort, list, pair, spread, fft+convolve, gather, correction,
+ * finalize.  ms must hold 8 floats. */
+int pmx_sys_stage_times(const pmx_sys *sys, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

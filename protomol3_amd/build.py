@@ -1,0 +1,35 @@
+"""Builds the gfx950 engine library (libpmx.so) in-tree with hipcc."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib", "libpmx.so")
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("pmx.hip", "pmx_kernels.h", "pmx_common.h")]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-munsafe-fp-atomics",
+    # packed FP32 arithmetic is kept off (only scalar/FMA FP32 VALU forms)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-Wno-unused-value",
+]
+
+
+def build_lib(verbose: bool = False) -> str:
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    cmd = ["hipcc", *HIPCC_FLAGS, "-I" + os.path.join(ROOT, "include"),
+           os.path.join(HERE, "csrc", "pmx.hip"), "-o", LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])
+    os.replace(LIB + ".tmp", LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build_lib(verbose=True)

@@ -1,0 +1,866 @@
+// pmx: C-ABI implementation (include/pmx.h) -- host orchestration of the
+// gfx950 kernels in pmx_kernels.h.  One HIP stream per context; all device
+// state of a system stays resident between calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <set>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/pmx.h"
+#include "pmx_kernels.h"
+
+using namespace pmx;
+
+struct pmx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct Sys {
+  pmx_ctx *ctx = nullptr;
+  pmx_sys_desc d{};
+  bool mixed = false;
+  int n = 0, npad = 0, ncl = 0, nsuper = 0;
+  // host copies
+  std::vector<double> q, lj;
+  std::vector<int> type;
+  std::vector<int4> etab;          // de-duplicated exclusion table a1 a2 cls
+  std::vector<int> eptr;           // CSR of partners per atom (orig index)
+  std::vector<int2> elist;
+  double alpha = 0, self_e = 0, charged_e = 0;
+  Box box{};
+  BinParams bin{};
+  PairParams pp{};
+  ListParams lp{};
+  PmeParams pme{};
+  FftPlan plx{}, ply{}, plz{};
+  bool pme_on = false;
+  int cap = 0, maxmaskwork = 0, ntiles_cap = 0;
+  long long nbins = 0;
+  int nscanblocks = 0;
+  int recip_blocks = 0, corr_blocks = 0;
+  int zl = 1, tky = 1, tkx = 1;
+  // device buffers
+  std::vector<DevBuf *> all;
+  DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
+      ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
+      pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec,
+      greal, theta, twx, twy, twz, exclmax;
+  // results
+  long long last_np = 0;
+  unsigned long long last_hash = 0;
+  float stage_ms[8] = {0};
+  hipEvent_t ev[9] = {};
+  bool have_positions = false;
+  bool cached = false;
+  double eout_h[4] = {0, 0, 0, 0};
+};
+
+int fail(pmx_ctx *c, int code, const std::string &m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) return fail(s->ctx, PMX_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+int dalloc(Sys *s, DevBuf &b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.p && b.bytes >= bytes) return PMX_OK;
+  if (b.p) hipFree(b.p);
+  b.p = nullptr;
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    b.p = nullptr;
+    return fail(s->ctx, PMX_ERR_OOM, "hipMalloc failed (" + std::to_string(bytes) + " bytes)");
+  }
+  b.bytes = bytes;
+  s->all.push_back(&b);
+  return PMX_OK;
+}
+
+template <class T>
+T *P(DevBuf &b) {
+  return reinterpret_cast<T *>(b.p);
+}
+
+unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Abramowitz-free alpha choice (NonbondedPMEwaldSystemForce.h:520-537).
+double choose_alpha(double rc, double acc) {
+  double a = 1.0;
+  while (std::erfc(a * rc) / rc >= acc) a *= 2.0;
+  double lo = 0., hi = a;
+  for (int i = 0; i < 100; ++i) {
+    a = 0.5 * (lo + hi);
+    if (std::erfc(a * rc) / rc >= acc)
+      lo = a;
+    else
+      hi = a;
+  }
+  return a;
+}
+
+bool make_plan(int n, FftPlan &pl) {
+  pl.n = n;
+  pl.nrad = 0;
+  int m = n;
+  while (m % 4 == 0 && pl.nrad < 16) {
+    pl.rad[pl.nrad++] = 4;
+    m /= 4;
+  }
+  for (int r : {2, 3, 5})
+    while (m % r == 0 && pl.nrad < 16) {
+      pl.rad[pl.nrad++] = r;
+      m /= r;
+    }
+  return m == 1;
+}
+
+void bspline_host(int order, double w, double *th, double *dth) {
+  th[order - 1] = 0.0;
+  th[1] = w;
+  th[0] = 1.0 - w;
+  for (int k = 2; k < order - 1; k++) {
+    double div = 1.0 / k;
+    th[k] = div * w * th[k - 1];
+    for (int j = 1; j < k; j++) th[k - j] = div * ((w + j) * th[k - j - 1] + (k + 1 - j - w) * th[k - j]);
+    th[0] = div * (1 - w) * th[0];
+  }
+  dth[0] = -th[0];
+  for (int j = 1; j <= order - 1; j++) dth[j] = th[j - 1] - th[j];
+  double div = 1.0 / (order - 1.0);
+  th[order - 1] = div * w * th[order - 2];
+  for (int j = 1; j < order - 1; j++)
+    th[order - 1 - j] = div * ((w + j) * th[order - j - 2] + (order - j - w) * th[order - 1 - j]);
+  th[0] = div * (1 - w) * th[0];
+}
+
+SwitchC make_switch(int kind, double on, double rc) {
+  SwitchC s{};
+  s.kind = kind;
+  s.rc2 = rc * rc;
+  s.on2 = on * on;
+  double t = rc * rc - on * on;
+  s.s1 = kind == SW_C2 ? 1.0 / (t * t * t) : 0.0;
+  s.s2 = rc * rc - 3.0 * on * on;
+  s.s3 = kind == SW_C2 ? 4.0 / (t * t * t) : 0.0;
+  s.c15_1 = 1.5 / rc;
+  s.c05_3 = 0.5 / (rc * rc * rc);
+  s.c15_3 = 1.5 / (rc * rc * rc);
+  return s;
+}
+
+int validate(pmx_ctx *c, const pmx_sys_desc *d) {
+  if (!d) return fail(c, PMX_ERR_ARG, "null descriptor");
+  if (d->n <= 0 || d->ntypes <= 0 || d->nexcl < 0) return fail(c, PMX_ERR_ARG, "bad sizes");
+  if (!d->charge || !d->type || !d->lj || (d->nexcl > 0 && !d->excl)) return fail(c, PMX_ERR_ARG, "null array");
+  if (d->ntypes > kMaxTypes) return fail(c, PMX_ERR_UNSUPPORTED, "more than 16 LJ types");
+  for (int k = 0; k < 3; ++k)
+    if (!(d->box[k] > 0)) return fail(c, PMX_ERR_ARG, "box edge must be > 0");
+  if (d->lj_switch < 0 || d->lj_switch > 3 || d->coul_kind < 0 || d->coul_kind > 2)
+    return fail(c, PMX_ERR_ARG, "bad force kind");
+  if (d->lj_switch == SW_NONE && d->coul_kind == COUL_NONE) return fail(c, PMX_ERR_ARG, "no force selected");
+  if (d->lj_switch != SW_NONE && !(d->lj_rc > 0)) return fail(c, PMX_ERR_ARG, "LJ cutoff must be > 0");
+  if (d->lj_switch == SW_C2 && !(d->lj_on >= 0 && d->lj_on < d->lj_rc))
+    return fail(c, PMX_ERR_ARG, "C2 switching function: 0 <= switchon < cutoff");
+  if (d->coul_kind != COUL_NONE && !(d->coul_rc > 0)) return fail(c, PMX_ERR_ARG, "Coulomb cutoff must be > 0");
+  if (d->coul_kind == COUL_CUTOFF && d->coul_switch == SW_C2 && !(d->coul_on >= 0 && d->coul_on < d->coul_rc))
+    return fail(c, PMX_ERR_ARG, "C2 switching function: 0 <= switchon < cutoff");
+  if (d->coul_kind == COUL_CUTOFF && (d->coul_switch < 1 || d->coul_switch > 3))
+    return fail(c, PMX_ERR_ARG, "bad Coulomb switching function");
+  if (d->coul_kind == COUL_PME) {
+    if (d->coul_switch != SW_CUTOFF && d->coul_switch != SW_C1)
+      return fail(c, PMX_ERR_ARG, "PME real term takes the Cutoff or C1 switching function");
+    const int p = d->pme_order;
+    if (p < 2 || p % 2 != 0 || p > 8) return fail(c, PMX_ERR_ARG, "PMEwald order must be even, 2..8");
+    for (int k = 0; k < 3; ++k)
+      if (d->pme_grid[k] < p) return fail(c, PMX_ERR_ARG, "PMEwald grid must be >= order");
+    if (!(d->pme_accuracy > 0) && !(d->pme_alpha > 0)) return fail(c, PMX_ERR_ARG, "PMEwald accuracy > 0");
+    if ((d->pme_terms & 7) == 0) return fail(c, PMX_ERR_ARG, "no PME term selected");
+  }
+  if (!(d->cellsize > 0)) return fail(c, PMX_ERR_ARG, "cellsize must be > 0");
+  if (d->precision != PMX_FP64 && d->precision != PMX_MIXED) return fail(c, PMX_ERR_ARG, "bad precision");
+  double rcmax = std::max(d->lj_switch ? d->lj_rc : 0.0, d->coul_kind ? d->coul_rc : 0.0);
+  for (int k = 0; k < 3; ++k)
+    if (rcmax > 0.5 * d->box[k])
+      return fail(c, PMX_ERR_UNSUPPORTED, "cutoff larger than half the box (multiple images per pair)");
+  for (int i = 0; i < d->n; ++i)
+    if (d->type[i] < 0 || d->type[i] >= d->ntypes) return fail(c, PMX_ERR_ARG, "atom type out of range");
+  for (int e = 0; e < d->nexcl; ++e) {
+    int a = d->excl[3 * e], b = d->excl[3 * e + 1], cl = d->excl[3 * e + 2];
+    if (a < 0 || b < 0 || a >= d->n || b >= d->n || cl < 0 || cl > 2)
+      return fail(c, PMX_ERR_ARG, "exclusion entry out of range");
+  }
+  return PMX_OK;
+}
+
+int setup(Sys *s) {
+  const pmx_sys_desc &d = s->d;
+  s->mixed = d.precision == PMX_MIXED;
+  s->n = d.n;
+  s->npad = (d.n + kSuperSize - 1) / kSuperSize * kSuperSize;
+  s->ncl = s->npad / kClusterSize;
+  s->nsuper = s->npad / kSuperSize;
+  s->q.assign(d.charge, d.charge + d.n);
+  s->type.assign(d.type, d.type + d.n);
+  s->lj.assign(d.lj, d.lj + (size_t)d.ntypes * d.ntypes * 4);
+
+  // Exclusions (ExclusionTable.cpp:103-120): table de-duplicated on
+  // (a1,a2,class) in insertion order; lookup class = last added.
+  std::set<std::tuple<int, int, int>> seen;
+  std::map<std::pair<int, int>, int> cls;
+  s->etab.clear();
+  for (int e = 0; e < d.nexcl; ++e) {
+    int a = d.excl[3 * e], b = d.excl[3 * e + 1], c = d.excl[3 * e + 2];
+    if (a > b) std::swap(a, b);
+    if (c != 0 && a != b && !seen.count({a, b, c})) {
+      s->etab.push_back(make_int4(a, b, c, 0));
+      seen.insert({a, b, c});
+    }
+    if (a != b) cls[{a, b}] = c;
+  }
+  std::vector<std::vector<int2>> part(d.n);
+  for (auto &kv : cls) {
+    if (kv.second == 0) continue;
+    part[kv.first.first].push_back(make_int2(kv.first.second, kv.second));
+    part[kv.first.second].push_back(make_int2(kv.first.first, kv.second));
+  }
+  s->eptr.assign(d.n + 1, 0);
+  s->elist.clear();
+  for (int a = 0; a < d.n; ++a) {
+    std::sort(part[a].begin(), part[a].end(), [](int2 x, int2 y) { return x.x < y.x; });
+    s->eptr[a] = (int)s->elist.size();
+    for (auto &v : part[a]) s->elist.push_back(v);
+  }
+  s->eptr[d.n] = (int)s->elist.size();
+
+  // Box (PeriodicBoundaryConditions.cpp:32-77)
+  Box &bx = s->box;
+  for (int k = 0; k < 3; ++k) {
+    bx.L[k] = d.box[k];
+    bx.Linv[k] = 1.0 / d.box[k];
+    bx.o[k] = d.origin[k];
+    double a = d.origin[k] - d.box[k] * 0.5, b = d.origin[k] + d.box[k] * 0.5;
+    bx.mn[k] = std::min(a, b);
+    double mx = std::max(a, b);
+    bx.H[k] = mx - bx.mn[k];
+    bx.H2[k] = bx.H[k] * 0.5;
+  }
+  bx.Lr[0] = (bx.L[1] * bx.L[2]) / (bx.L[0] * (bx.L[1] * bx.L[2]));
+  bx.Lr[1] = (bx.L[2] * bx.L[0]) / (bx.L[1] * (bx.L[2] * bx.L[0]));
+  bx.Lr[2] = (bx.L[0] * bx.L[1]) / (bx.L[2] * (bx.L[0] * bx.L[1]));
+  {
+    double dx = (bx.L[0] * 0.5) * (bx.L[0] * 0.5), dy = (bx.L[1] * 0.5) * (bx.L[1] * 0.5),
+           dz = (bx.L[2] * 0.5) * (bx.L[2] * 0.5);
+    bx.D = std::min(dx, std::min(dy, dz));
+  }
+
+  // Binning parameters: fine grid ~ one atom per cell; reference cell grid
+  // (CubicCellManager.cpp:46-65, ArrayCellListStructure.cpp:40-42).
+  BinParams &bp = s->bin;
+  bp.box = bx;
+  const double vol = bx.L[0] * bx.L[1] * bx.L[2];
+  const double edge = std::cbrt(vol / std::max(1, d.n));
+  int nfmax = 1;
+  for (int k = 0; k < 3; ++k) {
+    int nf = (int)std::lround(bx.L[k] / edge);
+    nf = std::max(1, std::min(128, nf));
+    bp.nf[k] = nf;
+    bp.finv[k] = nf / bx.L[k];
+    nfmax = std::max(nfmax, nf);
+    double H = bx.H[k];
+    double e = H / std::max(1.0, std::floor(H / d.cellsize));
+    bp.rc_inv[k] = 1.0 / e;
+    bp.rc_n[k] = std::max(1, (int)std::floor(H / e + 1.0e-14));
+    bp.delta[k] = bx.o[k] - bx.mn[k];
+  }
+  int p2 = 1;
+  while (p2 < nfmax) p2 <<= 1;
+  s->nbins = (long long)p2 * p2 * p2;
+  s->nscanblocks = (int)((s->nbins + kScanBlock - 1) / kScanBlock);
+
+  // Pair parameters
+  PairParams &pp = s->pp;
+  pp.box = bx;
+  pp.has_lj = d.lj_switch != SW_NONE;
+  pp.lj = make_switch(pp.has_lj ? d.lj_switch : SW_CUTOFF, d.lj_on, pp.has_lj ? d.lj_rc : 0.0);
+  pp.coul = d.coul_kind;
+  if (d.coul_kind == COUL_CUTOFF)
+    pp.co = make_switch(d.coul_switch, d.coul_on, d.coul_rc);
+  else if (d.coul_kind == COUL_PME)
+    pp.co = make_switch(d.coul_switch, 0.0, d.coul_rc);
+  else
+    pp.co = make_switch(SW_CUTOFF, 0.0, 0.0);
+  const double rcmax = std::max(pp.has_lj ? d.lj_rc : 0.0, d.coul_kind ? d.coul_rc : 0.0);
+  pp.rc2max = rcmax * rcmax;
+  const double Lmax = std::max(bx.L[0], std::max(bx.L[1], bx.L[2]));
+  const double eps = s->mixed ? std::ldexp(1.0, -24) : std::ldexp(1.0, -53);
+  // |delta r2| bound: coordinates rounded once (<= L eps), differences and
+  // image shift (<= 3 L eps per component), squares and sums (3 rc^2 eps).
+  pp.guard = 4.0 * (2.0 * std::sqrt(3.0) * rcmax * 3.0 * Lmax * eps + 3.0 * pp.rc2max * eps);
+  pp.csf = d.csf;
+  pp.ntypes = d.ntypes;
+
+  s->pme_on = d.coul_kind == COUL_PME;
+  s->alpha = 0;
+  s->self_e = s->charged_e = 0;
+  if (s->pme_on) {
+    s->alpha = d.pme_alpha > 0 ? d.pme_alpha : choose_alpha(d.coul_rc, d.pme_accuracy);
+    pp.alpha = s->alpha;
+    pp.a2pi = 2.0 * s->alpha / std::sqrt(M_PI);
+    if (d.pme_terms & PMX_PME_CORR) {
+      double q2 = 0, qs = 0;
+      for (int i = 0; i < d.n; ++i) q2 += d.charge[i] * d.charge[i];
+      for (int i = 0; i < d.n; ++i) qs += d.charge[i];
+      s->self_e = -q2 * s->alpha / std::sqrt(M_PI);
+      if (std::fabs(qs * 0.00268283) > 1.0e-5) s->charged_e = -M_PI / (2.0 * vol * s->alpha * s->alpha) * qs * qs;
+    }
+  }
+
+  // List parameters
+  ListParams &lp = s->lp;
+  const double margin = 4.0 * Lmax * std::ldexp(1.0, -23) + 1e-4;
+  for (int k = 0; k < 3; ++k) lp.L[k] = (float)bx.L[k];
+  lp.rlist2 = (float)((rcmax + margin) * (rcmax + margin));
+  lp.ncl = s->ncl;
+  lp.nsuper = s->nsuper;
+  lp.n = d.n;
+  {
+    // generous per-supercluster capacity from the cluster density
+    const double rho_cl = s->ncl / vol;
+    const double reach = 2.0 * rcmax + 24.0;
+    long long c = (long long)std::ceil(rho_cl * reach * reach * reach) + 64;
+    s->cap = (int)std::min<long long>(c, s->ncl);
+    s->cap = (s->cap + kTileEntries - 1) / kTileEntries * kTileEntries;
+  }
+  lp.cap = s->cap;
+  s->ntiles_cap = s->nsuper * (s->cap / kTileEntries);
+  s->maxmaskwork = std::max(1024, s->nsuper * 128);
+
+  int rc;
+  auto A = [&](DevBuf &b, size_t bytes) { return dalloc(s, b, bytes); };
+  const size_t N = d.n, NP = s->npad;
+  if ((rc = A(s->xorig, 24 * N)) || (rc = A(s->xwrap, 24 * N)) || (rc = A(s->q_d, 8 * N)) ||
+      (rc = A(s->type_d, 4 * N)) || (rc = A(s->key, 4 * N)) || (rc = A(s->hist, 4 * s->nbins)) ||
+      (rc = A(s->start, 4 * s->nbins)) || (rc = A(s->cursor, 4 * s->nbins)) ||
+      (rc = A(s->scansum, 4 * (size_t)s->nscanblocks)) || (rc = A(s->perm, 4 * N)) || (rc = A(s->permp, 4 * NP)) ||
+      (rc = A(s->slot, 4 * N)) || (rc = A(s->refcell, 4 * N)) || (rc = A(s->xs, 32 * NP)) ||
+      (rc = A(s->xs64, 32 * NP)) || (rc = A(s->ts, 4 * NP)) || (rc = A(s->ctr, 16 * (size_t)s->ncl)) ||
+      (rc = A(s->half, 16 * (size_t)s->ncl)) || (rc = A(s->entries, 16 * (size_t)s->nsuper * s->cap)) ||
+      (rc = A(s->count, 4 * (size_t)s->nsuper)) || (rc = A(s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
+      (rc = A(s->nmaskwork, 8)) || (rc = A(s->masks, 16 * 8 * (size_t)s->maxmaskwork)) ||
+      (rc = A(s->overflow, 8)) || (rc = A(s->tiles, 8 * (size_t)s->ntiles_cap)) || (rc = A(s->ntiles, 8)) ||
+      (rc = A(s->tilectr, 8)) || (rc = A(s->ljtab, 8 * s->lj.size())) || (rc = A(s->facc, 24 * NP)) ||
+      (rc = A(s->fout, 24 * N)) || (rc = A(s->pair_e, 16 * (size_t)s->ntiles_cap)) ||
+      (rc = A(s->npairs, 8 * (size_t)s->ntiles_cap)) || (rc = A(s->hash, 8 * (size_t)s->ntiles_cap)) ||
+      (rc = A(s->eptr_d, 4 * (N + 1))) || (rc = A(s->elist_d, 8 * s->elist.size())) ||
+      (rc = A(s->etab_d, 16 * s->etab.size())) || (rc = A(s->eout, 64)) || (rc = A(s->np_out, 8)) ||
+      (rc = A(s->hash_out, 8)) || (rc = A(s->exclmax, 8)))
+    return rc;
+  s->corr_blocks = std::max(1, (int)nblk((long long)s->etab.size(), 256));
+  if ((rc = A(s->corr_e, 8 * (size_t)s->corr_blocks))) return rc;
+
+  hipStream_t st = s->ctx->stream;
+  HIPCHK(hipMemcpyAsync(s->q_d.p, s->q.data(), 8 * N, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(s->type_d.p, s->type.data(), 4 * N, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(s->ljtab.p, s->lj.data(), 8 * s->lj.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(s->eptr_d.p, s->eptr.data(), 4 * (N + 1), hipMemcpyHostToDevice, st));
+  if (!s->elist.empty())
+    HIPCHK(hipMemcpyAsync(s->elist_d.p, s->elist.data(), 8 * s->elist.size(), hipMemcpyHostToDevice, st));
+  if (!s->etab.empty())
+    HIPCHK(hipMemcpyAsync(s->etab_d.p, s->etab.data(), 16 * s->etab.size(), hipMemcpyHostToDevice, st));
+
+  if (s->pme_on) {
+    PmeParams &pm = s->pme;
+    pm.nx = d.pme_grid[0];
+    pm.ny = d.pme_grid[1];
+    pm.nz = d.pme_grid[2];
+    pm.nzh = pm.nz / 2 + 1;
+    pm.order = d.pme_order;
+    pm.hxr = pm.nx / bx.L[0];
+    pm.hyr = pm.ny / bx.L[1];
+    pm.hzr = pm.nz / bx.L[2];
+    pm.offx = -(pm.order - 1) / 2 + pm.nx;
+    pm.offy = -(pm.order - 1) / 2 + pm.ny;
+    pm.offz = -(pm.order - 1) / 2 + pm.nz;
+    if (!make_plan(pm.nx, s->plx) || !make_plan(pm.ny, s->ply) || !make_plan(pm.nz, s->plz))
+      return fail(s->ctx, PMX_ERR_UNSUPPORTED, "PME grid sizes must be products of 2, 3 and 5");
+    const size_t G = (size_t)pm.nx * pm.ny * pm.nz, GH = (size_t)pm.nx * pm.ny * pm.nzh;
+    const size_t cxb = s->mixed ? 8 : 16;
+    if ((rc = A(s->grid64, s->mixed ? 8 : 8 * G)) || (rc = A(s->gridfx, s->mixed ? 8 * G : 8)) ||
+        (rc = A(s->spec, cxb * GH)) || (rc = A(s->greal, (s->mixed ? 4 : 8) * G)) ||
+        (rc = A(s->theta, (s->mixed ? 4 : 8) * GH)) || (rc = A(s->twx, cxb * pm.nx)) ||
+        (rc = A(s->twy, cxb * pm.ny)) || (rc = A(s->twz, cxb * pm.nz)))
+      return rc;
+    // Influence function on the half spectrum (Grid.h:237-341 with the
+    // tables of Grid.h:526-575 and dftmod :589-600).
+    const int p = pm.order;
+    std::vector<double> th0(p), dth0(p);
+    bspline_host(p, 0.0, th0.data(), dth0.data());
+    auto dftmod = [&](int nn, std::vector<double> &out) {
+      out.resize(nn);
+      for (int i = 0; i < nn; i++) {
+        double sc = 0.0, ss = 0.0;
+        for (int j = 0; j < p; j++) {
+          double x = M_PI * 2.0 * i * j / (double)nn;
+          sc += th0[j] * std::cos(x);
+          ss += th0[j] * std::sin(x);
+        }
+        out[i] = 1.0 / (sc * sc + ss * ss);
+      }
+    };
+    std::vector<double> bxm, bym, bzm;
+    dftmod(pm.nx, bxm);
+    dftmod(pm.ny, bym);
+    dftmod(pm.nz, bzm);
+    const double W = bx.L[0], Lh = bx.L[1], Hh = bx.L[2];
+    const double Wr = 1.0 / W, Lr = 1.0 / Lh, Hr = 1.0 / Hh, V = W * Lh * Hh;
+    const double fac = M_PI * M_PI / (s->alpha * s->alpha);
+    std::vector<double> ex(pm.nx), ey(pm.ny), ez(pm.nzh);
+    for (int i = 0; i < pm.nx; i++) {
+      int i0 = i <= pm.nx / 2 ? i : i - pm.nx;
+      ex[i] = std::exp(-fac * (i0 * Wr) * (i0 * Wr));
+    }
+    for (int j = 0; j < pm.ny; j++) {
+      int j0 = j <= pm.ny / 2 ? j : j - pm.ny;
+      ey[j] = std::exp(-fac * (j0 * Lr) * (j0 * Lr));
+    }
+    for (int k = 0; k < pm.nzh; k++) ez[k] = std::exp(-fac * (k * Hr) * (k * Hr));
+    std::vector<double> th(GH);
+    const double piVr = 1.0 / (M_PI * V);
+    for (int i = 0; i < pm.nx; i++) {
+      int i0 = i <= pm.nx / 2 ? i : i - pm.nx;
+      double mi = i0 * Wr, exi = ex[i] * piVr;
+      for (int j = 0; j < pm.ny; j++) {
+        int j0 = j <= pm.ny / 2 ? j : j - pm.ny;
+        double bxy = bxm[i] * bym[j], mj = j0 * Lr, mij = mi * mi + mj * mj, exy = exi * ey[j];
+        for (int k = 0; k < pm.nzh; k++) {
+          double v = 0.0;
+          if (i != 0 || j != 0 || k != 0) {
+            double mk = k * Hr, m2 = mij + mk * mk;
+            v = bxy * bzm[k] * exy * ez[k] / m2;
+          }
+          th[((size_t)i * pm.ny + j) * pm.nzh + k] = v;
+        }
+      }
+    }
+    auto tw = [&](int nn, DevBuf &b) -> int {
+      std::vector<double> t(2 * nn);
+      for (int m = 0; m < nn; ++m) {
+        t[2 * m] = std::cos(-2.0 * M_PI * m / nn);
+        t[2 * m + 1] = std::sin(-2.0 * M_PI * m / nn);
+      }
+      if (s->mixed) {
+        std::vector<float> tf(t.begin(), t.end());
+        return hipMemcpy(b.p, tf.data(), 4 * tf.size(), hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+      }
+      return hipMemcpy(b.p, t.data(), 8 * t.size(), hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+    };
+    if (tw(pm.nx, s->twx) || tw(pm.ny, s->twy) || tw(pm.nz, s->twz))
+      return fail(s->ctx, PMX_ERR_HIP, "twiddle upload failed");
+    if (s->mixed) {
+      std::vector<float> thf(th.begin(), th.end());
+      HIPCHK(hipMemcpy(s->theta.p, thf.data(), 4 * GH, hipMemcpyHostToDevice));
+    } else {
+      HIPCHK(hipMemcpy(s->theta.p, th.data(), 8 * GH, hipMemcpyHostToDevice));
+    }
+    // FFT blocking: two LDS buffers of lines_per_block * n complex <= 64 KiB
+    const int cpx = s->mixed ? 8 : 16;
+    const int budget = 61440 / (2 * cpx);  // 60 KiB of the 64 KiB workgroup LDS
+    s->zl = std::max(1, std::min(pm.nx * pm.ny, budget / pm.nz));
+    s->tky = std::max(1, std::min(pm.nzh, budget / pm.ny));
+    s->tkx = std::max(1, std::min(pm.nzh, budget / pm.nx));
+    s->recip_blocks = pm.ny * ((pm.nzh + s->tkx - 1) / s->tkx);
+    if ((rc = A(s->recip_e, 8 * (size_t)s->recip_blocks))) return rc;
+  }
+  for (auto &e : s->ev)
+    if (!e) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipStreamSynchronize(st));
+  s->cached = true;
+  return PMX_OK;
+}
+
+template <bool MIXED>
+int run_pair(Sys *s, bool stats) {
+  hipStream_t st = s->ctx->stream;
+  using R4 = typename Real4<typename std::conditional<MIXED, float, double>::type>::type;
+  int dev_cus = 256;
+  hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, s->ctx->device);
+  const unsigned grid = (unsigned)std::max(1, dev_cus * 8);
+  PairOut out{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash)};
+  auto *facc = P<typename AccT<MIXED>::type>(s->facc);
+#define PMX_LAUNCH_PAIR(C, STATS)                                                                               \
+  hipLaunchKernelGGL((k_pair<MIXED, C, STATS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, P<int2>(s->tiles), \
+                     P<int>(s->ntiles), P<int>(s->tilectr), P<int4>(s->entries), P<int>(s->count), s->cap, \
+                     P<unsigned long long>(s->masks), P<R4>(s->xs), P<int>(s->ts), P<int>(s->permp),         \
+                     P<double>(s->xorig), P<double>(s->ljtab), facc, out)
+  if (s->pp.coul == COUL_PME) {
+    if (stats) PMX_LAUNCH_PAIR(COUL_PME, true); else PMX_LAUNCH_PAIR(COUL_PME, false);
+  } else if (s->pp.coul == COUL_CUTOFF) {
+    if (stats) PMX_LAUNCH_PAIR(COUL_CUTOFF, true); else PMX_LAUNCH_PAIR(COUL_CUTOFF, false);
+  } else {
+    if (stats) PMX_LAUNCH_PAIR(COUL_NONE, true); else PMX_LAUNCH_PAIR(COUL_NONE, false);
+  }
+#undef PMX_LAUNCH_PAIR
+  return PMX_OK;
+}
+
+template <bool MIXED>
+int run_pme(Sys *s, bool timing) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  hipStream_t st = s->ctx->stream;
+  const PmeParams &pm = s->pme;
+  const size_t G = (size_t)pm.nx * pm.ny * pm.nz;
+  auto *facc = P<typename AccT<MIXED>::type>(s->facc);
+  if (s->d.pme_terms & PMX_PME_RECIP) {
+    if (MIXED)
+      hipLaunchKernelGGL(k_fill_u64, dim3(nblk(G, 256)), dim3(256), 0, st, P<unsigned long long>(s->gridfx), 0ull,
+                         (long long)G);
+    else
+      hipLaunchKernelGGL(k_fill_f64, dim3(nblk(G, 256)), dim3(256), 0, st, P<double>(s->grid64), 0.0, (long long)G);
+    hipLaunchKernelGGL((k_spread<MIXED>), dim3(nblk(s->n, 128)), dim3(128), 0, st, s->n, pm, P<double>(s->xorig),
+                       P<int>(s->permp), P<double>(s->q_d), P<double>(s->grid64),
+                       P<unsigned long long>(s->gridfx));
+    if (timing) hipEventRecord(s->ev[4], st);
+    const size_t cx = sizeof(Cx<T>);
+    const int zb = (pm.nx * pm.ny + s->zl - 1) / s->zl;
+    hipLaunchKernelGGL((k_fft_z_r2c<T, MIXED>), dim3(zb), dim3(256), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
+                       P<double>(s->grid64), P<unsigned long long>(s->gridfx), P<Cx<T>>(s->twz), P<Cx<T>>(s->spec));
+    const int yb = pm.nx * ((pm.nzh + s->tky - 1) / s->tky);
+    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(256), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, false,
+                       P<Cx<T>>(s->twy), P<Cx<T>>(s->spec));
+    hipLaunchKernelGGL((k_fft_x_conv<T>), dim3(s->recip_blocks), dim3(256), 2 * s->tkx * pm.nx * cx, st, pm, s->plx,
+                       s->tkx, P<Cx<T>>(s->twx), P<T>(s->theta), P<Cx<T>>(s->spec), P<double>(s->recip_e));
+    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(256), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, true,
+                       P<Cx<T>>(s->twy), P<Cx<T>>(s->spec));
+    hipLaunchKernelGGL((k_fft_z_c2r<T>), dim3(zb), dim3(256), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
+                       P<Cx<T>>(s->twz), P<Cx<T>>(s->spec), P<T>(s->greal));
+    if (timing) hipEventRecord(s->ev[5], st);
+    hipLaunchKernelGGL((k_gather<MIXED>), dim3(nblk(s->n, 128)), dim3(128), 0, st, s->n, pm, P<double>(s->xorig),
+                       P<int>(s->permp), P<double>(s->q_d), P<T>(s->greal), facc);
+  } else if (timing) {
+    hipEventRecord(s->ev[4], st);
+    hipEventRecord(s->ev[5], st);
+  }
+  if (timing) hipEventRecord(s->ev[6], st);
+  if ((s->d.pme_terms & PMX_PME_CORR) && !s->etab.empty())
+    hipLaunchKernelGGL((k_correction<MIXED>), dim3(s->corr_blocks), dim3(256), 0, st, (int)s->etab.size(),
+                       P<int4>(s->etab_d), P<double>(s->xorig), P<double>(s->q_d), P<int>(s->slot), s->box, s->alpha,
+                       s->d.csf, facc, P<double>(s->corr_e));
+  return PMX_OK;
+}
+
+int compute(Sys *s, unsigned flags) {
+  hipStream_t st = s->ctx->stream;
+  const bool timing = flags & PMX_EVAL_TIMING;
+  const bool stats = flags & PMX_EVAL_PAIRSTATS;
+  const int n = s->n, npad = s->npad;
+  if (timing) hipEventRecord(s->ev[0], st);
+  // --- sort
+  hipLaunchKernelGGL(k_bin, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<double>(s->xorig), s->bin, P<uint32_t>(s->key),
+                     P<double>(s->xwrap), P<int>(s->refcell));
+  hipLaunchKernelGGL(k_fill_u32, dim3(nblk(s->nbins, 256)), dim3(256), 0, st, P<uint32_t>(s->hist), 0u, s->nbins);
+  hipLaunchKernelGGL(k_hist, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<uint32_t>(s->key), P<uint32_t>(s->hist));
+  hipLaunchKernelGGL(k_scan_blocksums, dim3(s->nscanblocks), dim3(kScanBlock), 0, st, P<uint32_t>(s->hist), s->nbins,
+                     P<uint32_t>(s->scansum));
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, st, P<uint32_t>(s->scansum), s->nscanblocks);
+  hipLaunchKernelGGL(k_scan_final, dim3(s->nscanblocks), dim3(kScanBlock), 0, st, P<uint32_t>(s->hist), s->nbins,
+                     P<uint32_t>(s->scansum), P<uint32_t>(s->start));
+  HIPCHK(hipMemcpyAsync(s->cursor.p, s->start.p, 4 * s->nbins, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<uint32_t>(s->key), P<uint32_t>(s->cursor),
+                     P<int>(s->perm));
+  hipLaunchKernelGGL(k_binsort, dim3(nblk(s->nbins, 256)), dim3(256), 0, st, s->nbins, P<uint32_t>(s->start), n,
+                     P<int>(s->perm));
+  if (s->mixed)
+    hipLaunchKernelGGL(k_fill_sorted<float>, dim3(nblk(npad, 256)), dim3(256), 0, st, n, npad, P<int>(s->perm),
+                       P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<float4>(s->xs),
+                       P<double4>(s->xs64), P<int>(s->ts), P<int>(s->slot), P<int>(s->permp));
+  else
+    hipLaunchKernelGGL(k_fill_sorted<double>, dim3(nblk(npad, 256)), dim3(256), 0, st, n, npad, P<int>(s->perm),
+                       P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<double4>(s->xs),
+                       P<double4>(s->xs64), P<int>(s->ts), P<int>(s->slot), P<int>(s->permp));
+  if (timing) hipEventRecord(s->ev[1], st);
+  // --- cluster list
+  hipLaunchKernelGGL(k_cluster_bbox, dim3(nblk(s->ncl, 256)), dim3(256), 0, st, s->ncl, n, P<double4>(s->xs64),
+                     P<float4>(s->ctr), P<float4>(s->half));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, P<uint32_t>(s->exclmax), 0u, 2ll);
+  if (!s->etab.empty())
+    hipLaunchKernelGGL(k_excl_maxdist, dim3(nblk((long long)s->etab.size(), 256)), dim3(256), 0, st,
+                       (int)s->etab.size(), P<int4>(s->etab_d), P<double>(s->xwrap), s->box, P<unsigned>(s->exclmax));
+  hipLaunchKernelGGL(k_set_rexcl, dim3(1), dim3(1), 0, st, P<unsigned>(s->exclmax), P<float>(s->exclmax) + 1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->nmaskwork), 0, 2ll);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
+  hipLaunchKernelGGL(k_build_list, dim3(nblk((long long)s->nsuper * 64, 256)), dim3(256), 0, st, s->lp,
+                     P<float>(s->exclmax) + 1, P<float4>(s->ctr), P<float4>(s->half), P<int4>(s->entries),
+                     P<int>(s->count), P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork,
+                     P<int>(s->overflow));
+  hipLaunchKernelGGL(k_build_masks, dim3(nblk((long long)s->maxmaskwork * 64, 256)), dim3(256), 0, st,
+                     P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork, P<int4>(s->entries), s->cap, n,
+                     P<int>(s->permp), P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned long long>(s->masks));
+  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int2>(s->tiles),
+                     P<int>(s->ntiles));
+  if (timing) hipEventRecord(s->ev[2], st);
+  // --- forces
+  hipLaunchKernelGGL(k_fill_u64, dim3(nblk(3ll * npad, 256)), dim3(256), 0, st, P<unsigned long long>(s->facc), 0ull,
+                     3ll * npad);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->tilectr), 0, 2ll);
+  int rc = s->mixed ? run_pair<true>(s, stats) : run_pair<false>(s, stats);
+  if (rc) return rc;
+  if (timing) hipEventRecord(s->ev[3], st);
+  if (s->pme_on) {
+    rc = s->mixed ? run_pme<true>(s, timing) : run_pme<false>(s, timing);
+    if (rc) return rc;
+  } else if (timing) {
+    hipEventRecord(s->ev[4], st);
+    hipEventRecord(s->ev[5], st);
+    hipEventRecord(s->ev[6], st);
+  }
+  if (timing) hipEventRecord(s->ev[7], st);
+  // --- finalize
+  if (s->mixed)
+    hipLaunchKernelGGL(k_unsort_forces<true>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
+                       P<unsigned long long>(s->facc), P<double>(s->fout));
+  else
+    hipLaunchKernelGGL(k_unsort_forces<false>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
+                       P<double>(s->facc), P<double>(s->fout));
+  const bool recip = s->pme_on && (s->d.pme_terms & PMX_PME_RECIP);
+  const bool corr = s->pme_on && (s->d.pme_terms & PMX_PME_CORR) && !s->etab.empty();
+  hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(1024), 0, st, s->ntiles_cap, P<int>(s->ntiles),
+                     P<double>(s->pair_e), recip ? s->recip_blocks : 0, P<double>(s->recip_e),
+                     corr ? s->corr_blocks : 0, P<double>(s->corr_e), P<long long>(s->npairs),
+                     P<unsigned long long>(s->hash), stats ? 1 : 0, P<double>(s->eout), P<long long>(s->np_out),
+                     P<unsigned long long>(s->hash_out));
+  if (timing) hipEventRecord(s->ev[8], st);
+  HIPCHK(hipGetLastError());
+  return PMX_OK;
+}
+
+int finish(Sys *s, unsigned flags, double *f_aos, double *energies) {
+  hipStream_t st = s->ctx->stream;
+  int ovf = 0;
+  HIPCHK(hipMemcpyAsync(&ovf, s->overflow.p, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(s->eout_h, s->eout.p, 32, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&s->last_np, s->np_out.p, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&s->last_hash, s->hash_out.p, 8, hipMemcpyDeviceToHost, st));
+  std::vector<double> f;
+  if (f_aos) {
+    f.resize(3 * (size_t)s->n);
+    HIPCHK(hipMemcpyAsync(f.data(), s->fout.p, 24 * (size_t)s->n, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  if (ovf) return fail(s->ctx, PMX_ERR_OVERFLOW, "cluster pair list capacity exceeded");
+  if (flags & PMX_EVAL_TIMING)
+    for (int k = 0; k < 8; ++k) hipEventElapsedTime(&s->stage_ms[k], s->ev[k], s->ev[k + 1]);
+  if (f_aos)
+    for (size_t k = 0; k < f.size(); ++k) f_aos[k] += f[k];
+  if (energies) {
+    double ec = s->eout_h[0];
+    if (s->pme_on) ec += s->eout_h[2] + s->eout_h[3] + s->self_e + s->charged_e;
+    energies[PMX_E_COULOMB] += ec;
+    energies[PMX_E_LENNARDJONES] += s->eout_h[1];
+  }
+  return PMX_OK;
+}
+
+int grow_and_retry(Sys *s) {
+  // double the list capacity and the mask pool, then rebuild buffers
+  s->cap = std::min(s->ncl + kTileEntries, 2 * s->cap);
+  s->cap = (s->cap + kTileEntries - 1) / kTileEntries * kTileEntries;
+  s->lp.cap = s->cap;
+  s->ntiles_cap = s->nsuper * (s->cap / kTileEntries);
+  s->maxmaskwork *= 2;
+  int rc;
+  if ((rc = dalloc(s, s->entries, 16 * (size_t)s->nsuper * s->cap)) || (rc = dalloc(s, s->tiles, 8 * (size_t)s->ntiles_cap)) ||
+      (rc = dalloc(s, s->pair_e, 16 * (size_t)s->ntiles_cap)) || (rc = dalloc(s, s->npairs, 8 * (size_t)s->ntiles_cap)) ||
+      (rc = dalloc(s, s->hash, 8 * (size_t)s->ntiles_cap)) || (rc = dalloc(s, s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
+      (rc = dalloc(s, s->masks, 16 * 8 * (size_t)s->maxmaskwork)))
+    return rc;
+  return PMX_OK;
+}
+
+}  // namespace
+
+struct pmx_sys {
+  Sys s;
+};
+
+extern "C" {
+
+unsigned long long pmx_pair_hash(int i, int j) {
+  if (i > j) std::swap(i, j);
+  return pair_hash((uint32_t)i, (uint32_t)j);
+}
+
+int pmx_ctx_create(int device, pmx_ctx **out) {
+  if (!out) return PMX_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PMX_ERR_NODEV;
+  if (device < 0 || device >= ndev) return PMX_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return PMX_ERR_HIP;
+  // The code objects in this library are gfx950 only.
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return PMX_ERR_HIP;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PMX_ERR_NODEV;
+  pmx_ctx *c = new pmx_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return PMX_ERR_HIP;
+  }
+  *out = c;
+  return PMX_OK;
+}
+
+int pmx_ctx_destroy(pmx_ctx *ctx) {
+  if (!ctx) return PMX_ERR_ARG;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PMX_OK;
+}
+
+const char *pmx_last_error(const pmx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pmx_sys_create(pmx_ctx *ctx, const pmx_sys_desc *desc, pmx_sys **out) {
+  if (!ctx || !out) return PMX_ERR_ARG;
+  *out = nullptr;
+  int rc = validate(ctx, desc);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  pmx_sys *ps = new pmx_sys();
+  ps->s.ctx = ctx;
+  ps->s.d = *desc;
+  rc = setup(&ps->s);
+  if (rc) {
+    pmx_sys_destroy(ps);
+    return rc;
+  }
+  *out = ps;
+  return PMX_OK;
+}
+
+int pmx_sys_destroy(pmx_sys *sys) {
+  if (!sys) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  std::set<void *> freed;
+  for (DevBuf *b : s->all)
+    if (b->p && !freed.count(b->p)) {
+      freed.insert(b->p);
+      hipFree(b->p);
+    }
+  for (auto &e : s->ev)
+    if (e) hipEventDestroy(e);
+  delete sys;
+  return PMX_OK;
+}
+
+int pmx_sys_uncache(pmx_sys *sys) {
+  if (!sys) return PMX_ERR_ARG;
+  sys->s.cached = false;
+  return PMX_OK;
+}
+
+int pmx_sys_set_positions(pmx_sys *sys, const double *x_aos) {
+  if (!sys || !x_aos) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  hipSetDevice(s->ctx->device);
+  for (size_t k = 0; k < 3 * (size_t)s->n; ++k)
+    if (!std::isfinite(x_aos[k])) return fail(s->ctx, PMX_ERR_ARG, "non-finite position");
+  HIPCHK(hipMemcpyAsync(s->xorig.p, x_aos, 24 * (size_t)s->n, hipMemcpyHostToDevice, s->ctx->stream));
+  s->have_positions = true;
+  return PMX_OK;
+}
+
+int pmx_sys_compute(pmx_sys *sys, unsigned flags) {
+  if (!sys) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  if (!s->have_positions) return fail(s->ctx, PMX_ERR_ARG, "positions not set");
+  hipSetDevice(s->ctx->device);
+  if (!s->cached) {
+    int rc = setup(s);
+    if (rc) return rc;
+  }
+  return compute(s, flags);
+}
+
+int pmx_sys_get_forces(pmx_sys *sys, double *f_aos, double *energies) {
+  if (!sys) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  hipSetDevice(s->ctx->device);
+  return finish(s, 0, f_aos, energies);
+}
+
+int pmx_sys_sync(pmx_sys *sys) {
+  if (!sys) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return PMX_OK;
+}
+
+int pmx_sys_evaluate(pmx_sys *sys, const double *x_aos, double *f_aos, double *energies, unsigned flags) {
+  if (!sys || !x_aos || !f_aos || !energies) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  int rc = pmx_sys_set_positions(sys, x_aos);
+  if (rc) return rc;
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    rc = pmx_sys_compute(sys, flags);
+    if (rc) return rc;
+    rc = finish(s, flags, f_aos, energies);
+    if (rc != PMX_ERR_OVERFLOW) return rc;
+    if ((rc = grow_and_retry(s))) return rc;
+  }
+  return fail(s->ctx, PMX_ERR_OVERFLOW, "cluster pair list capacity exceeded after retries");
+}
+
+int pmx_sys_alpha(const pmx_sys *sys, double *alpha) {
+  if (!sys || !alpha) return PMX_ERR_ARG;
+  *alpha = sys->s.alpha;
+  return PMX_OK;
+}
+
+int pmx_sys_pair_stats(const pmx_sys *sys, long long *npairs, unsigned long long *hash) {
+  if (!sys || !npairs || !hash) return PMX_ERR_ARG;
+  *npairs = sys->s.last_np;
+  *hash = sys->s.last_hash;
+  return PMX_OK;
+}
+
+int pmx_sys_cells(pmx_sys *sys, int *cell_of_atom, int *dims) {
+  if (!sys || !cell_of_atom || !dims) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  hipSetDevice(s->ctx->device);
+  HIPCHK(hipMemcpyAsync(cell_of_atom, s->refcell.p, 4 * (size_t)s->n, hipMemcpyDeviceToHost, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  for (int k = 0; k < 3; ++k) dims[k] = s->bin.rc_n[k];
+  return PMX_OK;
+}
+
+int pmx_sys_stage_times(const pmx_sys *sys, float *ms) {
+  if (!sys || !ms) return PMX_ERR_ARG;
+  for (int k = 0; k < 8; ++k) ms[k] = sys->s.stage_ms[k];
+  return PMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1265 @@
+// HIP kernels for the ProtoMol nonbonded hot path on gfx950 (CDNA4).
+//
+// Data layout in HBM (per system):
+//   xorig  double[3N]  positions as the caller hands them (AoS, original order)
+//   perm   int[Npad]   sorted slot -> original atom (-1 for padding slots)
+//   slot   int[N]      original atom -> sorted slot
+//   xs     R4[Npad]    wrapped coordinates + scaled charge, sorted (R = double
+//                      for the FP64 form, float for the mixed form), SoA-by-atom
+//   ts     int[Npad]   LJ type per sorted slot (-1 padding)
+//   facc   Acc[3Npad]  force accumulators in sorted order (double, or int64
+//                      fixed point 2^-32 for the mixed form)
+// Clusters are 8 consecutive sorted slots; superclusters 64 (one wavefront).
+#pragma once
+#include <type_traits>
+
+#include "pmx_common.h"
+
+namespace pmx {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+template <class T>
+struct Real4;
+template <>
+struct Real4<double> {
+  using type = double4;
+};
+template <>
+struct Real4<float> {
+  using type = float4;
+};
+
+__device__ inline uint32_t expand_bits10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return (expand_bits10(x) << 2) | (expand_bits10(y) << 1) | expand_bits10(z);
+}
+
+// Reference minimal image in FP64 with the reference's operation order and
+// no FMA contraction (PeriodicBoundaryConditions.h:72-110).  Returns d^2.
+__device__ inline double exact_d2(const double *__restrict__ x, int a, int b, const Box &bx) {
+#pragma clang fp contract(off)
+  double d[3] = {x[3 * b] - x[3 * a], x[3 * b + 1] - x[3 * a + 1], x[3 * b + 2] - x[3 * a + 2]};
+  double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (d2 > bx.D) {
+    for (int k = 0; k < 3; ++k) {
+      if (d[k] < -bx.H2[k])
+        d[k] += bx.H[k];
+      else if (d[k] > bx.H2[k])
+        d[k] -= bx.H[k];
+    }
+    d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (d2 > bx.D) {
+      for (int k = 0; k < 3; ++k) d[k] -= bx.L[k] * rint(bx.Lr[k] * d[k]);
+      d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    }
+  }
+  return d2;
+}
+
+// Reference minimalPosition (PeriodicBoundaryConditions.h:113-147), FP64, no contraction.
+__device__ inline void exact_minpos(const double *p, const Box &bx, double *out) {
+#pragma clang fp contract(off)
+  double d[3] = {p[0] - bx.o[0], p[1] - bx.o[1], p[2] - bx.o[2]};
+  if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] <= bx.D) {
+    out[0] = d[0];
+    out[1] = d[1];
+    out[2] = d[2];
+    return;
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (d[k] < -bx.H2[k])
+      d[k] += bx.H[k];
+    else if (d[k] > bx.H2[k])
+      d[k] -= bx.H[k];
+  }
+  if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] <= bx.D) {
+    out[0] = d[0];
+    out[1] = d[1];
+    out[2] = d[2];
+    return;
+  }
+  for (int k = 0; k < 3; ++k) out[k] = d[k] - bx.L[k] * rint(bx.Lr[k] * d[k]);
+}
+
+template <class T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// 1. Binning: wrapped coordinates, fine-grid Morton key, reference cell.
+struct BinParams {
+  Box box;
+  int nf[3];          // fine grid for clustering
+  double finv[3];     // nf / L
+  int rc_n[3];        // reference cell grid
+  double rc_inv[3];   // 1 / reference cell edge
+  double delta[3];    // origin - min
+};
+
+__global__ void k_bin(int n, const double *__restrict__ xorig, BinParams p, uint32_t *__restrict__ key,
+                      double *__restrict__ xwrap, int *__restrict__ refcell) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  double w[3];
+  int f[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double x = xorig[3 * a + k];
+    double v = x - p.box.L[k] * floor(x * p.box.Linv[k]);
+    if (v >= p.box.L[k]) v -= p.box.L[k];
+    if (v < 0.0) v = 0.0;
+    w[k] = v;
+    int c = (int)(v * p.finv[k]);
+    f[k] = c < 0 ? 0 : (c >= p.nf[k] ? p.nf[k] - 1 : c);
+  }
+  key[a] = morton3(f[0], f[1], f[2]);
+  xwrap[3 * a] = w[0];
+  xwrap[3 * a + 1] = w[1];
+  xwrap[3 * a + 2] = w[2];
+  // Reference cell: findCell(delta + minimalPosition(x)) (Topology.h:64-66,
+  // CubicCellManager.cpp:40-44), an upper-face index wrapped to its image.
+  double mp[3];
+  exact_minpos(&xorig[3 * a], p.box, mp);
+  int c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double t;
+    {
+#pragma clang fp contract(off)
+      t = (p.delta[k] + mp[k]) * p.rc_inv[k];
+    }
+    int ci = (int)floor(t);
+    ci %= p.rc_n[k];
+    if (ci < 0) ci += p.rc_n[k];
+    c[k] = ci;
+  }
+  refcell[a] = (c[0] * p.rc_n[1] + c[1]) * p.rc_n[2] + c[2];
+}
+
+// Counting sort by key (one radix digit spanning the key range): histogram,
+// exclusive scan, scatter, then order each bin by atom index so the result is
+// deterministic and stable.
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_i32(int *p, int v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f64(double *p, double v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_hist(int n, const uint32_t *__restrict__ key, uint32_t *__restrict__ hist) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < n) atomicAdd(&hist[key[a]], 1u);
+}
+
+// Exclusive scan of m uint32 in 3 launches: per-block sums, scan of sums in
+// one block, then per-block rescan with the block offset.
+constexpr int kScanBlock = 1024;
+__global__ void k_scan_blocksums(const uint32_t *__restrict__ in, long long m, uint32_t *__restrict__ sums) {
+  __shared__ uint32_t s[kScanBlock / kWave];
+  long long i = (long long)blockIdx.x * kScanBlock + threadIdx.x;
+  uint32_t v = i < m ? in[i] : 0u;
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kScanBlock / kWave; ++w) t += s[w];
+    sums[blockIdx.x] = t;
+  }
+}
+__global__ void k_scan_sums(uint32_t *sums, int nb) {
+  // single block: serial over chunks of 1024 with a block-wide scan
+  __shared__ uint32_t s[kScanBlock];
+  uint32_t carry = 0;
+  for (int base = 0; base < nb; base += kScanBlock) {
+    int i = base + threadIdx.x;
+    uint32_t v = i < nb ? sums[i] : 0u;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+      uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) sums[i] = carry + s[threadIdx.x] - v;  // exclusive
+    __syncthreads();
+    carry += s[kScanBlock - 1];
+    __syncthreads();
+  }
+}
+__global__ void k_scan_final(const uint32_t *__restrict__ in, long long m, const uint32_t *__restrict__ sums,
+                             uint32_t *__restrict__ out) {
+  __shared__ uint32_t s[kScanBlock];
+  long long i = (long long)blockIdx.x * kScanBlock + threadIdx.x;
+  uint32_t v = i < m ? in[i] : 0u;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < m) out[i] = sums[blockIdx.x] + s[threadIdx.x] - v;
+}
+
+__global__ void k_scatter(int n, const uint32_t *__restrict__ key, uint32_t *__restrict__ cursor,
+                          int *__restrict__ perm) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < n) perm[atomicAdd(&cursor[key[a]], 1u)] = a;
+}
+
+// Orders each bin's slots by atom index (bins hold a handful of atoms).
+__global__ void k_binsort(long long nbins, const uint32_t *__restrict__ start, int n, int *__restrict__ perm) {
+  long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  int s0 = start[b];
+  int s1 = b + 1 < nbins ? (int)start[b + 1] : n;
+  for (int i = s0 + 1; i < s1; ++i) {
+    int v = perm[i], j = i - 1;
+    while (j >= s0 && perm[j] > v) {
+      perm[j + 1] = perm[j];
+      --j;
+    }
+    perm[j + 1] = v;
+  }
+}
+
+// Sorted SoA: wrapped coordinates + charge, type, inverse permutation.
+template <class R>
+__global__ void k_fill_sorted(int n, int npad, const int *__restrict__ perm, const double *__restrict__ xwrap,
+                              const double *__restrict__ q, const int *__restrict__ type,
+                              typename Real4<R>::type *__restrict__ xs, double4 *__restrict__ xs64,
+                              int *__restrict__ ts, int *__restrict__ slot, int *__restrict__ permp) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= npad) return;
+  typename Real4<R>::type v;
+  if (s < n) {
+    int a = perm[s];
+    slot[a] = s;
+    permp[s] = a;
+    v.x = (R)xwrap[3 * a];
+    v.y = (R)xwrap[3 * a + 1];
+    v.z = (R)xwrap[3 * a + 2];
+    v.w = (R)q[a];
+    ts[s] = type[a];
+    xs64[s] = make_double4(xwrap[3 * a], xwrap[3 * a + 1], xwrap[3 * a + 2], q[a]);
+  } else {
+    v.x = v.y = v.z = v.w = (R)0;
+    ts[s] = -1;
+    permp[s] = -1;
+    xs64[s] = make_double4(0, 0, 0, 0);
+  }
+  xs[s] = v;
+}
+
+// ---------------------------------------------------------------------------
+// 2. Cluster bounding boxes (FP32; list-building only) and the cluster list.
+__global__ void k_cluster_bbox(int ncl, int n, const double4 *__restrict__ xs64, float4 *__restrict__ ctr,
+                               float4 *__restrict__ half) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  int cnt = 0;
+  for (int t = 0; t < kClusterSize; ++t) {
+    int s = c * kClusterSize + t;
+    if (s >= n) break;
+    double4 p = xs64[s];
+    lo[0] = fmin(lo[0], p.x);
+    lo[1] = fmin(lo[1], p.y);
+    lo[2] = fmin(lo[2], p.z);
+    hi[0] = fmax(hi[0], p.x);
+    hi[1] = fmax(hi[1], p.y);
+    hi[2] = fmax(hi[2], p.z);
+    ++cnt;
+  }
+  if (cnt == 0) {
+    ctr[c] = make_float4(0, 0, 0, 0);
+    half[c] = make_float4(-1, -1, -1, 0);  // empty
+    return;
+  }
+  ctr[c] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])), (float)(0.5 * (lo[2] + hi[2])),
+                       (float)cnt);
+  // round the half extents up by a relative ulp margin so the FP32 box contains the FP64 atoms
+  half[c] = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.0000005f + 1e-6f,
+                        (float)(0.5 * (hi[1] - lo[1])) * 1.0000005f + 1e-6f,
+                        (float)(0.5 * (hi[2] - lo[2])) * 1.0000005f + 1e-6f, 0.f);
+}
+
+__device__ inline float bbox_dist2(float4 ca, float4 ha, float4 cb, float4 hb, const float *L) {
+  float d2 = 0.f;
+  float dc[3] = {fabsf(ca.x - cb.x), fabsf(ca.y - cb.y), fabsf(ca.z - cb.z)};
+  float hs[3] = {ha.x + hb.x, ha.y + hb.y, ha.z + hb.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float d = fminf(dc[k], L[k] - dc[k]);
+    float g = fmaxf(0.f, d - hs[k]);
+    d2 += g * g;
+  }
+  return d2;
+}
+
+struct ListParams {
+  float L[3];
+  float rlist2;      // (rc_max + margin)^2
+  float rexcl2;      // (max excluded-pair distance + margin)^2: closer cluster pairs get explicit masks
+  int ncl;           // clusters
+  int nsuper;
+  int cap;           // entries per supercluster
+  int n;             // real atoms
+};
+
+__global__ void k_excl_maxdist(int ne, const int4 *__restrict__ etab, const double *__restrict__ xwrap, Box box,
+                               unsigned int *__restrict__ out) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int4 t = etab[e];
+  double d2 = 0;
+  for (int k = 0; k < 3; ++k) {
+    double d = xwrap[3 * t.y + k] - xwrap[3 * t.x + k];
+    d -= box.L[k] * rint(d * box.Linv[k]);
+    d2 += d * d;
+  }
+  float r = (float)sqrt(d2);
+  atomicMax(out, __float_as_uint(r));
+}
+
+__global__ void k_set_rexcl(const unsigned int *__restrict__ in, float *__restrict__ out2) {
+  float r = __uint_as_float(*in) * 1.0001f + 1e-3f;
+  *out2 = r * r;
+}
+
+// The list kernel takes rexcl2 from device memory (computed each step).
+__global__ void __launch_bounds__(256) k_build_list(ListParams p, const float *__restrict__ rexcl2,
+                                                        const float4 *__restrict__ ctr,
+                                                        const float4 *__restrict__ half, int4 *__restrict__ entries,
+                                                        int *__restrict__ count, int *__restrict__ maskwork,
+                                                        int *__restrict__ nmaskwork, int maxmaskwork,
+                                                        int *__restrict__ overflow) {
+  // forward to the list builder with the device-resident exclusion radius
+  ListParams q = p;
+  q.rexcl2 = *rexcl2;
+  // body identical to k_build_list (inlined copy keeps one launch)
+  const int lane = threadIdx.x & 63;
+  const int S = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6);
+  if (S >= q.nsuper) return;
+  float4 ic[8], ih[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    int ci = S * 8 + c;
+    ic[c] = ci < q.ncl ? ctr[ci] : make_float4(0, 0, 0, 0);
+    ih[c] = ci < q.ncl ? half[ci] : make_float4(-1, -1, -1, 0);
+  }
+  int total = 0;
+  for (int base = S * 8; base < q.ncl; base += kWave) {
+    int cj = base + lane;
+    uint32_t imask = 0;
+    bool need = false;
+    if (cj < q.ncl) {
+      float4 jc = ctr[cj], jh = half[cj];
+      if (jh.x >= 0.f) {
+        bool jpartial = (cj + 1) * kClusterSize > q.n;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          int ci = S * 8 + c;
+          if (ih[c].x < 0.f || ci > cj) continue;
+          float d2 = bbox_dist2(ic[c], ih[c], jc, jh, q.L);
+          if (d2 <= q.rlist2) {
+            imask |= 1u << c;
+            bool ipartial = (ci + 1) * kClusterSize > q.n;
+            if (ci == cj || ipartial || jpartial || d2 <= q.rexcl2) need = true;
+          }
+        }
+      }
+    }
+    unsigned long long act = __ballot(imask != 0);
+    int rank = __popcll(act & ((1ull << lane) - 1ull));
+    if (imask != 0) {
+      int e = total + rank;
+      if (e < q.cap) {
+        int moff = -1;
+        if (need) {
+          int w = atomicAdd(nmaskwork, 1);
+          if (w < maxmaskwork) {
+            maskwork[w] = S * q.cap + e;
+            moff = -2;
+          } else {
+            atomicExch(overflow, 1);
+          }
+        }
+        entries[(size_t)S * q.cap + e] = make_int4(cj, (int)imask, moff, 0);
+      } else {
+        atomicExch(overflow, 1);
+      }
+    }
+    total += __popcll(act);
+  }
+  if (lane == 0) count[S] = total < q.cap ? total : q.cap;
+}
+
+// Binary search of b in the sorted exclusion partners of a (original indices).
+__device__ inline int excl_class(const int *__restrict__ eptr, const int2 *__restrict__ elist, int a, int b) {
+  int lo = eptr[a], hi = eptr[a + 1] - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int v = elist[mid].x;
+    if (v == b) return elist[mid].y;
+    if (v < b)
+      lo = mid + 1;
+    else
+      hi = mid - 1;
+  }
+  return 0;
+}
+
+// One wave per entry needing explicit masks: bit (tj*8 + ti) of mask[ci] set
+// when i = slot 8ci+ti, j = slot 8cj+tj are real atoms, i precedes j within
+// a shared cluster, and the pair is not FULL-excluded; modmask marks MODIFIED.
+__global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ maskwork, const int *__restrict__ nmaskwork,
+                                                     int maxmaskwork, int4 *__restrict__ entries, int cap, int n,
+                                                     const int *__restrict__ perm, const int *__restrict__ eptr,
+                                                     const int2 *__restrict__ elist,
+                                                     unsigned long long *__restrict__ masks) {
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6);
+  int nw = *nmaskwork;
+  if (nw > maxmaskwork) nw = maxmaskwork;
+  if (w >= nw) return;
+  const int eidx = maskwork[w];
+  const int S = eidx / cap;
+  int4 ent = entries[eidx];
+  const int cj = ent.x;
+  const int ti = lane & 7, tj = lane >> 3;
+  const int sj = cj * kClusterSize + tj;
+  const int aj = sj < n ? perm[sj] : -1;
+  for (int c = 0; c < 8; ++c) {
+    unsigned long long m = 0, mm = 0;
+    if (ent.y & (1 << c)) {
+      const int ci = S * 8 + c;
+      const int si = ci * kClusterSize + ti;
+      const int ai = si < n ? perm[si] : -1;
+      bool ok = ai >= 0 && aj >= 0 && (ci != cj || ti < tj);
+      int cls = 0;
+      if (ok) cls = excl_class(eptr, elist, ai, aj);
+      m = __ballot(ok && cls != 1);
+      mm = __ballot(ok && cls == 2);
+    }
+    if (lane == 0) {
+      masks[2 * ((size_t)w * 8 + c)] = m;
+      masks[2 * ((size_t)w * 8 + c) + 1] = mm;
+    }
+  }
+  if (lane == 0) entries[eidx].z = w * 8;
+}
+
+// Compact tile list: tiles of kTileEntries entries per supercluster.
+__global__ void k_make_tiles(int nsuper, const int *__restrict__ count, int2 *__restrict__ tiles,
+                             int *__restrict__ ntiles) {
+  // single block of 1024 threads; running offset over chunks
+  __shared__ int s[1024];
+  int carry = 0;
+  for (int base = 0; base < nsuper; base += 1024) {
+    int S = base + threadIdx.x;
+    int t = S < nsuper ? (count[S] + kTileEntries - 1) / kTileEntries : 0;
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int v = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += v;
+      __syncthreads();
+    }
+    int excl = carry + s[threadIdx.x] - t;
+    for (int k = 0; k < t; ++k) tiles[excl + k] = make_int2(S, k * kTileEntries);
+    __syncthreads();
+    carry += s[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *ntiles = carry;
+}
+
+// ---------------------------------------------------------------------------
+// 3. Pair kernel: LJ + Coulomb (cutoff or PME real space) with switching.
+
+template <class R>
+__device__ inline void switch_eval(const SwitchC &sc, R r2, R rinv, bool member, R &val, R &der) {
+  if (!member) {
+    val = 0;
+    der = 0;
+    return;
+  }
+  if (sc.kind == SW_C2) {
+    der = 0;
+    if (r2 >= (R)sc.on2) {
+      R c2 = (R)sc.rc2 - r2;
+      c2 = c2 > (R)0 ? c2 : (R)0;
+      R c4 = c2 * ((R)sc.s2 + (R)2 * r2);
+      val = (R)sc.s1 * (c2 * c4);
+      der = (R)sc.s3 * (c2 * c2 - c4);
+    } else {
+      val = 1;
+    }
+  } else if (sc.kind == SW_C1) {
+    R dist = r2 * rinv;
+    val = (R)1 - dist * ((R)sc.c15_1 - r2 * (R)sc.c05_3);
+    der = -(R)sc.c15_1 * rinv + dist * (R)sc.c15_3;
+  } else {
+    val = 1;
+    der = 0;
+  }
+}
+
+__device__ inline float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double fast_rsqrt(double x) { return 1.0 / sqrt(x); }
+__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double fast_rcp(double x) { return 1.0 / x; }
+__device__ inline float fast_exp(float x) { return __expf(x); }
+__device__ inline double fast_exp(double x) { return exp(x); }
+__device__ inline float fast_rint(float x) { return rintf(x); }
+__device__ inline double fast_rint(double x) { return rint(x); }
+
+template <class R>
+__device__ inline R poly5(R ar) {
+  // Abramowitz & Stegun 7.1.26 (reference: protomol/base/MathUtilities.cpp:39-49)
+  const R E1 = (R)0.254829592, E2 = (R)-0.284496736, E3 = (R)1.421413741, E4 = (R)-1.453152027,
+          E5 = (R)1.061405429, PP = (R)0.3275911;
+  R t = fast_rcp((R)1 + PP * ar);
+  return t * (E1 + t * (E2 + t * (E3 + t * (E4 + t * E5))));
+}
+
+template <bool MIXED>
+struct AccT;
+template <>
+struct AccT<false> {
+  using type = double;
+  __device__ static void add(double *p, double v) { unsafeAtomicAdd(p, v); }
+};
+template <>
+struct AccT<true> {
+  using type = unsigned long long;
+  __device__ static void add(unsigned long long *p, float v) {
+    long long q = __float2ll_rn(v * 4294967296.0f);
+    atomicAdd(p, (unsigned long long)q);
+  }
+  __device__ static void add(unsigned long long *p, double v) {
+    long long q = __double2ll_rn(v * kForceScale);
+    atomicAdd(p, (unsigned long long)q);
+  }
+};
+
+// Reduction over the 8 lanes of a cluster group (lanes differing in bits 0..2).
+template <class R>
+__device__ inline R group8_sum(R v) {
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 4);
+  return v;
+}
+// Reduction over lanes differing in bits 3..5.
+template <class R>
+__device__ inline R col8_sum(R v) {
+  v += __shfl_xor(v, 8);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+
+struct PairOut {
+  double *energy;           // [2 * ntiles_max]: per tile (coulomb, lj)
+  long long *npairs;        // [ntiles_max] when stats
+  unsigned long long *hash; // [ntiles_max]
+};
+
+template <bool MIXED, int COUL, bool STATS>
+__global__ void __launch_bounds__(kPairBlock) k_pair(
+    PairParams pp, const int2 *__restrict__ tiles, const int *__restrict__ ntiles_p, int *__restrict__ tile_counter,
+    const int4 *__restrict__ entries, const int *__restrict__ count, int cap, const unsigned long long *__restrict__ masks,
+    const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
+    const int *__restrict__ ts, const int *__restrict__ perm, const double *__restrict__ xorig,
+    const double *__restrict__ ljtab, typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using R4 = typename Real4<R>::type;
+  using Acc = AccT<MIXED>;
+  __shared__ R4 s_i[kPairBlock / kWave][kSuperSize];
+  __shared__ int s_ti[kPairBlock / kWave][kSuperSize];
+  __shared__ R4 s_lj[kMaxTypes * kMaxTypes];  // {A, B, A14, B14}
+
+  const int nt = pp.ntypes;
+  for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) {
+    R4 v;
+    v.x = (R)ljtab[4 * k];
+    v.y = (R)ljtab[4 * k + 1];
+    v.z = (R)ljtab[4 * k + 2];
+    v.w = (R)ljtab[4 * k + 3];
+    s_lj[k] = v;
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int ti = lane & 7, tj = lane >> 3;
+  const R Lx = (R)pp.box.L[0], Ly = (R)pp.box.L[1], Lz = (R)pp.box.L[2];
+  const R Lix = (R)pp.box.Linv[0], Liy = (R)pp.box.Linv[1], Liz = (R)pp.box.Linv[2];
+  const R rc2lj = (R)pp.lj.rc2, rc2c = (R)pp.co.rc2;
+  const R guard = (R)pp.guard;
+  const R alpha = (R)pp.alpha, a2pi = (R)pp.a2pi, csf = (R)pp.csf;
+  const int ntiles = *ntiles_p;
+
+  while (true) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(tile_counter, 1);
+    t = __shfl(t, 0);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= ntiles) break;
+    const int2 tile = tiles[t];
+    const int S = __builtin_amdgcn_readfirstlane(tile.x);
+    const int e0 = __builtin_amdgcn_readfirstlane(tile.y);
+    // i-supercluster into LDS
+    s_i[wv][lane] = xs[S * kSuperSize + lane];
+    s_ti[wv][lane] = ts[S * kSuperSize + lane];
+    __builtin_amdgcn_wave_barrier();
+
+    R fi[8][3];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) fi[c][0] = fi[c][1] = fi[c][2] = (R)0;
+    double elj = 0.0, eco = 0.0;
+    long long npair = 0;
+    unsigned long long hsum = 0;
+
+    const int eend = __builtin_amdgcn_readfirstlane(min(e0 + kTileEntries, count[S]));
+    for (int e = e0; e < eend; ++e) {
+      const int4 ent = entries[(size_t)S * cap + e];
+      const int cj = __builtin_amdgcn_readfirstlane(ent.x);
+      const int emask = __builtin_amdgcn_readfirstlane(ent.y);
+      const int moff = __builtin_amdgcn_readfirstlane(ent.z);
+      const int sj = cj * kClusterSize + tj;
+      const R4 xj = xs[sj];
+      const int tyj = ts[sj];
+      R fj0 = 0, fj1 = 0, fj2 = 0;
+      R el = 0, ec = 0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (!(emask & (1 << c))) continue;
+        bool cand = true, modified = false;
+        if (moff >= 0) {
+          const unsigned long long m = masks[2 * ((size_t)moff + c)];
+          const unsigned long long mm = masks[2 * ((size_t)moff + c) + 1];
+          cand = (m >> lane) & 1ull;
+          modified = (mm >> lane) & 1ull;
+        }
+        const R4 xi = s_i[wv][c * 8 + ti];
+        const int tyi = s_ti[wv][c * 8 + ti];
+        R dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        dx -= Lx * fast_rint(dx * Lix);
+        dy -= Ly * fast_rint(dy * Liy);
+        dz -= Lz * fast_rint(dz * Liz);
+        R r2 = dx * dx + dy * dy + dz * dz;
+        bool in_lj = cand && pp.has_lj && r2 <= rc2lj;
+        bool in_c = cand && COUL != COUL_NONE && r2 <= rc2c;
+        bool border = cand && ((pp.has_lj && fabs(r2 - rc2lj) <= guard) ||
+                               (COUL != COUL_NONE && fabs(r2 - rc2c) <= guard));
+        if (__ballot(border)) {
+          if (border) {
+            const int ai = perm[S * kSuperSize + c * 8 + ti], aj = perm[sj];
+            if (ai >= 0 && aj >= 0) {
+              double d2 = exact_d2(xorig, ai < aj ? ai : aj, ai < aj ? aj : ai, pp.box);
+              in_lj = pp.has_lj && d2 <= pp.lj.rc2;
+              in_c = COUL != COUL_NONE && d2 <= pp.co.rc2;
+            }
+          }
+        }
+        const bool member = in_lj || in_c;
+        if (STATS && member) {
+          const int ai = perm[S * kSuperSize + c * 8 + ti], aj = perm[sj];
+          ++npair;
+          hsum += pair_hash(ai < aj ? ai : aj, ai < aj ? aj : ai);
+        }
+        const R r2s = member ? r2 : (R)1;
+        const R rinv = fast_rsqrt(r2s);
+        const R rinv2 = rinv * rinv;
+        R fr = 0;
+        if (pp.has_lj) {
+          const R4 lj = s_lj[(tyi < 0 ? 0 : tyi) * nt + (tyj < 0 ? 0 : tyj)];
+          const R A = modified ? lj.z : lj.x, B = modified ? lj.w : lj.y;
+          const R r6 = rinv2 * rinv2 * rinv2;
+          const R r6B = B * r6, r12A = A * r6 * r6;
+          R e = r12A - r6B;
+          R f = ((R)12 * r12A - (R)6 * r6B) * rinv2;
+          R sv, sd;
+          switch_eval<R>(pp.lj, r2s, rinv, in_lj, sv, sd);
+          f = f * sv - e * sd;
+          e = e * sv;
+          el += e;
+          fr += f;
+        }
+        if (COUL == COUL_CUTOFF) {
+          R e = xi.w * xj.w * rinv;
+          if (modified) e *= csf;
+          R f = e * rinv2;
+          R sv, sd;
+          switch_eval<R>(pp.co, r2s, rinv, in_c, sv, sd);
+          f = f * sv - e * sd;
+          e = e * sv;
+          ec += e;
+          fr += f;
+        } else if (COUL == COUL_PME) {
+          R qq = xi.w * xj.w;
+          if (modified) qq *= csf;
+          const R r = r2s * rinv;
+          const R ar = alpha * r;
+          const R ex = qq * fast_exp(-ar * ar);
+          R e = poly5<R>(ar) * ex * rinv;
+          R f = (e + a2pi * ex) * rinv2;
+          R sv, sd;
+          switch_eval<R>(pp.co, r2s, rinv, in_c, sv, sd);
+          f = f * sv - e * sd;
+          e = e * sv;
+          ec += e;
+          fr += f;
+        }
+        const R fx = dx * fr, fy = dy * fr, fz = dz * fr;
+        fi[c][0] -= fx;
+        fi[c][1] -= fy;
+        fi[c][2] -= fz;
+        fj0 += fx;
+        fj1 += fy;
+        fj2 += fz;
+      }
+      // j forces: sum over the 8 i-lanes of each j atom, one component per lane
+      fj0 = group8_sum(fj0);
+      fj1 = group8_sum(fj1);
+      fj2 = group8_sum(fj2);
+      if (ti < 3 && tyj >= 0) {
+        const R v = ti == 0 ? fj0 : (ti == 1 ? fj1 : fj2);
+        Acc::add(&facc[3 * sj + ti], v);
+      }
+      elj += (double)el;
+      eco += (double)ec;
+    }
+    // i forces: sum over the 8 j-lanes; lane (ti, tj) then owns atom 8*tj + ti
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      fi[c][0] = col8_sum(fi[c][0]);
+      fi[c][1] = col8_sum(fi[c][1]);
+      fi[c][2] = col8_sum(fi[c][2]);
+    }
+    R my0 = 0, my1 = 0, my2 = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (tj == c) {
+        my0 = fi[c][0];
+        my1 = fi[c][1];
+        my2 = fi[c][2];
+      }
+    const int si = S * kSuperSize + tj * 8 + ti;
+    if (s_ti[wv][tj * 8 + ti] >= 0) {
+      Acc::add(&facc[3 * si], my0);
+      Acc::add(&facc[3 * si + 1], my1);
+      Acc::add(&facc[3 * si + 2], my2);
+    }
+    elj = wave_sum(elj);
+    eco = wave_sum(eco);
+    if (STATS) {
+      npair = wave_sum(npair);
+      hsum = wave_sum(hsum);
+    }
+    if (lane == 0) {
+      out.energy[2 * t] = eco;
+      out.energy[2 * t + 1] = elj;
+      if (STATS) {
+        out.npairs[t] = npair;
+        out.hash[t] = hsum;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 4. PME: B-spline spreading, LDS FFT passes with the influence function,
+//    force gather.  Grid [nx][ny][nz], z fastest (Grid.h:70).
+
+struct PmeParams {
+  int nx, ny, nz, nzh, order;
+  double hxr, hyr, hzr;   // K/L per axis (Grid.h:520-522)
+  int offx, offy, offz;   // -(p-1)/2 + K (Grid.h:509-511)
+};
+
+// BSpline::set (mdl/interface/force/nonbonded/BSpline.cpp:55-105), order <= 8.
+template <class R>
+__device__ inline void bspline_w(int order, R w, R *th, R *dth) {
+  th[order - 1] = 0;
+  th[1] = w;
+  th[0] = (R)1 - w;
+  for (int k = 2; k < order - 1; k++) {
+    R div = (R)1 / (R)k;
+    th[k] = div * w * th[k - 1];
+    for (int j = 1; j < k; j++) th[k - j] = div * ((w + (R)j) * th[k - j - 1] + ((R)(k + 1 - j) - w) * th[k - j]);
+    th[0] = div * ((R)1 - w) * th[0];
+  }
+  dth[0] = -th[0];
+  for (int j = 1; j <= order - 1; j++) dth[j] = th[j - 1] - th[j];
+  R div = (R)1 / (R)(order - 1);
+  th[order - 1] = div * w * th[order - 2];
+  for (int j = 1; j < order - 1; j++)
+    th[order - 1 - j] = div * ((w + (R)j) * th[order - j - 2] + ((R)(order - j) - w) * th[order - 1 - j]);
+  th[0] = div * ((R)1 - w) * th[0];
+}
+
+// Grid coordinate of one axis: u = x K/L wrapped to [0, K) (Grid.h:111-120).
+__device__ inline void grid_coord(double x, double hr, int K, int off, int &base, double &w) {
+  double u = x * hr;
+  while (u < 0.0) u += K;
+  while (u >= K) u -= K;
+  int iu = (int)u;
+  base = iu + off;
+  w = u - iu;
+}
+
+template <bool MIXED>
+__global__ void k_spread(int n, PmeParams p, const double *__restrict__ xorig, const int *__restrict__ perm,
+                         const double *__restrict__ q, double *__restrict__ grid64,
+                         unsigned long long *__restrict__ gridfx) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  const int a = perm[s];
+  int b[3];
+  double w[3];
+  grid_coord(xorig[3 * a], p.hxr, p.nx, p.offx, b[0], w[0]);
+  grid_coord(xorig[3 * a + 1], p.hyr, p.ny, p.offy, b[1], w[1]);
+  grid_coord(xorig[3 * a + 2], p.hzr, p.nz, p.offz, b[2], w[2]);
+  R tx[8], ty[8], tz[8], d[8];
+  bspline_w<R>(p.order, (R)w[0], tx, d);
+  bspline_w<R>(p.order, (R)w[1], ty, d);
+  bspline_w<R>(p.order, (R)w[2], tz, d);
+  const R qa = (R)q[a];
+  for (int i = 0; i < p.order; i++) {
+    const R aq = qa * tx[i];
+    const int i1 = (i + b[0]) % p.nx;
+    for (int j = 0; j < p.order; j++) {
+      const R ab = aq * ty[j];
+      const int j1 = (j + b[1]) % p.ny;
+      const size_t row = ((size_t)i1 * p.ny + j1) * p.nz;
+      for (int k = 0; k < p.order; k++) {
+        const int k1 = (k + b[2]) % p.nz;
+        const R v = ab * tz[k];
+        if (MIXED) {
+          long long fx = __double2ll_rn((double)v * kGridScale);
+          atomicAdd(&gridfx[row + k1], (unsigned long long)fx);
+        } else {
+          unsafeAtomicAdd(&grid64[row + k1], (double)v);
+        }
+      }
+    }
+  }
+}
+
+template <class T>
+struct Cx {
+  T x, y;
+};
+
+template <class T>
+__device__ inline Cx<T> cmul(Cx<T> a, Cx<T> b) {
+  return Cx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// Forward DFT of small radix R in registers, e^{-2 pi i rk/R}.
+template <class T>
+__device__ inline void dft_small(Cx<T> *v, int R) {
+  if (R == 2) {
+    Cx<T> a = v[0], b = v[1];
+    v[0] = Cx<T>{a.x + b.x, a.y + b.y};
+    v[1] = Cx<T>{a.x - b.x, a.y - b.y};
+  } else if (R == 4) {
+    Cx<T> a = v[0], b = v[1], c = v[2], d = v[3];
+    Cx<T> s0{a.x + c.x, a.y + c.y}, s1{a.x - c.x, a.y - c.y};
+    Cx<T> s2{b.x + d.x, b.y + d.y}, s3{b.x - d.x, b.y - d.y};
+    v[0] = Cx<T>{s0.x + s2.x, s0.y + s2.y};
+    v[2] = Cx<T>{s0.x - s2.x, s0.y - s2.y};
+    // -i * s3 = (s3.y, -s3.x)
+    v[1] = Cx<T>{s1.x + s3.y, s1.y - s3.x};
+    v[3] = Cx<T>{s1.x - s3.y, s1.y + s3.x};
+  } else if (R == 3) {
+    const T c = (T)-0.5, s = (T)0.86602540378443864676;
+    Cx<T> a = v[0], b = v[1], d = v[2];
+    Cx<T> t{b.x + d.x, b.y + d.y}, u{b.x - d.x, b.y - d.y};
+    v[0] = Cx<T>{a.x + t.x, a.y + t.y};
+    Cx<T> m{a.x + c * t.x, a.y + c * t.y};
+    // -i s u = (s u.y, -s u.x)
+    v[1] = Cx<T>{m.x + s * u.y, m.y - s * u.x};
+    v[2] = Cx<T>{m.x - s * u.y, m.y + s * u.x};
+  } else {  // R == 5
+    const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+    const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+    Cx<T> a = v[0];
+    Cx<T> t1{v[1].x + v[4].x, v[1].y + v[4].y}, u1{v[1].x - v[4].x, v[1].y - v[4].y};
+    Cx<T> t2{v[2].x + v[3].x, v[2].y + v[3].y}, u2{v[2].x - v[3].x, v[2].y - v[3].y};
+    v[0] = Cx<T>{a.x + t1.x + t2.x, a.y + t1.y + t2.y};
+    Cx<T> m1{a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y};
+    Cx<T> m2{a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y};
+    // y1 = m1 - i(s1 u1 + s2 u2); y4 = m1 + i(...); y2 = m2 - i(s2 u1 - s1 u2); y3 = m2 + i(...)
+    Cx<T> n1{s1 * u1.x + s2 * u2.x, s1 * u1.y + s2 * u2.y};
+    Cx<T> n2{s2 * u1.x - s1 * u2.x, s2 * u1.y - s1 * u2.y};
+    v[1] = Cx<T>{m1.x + n1.y, m1.y - n1.x};
+    v[4] = Cx<T>{m1.x - n1.y, m1.y + n1.x};
+    v[2] = Cx<T>{m2.x + n2.y, m2.y - n2.x};
+    v[3] = Cx<T>{m2.x - n2.y, m2.y + n2.x};
+  }
+}
+
+struct FftPlan {
+  int n;
+  int nrad;
+  int rad[16];
+};
+
+// Batched forward Stockham FFT of `nb` lines of length plan.n in LDS
+// (a -> b ping-pong); twiddle table tw[m] = e^{-2 pi i m/n}.  Returns the
+// buffer holding the result.  All threads of the block participate.
+template <class T>
+__device__ Cx<T> *lds_fft(Cx<T> *a, Cx<T> *b, int nb, const FftPlan &pl, const Cx<T> *__restrict__ tw) {
+  const int n = pl.n;
+  int Ns = 1;
+  for (int s = 0; s < pl.nrad; ++s) {
+    const int R = pl.rad[s];
+    const int nbut = n / R;
+    const int tstep = n / (Ns * R);
+    for (int t = threadIdx.x; t < nbut * nb; t += blockDim.x) {
+      const int line = t / nbut, j = t - line * nbut;
+      const Cx<T> *src = a + line * n;
+      Cx<T> *dst = b + line * n;
+      const int k = j % Ns;
+      Cx<T> v[5];
+      for (int r = 0; r < R; ++r) {
+        v[r] = src[j + r * nbut];
+        if (r > 0 && k > 0) v[r] = cmul(v[r], tw[(k * r * tstep) % n]);
+      }
+      dft_small(v, R);
+      const int idxD = (j / Ns) * Ns * R + k;
+      for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[r];
+    }
+    __syncthreads();
+    Cx<T> *tmp = a;
+    a = b;
+    b = tmp;
+    Ns *= R;
+  }
+  return a;
+}
+
+// z pass, real -> half complex: lines (i, j) contiguous along z.
+template <class T, bool FIXED>
+__global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int lines_per_block, const double *__restrict__ g64,
+                            const unsigned long long *__restrict__ gfx, const Cx<T> *__restrict__ tw,
+                            Cx<T> *__restrict__ spec) {
+  extern __shared__ unsigned char smem_raw[];
+  Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
+  Cx<T> *b = a + lines_per_block * p.nz;
+  const int nlines = p.nx * p.ny;
+  const int l0 = blockIdx.x * lines_per_block;
+  const int nb = min(lines_per_block, nlines - l0);
+  for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) {
+    const size_t gi = (size_t)l0 * p.nz + t;
+    T v;
+    if (FIXED)
+      v = (T)((double)(long long)gfx[gi] * (1.0 / kGridScale));
+    else
+      v = (T)g64[gi];
+    a[t] = Cx<T>{v, (T)0};
+  }
+  __syncthreads();
+  Cx<T> *r = lds_fft<T>(a, b, nb, pl, tw);
+  for (int t = threadIdx.x; t < nb * p.nzh; t += blockDim.x) {
+    const int line = t / p.nzh, k = t - line * p.nzh;
+    spec[(size_t)(l0 + line) * p.nzh + k] = r[line * p.nz + k];
+  }
+}
+
+// y pass (stride nzh) for a tile of TK consecutive k at fixed i.
+// inverse: conj-in / conj-out around the forward transform.
+template <class T>
+__global__ void k_fft_y(PmeParams p, FftPlan pl, int TK, bool inverse, const Cx<T> *__restrict__ tw,
+                        Cx<T> *__restrict__ spec) {
+  extern __shared__ unsigned char smem_raw[];
+  Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
+  Cx<T> *b = a + TK * p.ny;
+  const int ktiles = (p.nzh + TK - 1) / TK;
+  const int i = blockIdx.x / ktiles;
+  const int k0 = (blockIdx.x % ktiles) * TK;
+  const int nk = min(TK, p.nzh - k0);
+  for (int t = threadIdx.x; t < nk * p.ny; t += blockDim.x) {
+    const int j = t / nk, kk = t - j * nk;
+    Cx<T> v = spec[((size_t)i * p.ny + j) * p.nzh + k0 + kk];
+    if (inverse) v.y = -v.y;
+    a[kk * p.ny + j] = v;
+  }
+  __syncthreads();
+  Cx<T> *r = lds_fft<T>(a, b, nk, pl, tw);
+  for (int t = threadIdx.x; t < nk * p.ny; t += blockDim.x) {
+    const int j = t / nk, kk = t - j * nk;
+    Cx<T> v = r[kk * p.ny + j];
+    if (inverse) v.y = -v.y;
+    spec[((size_t)i * p.ny + j) * p.nzh + k0 + kk] = v;
+  }
+}
+
+// x pass forward, influence function multiply with energy (Grid.h:237-341),
+// x pass inverse, for a tile of TK consecutive k at fixed j.
+template <class T>
+__global__ void k_fft_x_conv(PmeParams p, FftPlan pl, int TK, const Cx<T> *__restrict__ tw,
+                             const T *__restrict__ theta, Cx<T> *__restrict__ spec, double *__restrict__ epart) {
+  extern __shared__ unsigned char smem_raw[];
+  Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
+  Cx<T> *b = a + TK * p.nx;
+  __shared__ double s_e[kPairBlock / kWave];
+  const int ktiles = (p.nzh + TK - 1) / TK;
+  const int j = blockIdx.x / ktiles;
+  const int k0 = (blockIdx.x % ktiles) * TK;
+  const int nk = min(TK, p.nzh - k0);
+  const size_t stride = (size_t)p.ny * p.nzh;
+  for (int t = threadIdx.x; t < nk * p.nx; t += blockDim.x) {
+    const int i = t / nk, kk = t - i * nk;
+    a[kk * p.nx + i] = spec[(size_t)i * stride + (size_t)j * p.nzh + k0 + kk];
+  }
+  __syncthreads();
+  Cx<T> *r = lds_fft<T>(a, b, nk, pl, tw);
+  double e = 0.0;
+  for (int t = threadIdx.x; t < nk * p.nx; t += blockDim.x) {
+    const int kk = t / p.nx, i = t - kk * p.nx;
+    const int k = k0 + kk;
+    const T th = theta[((size_t)i * p.ny + j) * p.nzh + k];
+    Cx<T> v = r[kk * p.nx + i];
+    const double wgt = (k > 0 && ((k != p.nz / 2) || (p.nz & 1))) ? 2.0 : 1.0;
+    e += wgt * (double)th * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+    // conj for the inverse transform (conj-in / conj-out)
+    r[kk * p.nx + i] = Cx<T>{v.x * th, -(v.y * th)};
+  }
+  __syncthreads();
+  Cx<T> *o = (r == a) ? b : a;
+  Cx<T> *r2 = lds_fft<T>(r, o, nk, pl, tw);
+  for (int t = threadIdx.x; t < nk * p.nx; t += blockDim.x) {
+    const int i = t / nk, kk = t - i * nk;
+    Cx<T> v = r2[kk * p.nx + i];
+    v.y = -v.y;
+    spec[(size_t)i * stride + (size_t)j * p.nzh + k0 + kk] = v;
+  }
+  e = wave_sum(e);
+  if ((threadIdx.x & 63) == 0) s_e[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) tot += s_e[w];
+    epart[blockIdx.x] = tot;
+  }
+}
+
+// z pass, half complex -> real (Hermitian completion along z), inverse sign.
+template <class T>
+__global__ void k_fft_z_c2r(PmeParams p, FftPlan pl, int lines_per_block, const Cx<T> *__restrict__ tw,
+                            const Cx<T> *__restrict__ spec, T *__restrict__ greal) {
+  extern __shared__ unsigned char smem_raw[];
+  Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
+  Cx<T> *b = a + lines_per_block * p.nz;
+  const int nlines = p.nx * p.ny;
+  const int l0 = blockIdx.x * lines_per_block;
+  const int nb = min(lines_per_block, nlines - l0);
+  for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) {
+    const int line = t / p.nz, k = t - line * p.nz;
+    Cx<T> v;
+    if (k < p.nzh) {
+      v = spec[(size_t)(l0 + line) * p.nzh + k];
+      v.y = -v.y;  // conj-in
+    } else {
+      v = spec[(size_t)(l0 + line) * p.nzh + (p.nz - k)];  // conj(conj(X[nz-k]))
+    }
+    a[t] = v;
+  }
+  __syncthreads();
+  Cx<T> *r = lds_fft<T>(a, b, nb, pl, tw);
+  for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) greal[(size_t)l0 * p.nz + t] = r[t].x;
+}
+
+// Force interpolation (Grid.h:159-189) into the sorted accumulators.
+template <bool MIXED>
+__global__ void k_gather(int n, PmeParams p, const double *__restrict__ xorig, const int *__restrict__ perm,
+                         const double *__restrict__ q,
+                         const typename std::conditional<MIXED, float, double>::type *__restrict__ greal,
+                         typename AccT<MIXED>::type *__restrict__ facc) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  const int a = perm[s];
+  int b[3];
+  double w[3];
+  grid_coord(xorig[3 * a], p.hxr, p.nx, p.offx, b[0], w[0]);
+  grid_coord(xorig[3 * a + 1], p.hyr, p.ny, p.offy, b[1], w[1]);
+  grid_coord(xorig[3 * a + 2], p.hzr, p.nz, p.offz, b[2], w[2]);
+  R tx[8], ty[8], tz[8], dx[8], dy[8], dz[8];
+  bspline_w<R>(p.order, (R)w[0], tx, dx);
+  bspline_w<R>(p.order, (R)w[1], ty, dy);
+  bspline_w<R>(p.order, (R)w[2], tz, dz);
+  R fx = 0, fy = 0, fz = 0;
+  for (int i = 0; i < p.order; i++) {
+    const int i1 = (i + b[0]) % p.nx;
+    for (int j = 0; j < p.order; j++) {
+      const int j1 = (j + b[1]) % p.ny;
+      const R xij = dx[i] * ty[j], yij = tx[i] * dy[j], zij = tx[i] * ty[j];
+      const size_t row = ((size_t)i1 * p.ny + j1) * p.nz;
+      for (int k = 0; k < p.order; k++) {
+        const int k1 = (k + b[2]) % p.nz;
+        const R term = greal[row + k1];
+        fx -= term * xij * tz[k];
+        fy -= term * yij * tz[k];
+        fz -= term * zij * dz[k];
+      }
+    }
+  }
+  const R qa = (R)q[a];
+  AccT<MIXED>::add(&facc[3 * s], fx * (R)p.hxr * qa);
+  AccT<MIXED>::add(&facc[3 * s + 1], fy * (R)p.hyr * qa);
+  AccT<MIXED>::add(&facc[3 * s + 2], fz * (R)p.hzr * qa);
+}
+
+// ---------------------------------------------------------------------------
+// 5. Exclusion correction (NonbondedPMEwaldSystemForce.h:874-911), FP64, exact erf.
+template <bool MIXED>
+__global__ void k_correction(int ne, const int4 *__restrict__ etab, const double *__restrict__ xorig,
+                             const double *__restrict__ q, const int *__restrict__ slot, Box box, double alpha,
+                             double csf, typename AccT<MIXED>::type *__restrict__ facc, double *__restrict__ epart) {
+  __shared__ double s_e[kPairBlock / kWave];
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double en = 0.0;
+  if (e < ne) {
+    const int4 t = etab[e];
+    const int a1 = t.x, a2 = t.y;
+    double d[3] = {xorig[3 * a2] - xorig[3 * a1], xorig[3 * a2 + 1] - xorig[3 * a1 + 1],
+                   xorig[3 * a2 + 2] - xorig[3 * a1 + 2]};
+    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (r2 > box.D) {
+      for (int k = 0; k < 3; ++k) {
+        if (d[k] < -box.H2[k])
+          d[k] += box.H[k];
+        else if (d[k] > box.H2[k])
+          d[k] -= box.H[k];
+      }
+      r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (r2 > box.D) {
+        for (int k = 0; k < 3; ++k) d[k] -= box.L[k] * rint(box.Lr[k] * d[k]);
+        r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      }
+    }
+    double qq = q[a1] * q[a2];
+    if (t.z == 2) qq *= 1 - csf;
+    const double r = sqrt(r2), rr = 1.0 / r;
+    const double ee = erf(alpha * r) * rr;
+    en = -qq * ee;
+    const double fr = qq * (2.0 * alpha / sqrt(M_PI) * exp(-alpha * alpha * r2) - ee) * rr * rr;
+    const int s1 = slot[a1], s2 = slot[a2];
+    for (int k = 0; k < 3; ++k) {
+      AccT<MIXED>::add(&facc[3 * s1 + k], -d[k] * fr);
+      AccT<MIXED>::add(&facc[3 * s2 + k], d[k] * fr);
+    }
+  }
+  en = wave_sum(en);
+  if ((threadIdx.x & 63) == 0) s_e[threadIdx.x >> 6] = en;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) tot += s_e[w];
+    epart[blockIdx.x] = tot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 6. Finalize: sorted accumulators -> caller's AoS order, FP64.
+template <bool MIXED>
+__global__ void k_unsort_forces(int n, const int *__restrict__ slot, const typename AccT<MIXED>::type *__restrict__ facc,
+                                double *__restrict__ fout) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int s = slot[a];
+  for (int k = 0; k < 3; ++k) {
+    if (MIXED)
+      fout[3 * a + k] = (double)(long long)facc[3 * s + k] * (1.0 / kForceScale);
+    else
+      fout[3 * a + k] = ((const double *)facc)[3 * s + k];
+  }
+}
+
+// Deterministic energy/pair-stat reduction in one block, in index order.
+__global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p, const double *__restrict__ pair_e,
+                                int nrecip, const double *__restrict__ recip_e, int ncorr,
+                                const double *__restrict__ corr_e, const long long *__restrict__ npairs,
+                                const unsigned long long *__restrict__ hash, int stats, double *__restrict__ out,
+                                long long *__restrict__ out_np, unsigned long long *__restrict__ out_hash) {
+  __shared__ double s0[1024], s1[1024], s2[1024], s3[1024];
+  __shared__ long long sn[1024];
+  __shared__ unsigned long long sh[1024];
+  const int nt = min(*ntiles_p, ntiles_cap);
+  double c = 0, l = 0, r = 0, k = 0;
+  long long np = 0;
+  unsigned long long h = 0;
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+    c += pair_e[2 * i];
+    l += pair_e[2 * i + 1];
+    if (stats) {
+      np += npairs[i];
+      h += hash[i];
+    }
+  }
+  for (int i = threadIdx.x; i < nrecip; i += blockDim.x) r += recip_e[i];
+  for (int i = threadIdx.x; i < ncorr; i += blockDim.x) k += corr_e[i];
+  s0[threadIdx.x] = c;
+  s1[threadIdx.x] = l;
+  s2[threadIdx.x] = r;
+  s3[threadIdx.x] = k;
+  sn[threadIdx.x] = np;
+  sh[threadIdx.x] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double C = 0, Lj = 0, Rr = 0, K = 0;
+    long long N = 0;
+    unsigned long long H = 0;
+    for (int i = 0; i < (int)blockDim.x; ++i) {
+      C += s0[i];
+      Lj += s1[i];
+      Rr += s2[i];
+      K += s3[i];
+      N += sn[i];
+      H += sh[i];
+    }
+    out[0] = C;
+    out[1] = Lj;
+    out[2] = 0.5 * Rr;
+    out[3] = K;
+    *out_np = N;
+    *out_hash = H;
+  }
+}
+
+}  // namespace pmx

@@ -1,0 +1,245 @@
+"""ctypes binding of the pmx C ABI (include/pmx.h).
+
+The library is the in-tree build protomol3_amd/lib/libpmx.so (gfx950 code
+objects).  There is no fallback: if the library or a gfx950 device is
+missing, every call raises PmxError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+from .systems import MolSystem
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpmx.so")
+
+FP64, MIXED = 0, 1
+SW_NONE, SW_CUTOFF, SW_C1, SW_C2 = 0, 1, 2, 3
+COUL_NONE, COUL_CUTOFF, COUL_PME = 0, 1, 2
+PME_REAL, PME_RECIP, PME_CORR = 1, 2, 4
+EVAL_PAIRSTATS, EVAL_TIMING = 1, 2
+STAGES = ("sort", "list", "pair", "spread", "fft", "gather", "correction", "finalize")
+
+ERRORS = {-1: "PMX_ERR_ARG", -2: "PMX_ERR_OOM", -3: "PMX_ERR_HIP", -4: "PMX_ERR_UNSUPPORTED",
+          -5: "PMX_ERR_NODEV", -6: "PMX_ERR_OVERFLOW"}
+
+
+class PmxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SysDesc(C.Structure):
+    _fields_ = [("n", C.c_int), ("ntypes", C.c_int), ("nexcl", C.c_int),
+                ("charge", C.POINTER(C.c_double)), ("type", C.POINTER(C.c_int)),
+                ("molecule", C.POINTER(C.c_int)), ("lj", C.POINTER(C.c_double)),
+                ("excl", C.POINTER(C.c_int)), ("box", C.c_double * 3), ("origin", C.c_double * 3),
+                ("csf", C.c_double), ("cellsize", C.c_double), ("lj_switch", C.c_int),
+                ("lj_on", C.c_double), ("lj_rc", C.c_double), ("coul_kind", C.c_int),
+                ("coul_switch", C.c_int), ("coul_on", C.c_double), ("coul_rc", C.c_double),
+                ("pme_grid", C.c_int * 3), ("pme_order", C.c_int), ("pme_accuracy", C.c_double),
+                ("pme_alpha", C.c_double), ("pme_terms", C.c_int), ("precision", C.c_int)]
+
+
+EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_create", "pmx_sys_destroy",
+            "pmx_sys_uncache", "pmx_sys_evaluate", "pmx_sys_set_positions", "pmx_sys_compute",
+            "pmx_sys_get_forces", "pmx_sys_sync", "pmx_sys_alpha", "pmx_sys_pair_stats", "pmx_pair_hash",
+            "pmx_sys_cells", "pmx_sys_stage_times"]
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libpmx.so; raises PmxError when it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PmxError(-5, f"{LIB_PATH} not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    L.pmx_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.pmx_ctx_destroy.argtypes = [vp]
+    L.pmx_last_error.argtypes = [vp]
+    L.pmx_last_error.restype = C.c_char_p
+    L.pmx_sys_create.argtypes = [vp, C.POINTER(SysDesc), C.POINTER(vp)]
+    L.pmx_sys_destroy.argtypes = [vp]
+    L.pmx_sys_uncache.argtypes = [vp]
+    L.pmx_sys_evaluate.argtypes = [vp, dp, dp, dp, C.c_uint]
+    L.pmx_sys_set_positions.argtypes = [vp, dp]
+    L.pmx_sys_compute.argtypes = [vp, C.c_uint]
+    L.pmx_sys_get_forces.argtypes = [vp, dp, dp]
+    L.pmx_sys_sync.argtypes = [vp]
+    L.pmx_sys_alpha.argtypes = [vp, dp]
+    L.pmx_sys_pair_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_ulonglong)]
+    L.pmx_pair_hash.argtypes = [C.c_int, C.c_int]
+    L.pmx_pair_hash.restype = C.c_ulonglong
+    L.pmx_sys_cells.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.pmx_sys_stage_times.argtypes = [vp, C.POINTER(C.c_float)]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def pair_hash_set(pairs: np.ndarray) -> int:
+    """The library's order-independent pair-set hash, computed on the host
+    (splitmix64 of (min << 32 | max), summed mod 2^64)."""
+    p = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    lo = np.minimum(p[:, 0], p[:, 1]).astype(np.uint64)
+    hi = np.maximum(p[:, 0], p[:, 1]).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        z = (lo << np.uint64(32)) | hi
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+        return int(np.sum(z, dtype=np.uint64))
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        rc = self.lib.pmx_ctx_create(device, C.byref(self.h))
+        if rc != 0:
+            raise PmxError(rc, f"pmx_ctx_create(device={device}) failed")
+
+    def check(self, rc: int, what: str):
+        if rc != 0:
+            raise PmxError(rc, f"{what}: {self.lib.pmx_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.pmx_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ForceSpec:
+    """The force lines of a ProtoMol .conf for this engine:
+    lj = (switch, switchon, cutoff) for `LennardJones -algorithm NonbondedCutoff`;
+    coul = (switch, switchon, cutoff) for `Coulomb -algorithm NonbondedCutoff`;
+    pme = dict(rc, grid, order, accuracy, alpha, terms, switch) for
+    `Coulomb -algorithm PMEwald`."""
+
+    def __init__(self, cellsize: float, lj=None, coul=None, pme=None):
+        self.cellsize, self.lj, self.coul, self.pme = cellsize, lj, coul, pme
+
+
+class System:
+    """A resident device system (pmx_sys)."""
+
+    def __init__(self, ctx: Context, s: MolSystem, spec: ForceSpec, precision: int = FP64):
+        self.ctx, self.n = ctx, s.n
+        self._keep = [np.ascontiguousarray(s.charge, np.float64), np.ascontiguousarray(s.atype, np.int32),
+                      np.ascontiguousarray(s.molecule, np.int32), np.ascontiguousarray(s.lj, np.float64),
+                      np.ascontiguousarray(s.exclusions, np.int32).reshape(-1, 3)]
+        q, t, m, lj, ex = self._keep
+        d = SysDesc()
+        d.n, d.ntypes, d.nexcl = s.n, lj.shape[0], ex.shape[0]
+        d.charge = _dp(q)
+        d.type = t.ctypes.data_as(C.POINTER(C.c_int))
+        d.molecule = m.ctypes.data_as(C.POINTER(C.c_int))
+        d.lj = _dp(lj)
+        d.excl = ex.ctypes.data_as(C.POINTER(C.c_int))
+        for k in range(3):
+            d.box[k] = float(s.box[k])
+            d.origin[k] = float(s.origin[k])
+        d.csf = float(s.csf)
+        d.cellsize = float(spec.cellsize)
+        if spec.lj is not None:
+            d.lj_switch, d.lj_on, d.lj_rc = spec.lj
+        if spec.coul is not None:
+            d.coul_kind = COUL_CUTOFF
+            d.coul_switch, d.coul_on, d.coul_rc = spec.coul
+        if spec.pme is not None:
+            p = spec.pme
+            d.coul_kind = COUL_PME
+            d.coul_switch = p.get("switch", SW_CUTOFF)
+            d.coul_rc = p["rc"]
+            for k in range(3):
+                d.pme_grid[k] = p["grid"][k]
+            d.pme_order = p.get("order", 4)
+            d.pme_accuracy = p.get("accuracy", 1e-6)
+            d.pme_alpha = p.get("alpha", -1.0)
+            d.pme_terms = p.get("terms", PME_REAL | PME_RECIP | PME_CORR)
+        d.precision = precision
+        self.desc = d
+        self.h = C.c_void_p()
+        ctx.check(ctx.lib.pmx_sys_create(ctx.h, C.byref(d), C.byref(self.h)), "pmx_sys_create")
+
+    def evaluate(self, pos: np.ndarray, flags: int = 0):
+        """Returns (forces (N,3), e_coulomb, e_lj) for positions (N,3)."""
+        x = np.ascontiguousarray(pos, dtype=np.float64)
+        f = np.zeros((self.n, 3))
+        e = np.zeros(2)
+        self.ctx.check(self.ctx.lib.pmx_sys_evaluate(self.h, _dp(x), _dp(f), _dp(e), flags), "pmx_sys_evaluate")
+        return f, float(e[0]), float(e[1])
+
+    def set_positions(self, pos: np.ndarray):
+        x = np.ascontiguousarray(pos, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.pmx_sys_set_positions(self.h, _dp(x)), "pmx_sys_set_positions")
+
+    def compute(self, flags: int = 0):
+        self.ctx.check(self.ctx.lib.pmx_sys_compute(self.h, flags), "pmx_sys_compute")
+
+    def sync(self):
+        self.ctx.check(self.ctx.lib.pmx_sys_sync(self.h), "pmx_sys_sync")
+
+    def get_forces(self):
+        f = np.zeros((self.n, 3))
+        e = np.zeros(2)
+        self.ctx.check(self.ctx.lib.pmx_sys_get_forces(self.h, _dp(f), _dp(e)), "pmx_sys_get_forces")
+        return f, float(e[0]), float(e[1])
+
+    def alpha(self) -> float:
+        a = C.c_double()
+        self.ctx.check(self.ctx.lib.pmx_sys_alpha(self.h, C.byref(a)), "pmx_sys_alpha")
+        return a.value
+
+    def pair_stats(self):
+        n = C.c_longlong()
+        h = C.c_ulonglong()
+        self.ctx.check(self.ctx.lib.pmx_sys_pair_stats(self.h, C.byref(n), C.byref(h)), "pmx_sys_pair_stats")
+        return n.value, h.value
+
+    def cells(self):
+        out = np.zeros(self.n, np.int32)
+        dims = np.zeros(3, np.int32)
+        self.ctx.check(self.ctx.lib.pmx_sys_cells(self.h, out.ctypes.data_as(C.POINTER(C.c_int)),
+                                                  dims.ctypes.data_as(C.POINTER(C.c_int))), "pmx_sys_cells")
+        return out, tuple(int(v) for v in dims)
+
+    def stage_times(self) -> dict:
+        ms = (C.c_float * 8)()
+        self.ctx.check(self.ctx.lib.pmx_sys_stage_times(self.h, ms), "pmx_sys_stage_times")
+        return dict(zip(STAGES, [float(v) for v in ms]))
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.pmx_sys_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
