@@ -129,6 +129,11 @@ int pmx_sys_cells(pmx_sys *sys, int *cell_of_atom, int *dims);
  * order: sort, list, pair, spread, fft+convolve, gather, correction,
  * finalize.  ms must hold 8 floats. */
 int pmx_sys_stage_times(const pmx_sys *sys, float *ms);
+/* Per-stage device times averaged over every pmx_sys_compute call made with
+ * PMX_EVAL_TIMING since pmx_sys_timing_begin (HIP events on the system's
+ * stream, read after one synchronisation). avg_ms holds 8 floats. */
+int pmx_sys_timing_begin(pmx_sys *sys);
+int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,7 @@ using namespace ProtoMol::Report;
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <fstream>
 #include <iostream>
 #include <sstream>
@@ -252,8 +253,20 @@ int main(int argc, char **argv) {
           ++npairs;
         }
     }
-    std::fprintf(pf, "pairs %ld\n", npairs);
-    for (size_t n = 0; n < out.size(); ++n) std::fprintf(pf, "%d %d\n", out[n].first, out[n].second);
+    // order-independent set hash: sum of splitmix64(min << 32 | max), mod 2^64
+    unsigned long long hsum = 0;
+    for (size_t n = 0; n < out.size(); ++n) {
+      unsigned long long a = (unsigned)std::min(out[n].first, out[n].second);
+      unsigned long long b = (unsigned)std::max(out[n].first, out[n].second);
+      unsigned long long z = (a << 32) | b;
+      z += 0x9E3779B97F4A7C15ull;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      hsum += z ^ (z >> 31);
+    }
+    std::fprintf(pf, "pairs %ld %llu\n", npairs, hsum);
+    if (!std::getenv("PM_REF_PAIRS_NOLIST"))
+      for (size_t n = 0; n < out.size(); ++n) std::fprintf(pf, "%d %d\n", out[n].first, out[n].second);
     std::fclose(pf);
   }
 

@@ -21,6 +21,7 @@ using namespace pmx;
 
 struct pmx_ctx {
   int device = 0;
+  int cus = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   std::string err;
 };
@@ -61,12 +62,15 @@ struct Sys {
   DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
       ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
       pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec,
-      greal, theta, twx, twy, twz, exclmax;
+      greal, theta, twx, twy, twz, exclmax, border, nborder, border_acc;
+  int maxborder = 65536;
   // results
   long long last_np = 0;
   unsigned long long last_hash = 0;
   float stage_ms[8] = {0};
-  hipEvent_t ev[9] = {};
+  hipEvent_t *ev = nullptr;                    // event set of the current timed call
+  std::vector<std::vector<hipEvent_t>> evsets;  // one set of 9 per timed call since timing_begin
+  size_t evused = 0;
   bool have_positions = false;
   bool cached = false;
   double eout_h[4] = {0, 0, 0, 0};
@@ -374,7 +378,8 @@ int setup(Sys *s) {
       (rc = A(s->npairs, 8 * (size_t)s->ntiles_cap)) || (rc = A(s->hash, 8 * (size_t)s->ntiles_cap)) ||
       (rc = A(s->eptr_d, 4 * (N + 1))) || (rc = A(s->elist_d, 8 * s->elist.size())) ||
       (rc = A(s->etab_d, 16 * s->etab.size())) || (rc = A(s->eout, 64)) || (rc = A(s->np_out, 8)) ||
-      (rc = A(s->hash_out, 8)) || (rc = A(s->exclmax, 8)))
+      (rc = A(s->hash_out, 8)) || (rc = A(s->exclmax, 8)) || (rc = A(s->border, 8 * (size_t)s->maxborder)) ||
+      (rc = A(s->nborder, 8)) || (rc = A(s->border_acc, 32)))
     return rc;
   s->corr_blocks = std::max(1, (int)nblk((long long)s->etab.size(), 256));
   if ((rc = A(s->corr_e, 8 * (size_t)s->corr_blocks))) return rc;
@@ -486,14 +491,14 @@ int setup(Sys *s) {
     // FFT blocking: two LDS buffers of lines_per_block * n complex <= 64 KiB
     const int cpx = s->mixed ? 8 : 16;
     const int budget = 61440 / (2 * cpx);  // 60 KiB of the 64 KiB workgroup LDS
-    s->zl = std::max(1, std::min(pm.nx * pm.ny, budget / pm.nz));
-    s->tky = std::max(1, std::min(pm.nzh, budget / pm.ny));
-    s->tkx = std::max(1, std::min(pm.nzh, budget / pm.nx));
+    // Many small blocks: ~128 butterflies per stage per block, >= 4 blocks per CU.
+    s->zl = std::max(1, std::min(std::min(pm.nx * pm.ny, budget / pm.nz), 512 / pm.nz));
+    const int tk = s->mixed ? 8 : 4;  // 64-byte row segments
+    s->tky = std::max(1, std::min(std::min(pm.nzh, budget / pm.ny), tk));
+    s->tkx = std::max(1, std::min(std::min(pm.nzh, budget / pm.nx), tk));
     s->recip_blocks = pm.ny * ((pm.nzh + s->tkx - 1) / s->tkx);
     if ((rc = A(s->recip_e, 8 * (size_t)s->recip_blocks))) return rc;
   }
-  for (auto &e : s->ev)
-    if (!e) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipStreamSynchronize(st));
   s->cached = true;
   return PMX_OK;
@@ -503,22 +508,42 @@ template <bool MIXED>
 int run_pair(Sys *s, bool stats) {
   hipStream_t st = s->ctx->stream;
   using R4 = typename Real4<typename std::conditional<MIXED, float, double>::type>::type;
-  int dev_cus = 256;
-  hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, s->ctx->device);
-  const unsigned grid = (unsigned)std::max(1, dev_cus * 8);
-  PairOut out{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash)};
+  const unsigned grid = (unsigned)std::max(1, s->ctx->cus * 8);
+  PairOut out{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash),
+              P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow)};
   auto *facc = P<typename AccT<MIXED>::type>(s->facc);
-#define PMX_LAUNCH_PAIR(C, STATS)                                                                               \
-  hipLaunchKernelGGL((k_pair<MIXED, C, STATS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, P<int2>(s->tiles), \
-                     P<int>(s->ntiles), P<int>(s->tilectr), P<int4>(s->entries), P<int>(s->count), s->cap, \
-                     P<unsigned long long>(s->masks), P<R4>(s->xs), P<int>(s->ts), P<int>(s->permp),         \
-                     P<double>(s->xorig), P<double>(s->ljtab), facc, out)
+  const int si = stats ? 1 : 0;
+  // border kernel grid: upper bound from the capacity; threads past the count exit
+  const unsigned bgrid = nblk(s->maxborder, 256);
+#define PMX_LAUNCH_PAIR(C, LJ, CS)                                                                                  \
+  do {                                                                                                             \
+    hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, si,                 \
+                       P<int2>(s->tiles), P<int>(s->ntiles), P<int>(s->tilectr), P<int4>(s->entries),               \
+                       P<int>(s->count), s->cap, P<unsigned long long>(s->masks), P<R4>(s->xs), P<int>(s->ts),      \
+                       P<int>(s->permp), P<double>(s->ljtab), facc, out);                                           \
+    hipLaunchKernelGGL((k_border<MIXED, C, LJ, CS>), dim3(bgrid), dim3(256), 0, st, s->pp, si, P<int2>(s->border), \
+                       P<int>(s->nborder), s->maxborder, P<unsigned long long>(s->masks), P<R4>(s->xs),           \
+                       P<int>(s->ts), P<int>(s->permp), P<double>(s->xorig), P<double>(s->ljtab),                \
+                       P<int>(s->eptr_d), P<int2>(s->elist_d), facc, P<unsigned long long>(s->border_acc),         \
+                       P<unsigned long long>(s->border_acc) + 2);                                                  \
+  } while (0)
+  const int lj = s->d.lj_switch, cs = s->d.coul_switch;
   if (s->pp.coul == COUL_PME) {
-    if (stats) PMX_LAUNCH_PAIR(COUL_PME, true); else PMX_LAUNCH_PAIR(COUL_PME, false);
+    if (lj == SW_C2 && cs == SW_CUTOFF)
+      PMX_LAUNCH_PAIR(COUL_PME, SW_C2, SW_CUTOFF);
+    else if (lj == SW_NONE)
+      PMX_LAUNCH_PAIR(COUL_PME, SW_NONE, SW_ANY);
+    else
+      PMX_LAUNCH_PAIR(COUL_PME, SW_ANY, SW_ANY);
   } else if (s->pp.coul == COUL_CUTOFF) {
-    if (stats) PMX_LAUNCH_PAIR(COUL_CUTOFF, true); else PMX_LAUNCH_PAIR(COUL_CUTOFF, false);
+    if (lj == SW_C2 && cs == SW_C1)
+      PMX_LAUNCH_PAIR(COUL_CUTOFF, SW_C2, SW_C1);
+    else if (lj == SW_NONE)
+      PMX_LAUNCH_PAIR(COUL_CUTOFF, SW_NONE, SW_ANY);
+    else
+      PMX_LAUNCH_PAIR(COUL_CUTOFF, SW_ANY, SW_ANY);
   } else {
-    if (stats) PMX_LAUNCH_PAIR(COUL_NONE, true); else PMX_LAUNCH_PAIR(COUL_NONE, false);
+    PMX_LAUNCH_PAIR(COUL_NONE, SW_ANY, SW_ANY);
   }
 #undef PMX_LAUNCH_PAIR
   return PMX_OK;
@@ -543,16 +568,16 @@ int run_pme(Sys *s, bool timing) {
     if (timing) hipEventRecord(s->ev[4], st);
     const size_t cx = sizeof(Cx<T>);
     const int zb = (pm.nx * pm.ny + s->zl - 1) / s->zl;
-    hipLaunchKernelGGL((k_fft_z_r2c<T, MIXED>), dim3(zb), dim3(256), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
+    hipLaunchKernelGGL((k_fft_z_r2c<T, MIXED>), dim3(zb), dim3(128), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
                        P<double>(s->grid64), P<unsigned long long>(s->gridfx), P<Cx<T>>(s->twz), P<Cx<T>>(s->spec));
     const int yb = pm.nx * ((pm.nzh + s->tky - 1) / s->tky);
-    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(256), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, false,
+    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(128), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, false,
                        P<Cx<T>>(s->twy), P<Cx<T>>(s->spec));
-    hipLaunchKernelGGL((k_fft_x_conv<T>), dim3(s->recip_blocks), dim3(256), 2 * s->tkx * pm.nx * cx, st, pm, s->plx,
+    hipLaunchKernelGGL((k_fft_x_conv<T>), dim3(s->recip_blocks), dim3(128), 2 * s->tkx * pm.nx * cx, st, pm, s->plx,
                        s->tkx, P<Cx<T>>(s->twx), P<T>(s->theta), P<Cx<T>>(s->spec), P<double>(s->recip_e));
-    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(256), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, true,
+    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(128), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, true,
                        P<Cx<T>>(s->twy), P<Cx<T>>(s->spec));
-    hipLaunchKernelGGL((k_fft_z_c2r<T>), dim3(zb), dim3(256), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
+    hipLaunchKernelGGL((k_fft_z_c2r<T>), dim3(zb), dim3(128), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
                        P<Cx<T>>(s->twz), P<Cx<T>>(s->spec), P<T>(s->greal));
     if (timing) hipEventRecord(s->ev[5], st);
     hipLaunchKernelGGL((k_gather<MIXED>), dim3(nblk(s->n, 128)), dim3(128), 0, st, s->n, pm, P<double>(s->xorig),
@@ -574,7 +599,15 @@ int compute(Sys *s, unsigned flags) {
   const bool timing = flags & PMX_EVAL_TIMING;
   const bool stats = flags & PMX_EVAL_PAIRSTATS;
   const int n = s->n, npad = s->npad;
-  if (timing) hipEventRecord(s->ev[0], st);
+  if (timing) {
+    if (s->evused == s->evsets.size()) {
+      std::vector<hipEvent_t> set(9);
+      for (auto &e : set) HIPCHK(hipEventCreate(&e));
+      s->evsets.push_back(set);
+    }
+    s->ev = s->evsets[s->evused++].data();
+    hipEventRecord(s->ev[0], st);
+  }
   // --- sort
   hipLaunchKernelGGL(k_bin, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<double>(s->xorig), s->bin, P<uint32_t>(s->key),
                      P<double>(s->xwrap), P<int>(s->refcell));
@@ -613,7 +646,7 @@ int compute(Sys *s, unsigned flags) {
                      P<float>(s->exclmax) + 1, P<float4>(s->ctr), P<float4>(s->half), P<int4>(s->entries),
                      P<int>(s->count), P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork,
                      P<int>(s->overflow));
-  hipLaunchKernelGGL(k_build_masks, dim3(nblk((long long)s->maxmaskwork * 64, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_build_masks, dim3(s->ctx->cus * 4), dim3(256), 0, st,
                      P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork, P<int4>(s->entries), s->cap, n,
                      P<int>(s->permp), P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned long long>(s->masks));
   hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int2>(s->tiles),
@@ -623,6 +656,8 @@ int compute(Sys *s, unsigned flags) {
   hipLaunchKernelGGL(k_fill_u64, dim3(nblk(3ll * npad, 256)), dim3(256), 0, st, P<unsigned long long>(s->facc), 0ull,
                      3ll * npad);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->tilectr), 0, 2ll);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->nborder), 0, 2ll);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, P<unsigned long long>(s->border_acc), 0ull, 4ll);
   int rc = s->mixed ? run_pair<true>(s, stats) : run_pair<false>(s, stats);
   if (rc) return rc;
   if (timing) hipEventRecord(s->ev[3], st);
@@ -647,7 +682,8 @@ int compute(Sys *s, unsigned flags) {
   hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(1024), 0, st, s->ntiles_cap, P<int>(s->ntiles),
                      P<double>(s->pair_e), recip ? s->recip_blocks : 0, P<double>(s->recip_e),
                      corr ? s->corr_blocks : 0, P<double>(s->corr_e), P<long long>(s->npairs),
-                     P<unsigned long long>(s->hash), stats ? 1 : 0, P<double>(s->eout), P<long long>(s->np_out),
+                     P<unsigned long long>(s->hash), stats ? 1 : 0, P<unsigned long long>(s->border_acc),
+                     P<unsigned long long>(s->border_acc) + 2, P<double>(s->eout), P<long long>(s->np_out),
                      P<unsigned long long>(s->hash_out));
   if (timing) hipEventRecord(s->ev[8], st);
   HIPCHK(hipGetLastError());
@@ -668,7 +704,7 @@ int finish(Sys *s, unsigned flags, double *f_aos, double *energies) {
   }
   HIPCHK(hipStreamSynchronize(st));
   if (ovf) return fail(s->ctx, PMX_ERR_OVERFLOW, "cluster pair list capacity exceeded");
-  if (flags & PMX_EVAL_TIMING)
+  if ((flags & PMX_EVAL_TIMING) && s->ev)
     for (int k = 0; k < 8; ++k) hipEventElapsedTime(&s->stage_ms[k], s->ev[k], s->ev[k + 1]);
   if (f_aos)
     for (size_t k = 0; k < f.size(); ++k) f_aos[k] += f[k];
@@ -688,6 +724,8 @@ int grow_and_retry(Sys *s) {
   s->lp.cap = s->cap;
   s->ntiles_cap = s->nsuper * (s->cap / kTileEntries);
   s->maxmaskwork *= 2;
+  s->maxborder *= 2;
+  if (int rc0 = dalloc(s, s->border, 8 * (size_t)s->maxborder)) return rc0;
   int rc;
   if ((rc = dalloc(s, s->entries, 16 * (size_t)s->nsuper * s->cap)) || (rc = dalloc(s, s->tiles, 8 * (size_t)s->ntiles_cap)) ||
       (rc = dalloc(s, s->pair_e, 16 * (size_t)s->ntiles_cap)) || (rc = dalloc(s, s->npairs, 8 * (size_t)s->ntiles_cap)) ||
@@ -723,6 +761,7 @@ int pmx_ctx_create(int device, pmx_ctx **out) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PMX_ERR_NODEV;
   pmx_ctx *c = new pmx_ctx();
   c->device = device;
+  c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return PMX_ERR_HIP;
@@ -770,8 +809,8 @@ int pmx_sys_destroy(pmx_sys *sys) {
       freed.insert(b->p);
       hipFree(b->p);
     }
-  for (auto &e : s->ev)
-    if (e) hipEventDestroy(e);
+  for (auto &set : s->evsets)
+    for (auto &e : set) hipEventDestroy(e);
   delete sys;
   return PMX_OK;
 }
@@ -825,6 +864,7 @@ int pmx_sys_evaluate(pmx_sys *sys, const double *x_aos, double *f_aos, double *e
   int rc = pmx_sys_set_positions(sys, x_aos);
   if (rc) return rc;
   for (int attempt = 0; attempt < 6; ++attempt) {
+    s->evused = 0;
     rc = pmx_sys_compute(sys, flags);
     if (rc) return rc;
     rc = finish(s, flags, f_aos, energies);
@@ -854,6 +894,30 @@ int pmx_sys_cells(pmx_sys *sys, int *cell_of_atom, int *dims) {
   HIPCHK(hipMemcpyAsync(cell_of_atom, s->refcell.p, 4 * (size_t)s->n, hipMemcpyDeviceToHost, s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
   for (int k = 0; k < 3; ++k) dims[k] = s->bin.rc_n[k];
+  return PMX_OK;
+}
+
+int pmx_sys_timing_begin(pmx_sys *sys) {
+  if (!sys) return PMX_ERR_ARG;
+  sys->s.evused = 0;
+  return PMX_OK;
+}
+
+int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls) {
+  if (!sys || !avg_ms || !ncalls) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  for (int k = 0; k < 8; ++k) avg_ms[k] = 0.f;
+  const size_t m = s->evused;
+  for (size_t c = 0; c < m; ++c)
+    for (int k = 0; k < 8; ++k) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, s->evsets[c][k], s->evsets[c][k + 1]));
+      avg_ms[k] += t;
+    }
+  for (int k = 0; k < 8; ++k) avg_ms[k] = m ? avg_ms[k] / (float)m : 0.f;
+  *ncalls = (int)m;
+  s->evused = 0;
   return PMX_OK;
 }
 

@@ -444,10 +444,10 @@ __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ mas
                                                      const int2 *__restrict__ elist,
                                                      unsigned long long *__restrict__ masks) {
   const int lane = threadIdx.x & 63;
-  const int w = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6);
   int nw = *nmaskwork;
   if (nw > maxmaskwork) nw = maxmaskwork;
-  if (w >= nw) return;
+  const int nwaves = gridDim.x * (blockDim.x / kWave);
+  for (int w = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6); w < nw; w += nwaves) {
   const int eidx = maskwork[w];
   const int S = eidx / cap;
   int4 ent = entries[eidx];
@@ -473,6 +473,7 @@ __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ mas
     }
   }
   if (lane == 0) entries[eidx].z = w * 8;
+  }
 }
 
 // Compact tile list: tiles of kTileEntries entries per supercluster.
@@ -503,34 +504,18 @@ __global__ void k_make_tiles(int nsuper, const int *__restrict__ count, int2 *__
 
 // ---------------------------------------------------------------------------
 // 3. Pair kernel: LJ + Coulomb (cutoff or PME real space) with switching.
-
-template <class R>
-__device__ inline void switch_eval(const SwitchC &sc, R r2, R rinv, bool member, R &val, R &der) {
-  if (!member) {
-    val = 0;
-    der = 0;
-    return;
-  }
-  if (sc.kind == SW_C2) {
-    der = 0;
-    if (r2 >= (R)sc.on2) {
-      R c2 = (R)sc.rc2 - r2;
-      c2 = c2 > (R)0 ? c2 : (R)0;
-      R c4 = c2 * ((R)sc.s2 + (R)2 * r2);
-      val = (R)sc.s1 * (c2 * c4);
-      der = (R)sc.s3 * (c2 * c2 - c4);
-    } else {
-      val = 1;
-    }
-  } else if (sc.kind == SW_C1) {
-    R dist = r2 * rinv;
-    val = (R)1 - dist * ((R)sc.c15_1 - r2 * (R)sc.c05_3);
-    der = -(R)sc.c15_1 * rinv + dist * (R)sc.c15_3;
-  } else {
-    val = 1;
-    der = 0;
-  }
-}
+//
+// Wavefront layout: lane = 8*tj + ti.  A work tile is (i-supercluster S,
+// up to kTileEntries list entries).  Each entry names one j-cluster (8 atoms,
+// lane tj holds atom tj) and an 8-bit mask of the i-clusters of S it is in
+// range of; for every set i-cluster c the 64 lanes evaluate the 8x8 atom
+// pairs (8c+ti, tj).  i-forces stay in registers for the whole tile, j-forces
+// are summed over the 8 i-lanes with DPP and added once per entry.
+//
+// Membership is decided exactly: a pair whose FP32/FP64 r^2 lies within
+// `guard` of a cutoff^2 is not evaluated here but recorded, and k_border
+// re-tests it in FP64 with the reference's operation order on the original
+// coordinates before evaluating it.
 
 __device__ inline float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ inline double fast_rsqrt(double x) { return 1.0 / sqrt(x); }
@@ -570,15 +555,111 @@ struct AccT<true> {
   }
 };
 
-// Reduction over the 8 lanes of a cluster group (lanes differing in bits 0..2).
+// Switch constants in the arithmetic type of the kernel.
 template <class R>
-__device__ inline R group8_sum(R v) {
+struct SwR {
+  R rc2, on2, s1, s2, s3, c15_1, c05_3, c15_3;
+  __device__ void load(const SwitchC &s) {
+    rc2 = (R)s.rc2;
+    on2 = (R)s.on2;
+    s1 = (R)s.s1;
+    s2 = (R)s.s2;
+    s3 = (R)s.s3;
+    c15_1 = (R)s.c15_1;
+    c05_3 = (R)s.c05_3;
+    c15_3 = (R)s.c15_3;
+  }
+};
+
+// Switching function value and (1/r) dS/dr at r2 (in range), branch-free.
+// KIND = SW_ANY selects at run time from `kind`.
+constexpr int SW_ANY = 9;
+template <int KIND, class R>
+__device__ inline void switch_val(int kind, const SwR<R> &s, R r2, R rinv, R &val, R &der) {
+  const int k = KIND == SW_ANY ? kind : KIND;
+  if (k == SW_C2) {
+    // C2SwitchingFunction.h:42-53: 1 below r_on, polynomial on [r_on, rc]
+    R c2 = s.rc2 - r2;
+    c2 = c2 > (R)0 ? c2 : (R)0;
+    const R c4 = c2 * (s.s2 + (R)2 * r2);
+    const bool on = r2 >= s.on2;
+    val = on ? s.s1 * (c2 * c4) : (R)1;
+    der = on ? s.s3 * (c2 * c2 - c4) : (R)0;
+  } else if (k == SW_C1) {
+    // C1SwitchingFunction.h:41-50
+    const R dist = r2 * rinv;
+    val = (R)1 - dist * (s.c15_1 - r2 * s.c05_3);
+    der = -s.c15_1 * rinv + dist * s.c15_3;
+  } else {
+    val = (R)1;
+    der = (R)0;
+  }
+}
+
+// One pair's energies and scalar force (F = f * d), given membership flags.
+template <int COUL, int LJSW, int CSW, class R>
+__device__ inline void pair_eval(const PairParams &pp, const SwR<R> &sl, const SwR<R> &sc, R alpha, R a2pi, R csf,
+                                 R r2, R qq, R A, R B, bool in_lj, bool in_c, R &elj, R &eco, R &fr) {
+  const R r2s = (in_lj || in_c) ? r2 : (R)1;
+  const R rinv = fast_rsqrt(r2s);
+  const R rinv2 = rinv * rinv;
+  fr = (R)0;
+  elj = (R)0;
+  eco = (R)0;
+  if (LJSW != SW_NONE) {
+    // LennardJonesForce.h:28-60
+    const R r6 = rinv2 * rinv2 * rinv2;
+    const R r6B = B * r6, r12A = A * r6 * r6;
+    R e = r12A - r6B;
+    R f = ((R)12 * r12A - (R)6 * r6B) * rinv2;
+    R sv, sd;
+    switch_val<LJSW, R>(pp.lj.kind, sl, r2s, rinv, sv, sd);
+    f = f * sv - e * sd;
+    e = e * sv;
+    elj = in_lj ? e : (R)0;
+    fr += in_lj ? f : (R)0;
+  }
+  if (COUL == COUL_CUTOFF) {
+    // CoulombForce.h:29-40
+    R e = qq * rinv;
+    R f = e * rinv2;
+    R sv, sd;
+    switch_val<CSW, R>(pp.co.kind, sc, r2s, rinv, sv, sd);
+    f = f * sv - e * sd;
+    e = e * sv;
+    eco = in_c ? e : (R)0;
+    fr += in_c ? f : (R)0;
+  } else if (COUL == COUL_PME) {
+    // NonbondedPMEwaldSystemForce.h:657-683 (poly5 form)
+    const R r = r2s * rinv;
+    const R ar = alpha * r;
+    const R ex = qq * fast_exp(-ar * ar);
+    R e = poly5<R>(ar) * ex * rinv;
+    R f = (e + a2pi * ex) * rinv2;
+    R sv, sd;
+    switch_val<CSW, R>(pp.co.kind, sc, r2s, rinv, sv, sd);
+    f = f * sv - e * sd;
+    e = e * sv;
+    eco = in_c ? e : (R)0;
+    fr += in_c ? f : (R)0;
+  }
+}
+
+// Sum over the 8 lanes of a cluster group (lanes differing in bits 0..2).
+__device__ inline float group8_sum(float v) {
+  // quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, true));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, true));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, true));
+  return v;
+}
+__device__ inline double group8_sum(double v) {
   v += __shfl_xor(v, 1);
   v += __shfl_xor(v, 2);
   v += __shfl_xor(v, 4);
   return v;
 }
-// Reduction over lanes differing in bits 3..5.
+// Sum over lanes differing in bits 3..5.
 template <class R>
 __device__ inline R col8_sum(R v) {
   v += __shfl_xor(v, 8);
@@ -588,23 +669,27 @@ __device__ inline R col8_sum(R v) {
 }
 
 struct PairOut {
-  double *energy;           // [2 * ntiles_max]: per tile (coulomb, lj)
-  long long *npairs;        // [ntiles_max] when stats
-  unsigned long long *hash; // [ntiles_max]
+  double *energy;            // [2 * ntiles_max]: per tile (coulomb, lj)
+  long long *npairs;         // [ntiles_max] when stats
+  unsigned long long *hash;  // [ntiles_max]
+  int2 *border;              // recorded boundary pairs (sorted slots i, j)
+  int *nborder;
+  int maxborder;
+  int *overflow;
 };
 
-template <bool MIXED, int COUL, bool STATS>
+template <bool MIXED, int COUL, int LJSW, int CSW>
 __global__ void __launch_bounds__(kPairBlock) k_pair(
-    PairParams pp, const int2 *__restrict__ tiles, const int *__restrict__ ntiles_p, int *__restrict__ tile_counter,
-    const int4 *__restrict__ entries, const int *__restrict__ count, int cap, const unsigned long long *__restrict__ masks,
+    PairParams pp, int stats, const int2 *__restrict__ tiles, const int *__restrict__ ntiles_p,
+    int *__restrict__ tile_counter, const int4 *__restrict__ entries, const int *__restrict__ count, int cap,
+    const unsigned long long *__restrict__ masks,
     const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
-    const int *__restrict__ ts, const int *__restrict__ perm, const double *__restrict__ xorig,
-    const double *__restrict__ ljtab, typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
+    const int *__restrict__ ts, const int *__restrict__ perm, const double *__restrict__ ljtab,
+    typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
   using R = typename std::conditional<MIXED, float, double>::type;
   using R4 = typename Real4<R>::type;
   using Acc = AccT<MIXED>;
   __shared__ R4 s_i[kPairBlock / kWave][kSuperSize];
-  __shared__ int s_ti[kPairBlock / kWave][kSuperSize];
   __shared__ R4 s_lj[kMaxTypes * kMaxTypes];  // {A, B, A14, B14}
 
   const int nt = pp.ntypes;
@@ -623,7 +708,9 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   const int ti = lane & 7, tj = lane >> 3;
   const R Lx = (R)pp.box.L[0], Ly = (R)pp.box.L[1], Lz = (R)pp.box.L[2];
   const R Lix = (R)pp.box.Linv[0], Liy = (R)pp.box.Linv[1], Liz = (R)pp.box.Linv[2];
-  const R rc2lj = (R)pp.lj.rc2, rc2c = (R)pp.co.rc2;
+  SwR<R> sl, sc;
+  sl.load(pp.lj);
+  sc.load(pp.co);
   const R guard = (R)pp.guard;
   const R alpha = (R)pp.alpha, a2pi = (R)pp.a2pi, csf = (R)pp.csf;
   const int ntiles = *ntiles_p;
@@ -631,22 +718,25 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   while (true) {
     int t = 0;
     if (lane == 0) t = atomicAdd(tile_counter, 1);
-    t = __shfl(t, 0);
-    t = __builtin_amdgcn_readfirstlane(t);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
     if (t >= ntiles) break;
     const int2 tile = tiles[t];
     const int S = __builtin_amdgcn_readfirstlane(tile.x);
     const int e0 = __builtin_amdgcn_readfirstlane(tile.y);
-    // i-supercluster into LDS
     s_i[wv][lane] = xs[S * kSuperSize + lane];
-    s_ti[wv][lane] = ts[S * kSuperSize + lane];
+    int tyi[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int v = ts[S * kSuperSize + c * 8 + ti];
+      tyi[c] = v < 0 ? 0 : v;
+    }
     __builtin_amdgcn_wave_barrier();
 
     R fi[8][3];
 #pragma unroll
     for (int c = 0; c < 8; ++c) fi[c][0] = fi[c][1] = fi[c][2] = (R)0;
     double elj = 0.0, eco = 0.0;
-    long long npair = 0;
+    int npair = 0;
     unsigned long long hsum = 0;
 
     const int eend = __builtin_amdgcn_readfirstlane(min(e0 + kTileEntries, count[S]));
@@ -657,7 +747,8 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
       const int moff = __builtin_amdgcn_readfirstlane(ent.z);
       const int sj = cj * kClusterSize + tj;
       const R4 xj = xs[sj];
-      const int tyj = ts[sj];
+      const int tyj0 = ts[sj];
+      const int tyj = tyj0 < 0 ? 0 : tyj0;
       R fj0 = 0, fj1 = 0, fj2 = 0;
       R el = 0, ec = 0;
 #pragma unroll
@@ -671,75 +762,42 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
           modified = (mm >> lane) & 1ull;
         }
         const R4 xi = s_i[wv][c * 8 + ti];
-        const int tyi = s_ti[wv][c * 8 + ti];
         R dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         dx -= Lx * fast_rint(dx * Lix);
         dy -= Ly * fast_rint(dy * Liy);
         dz -= Lz * fast_rint(dz * Liz);
-        R r2 = dx * dx + dy * dy + dz * dz;
-        bool in_lj = cand && pp.has_lj && r2 <= rc2lj;
-        bool in_c = cand && COUL != COUL_NONE && r2 <= rc2c;
-        bool border = cand && ((pp.has_lj && fabs(r2 - rc2lj) <= guard) ||
-                               (COUL != COUL_NONE && fabs(r2 - rc2c) <= guard));
-        if (__ballot(border)) {
-          if (border) {
-            const int ai = perm[S * kSuperSize + c * 8 + ti], aj = perm[sj];
-            if (ai >= 0 && aj >= 0) {
-              double d2 = exact_d2(xorig, ai < aj ? ai : aj, ai < aj ? aj : ai, pp.box);
-              in_lj = pp.has_lj && d2 <= pp.lj.rc2;
-              in_c = COUL != COUL_NONE && d2 <= pp.co.rc2;
-            }
-          }
+        const R r2 = dx * dx + dy * dy + dz * dz;
+        bool border = false;
+        if (LJSW != SW_NONE) border |= fabs(r2 - sl.rc2) <= guard;
+        if (COUL != COUL_NONE) border |= fabs(r2 - sc.rc2) <= guard;
+        border = border && cand;
+        const bool ok = cand && !border;
+        const bool in_lj = LJSW != SW_NONE && ok && r2 <= sl.rc2;
+        const bool in_c = COUL != COUL_NONE && ok && r2 <= sc.rc2;
+        if (border) {
+          const int b = atomicAdd(out.nborder, 1);
+          if (b < out.maxborder)
+            out.border[b] = make_int2(S * kSuperSize + c * 8 + ti, sj);
+          else
+            atomicExch(out.overflow, 1);
         }
-        const bool member = in_lj || in_c;
-        if (STATS && member) {
+        if (stats && (in_lj || in_c)) {
           const int ai = perm[S * kSuperSize + c * 8 + ti], aj = perm[sj];
           ++npair;
           hsum += pair_hash(ai < aj ? ai : aj, ai < aj ? aj : ai);
         }
-        const R r2s = member ? r2 : (R)1;
-        const R rinv = fast_rsqrt(r2s);
-        const R rinv2 = rinv * rinv;
-        R fr = 0;
-        if (pp.has_lj) {
-          const R4 lj = s_lj[(tyi < 0 ? 0 : tyi) * nt + (tyj < 0 ? 0 : tyj)];
-          const R A = modified ? lj.z : lj.x, B = modified ? lj.w : lj.y;
-          const R r6 = rinv2 * rinv2 * rinv2;
-          const R r6B = B * r6, r12A = A * r6 * r6;
-          R e = r12A - r6B;
-          R f = ((R)12 * r12A - (R)6 * r6B) * rinv2;
-          R sv, sd;
-          switch_eval<R>(pp.lj, r2s, rinv, in_lj, sv, sd);
-          f = f * sv - e * sd;
-          e = e * sv;
-          el += e;
-          fr += f;
+        R A = 0, B = 0;
+        if (LJSW != SW_NONE) {
+          const R4 lj = s_lj[tyi[c] * nt + tyj];
+          A = modified ? lj.z : lj.x;
+          B = modified ? lj.w : lj.y;
         }
-        if (COUL == COUL_CUTOFF) {
-          R e = xi.w * xj.w * rinv;
-          if (modified) e *= csf;
-          R f = e * rinv2;
-          R sv, sd;
-          switch_eval<R>(pp.co, r2s, rinv, in_c, sv, sd);
-          f = f * sv - e * sd;
-          e = e * sv;
-          ec += e;
-          fr += f;
-        } else if (COUL == COUL_PME) {
-          R qq = xi.w * xj.w;
-          if (modified) qq *= csf;
-          const R r = r2s * rinv;
-          const R ar = alpha * r;
-          const R ex = qq * fast_exp(-ar * ar);
-          R e = poly5<R>(ar) * ex * rinv;
-          R f = (e + a2pi * ex) * rinv2;
-          R sv, sd;
-          switch_eval<R>(pp.co, r2s, rinv, in_c, sv, sd);
-          f = f * sv - e * sd;
-          e = e * sv;
-          ec += e;
-          fr += f;
-        }
+        R qq = xi.w * xj.w;
+        if (modified) qq *= csf;
+        R e1, e2, fr;
+        pair_eval<COUL, LJSW, CSW, R>(pp, sl, sc, alpha, a2pi, csf, r2, qq, A, B, in_lj, in_c, e1, e2, fr);
+        el += e1;
+        ec += e2;
         const R fx = dx * fr, fy = dy * fr, fz = dz * fr;
         fi[c][0] -= fx;
         fi[c][1] -= fy;
@@ -748,18 +806,16 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
         fj1 += fy;
         fj2 += fz;
       }
-      // j forces: sum over the 8 i-lanes of each j atom, one component per lane
       fj0 = group8_sum(fj0);
       fj1 = group8_sum(fj1);
       fj2 = group8_sum(fj2);
-      if (ti < 3 && tyj >= 0) {
+      if (ti < 3 && tyj0 >= 0) {
         const R v = ti == 0 ? fj0 : (ti == 1 ? fj1 : fj2);
         Acc::add(&facc[3 * sj + ti], v);
       }
       elj += (double)el;
       eco += (double)ec;
     }
-    // i forces: sum over the 8 j-lanes; lane (ti, tj) then owns atom 8*tj + ti
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       fi[c][0] = col8_sum(fi[c][0]);
@@ -775,26 +831,102 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
         my2 = fi[c][2];
       }
     const int si = S * kSuperSize + tj * 8 + ti;
-    if (s_ti[wv][tj * 8 + ti] >= 0) {
+    if (ts[si] >= 0) {
       Acc::add(&facc[3 * si], my0);
       Acc::add(&facc[3 * si + 1], my1);
       Acc::add(&facc[3 * si + 2], my2);
     }
     elj = wave_sum(elj);
     eco = wave_sum(eco);
-    if (STATS) {
-      npair = wave_sum(npair);
+    long long np = 0;
+    if (stats) {
+      np = wave_sum((long long)npair);
       hsum = wave_sum(hsum);
     }
     if (lane == 0) {
       out.energy[2 * t] = eco;
       out.energy[2 * t + 1] = elj;
-      if (STATS) {
-        out.npairs[t] = npair;
+      if (stats) {
+        out.npairs[t] = np;
         out.hash[t] = hsum;
       }
     }
     __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Boundary pairs: exact FP64 membership on the original coordinates
+// (reference operation order), then the same pair evaluation.  Energies go
+// to int64 fixed point (2^-32) so the total is independent of record order.
+template <bool MIXED, int COUL, int LJSW, int CSW>
+__global__ void k_border(PairParams pp, int stats, const int2 *__restrict__ border, const int *__restrict__ nborder,
+                         int maxborder, const unsigned long long *__restrict__ masks_unused,
+                         const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
+                         const int *__restrict__ ts, const int *__restrict__ perm, const double *__restrict__ xorig,
+                         const double *__restrict__ ljtab, const int *__restrict__ eptr, const int2 *__restrict__ elist,
+                         typename AccT<MIXED>::type *__restrict__ facc, unsigned long long *__restrict__ eacc,
+                         unsigned long long *__restrict__ stat) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  const int nb = min(*nborder, maxborder);
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int2 rec = border[b];
+  const int si = rec.x, sj = rec.y;
+  const int ai = perm[si], aj = perm[sj];
+  const int lo = ai < aj ? ai : aj, hi = ai < aj ? aj : ai;
+  const double d2 = exact_d2(xorig, lo, hi, pp.box);
+  const bool in_lj = LJSW != SW_NONE && d2 <= pp.lj.rc2;
+  const bool in_c = COUL != COUL_NONE && d2 <= pp.co.rc2;
+  if (!(in_lj || in_c)) return;
+  // exclusion class (recorded pairs already passed the FULL mask)
+  int cls = 0;
+  {
+    int l = eptr[ai], h = eptr[ai + 1] - 1;
+    while (l <= h) {
+      const int mid = (l + h) >> 1;
+      const int v = elist[mid].x;
+      if (v == aj) {
+        cls = elist[mid].y;
+        break;
+      }
+      if (v < aj)
+        l = mid + 1;
+      else
+        h = mid - 1;
+    }
+  }
+  const bool modified = cls == 2;
+  SwR<R> sl, sc;
+  sl.load(pp.lj);
+  sc.load(pp.co);
+  const auto xi = xs[si], xj = xs[sj];
+  R dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+  dx -= (R)pp.box.L[0] * fast_rint(dx * (R)pp.box.Linv[0]);
+  dy -= (R)pp.box.L[1] * fast_rint(dy * (R)pp.box.Linv[1]);
+  dz -= (R)pp.box.L[2] * fast_rint(dz * (R)pp.box.Linv[2]);
+  R r2 = dx * dx + dy * dy + dz * dz;
+  // evaluate at the cutoff side the exact test chose
+  R A = 0, B = 0;
+  if (LJSW != SW_NONE) {
+    const int k = 4 * (ts[si] * pp.ntypes + ts[sj]);
+    A = (R)ljtab[k + (modified ? 2 : 0)];
+    B = (R)ljtab[k + (modified ? 3 : 1)];
+  }
+  R qq = xi.w * xj.w;
+  if (modified) qq *= (R)pp.csf;
+  R e1, e2, fr;
+  pair_eval<COUL, LJSW, CSW, R>(pp, sl, sc, (R)pp.alpha, (R)pp.a2pi, (R)pp.csf, r2, qq, A, B, in_lj, in_c, e1, e2, fr);
+  AccT<MIXED>::add(&facc[3 * si], -dx * fr);
+  AccT<MIXED>::add(&facc[3 * si + 1], -dy * fr);
+  AccT<MIXED>::add(&facc[3 * si + 2], -dz * fr);
+  AccT<MIXED>::add(&facc[3 * sj], dx * fr);
+  AccT<MIXED>::add(&facc[3 * sj + 1], dy * fr);
+  AccT<MIXED>::add(&facc[3 * sj + 2], dz * fr);
+  atomicAdd(&eacc[0], (unsigned long long)__double2ll_rn((double)e2 * kForceScale));
+  atomicAdd(&eacc[1], (unsigned long long)__double2ll_rn((double)e1 * kForceScale));
+  if (stats) {
+    atomicAdd(&stat[0], 1ull);
+    atomicAdd(&stat[1], pair_hash(lo, hi));
   }
 }
 
@@ -1215,11 +1347,13 @@ __global__ void k_unsort_forces(int n, const int *__restrict__ slot, const typen
 __global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p, const double *__restrict__ pair_e,
                                 int nrecip, const double *__restrict__ recip_e, int ncorr,
                                 const double *__restrict__ corr_e, const long long *__restrict__ npairs,
-                                const unsigned long long *__restrict__ hash, int stats, double *__restrict__ out,
+                                const unsigned long long *__restrict__ hash, int stats,
+                                const unsigned long long *__restrict__ border_e,
+                                const unsigned long long *__restrict__ border_stat, double *__restrict__ out,
                                 long long *__restrict__ out_np, unsigned long long *__restrict__ out_hash) {
-  __shared__ double s0[1024], s1[1024], s2[1024], s3[1024];
-  __shared__ long long sn[1024];
-  __shared__ unsigned long long sh[1024];
+  __shared__ double sd[4][1024 / kWave];
+  __shared__ long long sn[1024 / kWave];
+  __shared__ unsigned long long sh[1024 / kWave];
   const int nt = min(*ntiles_p, ntiles_cap);
   double c = 0, l = 0, r = 0, k = 0;
   long long np = 0;
@@ -1234,31 +1368,40 @@ __global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p
   }
   for (int i = threadIdx.x; i < nrecip; i += blockDim.x) r += recip_e[i];
   for (int i = threadIdx.x; i < ncorr; i += blockDim.x) k += corr_e[i];
-  s0[threadIdx.x] = c;
-  s1[threadIdx.x] = l;
-  s2[threadIdx.x] = r;
-  s3[threadIdx.x] = k;
-  sn[threadIdx.x] = np;
-  sh[threadIdx.x] = h;
+  c = wave_sum(c);
+  l = wave_sum(l);
+  r = wave_sum(r);
+  k = wave_sum(k);
+  np = wave_sum(np);
+  h = wave_sum(h);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sd[0][w] = c;
+    sd[1][w] = l;
+    sd[2][w] = r;
+    sd[3][w] = k;
+    sn[w] = np;
+    sh[w] = h;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     double C = 0, Lj = 0, Rr = 0, K = 0;
     long long N = 0;
     unsigned long long H = 0;
-    for (int i = 0; i < (int)blockDim.x; ++i) {
-      C += s0[i];
-      Lj += s1[i];
-      Rr += s2[i];
-      K += s3[i];
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) {
+      C += sd[0][i];
+      Lj += sd[1][i];
+      Rr += sd[2][i];
+      K += sd[3][i];
       N += sn[i];
       H += sh[i];
     }
-    out[0] = C;
-    out[1] = Lj;
+    out[0] = C + (double)(long long)border_e[0] * (1.0 / kForceScale);
+    out[1] = Lj + (double)(long long)border_e[1] * (1.0 / kForceScale);
     out[2] = 0.5 * Rr;
     out[3] = K;
-    *out_np = N;
-    *out_hash = H;
+    *out_np = N + (long long)border_stat[0];
+    *out_hash = H + border_stat[1];
   }
 }
 

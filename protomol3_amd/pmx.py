@@ -49,7 +49,7 @@ class SysDesc(C.Structure):
 EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_create", "pmx_sys_destroy",
             "pmx_sys_uncache", "pmx_sys_evaluate", "pmx_sys_set_positions", "pmx_sys_compute",
             "pmx_sys_get_forces", "pmx_sys_sync", "pmx_sys_alpha", "pmx_sys_pair_stats", "pmx_pair_hash",
-            "pmx_sys_cells", "pmx_sys_stage_times"]
+            "pmx_sys_cells", "pmx_sys_stage_times", "pmx_sys_timing_begin", "pmx_sys_timing_end"]
 
 _lib = None
 
@@ -82,6 +82,8 @@ def load() -> C.CDLL:
     L.pmx_pair_hash.restype = C.c_ulonglong
     L.pmx_sys_cells.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.pmx_sys_stage_times.argtypes = [vp, C.POINTER(C.c_float)]
+    L.pmx_sys_timing_begin.argtypes = [vp]
+    L.pmx_sys_timing_end.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -230,6 +232,16 @@ class System:
         ms = (C.c_float * 8)()
         self.ctx.check(self.ctx.lib.pmx_sys_stage_times(self.h, ms), "pmx_sys_stage_times")
         return dict(zip(STAGES, [float(v) for v in ms]))
+
+    def timing_begin(self):
+        self.ctx.check(self.ctx.lib.pmx_sys_timing_begin(self.h), "pmx_sys_timing_begin")
+
+    def timing_end(self):
+        """(stage -> average ms, number of timed calls)."""
+        ms = (C.c_float * 8)()
+        n = C.c_int()
+        self.ctx.check(self.ctx.lib.pmx_sys_timing_end(self.h, ms, C.byref(n)), "pmx_sys_timing_end")
+        return dict(zip(STAGES, [float(v) for v in ms])), n.value
 
     def close(self):
         if self.h:

@@ -1,0 +1,80 @@
+"""Loader for the reference fixtures in tests/golden/*.npz (written by
+make_fixtures.py from the reference build).  Pure data access: rebuilds the
+MolSystem each fixture was made from and translates its force spec."""
+from __future__ import annotations
+
+import glob
+import hashlib
+import json
+import os
+
+import numpy as np
+
+from protomol3_amd import charmm, systems
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+W423 = os.path.join(HERE, "water_423")
+
+SW_NONE, SW_CUTOFF, SW_C1, SW_C2 = 0, 1, 2, 3
+
+
+def names() -> list[str]:
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz")))
+
+
+def load(name: str) -> dict:
+    with np.load(os.path.join(HERE, f"{name}.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def system(fx: dict):
+    gen = str(fx["generator"])
+    args = json.loads(str(fx["generator_args"]))
+    if gen == "charmm_water423":
+        s = charmm.build_system(os.path.join(W423, "water_423.psf"), os.path.join(W423, "water_423.pos.pdb"),
+                                os.path.join(W423, "par_all27_prot_lipid.inp"), box=(28.0, 28.0, 28.0))
+    else:
+        s = getattr(systems, gen)(**args)
+    if fx["pos"].shape[0]:
+        s.pos = np.array(fx["pos"])
+    digest = hashlib.sha256(np.ascontiguousarray(s.pos).tobytes()).hexdigest()
+    if digest != str(fx["pos_sha256"]):
+        raise RuntimeError("regenerated positions differ from the fixture's (generator changed?)")
+    return s
+
+
+def spec(fx: dict) -> dict:
+    """Force spec of the fixture as keyword arguments: cellsize, lj, coul, pme
+    (tuples/dicts in the form oracle.make_spec and pmx.ForceSpec take)."""
+    tok = str(fx["spec"]).split()
+    out = dict(cellsize=float(fx["cellsize"]), lj=None, coul=None, pme=None)
+    i = 0
+    while i < len(tok):
+        k = tok[i]
+        if k == "lj_c2":
+            out["lj"] = (SW_C2, float(tok[i + 1]), float(tok[i + 2])); i += 3
+        elif k == "lj_cutoff":
+            out["lj"] = (SW_CUTOFF, 0.0, float(tok[i + 1])); i += 2
+        elif k == "coul_c1":
+            out["coul"] = (SW_C1, 0.0, float(tok[i + 1])); i += 2
+        elif k == "coul_c2":
+            out["coul"] = (SW_C2, float(tok[i + 1]), float(tok[i + 2])); i += 3
+        elif k == "coul_cutoff":
+            out["coul"] = (SW_CUTOFF, 0.0, float(tok[i + 1])); i += 2
+        elif k in ("pme", "pme_c1"):
+            g = tuple(int(v) for v in tok[i + 1:i + 4])
+            mask = tok[i + 8]
+            out["pme"] = dict(grid=g, order=int(tok[i + 4]), rc=float(tok[i + 5]), accuracy=float(tok[i + 6]),
+                              alpha=float(tok[i + 7]), switch=SW_C1 if k == "pme_c1" else SW_CUTOFF,
+                              terms=(1 if mask[0] == "1" else 0) | (2 if mask[1] == "1" else 0)
+                              | (4 if mask[2] == "1" else 0))
+            i += 9
+        else:
+            raise ValueError(f"unknown spec token {k}")
+    return out
+
+
+def oracle_pme(p: dict | None) -> dict | None:
+    if p is None:
+        return None
+    return dict(p, real=bool(p["terms"] & 1), recip=bool(p["terms"] & 2), corr=bool(p["terms"] & 4))

@@ -1,0 +1,163 @@
+"""The oracle pinned by mathematics, independent of the reference's code:
+closed forms (Madelung constant, Lennard-Jones minimum, B-spline cubic,
+Abramowitz-Stegun bound), brute-force checks (pair sets, DFT), exact
+invariants (grid-shift invariance, partition of unity) and finite
+differences.  Each is chosen so that a plausible slip (a dropped term, a
+transposed index, a wrong sign) fails it."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from protomol3_amd import systems
+from protomol3_amd.systems import MolSystem, charmm_table, rmin_half
+
+
+def test_poly5_within_abramowitz_stegun_bound():
+    # A&S 7.1.26: |erfc(x) - t(a1 + t(a2 + ...)) e^{-x^2}| <= 1.5e-7
+    for x in np.linspace(0.0, 6.0, 601):
+        assert abs(math.erfc(x) - O.poly5(x) * math.exp(-x * x)) <= 1.5e-7
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+def test_bspline_partition_of_unity_and_derivative(order):
+    rng = np.random.default_rng(order)
+    for w in rng.uniform(0, 1, 20):
+        th, dth = O.bspline(order, w)
+        assert abs(th.sum() - 1.0) < 1e-14
+        assert abs(dth.sum()) < 1e-14
+        assert (th >= -1e-15).all()
+        h = 1e-6
+        thp, _ = O.bspline(order, w + h)
+        thm, _ = O.bspline(order, w - h)
+        assert np.allclose((thp - thm) / (2 * h), dth, atol=1e-8)
+
+
+def test_bspline_order4_closed_form():
+    # cubic B-spline pieces at offsets 1+w, w, 1-w, 2-w from the particle
+    for w in np.linspace(0, 1, 11):
+        th, _ = O.bspline(4, w)
+        ref = [(1 - w) ** 3 / 6, (3 * w ** 3 - 6 * w ** 2 + 4) / 6, (-3 * w ** 3 + 3 * w ** 2 + 3 * w + 1) / 6,
+               w ** 3 / 6]
+        assert np.allclose(th, ref, atol=1e-15)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (6, 10, 8), (5, 9, 12), (7, 4, 3)])
+def test_fft_matches_direct_dft(shape):
+    rng = np.random.default_rng(sum(shape))
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    assert np.allclose(O.fft3d(a, -1), np.fft.fftn(a), atol=1e-10)
+    assert np.allclose(O.fft3d(a, +1), np.fft.ifftn(a) * a.size, atol=1e-10)
+
+
+def _point_system(pos, q, box, types=None, lj=None, excl=None):
+    n = len(pos)
+    return MolSystem(name="pts", pos=np.asarray(pos, float), charge=np.asarray(q, float),
+                     atype=np.zeros(n, np.int32) if types is None else np.asarray(types, np.int32),
+                     molecule=np.arange(n, dtype=np.int32), mass=np.ones(n),
+                     lj=np.zeros((1, 1, 4)) if lj is None else lj, box=np.asarray(box, float), origin=np.zeros(3),
+                     bonds=np.zeros((0, 2), np.int32), bond_par=np.zeros((0, 2)),
+                     angles=np.zeros((0, 3), np.int32), angle_par=np.zeros((0, 2)),
+                     exclusions=np.zeros((0, 3), np.int32) if excl is None else np.asarray(excl, np.int32))
+
+
+def test_pme_reproduces_rock_salt_madelung_constant():
+    """64 ions of alternating sign on a simple cubic lattice (rock salt),
+    spacing a: E = -(N/2) M q^2 / a with M = 1.747564594633."""
+    a, m = 2.5, 4
+    g = np.arange(m) * a
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + 0.3
+    idx = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+    q = np.where(idx.sum(1) % 2 == 0, 1.0, -1.0) * 18.2226123264
+    s = _point_system(pos, q, [m * a] * 3)
+    rc = 4.99
+    alpha = 1.4   # erfc(alpha rc)/rc ~ 1e-10
+    sp = O.make_spec(2.0, pme=dict(rc=rc, grid=(48, 48, 48), order=8, alpha=alpha))
+    _, ec, _ = O.evaluate(s, sp)
+    ref = -(len(q) / 2) * 1.747564594633 * 18.2226123264 ** 2 / a
+    # the real-space term uses the reference's poly5 erfc (|error| <= 1.5e-7
+    # absolute per pair), which bounds the agreement at ~1e-6
+    assert abs(ec / ref - 1) < 1e-6
+
+
+def test_pme_converges_to_direct_ewald():
+    """Same alpha: PME (poly5 real space) against classic Ewald with exact
+    erfc; the error must shrink with grid size and order."""
+    s = systems.water_box(n_side=4, box=12.4, seed=11)
+    alpha, rc = 0.5, 6.2
+    f_ew, e_ew = O.ewald_direct(s, alpha, rc, 12)
+    errs = []
+    for grid, order in [(16, 4), (32, 6), (48, 8)]:
+        sp = O.make_spec(3.0, pme=dict(rc=rc, grid=(grid,) * 3, order=order, alpha=alpha))
+        f, e, _ = O.evaluate(s, sp)
+        rms = np.sqrt(np.mean(np.sum((f - f_ew) ** 2, 1)) / np.mean(np.sum(f_ew ** 2, 1)))
+        errs.append((abs(e - e_ew) / abs(e_ew), rms))
+    assert errs[0][1] > errs[1][1] > errs[2][1]
+    # residual: poly5 (1.5e-7 per pair) in the real space term vs exact erfc
+    assert errs[2][0] < 3e-5 and errs[2][1] < 1e-4
+
+
+def test_pme_reciprocal_energy_positive_and_grid_shift_invariant():
+    s = systems.water_box(n_side=4, box=12.4, seed=12)
+    sp = O.make_spec(3.0, pme=dict(rc=6.0, grid=(16, 16, 16), order=4, alpha=0.45, real=False, corr=False))
+    f0, e0, _ = O.evaluate(s, sp)
+    assert e0 > 0
+    h = 12.4 / 16
+    f1, e1, _ = O.evaluate(s, sp, pos=s.pos + np.array([h, 2 * h, -3 * h]))
+    assert abs(e1 - e0) < 1e-10 * abs(e0)
+    assert np.abs(f1 - f0).max() < 1e-9 * np.abs(f0).max()
+
+
+def test_lj_minimum_and_c2_switch_limits():
+    eps, sig = 0.1521, 3.1507
+    lj = charmm_table([eps], [rmin_half(sig)])
+    rmin = 2 * rmin_half(sig)
+    s = _point_system([[1.0, 1.0, 1.0], [1.0 + rmin, 1.0, 1.0]], [0.0, 0.0], [30.0] * 3, lj=lj)
+    f, _, elj = O.evaluate(s, O.make_spec(5.0, lj=(O.SW_C2, 8.0, 10.0)))
+    assert abs(elj + eps) < 1e-12
+    assert np.abs(f).max() < 1e-10
+    # C2 switch: S(r_on)=1, S(rc)=0, energy continuous across r_on and rc
+    es = []
+    for r in [8.0 - 1e-7, 8.0 + 1e-7, 10.0 - 1e-7]:
+        s.pos[1] = [1.0 + r, 1.0, 1.0]
+        es.append(O.evaluate(s, O.make_spec(5.0, lj=(O.SW_C2, 8.0, 10.0)))[2])
+    assert abs(es[0] - es[1]) < 1e-6 * abs(es[0])  # slope x 2e-7 step
+    assert abs(es[2]) < 1e-12
+
+
+def test_force_is_minus_energy_gradient():
+    """Central differences of the total energy (LJ C2 + Coulomb C1, and LJ C2
+    + full PME) against the oracle's forces."""
+    s = systems.water_box(n_side=3, box=10.2, seed=13, jitter=0.3)
+    specs = [O.make_spec(3.0, lj=(O.SW_C2, 3.0, 5.0), coul=(O.SW_C1, 0.0, 5.0)),
+             O.make_spec(3.0, lj=(O.SW_C2, 3.0, 5.0), pme=dict(rc=5.0, grid=(12, 12, 12), order=4, alpha=0.6))]
+    rng = np.random.default_rng(0)
+    for sp in specs:
+        f, _, _ = O.evaluate(s, sp)
+        for atom in rng.choice(s.n, 6, replace=False):
+            for k in range(3):
+                h = 1e-5
+                p = s.pos.copy()
+                p[atom, k] += h
+                _, ec1, el1 = O.evaluate(s, sp, pos=p)
+                p[atom, k] -= 2 * h
+                _, ec2, el2 = O.evaluate(s, sp, pos=p)
+                g = ((ec1 + el1) - (ec2 + el2)) / (2 * h)
+                assert abs(-g - f[atom, k]) < 2e-4 * max(1.0, abs(f[atom, k]))
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_cell_enumerator_pair_set_equals_brute_force(seed):
+    """Every in-cutoff non-excluded pair exactly once, for boxes with 1-4
+    cells per axis and atoms on the +L/2 face."""
+    rng = np.random.default_rng(seed)
+    s = systems.water_box(n_side=3, box=float(rng.uniform(9.0, 11.0)), seed=seed, jitter=0.5)
+    s.pos[0] = s.box / 2  # exactly on the upper face of the minimal image
+    rc = float(rng.uniform(0.3, 0.5) * s.box[0])
+    cellsize = float(rng.uniform(0.25, 1.2) * rc)
+    p = O.pairs(s, cellsize, rc)
+    b = O.pairs_bruteforce(s, rc)
+    key = lambda arr: set(map(tuple, np.sort(arr, axis=1).tolist()))
+    assert len(p) == len(b)
+    assert key(p) == key(b)

@@ -1,0 +1,104 @@
+"""The CPU oracle pinned against the reference ProtoMol path:
+
+* the reference's own golden file test/tests/expected/water_CHARMM_PERIODIC_L.energy
+  (step-0 E_VdW and E_coulomb of the 423-atom periodic water test);
+* fixtures written by tests/golden/make_fixtures.py from the reference
+  compiled from its sources (forces, energies, PME alpha, cell of every atom,
+  in-cutoff pair count and set hash);
+* live runs of that reference build on random inputs, when it is built here
+  (oracle/_ref/pm_ref; absent on machines without the reference tree).
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from golden import fixtures as FX
+from oracle import oracle as O
+from protomol3_amd import charmm, pmx, systems, sysfile
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PM_REF = os.path.join(ROOT, "oracle", "_ref", "pm_ref")
+
+
+def oracle_spec(sp):
+    return O.make_spec(sp["cellsize"], lj=sp["lj"], coul=sp["coul"], pme=FX.oracle_pme(sp["pme"]))
+
+
+def test_water423_reference_golden_energies():
+    """Step-0 energies of the reference's own periodic water regression test
+    (water_CHARMM_PERIODIC_L.conf: LJ C2 switchon 0.1 cutoff 6.5, Coulomb C1
+    6.5, cellsize 6.5), printed there to 8 significant digits."""
+    w = FX.W423
+    s = charmm.build_system(os.path.join(w, "water_423.psf"), os.path.join(w, "water_423.pos.pdb"),
+                            os.path.join(w, "par_all27_prot_lipid.inp"), box=(28.0, 28.0, 28.0))
+    with open(os.path.join(w, "expected.energy")) as f:
+        t0 = [float(v) for v in f.readline().split()]
+    e_vdw_ref, e_coul_ref = t0[9], t0[10]
+    _, ec, elj = O.evaluate(s, O.make_spec(6.5, lj=(O.SW_C2, 0.1, 6.5), coul=(O.SW_C1, 0.0, 6.5)))
+    assert abs(elj - e_vdw_ref) <= 5e-8 * abs(e_vdw_ref) + 1e-4
+    assert abs(ec - e_coul_ref) <= 5e-8 * abs(e_coul_ref) + 1e-4
+
+
+@pytest.mark.parametrize("name", FX.names())
+def test_oracle_matches_reference_fixture(name):
+    fx = FX.load(name)
+    s = FX.system(fx)
+    sp = FX.spec(fx)
+    f, ec, elj = O.evaluate(s, oracle_spec(sp), nthreads=8)
+    idx = fx["force_index"]
+    fref = fx["forces"]
+    scale = max(1.0, np.abs(fref).max())
+    assert np.abs(f[idx] - fref).max() <= 1e-10 * scale
+    assert abs(ec - float(fx["e_coulomb"])) <= 1e-10 * max(1.0, abs(float(fx["e_coulomb"])))
+    assert abs(elj - float(fx["e_lj"])) <= 1e-10 * max(1.0, abs(float(fx["e_lj"])))
+    if sp["pme"] is not None:
+        terms = O.pme_terms(s, oracle_spec(sp))
+        # the reference prints alpha with 15 significant digits
+        assert abs(terms["alpha"] - float(fx["alpha"])) <= 1e-14 * float(fx["alpha"])
+    cells, dims = O.cells(s, sp["cellsize"])
+    assert dims == tuple(int(v) for v in fx["cell_dims"])
+    assert np.array_equal(cells, fx["cells"])
+    pairs = O.pairs(s, sp["cellsize"], float(fx["rc"]))
+    assert len(pairs) == int(fx["npairs"])
+    assert pmx.pair_hash_set(pairs) == int(fx["pair_hash"])
+
+
+def _run_ref(s, cellsize, spec_tokens):
+    with tempfile.TemporaryDirectory() as td:
+        sf, of = os.path.join(td, "sys"), os.path.join(td, "out")
+        sysfile.write_sysfile(sf, s)
+        subprocess.run([PM_REF, sf, of, "cellsize", repr(cellsize), *spec_tokens], check=True,
+                       capture_output=True)
+        return sysfile.read_forces(of)
+
+
+@pytest.mark.skipif(not os.path.exists(PM_REF), reason="reference build oracle/_ref/pm_ref not present")
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_matches_live_reference_random(seed):
+    """Random small water boxes, random cell sizes (including boxes with only
+    1-3 cells per axis), random LJ/Coulomb switch choices and PME settings."""
+    rng = np.random.default_rng(100 + seed)
+    n_side = int(rng.integers(3, 6))
+    box = float(n_side * rng.uniform(3.0, 3.4))
+    s = systems.water_box(n_side=n_side, box=box, seed=seed, jitter=0.4)
+    rc = float(rng.uniform(0.35, 0.5) * box)
+    cellsize = float(rng.uniform(0.3, 1.1) * rc)
+    choice = seed % 3
+    if choice == 0:
+        toks = ["lj_c2", repr(0.6 * rc), repr(rc), "coul_c1", repr(rc)]
+        sp = O.make_spec(cellsize, lj=(O.SW_C2, 0.6 * rc, rc), coul=(O.SW_C1, 0.0, rc))
+    elif choice == 1:
+        toks = ["lj_cutoff", repr(rc), "coul_c2", repr(0.5 * rc), repr(rc)]
+        sp = O.make_spec(cellsize, lj=(O.SW_CUTOFF, 0.0, rc), coul=(O.SW_C2, 0.5 * rc, rc))
+    else:
+        g = int(rng.choice([12, 16, 18, 20]))
+        toks = ["lj_c2", repr(0.7 * rc), repr(rc), "pme", str(g), str(g), str(g), "4", repr(rc), "1e-6", "-1", "111"]
+        sp = O.make_spec(cellsize, lj=(O.SW_C2, 0.7 * rc, rc), pme=dict(rc=rc, grid=(g, g, g), order=4, alpha=-1))
+    ec_r, elj_r, f_r = _run_ref(s, cellsize, toks)
+    f, ec, elj = O.evaluate(s, sp)
+    assert abs(ec - ec_r) <= 1e-11 * max(1.0, abs(ec_r))
+    assert abs(elj - elj_r) <= 1e-11 * max(1.0, abs(elj_r))
+    assert np.abs(f - f_r).max() <= 1e-11 * max(1.0, np.abs(f_r).max())
