@@ -4,18 +4,20 @@
 Workload: the 26,000-atom solvated bead polymer in a 64 A periodic box
 (500-bead chain + 8,500 flexible 3-site waters), LJ C2 switched 10-12 A,
 PME real space Cutoff 12 A, PME 64^3 grid order 4, alpha from accuracy 1e-6
-(= 0.257951837636369), 1 fs per step.
+(= 0.257951837636369), leapfrog NVE at 1 fs, Maxwell-Boltzmann 300 K start.
 
-A step is one full nonbonded force pass over the resident system: cell sort,
-cluster-pair list build, LJ + PME real-space pair kernel, B-spline spread,
-3-D FFT convolution, force gather, FP64 exclusion correction, and the
-accumulator-to-FP64 force conversion.  Positions stay in HBM; nothing is
-cached between steps (the sort, list and grids are rebuilt every step, as the
-reference rebuilds its cell lists every evaluation).
+A step is one leapfrog MD step run entirely on the device: kick + drift,
+the full nonbonded force pass (cluster-pair LJ + PME real space with exact
+cutoff membership, B-spline spread, 3-D FFT convolution, force gather, FP64
+exclusion correction), harmonic bonds and angles.  The cell sort and
+cluster-pair list are rebuilt whenever an atom has moved more than half the
+pair-list skin (chunks that cross it are rolled back and replayed), so no
+step uses a stale list.  Positions and velocities stay in HBM.
 
-Prints one JSON line.  value = ns/day of the mixed form (FP32 pair/spline
-math, int64 fixed-point accumulation); the FP64 form is measured in the same
-run and reported beside it.
+value = ns/day of the mixed form (FP32 pair/spline math, int64 fixed-point
+accumulation, FP64 corrections/bonded/integrator); the FP64 form is
+measured in the same run and reported beside it, with the RMS relative force
+error of the mixed form against it and both forms' NVE energy drift.
 """
 from __future__ import annotations
 
@@ -33,69 +35,94 @@ sys.path.insert(0, ROOT)
 FLOP_PER_PAIR = 69          # algorithmic FLOPs per in-cutoff pair (LJ C2 + PME real, counted once)
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
-HBM_PEAK_GBS = 8000.0
+TIMESTEP_FS = 1.0
 
 
-def workload():
-    from protomol3_amd import systems
+def workload(ctx):
+    """The generated structure, relaxed by 300 capped steepest-descent steps
+    on the full force field (removes lattice overlaps), then 300 K
+    Maxwell-Boltzmann velocities."""
+    from protomol3_amd import pmx, prep, systems
     s = systems.polymer_water(n_total=26000, box=64.0, n_beads=500, seed=1)
-    spec = dict(cellsize=6.0, lj=(3, 10.0, 12.0), pme=dict(rc=12.0, grid=(64, 64, 64), order=4, alpha=-1.0,
-                                                            accuracy=1e-6))
+    spec = pmx.ForceSpec(6.0, lj=(pmx.SW_C2, 10.0, 12.0),
+                         pme=dict(rc=12.0, grid=(64, 64, 64), order=4, alpha=-1.0, accuracy=1e-6))
+    s.pos = prep.relax(ctx, s, spec)
+    s.with_velocities(300.0, seed=2)
     return s, spec
 
 
-def run_form(ctx, s, spec, precision, steps, warmup):
+def ns_day(ms_per_step: float) -> float:
+    return 86400.0 / (ms_per_step * 1e-3) * TIMESTEP_FS * 1e-6
+
+
+def run_form(ctx, s, spec, precision, steps, warmup, skin, drift_steps):
     from protomol3_amd import pmx
-    gs = pmx.ForceSpec(spec["cellsize"], lj=spec["lj"], pme=spec["pme"])
-    sysh = pmx.System(ctx, s, gs, precision)
-    sysh.set_positions(s.pos)
-    # pair count for the algorithmic FLOP figure (outside the timed region)
-    f, ec, elj = sysh.evaluate(s.pos, flags=pmx.EVAL_PAIRSTATS)
-    npairs, _ = sysh.pair_stats()
-    sysh.set_positions(s.pos)
-    for _ in range(warmup):
-        sysh.compute()
-    sysh.sync()
-    # timed region: K back-to-back steps, events per stage on the system's stream
-    sysh.timing_begin()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        sysh.compute(pmx.EVAL_TIMING)
-    sysh.sync()
-    t1 = time.perf_counter()
-    stages, ncalls = sysh.timing_end()
-    f2, ec2, elj2 = sysh.get_forces()
-    sysh.close()
+    sy = pmx.System(ctx, s, spec, precision)
+    try:
+        sy.md_init(s, TIMESTEP_FS, skin=skin)
+        sy.md_set_state(s.pos, s.vel)
+        sy.md_run(warmup)
+        st0 = sy.md_stats()
+        t0 = time.perf_counter()
+        sy.md_run(steps)
+        t1 = time.perf_counter()
+        st1 = sy.md_stats()
+        # same number of steps again with per-stage HIP events recorded
+        sy.timing_begin()
+        sy.md_run(steps)
+        stages, ncalls = sy.timing_end()
+        x, v, e = sy.md_get_state()
+        # in-cutoff pair count at this configuration (for the FLOP figure)
+        sy.evaluate(x, flags=pmx.EVAL_PAIRSTATS)
+        npairs, _ = sy.pair_stats()
+        # NVE drift over drift_steps from here
+        sy.md_set_state(x, v)
+        sy.md_run(1)
+        _, _, e0 = sy.md_get_state()
+        etot = [sum(e0.values())]
+        chunk = max(1, drift_steps // 10)
+        for _ in range(10):
+            sy.md_run(chunk)
+            _, _, ek = sy.md_get_state()
+            etot.append(sum(ek.values()))
+        tps = np.arange(len(etot)) * chunk * TIMESTEP_FS * 1e-6  # ns
+        slope = float(np.polyfit(tps, etot, 1)[0])
+    finally:
+        sy.close()
     return dict(ms_per_step=(t1 - t0) * 1e3 / steps, stages=stages, ncalls=ncalls, npairs=npairs,
-                forces=f2, ecoul=ec2, elj=elj2)
+                rebuilds=st1["rebuilds"] - st0["rebuilds"], rollbacks=st1["rollbacks"] - st0["rollbacks"],
+                x=x, drift_kcal_mol_ns=slope, etot_first=etot[0], etot_last=etot[-1])
 
 
-def cpu_baseline(s, spec, budget_s: float = 20.0):
-    """The oracle (FP64 C++ restatement of the reference path) on the host's
-    cores: whole force evaluations of the same workload until ~budget_s."""
+def cpu_baseline(s, budget_s: float = 20.0):
+    """The oracle (FP64 C++ restatement of the reference path + leapfrog) on
+    the host's cores: whole MD steps of the same workload until ~budget_s."""
+    from oracle import md
     from oracle import oracle as O
     cores = min(os.cpu_count() or 1, 16)
-    p = spec["pme"]
-    ospec = O.make_spec(spec["cellsize"], lj=spec["lj"], pme=dict(p))
-    O.evaluate(s, ospec, nthreads=cores)  # warm
+    ospec = O.make_spec(6.0, lj=(O.SW_C2, 10.0, 12.0),
+                        pme=dict(rc=12.0, grid=(64, 64, 64), order=4, alpha=-1.0, accuracy=1e-6))
+    x, v, f, _ = md.leapfrog_run(s, ospec, s.pos, s.vel, TIMESTEP_FS, 1, nthreads=cores)  # relaxed start
     n = 0
     t0 = time.perf_counter()
     while True:
-        O.evaluate(s, ospec, nthreads=cores)
+        x, v, f, _ = md.leapfrog_run(s, ospec, x, v, TIMESTEP_FS, 1, f=f, nthreads=cores)
         n += 1
         if time.perf_counter() - t0 > budget_s or n >= 50:
             break
     dt = (time.perf_counter() - t0) / n
-    return dict(value=86400.0 / dt * 1e-6, unit="ns/day", cores=cores, kind="port",
-                sample=f"{n} full force evaluations (LJ C2 + PME real/recip/correction) of the 26,000-atom "
-                       f"workload on {cores} threads, {dt:.3f} s each")
+    return dict(value=round(ns_day(dt * 1e3), 5), unit="ns/day", cores=cores, kind="port",
+                sample=f"{n} leapfrog MD steps of the 26,000-atom workload (oracle FP64 nonbonded on {cores} "
+                       f"threads + bonded + integrator), {dt:.3f} s per step")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--skin", type=float, default=-1.0, help="pair-list skin in A (<0: autotune)")
+    ap.add_argument("--drift-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     a = ap.parse_args()
@@ -103,22 +130,40 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
               "the multi-GPU sharding design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
-    from protomol3_amd import pmx
-    s, spec = workload()
+    from protomol3_amd import pmx, tune
     ctx = pmx.Context(0)
-    print(f"bench: {s.n} atoms, steps={a.steps} warmup={a.warmup}", file=sys.stderr, flush=True)
-    mixed = run_form(ctx, s, spec, pmx.MIXED, a.steps, a.warmup)
-    print(f"bench: mixed {mixed['ms_per_step']:.4f} ms/step stages {mixed['stages']}", file=sys.stderr, flush=True)
-    fp64 = run_form(ctx, s, spec, pmx.FP64, max(1, a.steps // 2), max(1, a.warmup // 2))
-    print(f"bench: fp64 {fp64['ms_per_step']:.4f} ms/step stages {fp64['stages']}", file=sys.stderr, flush=True)
+    s, spec = workload(ctx)
+    log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
+    log(f"{s.n} atoms, steps={a.steps} warmup={a.warmup}")
+    skins = {}
+    if a.skin < 0:
+        skin_m, times_m = tune.tune_skin(ctx, s, spec, pmx.MIXED, s.pos, s.vel)
+        skin_d, times_d = tune.tune_skin(ctx, s, spec, pmx.FP64, s.pos, s.vel)
+        skins = {"mixed": times_m, "fp64": times_d}
+        log(f"skin tuning (ms/step) mixed {times_m} fp64 {times_d}")
+    else:
+        skin_m = skin_d = a.skin
+    mixed = run_form(ctx, s, spec, pmx.MIXED, a.steps, a.warmup, skin_m, a.drift_steps)
+    log(f"mixed {mixed['ms_per_step']:.4f} ms/step rebuilds {mixed['rebuilds']} rollbacks {mixed['rollbacks']} "
+        f"stages {mixed['stages']}")
+    fp64 = run_form(ctx, s, spec, pmx.FP64, a.steps, a.warmup, skin_d, a.drift_steps)
+    log(f"fp64 {fp64['ms_per_step']:.4f} ms/step rebuilds {fp64['rebuilds']} stages {fp64['stages']}")
+    # force accuracy of the mixed form: nonbonded forces of both forms at the
+    # same configuration (the end of the mixed run)
+    forms = {}
+    for prec in (pmx.FP64, pmx.MIXED):
+        sy = pmx.System(ctx, s, spec, prec)
+        forms[prec] = sy.evaluate(mixed["x"])
+        sy.close()
     ctx.close()
-
-    ref = fp64["forces"]
-    err = np.sqrt(np.mean(np.sum((mixed["forces"] - ref) ** 2, 1))) / np.sqrt(np.mean(np.sum(ref ** 2, 1)))
-    ns_day = lambda ms: 86400.0 / (ms * 1e-3) * 1e-6  # 1 fs per step
+    f64, fm = forms[pmx.FP64][0], forms[pmx.MIXED][0]
+    e64, em = forms[pmx.FP64], forms[pmx.MIXED]
+    rel_e = {"coulomb": float(f"{abs(em[1] - e64[1]) / abs(e64[1]):.3e}"),
+             "lj": float(f"{abs(em[2] - e64[2]) / abs(e64[2]):.3e}")}
+    err = float(np.sqrt(np.mean(np.sum((fm - f64) ** 2, 1))) / np.sqrt(np.mean(np.sum(f64 ** 2, 1))))
     pair_ms = mixed["stages"]["pair"]
     achieved = FLOP_PER_PAIR * mixed["npairs"] / (pair_ms * 1e-3) / 1e12
-    cpu = None if a.no_cpu else cpu_baseline(s, spec, a.cpu_budget)
+    cpu = None if a.no_cpu else cpu_baseline(s, a.cpu_budget)
     out = {
         "metric": "ns/day (26k-atom PME) mixed vs FP64 GPU at <=1e-5 force err; p50/p99 latency",
         "value": round(ns_day(mixed["ms_per_step"]), 3),
@@ -130,22 +175,32 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+i64 (mixed); f64 reference form measured alongside",
-        "data": "synthetic (seeded 26,000-atom polymer + water, random orientations)",
+        "dtype": "f32 pair/spline math + i64 fixed-point accumulation (mixed); f64 form measured alongside",
+        "data": "synthetic (seeded 26,000-atom bead polymer + water, random orientations, relaxed by 300 "
+                "capped steepest-descent steps, 300 K velocities)",
         "config": {"workload": "polymer_water26000: 64 A box, LJ C2 10-12 A, PME 64^3 order 4 rc 12 A "
-                               "alpha 0.257951837636369, 1 fs/step; step = full nonbonded force pass",
+                               "alpha 0.257951837636369, leapfrog NVE 1 fs; step = full device MD step",
                    "global_batch": 1, "seq_len": s.n, "parallelism": "single GPU"},
         "fp64_ns_per_day": round(ns_day(fp64["ms_per_step"]), 3),
         "fp64_ms_per_step": round(fp64["ms_per_step"], 5),
         "mixed_over_fp64": round(fp64["ms_per_step"] / mixed["ms_per_step"], 3),
         "rms_rel_force_err_mixed_vs_fp64": float(f"{err:.3e}"),
+        "rel_energy_err_mixed_vs_fp64": rel_e,
+        "nve_drift_kcal_mol_per_ns": {"mixed": round(mixed["drift_kcal_mol_ns"], 3),
+                                      "fp64": round(fp64["drift_kcal_mol_ns"], 3),
+                                      "steps": a.drift_steps},
+        "pair_list": {"skin_A": {"mixed": skin_m, "fp64": skin_d}, "tuning_ms_per_step": skins,
+                      "rebuilds_in_timed_steps": {"mixed": mixed["rebuilds"], "fp64": fp64["rebuilds"]},
+                      "rollbacks_in_timed_steps": {"mixed": mixed["rollbacks"], "fp64": fp64["rollbacks"]}},
         "pairs_per_step": int(mixed["npairs"]),
         "pair_interactions_per_s": float(f"{mixed['npairs'] / (mixed['ms_per_step'] * 1e-3):.4e}"),
         "stage_ms_mixed": {k: round(v, 5) for k, v in mixed["stages"].items()},
         "stage_ms_fp64": {k: round(v, 5) for k, v in fp64["stages"].items()},
-        "roofline": {"bound": "valu_fp32", "kernel": "k_pair<mixed,PME>", "achieved": round(achieved, 3),
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                     "traffic": None, "flop_per_pair": FLOP_PER_PAIR},
+        "roofline": {"bound": "valu", "kernel": "k_pair<mixed,PME,C2,Cutoff> + k_border",
+                     "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": int(mixed["npairs"]),
+                     "avg_launch_ms": round(pair_ms, 5)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -135,6 +135,48 @@ int pmx_sys_stage_times(const pmx_sys *sys, float *ms);
 int pmx_sys_timing_begin(pmx_sys *sys);
 int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls);
 
+/* ---------------------------------------------------------------------
+ * Device-resident MD (the caller of the hot path: StandardIntegrator /
+ * LeapfrogIntegrator, protomol/src/protomol/integrator/leapfrog/
+ * LeapfrogIntegrator.cpp:89-104).  Positions, velocities and forces stay in
+ * HBM for whole runs; the bonded terms of the systems in this engine
+ * (harmonic bonds and angles, force/bonded/BondSystemForce.h:97-118,
+ * AngleSystemForce.h:95-147) are evaluated on the device too.
+ *
+ * The cluster-pair list is reused between steps while no atom has moved
+ * more than skin/2 since it was built; membership is still decided exactly
+ * against the cutoff every step.  Steps run in chunks; if an atom crossed
+ * the skin/2 bound inside a chunk, the chunk is rolled back and replayed
+ * with an earlier rebuild, so every committed step used a valid list.
+ */
+typedef struct {
+  int nbonds, nangles;
+  const int *bonds;          /* [2*nbonds] atom1 atom2 */
+  const double *bond_par;    /* [2*nbonds] k, r0; E = k (r - r0)^2 */
+  const int *angles;         /* [3*nangles] atom1 atom2(vertex) atom3 */
+  const double *angle_par;   /* [2*nangles] k, theta0 (rad); E = k (theta - theta0)^2 */
+  const double *mass;        /* [n] amu */
+  double timestep_fs;
+  double skin;               /* pair-list buffer in A (0: rebuild every step) */
+} pmx_md_desc;
+
+/* Energy slots of pmx_md_get_state. */
+enum { PMX_MD_E_COULOMB = 0, PMX_MD_E_LJ = 1, PMX_MD_E_BOND = 2, PMX_MD_E_ANGLE = 3, PMX_MD_E_KINETIC = 4,
+       PMX_MD_E_COUNT = 5 };
+
+int pmx_md_init(pmx_sys *sys, const pmx_md_desc *desc);
+/* x, v: [3n] AoS, v in ProtoMol internal units (A per internal time unit). */
+int pmx_md_set_state(pmx_sys *sys, const double *x, const double *v);
+/* LeapfrogIntegrator::run(nsteps): half kick + drift, forces, (kick + drift,
+ * forces) x (nsteps-1), half kick.  Returns after queueing and checking the
+ * chunks (synchronous). */
+int pmx_md_run(pmx_sys *sys, int nsteps);
+/* State and energies after the last run (energies[PMX_MD_E_COUNT]; any
+ * pointer may be NULL). */
+int pmx_md_get_state(pmx_sys *sys, double *x, double *v, double *energies);
+/* Counters: list rebuilds and rolled-back chunks since pmx_md_init. */
+int pmx_md_stats(const pmx_sys *sys, long long *steps, long long *rebuilds, long long *rollbacks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,5 +31,24 @@ def build_lib(verbose: bool = False) -> str:
     return LIB
 
 
+HOST_EXE = os.path.join(HERE, "lib", "pmx_force")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ ProtoMol-boundary layer and its driver, linked to libpmx.so."""
+    host = os.path.join(HERE, "host")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(host, "pmx_protomol.cpp"), os.path.join(host, "pmx_force.cpp"),
+           "-L" + os.path.dirname(LIB), "-lpmx", "-Wl,-rpath,$ORIGIN", "-o", HOST_EXE + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + r.stderr[-4000:])
+    os.replace(HOST_EXE + ".tmp", HOST_EXE)
+    if verbose:
+        print("built", HOST_EXE)
+    return HOST_EXE
+
+
 if __name__ == "__main__":
     build_lib(verbose=True)
+    build_host(verbose=True)

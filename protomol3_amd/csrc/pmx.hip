@@ -71,10 +71,23 @@ struct Sys {
   hipEvent_t *ev = nullptr;                    // event set of the current timed call
   std::vector<std::vector<hipEvent_t>> evsets;  // one set of 9 per timed call since timing_begin
   size_t evused = 0;
+  bool timing_active = false;                   // MD steps record events (timing_begin .. timing_end)
   bool have_positions = false;
   bool cached = false;
   double eout_h[4] = {0, 0, 0, 0};
+  // device-resident MD state
+  struct Md {
+    bool on = false;
+    int nb = 0, na = 0;
+    double h = 0, skin = 0;
+    DevBuf bonds, bpar, angles, apar, minv, v, xbuild, xbak, vbak, fbak, dmax, bond_e, ke_e;
+    int chunk = 16;
+    long long steps = 0, rebuilds = 0, rollbacks = 0;
+    bool forces_valid = false;
+  } md;
 };
+constexpr int kMaxChunk = 128;
+constexpr double kInvTimeFactor = 0.02045482828087295;  // protomol/base/PMConstants.cpp:54
 
 int fail(pmx_ctx *c, int code, const std::string &m) {
   if (c) c->err = m;
@@ -344,7 +357,7 @@ int setup(Sys *s) {
   ListParams &lp = s->lp;
   const double margin = 4.0 * Lmax * std::ldexp(1.0, -23) + 1e-4;
   for (int k = 0; k < 3; ++k) lp.L[k] = (float)bx.L[k];
-  lp.rlist2 = (float)((rcmax + margin) * (rcmax + margin));
+  lp.rlist2 = (float)((rcmax + margin + s->md.skin) * (rcmax + margin + s->md.skin));
   lp.ncl = s->ncl;
   lp.nsuper = s->nsuper;
   lp.n = d.n;
@@ -378,7 +391,7 @@ int setup(Sys *s) {
       (rc = A(s->npairs, 8 * (size_t)s->ntiles_cap)) || (rc = A(s->hash, 8 * (size_t)s->ntiles_cap)) ||
       (rc = A(s->eptr_d, 4 * (N + 1))) || (rc = A(s->elist_d, 8 * s->elist.size())) ||
       (rc = A(s->etab_d, 16 * s->etab.size())) || (rc = A(s->eout, 64)) || (rc = A(s->np_out, 8)) ||
-      (rc = A(s->hash_out, 8)) || (rc = A(s->exclmax, 8)) || (rc = A(s->border, 8 * (size_t)s->maxborder)) ||
+      (rc = A(s->hash_out, 8)) || (rc = A(s->exclmax, 16)) || (rc = A(s->border, 8 * (size_t)s->maxborder)) ||
       (rc = A(s->nborder, 8)) || (rc = A(s->border_acc, 32)))
     return rc;
   s->corr_blocks = std::max(1, (int)nblk((long long)s->etab.size(), 256));
@@ -554,14 +567,9 @@ int run_pme(Sys *s, bool timing) {
   using T = typename std::conditional<MIXED, float, double>::type;
   hipStream_t st = s->ctx->stream;
   const PmeParams &pm = s->pme;
-  const size_t G = (size_t)pm.nx * pm.ny * pm.nz;
   auto *facc = P<typename AccT<MIXED>::type>(s->facc);
   if (s->d.pme_terms & PMX_PME_RECIP) {
-    if (MIXED)
-      hipLaunchKernelGGL(k_fill_u64, dim3(nblk(G, 256)), dim3(256), 0, st, P<unsigned long long>(s->gridfx), 0ull,
-                         (long long)G);
-    else
-      hipLaunchKernelGGL(k_fill_f64, dim3(nblk(G, 256)), dim3(256), 0, st, P<double>(s->grid64), 0.0, (long long)G);
+    // the charge grid was zeroed by the k_zero launch at the start of the pass
     hipLaunchKernelGGL((k_spread<MIXED>), dim3(nblk(s->n, 128)), dim3(128), 0, st, s->n, pm, P<double>(s->xorig),
                        P<int>(s->permp), P<double>(s->q_d), P<double>(s->grid64),
                        P<unsigned long long>(s->gridfx));
@@ -594,21 +602,34 @@ int run_pme(Sys *s, bool timing) {
   return PMX_OK;
 }
 
-int compute(Sys *s, unsigned flags) {
-  hipStream_t st = s->ctx->stream;
-  const bool timing = flags & PMX_EVAL_TIMING;
-  const bool stats = flags & PMX_EVAL_PAIRSTATS;
-  const int n = s->n, npad = s->npad;
-  if (timing) {
-    if (s->evused == s->evsets.size()) {
-      std::vector<hipEvent_t> set(9);
-      for (auto &e : set) HIPCHK(hipEventCreate(&e));
-      s->evsets.push_back(set);
-    }
-    s->ev = s->evsets[s->evused++].data();
-    hipEventRecord(s->ev[0], st);
+void reduce_energies(Sys *s, bool stats) {
+  const bool recip = s->pme_on && (s->d.pme_terms & PMX_PME_RECIP);
+  const bool corr = s->pme_on && (s->d.pme_terms & PMX_PME_CORR) && !s->etab.empty();
+  hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(1024), 0, s->ctx->stream, s->ntiles_cap, P<int>(s->ntiles),
+                     P<double>(s->pair_e), recip ? s->recip_blocks : 0, P<double>(s->recip_e),
+                     corr ? s->corr_blocks : 0, P<double>(s->corr_e), P<long long>(s->npairs),
+                     P<unsigned long long>(s->hash), stats ? 1 : 0, P<unsigned long long>(s->border_acc),
+                     P<unsigned long long>(s->border_acc) + 2, P<double>(s->eout), P<long long>(s->np_out),
+                     P<unsigned long long>(s->hash_out));
+}
+
+void timing_begin_call(Sys *s, bool timing) {
+  if (!timing) return;
+  if (s->evused == s->evsets.size()) {
+    std::vector<hipEvent_t> set(9);
+    for (auto &e : set) hipEventCreate(&e);
+    s->evsets.push_back(set);
   }
-  // --- sort
+  s->ev = s->evsets[s->evused++].data();
+  hipEventRecord(s->ev[0], s->ctx->stream);
+}
+
+// Cell sort of the current positions and the cluster-pair list built from it
+// (the reference rebuilds its cell lists at every force evaluation,
+// ForceGroup.cpp:57; the MD path rebuilds when the skin requires it).
+int rebuild(Sys *s, bool timing) {
+  hipStream_t st = s->ctx->stream;
+  const int n = s->n, npad = s->npad;
   hipLaunchKernelGGL(k_bin, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<double>(s->xorig), s->bin, P<uint32_t>(s->key),
                      P<double>(s->xwrap), P<int>(s->refcell));
   hipLaunchKernelGGL(k_fill_u32, dim3(nblk(s->nbins, 256)), dim3(256), 0, st, P<uint32_t>(s->hist), 0u, s->nbins);
@@ -632,32 +653,53 @@ int compute(Sys *s, unsigned flags) {
                        P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<double4>(s->xs),
                        P<double4>(s->xs64), P<int>(s->ts), P<int>(s->slot), P<int>(s->permp));
   if (timing) hipEventRecord(s->ev[1], st);
-  // --- cluster list
   hipLaunchKernelGGL(k_cluster_bbox, dim3(nblk(s->ncl, 256)), dim3(256), 0, st, s->ncl, n, P<double4>(s->xs64),
                      P<float4>(s->ctr), P<float4>(s->half));
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, P<uint32_t>(s->exclmax), 0u, 2ll);
-  if (!s->etab.empty())
-    hipLaunchKernelGGL(k_excl_maxdist, dim3(nblk((long long)s->etab.size(), 256)), dim3(256), 0, st,
-                       (int)s->etab.size(), P<int4>(s->etab_d), P<double>(s->xwrap), s->box, P<unsigned>(s->exclmax));
-  hipLaunchKernelGGL(k_set_rexcl, dim3(1), dim3(1), 0, st, P<unsigned>(s->exclmax), P<float>(s->exclmax) + 1);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->nmaskwork), 0, 2ll);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
+  // exclmax words: [1] unused list radius slot, [2] mask work counter
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, P<uint32_t>(s->exclmax), 0u, 4ll);
   hipLaunchKernelGGL(k_build_list, dim3(nblk((long long)s->nsuper * 64, 256)), dim3(256), 0, st, s->lp,
                      P<float>(s->exclmax) + 1, P<float4>(s->ctr), P<float4>(s->half), P<int4>(s->entries),
-                     P<int>(s->count), P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork,
+                     P<int>(s->count), P<int>(s->maskwork), P<int>(s->exclmax) + 2, s->maxmaskwork,
                      P<int>(s->overflow));
-  hipLaunchKernelGGL(k_build_masks, dim3(s->ctx->cus * 4), dim3(256), 0, st,
-                     P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork, P<int4>(s->entries), s->cap, n,
-                     P<int>(s->permp), P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned long long>(s->masks));
+  if (!s->etab.empty())
+    hipLaunchKernelGGL(k_mark_exclusions, dim3(nblk((long long)s->etab.size(), 256)), dim3(256), 0, st,
+                       (int)s->etab.size(), P<int4>(s->etab_d), P<int>(s->slot), P<int4>(s->entries),
+                       P<int>(s->count), s->cap, P<int>(s->maskwork), P<int>(s->exclmax) + 2, s->maxmaskwork,
+                       P<int>(s->overflow));
+  hipLaunchKernelGGL(k_build_masks, dim3(s->ctx->cus * 4), dim3(256), 0, st, P<int>(s->maskwork),
+                     P<int>(s->exclmax) + 2, s->maxmaskwork, P<int4>(s->entries), s->cap, n, P<int>(s->permp),
+                     P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned long long>(s->masks));
   hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int2>(s->tiles),
                      P<int>(s->ntiles));
+  if (s->md.on) HIPCHK(hipMemcpyAsync(s->md.xbuild.p, s->xorig.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
   if (timing) hipEventRecord(s->ev[2], st);
-  // --- forces
-  hipLaunchKernelGGL(k_fill_u64, dim3(nblk(3ll * npad, 256)), dim3(256), 0, st, P<unsigned long long>(s->facc), 0ull,
-                     3ll * npad);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->tilectr), 0, 2ll);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->nborder), 0, 2ll);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, P<unsigned long long>(s->border_acc), 0ull, 4ll);
+  return PMX_OK;
+}
+
+// Forces (and optionally energies) at the current positions with the
+// current list: pair + boundary pairs, PME, correction, bonded terms, then
+// the FP64 force array in the caller's atom order.
+int forces(Sys *s, unsigned flags, bool energies) {
+  hipStream_t st = s->ctx->stream;
+  const bool timing = flags & PMX_EVAL_TIMING;
+  const bool stats = flags & PMX_EVAL_PAIRSTATS;
+  const int n = s->n, npad = s->npad;
+  ZeroRanges z{};
+  int nr = 0;
+  z.p[nr] = P<unsigned long long>(s->facc);
+  z.n[nr++] = 3ll * npad;
+  z.p[nr] = P<unsigned long long>(s->tilectr);
+  z.n[nr++] = 1;
+  z.p[nr] = P<unsigned long long>(s->nborder);
+  z.n[nr++] = 1;
+  z.p[nr] = P<unsigned long long>(s->border_acc);
+  z.n[nr++] = 4;
+  if (s->pme_on && (s->d.pme_terms & PMX_PME_RECIP)) {
+    const long long G = (long long)s->pme.nx * s->pme.ny * s->pme.nz;
+    z.p[nr] = s->mixed ? P<unsigned long long>(s->gridfx) : P<unsigned long long>(s->grid64);
+    z.n[nr++] = G;
+  }
+  hipLaunchKernelGGL(k_zero, dim3(std::max(1u, nblk(3ll * npad, 256)), nr), dim3(256), 0, st, z);
   int rc = s->mixed ? run_pair<true>(s, stats) : run_pair<false>(s, stats);
   if (rc) return rc;
   if (timing) hipEventRecord(s->ev[3], st);
@@ -669,25 +711,38 @@ int compute(Sys *s, unsigned flags) {
     hipEventRecord(s->ev[5], st);
     hipEventRecord(s->ev[6], st);
   }
+  if (s->md.on && s->md.nb + s->md.na > 0) {
+    const int nt = s->md.nb + s->md.na;
+    if (s->mixed)
+      hipLaunchKernelGGL(k_bonded<true>, dim3(nblk(nt, 256)), dim3(256), 0, st, s->md.nb, s->md.na,
+                         P<int2>(s->md.bonds), P<double2>(s->md.bpar), P<int4>(s->md.angles), P<double2>(s->md.apar),
+                         P<double>(s->xorig), P<int>(s->slot), s->box, P<unsigned long long>(s->facc),
+                         P<double>(s->md.bond_e));
+    else
+      hipLaunchKernelGGL(k_bonded<false>, dim3(nblk(nt, 256)), dim3(256), 0, st, s->md.nb, s->md.na,
+                         P<int2>(s->md.bonds), P<double2>(s->md.bpar), P<int4>(s->md.angles), P<double2>(s->md.apar),
+                         P<double>(s->xorig), P<int>(s->slot), s->box, P<double>(s->facc), P<double>(s->md.bond_e));
+  }
   if (timing) hipEventRecord(s->ev[7], st);
-  // --- finalize
   if (s->mixed)
     hipLaunchKernelGGL(k_unsort_forces<true>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
                        P<unsigned long long>(s->facc), P<double>(s->fout));
   else
     hipLaunchKernelGGL(k_unsort_forces<false>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
                        P<double>(s->facc), P<double>(s->fout));
-  const bool recip = s->pme_on && (s->d.pme_terms & PMX_PME_RECIP);
-  const bool corr = s->pme_on && (s->d.pme_terms & PMX_PME_CORR) && !s->etab.empty();
-  hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(1024), 0, st, s->ntiles_cap, P<int>(s->ntiles),
-                     P<double>(s->pair_e), recip ? s->recip_blocks : 0, P<double>(s->recip_e),
-                     corr ? s->corr_blocks : 0, P<double>(s->corr_e), P<long long>(s->npairs),
-                     P<unsigned long long>(s->hash), stats ? 1 : 0, P<unsigned long long>(s->border_acc),
-                     P<unsigned long long>(s->border_acc) + 2, P<double>(s->eout), P<long long>(s->np_out),
-                     P<unsigned long long>(s->hash_out));
+  if (energies) reduce_energies(s, stats);
   if (timing) hipEventRecord(s->ev[8], st);
   HIPCHK(hipGetLastError());
   return PMX_OK;
+}
+
+int compute(Sys *s, unsigned flags) {
+  const bool timing = flags & PMX_EVAL_TIMING;
+  timing_begin_call(s, timing);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s->ctx->stream, P<int>(s->overflow), 0, 2ll);
+  int rc = rebuild(s, timing);
+  if (rc) return rc;
+  return forces(s, flags, true);
 }
 
 int finish(Sys *s, unsigned flags, double *f_aos, double *energies) {
@@ -900,6 +955,7 @@ int pmx_sys_cells(pmx_sys *sys, int *cell_of_atom, int *dims) {
 int pmx_sys_timing_begin(pmx_sys *sys) {
   if (!sys) return PMX_ERR_ARG;
   sys->s.evused = 0;
+  sys->s.timing_active = true;
   return PMX_OK;
 }
 
@@ -918,12 +974,261 @@ int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls) {
   for (int k = 0; k < 8; ++k) avg_ms[k] = m ? avg_ms[k] / (float)m : 0.f;
   *ncalls = (int)m;
   s->evused = 0;
+  s->timing_active = false;
   return PMX_OK;
 }
 
 int pmx_sys_stage_times(const pmx_sys *sys, float *ms) {
   if (!sys || !ms) return PMX_ERR_ARG;
   for (int k = 0; k < 8; ++k) ms[k] = sys->s.stage_ms[k];
+  return PMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident MD
+
+int pmx_md_init(pmx_sys *sys, const pmx_md_desc *d) {
+  if (!sys || !d) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  hipSetDevice(s->ctx->device);
+  if (d->nbonds < 0 || d->nangles < 0 || !d->mass || !(d->timestep_fs > 0) || d->skin < 0 || d->skin > 4.0 ||
+      (d->nbonds > 0 && (!d->bonds || !d->bond_par)) || (d->nangles > 0 && (!d->angles || !d->angle_par)))
+    return fail(s->ctx, PMX_ERR_ARG, "bad MD descriptor");
+  for (int i = 0; i < s->n; ++i)
+    if (!(d->mass[i] > 0)) return fail(s->ctx, PMX_ERR_ARG, "masses must be > 0");
+  for (int b = 0; b < 2 * d->nbonds; ++b)
+    if (d->bonds[b] < 0 || d->bonds[b] >= s->n) return fail(s->ctx, PMX_ERR_ARG, "bond atom out of range");
+  for (int a = 0; a < 3 * d->nangles; ++a)
+    if (d->angles[a] < 0 || d->angles[a] >= s->n) return fail(s->ctx, PMX_ERR_ARG, "angle atom out of range");
+  double rcmax = std::max(s->pp.has_lj ? s->d.lj_rc : 0.0, s->d.coul_kind ? s->d.coul_rc : 0.0);
+  for (int k = 0; k < 3; ++k)
+    if (rcmax + d->skin > 0.5 * s->box.L[k])
+      return fail(s->ctx, PMX_ERR_UNSUPPORTED, "cutoff + skin larger than half the box");
+  Sys::Md &m = s->md;
+  m.on = true;
+  m.nb = d->nbonds;
+  m.na = d->nangles;
+  m.h = d->timestep_fs * kInvTimeFactor;
+  m.skin = d->skin;
+  m.chunk = 16;
+  m.steps = m.rebuilds = m.rollbacks = 0;
+  m.forces_valid = false;
+  const size_t N = s->n;
+  std::vector<int2> b(m.nb);
+  std::vector<double2> bp(m.nb);
+  for (int i = 0; i < m.nb; ++i) {
+    b[i] = make_int2(d->bonds[2 * i], d->bonds[2 * i + 1]);
+    bp[i] = make_double2(d->bond_par[2 * i], d->bond_par[2 * i + 1]);
+  }
+  std::vector<int4> a(m.na);
+  std::vector<double2> ap(m.na);
+  for (int i = 0; i < m.na; ++i) {
+    a[i] = make_int4(d->angles[3 * i], d->angles[3 * i + 1], d->angles[3 * i + 2], 0);
+    ap[i] = make_double2(d->angle_par[2 * i], d->angle_par[2 * i + 1]);
+  }
+  std::vector<double> minv(N);
+  for (size_t i = 0; i < N; ++i) minv[i] = 1.0 / d->mass[i];
+  const int nbonded_blocks = std::max(1, (int)nblk(m.nb + m.na, 256));
+  int rc;
+  if ((rc = dalloc(s, m.bonds, 8 * b.size())) || (rc = dalloc(s, m.bpar, 16 * bp.size())) ||
+      (rc = dalloc(s, m.angles, 16 * a.size())) || (rc = dalloc(s, m.apar, 16 * ap.size())) ||
+      (rc = dalloc(s, m.minv, 8 * N)) || (rc = dalloc(s, m.v, 24 * N)) || (rc = dalloc(s, m.xbuild, 24 * N)) ||
+      (rc = dalloc(s, m.xbak, 24 * N)) || (rc = dalloc(s, m.vbak, 24 * N)) || (rc = dalloc(s, m.fbak, 24 * N)) ||
+      (rc = dalloc(s, m.dmax, 4 * kMaxChunk)) || (rc = dalloc(s, m.bond_e, 16 * (size_t)nbonded_blocks)) ||
+      (rc = dalloc(s, m.ke_e, 8 * (size_t)nblk(N, 256))))
+    return rc;
+  hipStream_t st = s->ctx->stream;
+  if (m.nb) HIPCHK(hipMemcpyAsync(m.bonds.p, b.data(), 8 * b.size(), hipMemcpyHostToDevice, st));
+  if (m.nb) HIPCHK(hipMemcpyAsync(m.bpar.p, bp.data(), 16 * bp.size(), hipMemcpyHostToDevice, st));
+  if (m.na) HIPCHK(hipMemcpyAsync(m.angles.p, a.data(), 16 * a.size(), hipMemcpyHostToDevice, st));
+  if (m.na) HIPCHK(hipMemcpyAsync(m.apar.p, ap.data(), 16 * ap.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(m.minv.p, minv.data(), 8 * N, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // the pair-list radius includes the skin
+  s->cached = false;
+  return setup(s);
+}
+
+int pmx_md_set_state(pmx_sys *sys, const double *x, const double *v) {
+  if (!sys || !x || !v) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  if (!s->md.on) return fail(s->ctx, PMX_ERR_ARG, "pmx_md_init first");
+  int rc = pmx_sys_set_positions(sys, x);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(s->md.v.p, v, 24 * (size_t)s->n, hipMemcpyHostToDevice, s->ctx->stream));
+  s->md.forces_valid = false;
+  return PMX_OK;
+}
+
+namespace {
+
+int md_check_overflow(Sys *s) {
+  int ovf = 0;
+  HIPCHK(hipMemcpy(&ovf, s->overflow.p, 4, hipMemcpyDeviceToHost));
+  if (ovf) return fail(s->ctx, PMX_ERR_OVERFLOW, "pair list capacity exceeded during MD");
+  return PMX_OK;
+}
+
+// One leapfrog step: kick + drift (+ optional list rebuild at the new
+// positions) + forces.  dslot: per-step displacement slot.
+int md_step(Sys *s, double kick, bool rebuild_here, int dslot) {
+  hipStream_t st = s->ctx->stream;
+  Sys::Md &m = s->md;
+  const int n = s->n;
+  const bool timing = s->timing_active;
+  // stage 0 of a timed MD step is integrate (+ sort when the list is rebuilt)
+  timing_begin_call(s, timing);
+  if (s->mixed)
+    hipLaunchKernelGGL(k_kick_drift<float>, dim3(nblk(n, 256)), dim3(256), 0, st, n, kick * m.h, m.h,
+                       P<int>(s->permp), P<double>(s->fout), P<double>(m.minv), P<double>(m.v), P<double>(s->xorig),
+                       P<double>(m.xbuild), s->box, P<float4>(s->xs), P<double4>(s->xs64),
+                       P<unsigned>(m.dmax) + dslot);
+  else
+    hipLaunchKernelGGL(k_kick_drift<double>, dim3(nblk(n, 256)), dim3(256), 0, st, n, kick * m.h, m.h,
+                       P<int>(s->permp), P<double>(s->fout), P<double>(m.minv), P<double>(m.v), P<double>(s->xorig),
+                       P<double>(m.xbuild), s->box, P<double4>(s->xs), P<double4>(s->xs64),
+                       P<unsigned>(m.dmax) + dslot);
+  if (rebuild_here) {
+    int rc = rebuild(s, timing);
+    if (rc) return rc;
+    ++m.rebuilds;
+  } else if (timing) {
+    hipEventRecord(s->ev[1], st);
+    hipEventRecord(s->ev[2], st);
+  }
+  return forces(s, timing ? PMX_EVAL_TIMING : 0u, false);
+}
+
+}  // namespace
+
+int pmx_md_run(pmx_sys *sys, int nsteps) {
+  if (!sys || nsteps < 0) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  Sys::Md &m = s->md;
+  if (!m.on || !s->have_positions) return fail(s->ctx, PMX_ERR_ARG, "pmx_md_init and pmx_md_set_state first");
+  if (nsteps == 0) return PMX_OK;
+  hipSetDevice(s->ctx->device);
+  hipStream_t st = s->ctx->stream;
+  const int n = s->n;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
+  if (!m.forces_valid) {  // StandardIntegrator::initializeForces
+    int rc = rebuild(s, false);
+    if (rc) return rc;
+    ++m.rebuilds;
+    if ((rc = forces(s, 0, false))) return rc;
+    m.forces_valid = true;
+  }
+  const float lim2 = (float)(0.25 * m.skin * m.skin);
+  int overflow_retries = 0;
+  int done = 0;
+  bool first = true;
+  std::vector<float> dm(kMaxChunk);
+  while (done < nsteps) {
+    int K = std::min(m.chunk, nsteps - done);
+    // snapshot of the committed state at the chunk start
+    HIPCHK(hipMemcpyAsync(m.xbak.p, s->xorig.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(m.vbak.p, m.v.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(m.fbak.p, s->fout.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kMaxChunk), 0, st, P<uint32_t>(m.dmax), 0u, (long long)kMaxChunk);
+    for (int k = 0; k < K; ++k) {
+      int rc = md_step(s, (first && k == 0) ? 0.5 : 1.0, k == 0 || m.skin == 0.0, k);
+      if (rc) return rc;
+    }
+    int ovf = 0;
+    HIPCHK(hipMemcpyAsync(dm.data(), m.dmax.p, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&ovf, s->overflow.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (ovf) {
+      // a list or mask capacity was exceeded: grow, roll back, replay the chunk
+      if (++overflow_retries > 6) return fail(s->ctx, PMX_ERR_OVERFLOW, "pair list capacity exceeded during MD");
+      int rc = grow_and_retry(s);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
+      HIPCHK(hipMemcpyAsync(s->xorig.p, m.xbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(m.v.p, m.vbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(s->fout.p, m.fbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+      continue;
+    }
+    int bad = -1;
+    float worst = 0.f;
+    for (int k = 1; k < K; ++k) {
+      worst = std::max(worst, dm[k]);
+      if (m.skin > 0 && dm[k] > lim2) {
+        bad = k;
+        break;
+      }
+    }
+    if (bad < 0) {
+      done += K;
+      first = false;
+      m.steps += K;
+      if (m.skin > 0 && K == m.chunk && worst < 0.25f * lim2) m.chunk = std::min(kMaxChunk, 2 * m.chunk);
+      continue;
+    }
+    // roll back and replay the steps that used a valid list
+    ++m.rollbacks;
+    HIPCHK(hipMemcpyAsync(s->xorig.p, m.xbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(m.v.p, m.vbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(s->fout.p, m.fbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    for (int k = 0; k < bad; ++k) {
+      int rc = md_step(s, (first && k == 0) ? 0.5 : 1.0, k == 0, k);
+      if (rc) return rc;
+    }
+    done += bad;
+    m.steps += bad;
+    first = false;
+    m.chunk = std::max(1, bad);
+  }
+  hipLaunchKernelGGL(k_half_kick, dim3(nblk(n, 256)), dim3(256), 0, st, n, 0.5 * m.h, P<double>(s->fout),
+                     P<double>(m.minv), P<double>(m.v));
+  HIPCHK(hipStreamSynchronize(st));
+  return md_check_overflow(s);
+}
+
+int pmx_md_get_state(pmx_sys *sys, double *x, double *v, double *energies) {
+  if (!sys) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  Sys::Md &m = s->md;
+  if (!m.on) return fail(s->ctx, PMX_ERR_ARG, "pmx_md_init first");
+  hipSetDevice(s->ctx->device);
+  hipStream_t st = s->ctx->stream;
+  const int n = s->n;
+  if (x) HIPCHK(hipMemcpyAsync(x, s->xorig.p, 24 * (size_t)n, hipMemcpyDeviceToHost, st));
+  if (v) HIPCHK(hipMemcpyAsync(v, m.v.p, 24 * (size_t)n, hipMemcpyDeviceToHost, st));
+  if (energies) {
+    // energies of the last force evaluation (positions x) and kinetic energy of v
+    reduce_energies(s, false);
+    hipLaunchKernelGGL(k_kinetic, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<double>(m.v), P<double>(m.minv),
+                       P<double>(m.ke_e));
+    const int nbb = std::max(1, (int)nblk(m.nb + m.na, 256));
+    std::vector<double> be(2 * nbb, 0.0), ke(nblk(n, 256));
+    if (m.nb + m.na > 0) HIPCHK(hipMemcpyAsync(be.data(), m.bond_e.p, 16 * (size_t)nbb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ke.data(), m.ke_e.p, 8 * ke.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(s->eout_h, s->eout.p, 32, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double eb = 0, ea = 0, ek = 0;
+    for (int i = 0; i < nbb; ++i) {
+      eb += be[2 * i];
+      ea += be[2 * i + 1];
+    }
+    for (double e : ke) ek += e;
+    double ec = s->eout_h[0];
+    if (s->pme_on) ec += s->eout_h[2] + s->eout_h[3] + s->self_e + s->charged_e;
+    energies[PMX_MD_E_COULOMB] = ec;
+    energies[PMX_MD_E_LJ] = s->eout_h[1];
+    energies[PMX_MD_E_BOND] = eb;
+    energies[PMX_MD_E_ANGLE] = ea;
+    energies[PMX_MD_E_KINETIC] = ek;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  return PMX_OK;
+}
+
+int pmx_md_stats(const pmx_sys *sys, long long *steps, long long *rebuilds, long long *rollbacks) {
+  if (!sys || !steps || !rebuilds || !rollbacks) return PMX_ERR_ARG;
+  *steps = sys->s.md.steps;
+  *rebuilds = sys->s.md.rebuilds;
+  *rollbacks = sys->s.md.rollbacks;
   return PMX_OK;
 }
 

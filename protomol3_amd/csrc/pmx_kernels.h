@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(256) k_build_list(ListParams p, const float *_
           if (d2 <= q.rlist2) {
             imask |= 1u << c;
             bool ipartial = (ci + 1) * kClusterSize > q.n;
-            if (ci == cj || ipartial || jpartial || d2 <= q.rexcl2) need = true;
+            if (ci == cj || ipartial || jpartial) need = true;
           }
         }
       }
@@ -418,6 +418,45 @@ __global__ void __launch_bounds__(256) k_build_list(ListParams p, const float *_
     total += __popcll(act);
   }
   if (lane == 0) count[S] = total < q.cap ? total : q.cap;
+}
+
+// Flags the list entry that holds each excluded pair (FULL or MODIFIED) so
+// that k_build_masks gives it explicit pair masks.  Entries of supercluster S
+// are sorted by j-cluster, so the entry is found by binary search; pairs whose
+// clusters are not in the list are beyond the list radius and never evaluated.
+__global__ void k_mark_exclusions(int ne, const int4 *__restrict__ etab, const int *__restrict__ slot,
+                                  int4 *__restrict__ entries, const int *__restrict__ count, int cap,
+                                  int *__restrict__ maskwork, int *__restrict__ nmaskwork, int maxmaskwork,
+                                  int *__restrict__ overflow) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int4 t = etab[e];
+  const int c1 = slot[t.x] / kClusterSize, c2 = slot[t.y] / kClusterSize;
+  if (c1 == c2) return;  // diagonal entries always carry masks
+  const int ci = c1 < c2 ? c1 : c2, cj = c1 < c2 ? c2 : c1;
+  const int S = ci / 8;
+  int lo = 0, hi = count[S] - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int v = entries[(size_t)S * cap + mid].x;
+    if (v == cj) {
+      int *z = &entries[(size_t)S * cap + mid].z;
+      if (atomicCAS(z, -1, -3) == -1) {
+        const int w = atomicAdd(nmaskwork, 1);
+        if (w < maxmaskwork) {
+          maskwork[w] = S * cap + mid;
+          atomicExch(z, -2);
+        } else {
+          atomicExch(overflow, 1);
+        }
+      }
+      return;
+    }
+    if (v < cj)
+      lo = mid + 1;
+    else
+      hi = mid - 1;
+  }
 }
 
 // Binary search of b in the sorted exclusion partners of a (original indices).
@@ -1402,6 +1441,178 @@ __global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p
     out[3] = K;
     *out_np = N + (long long)border_stat[0];
     *out_hash = H + border_stat[1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 7. Device-resident MD: bonded terms, leapfrog kick/drift, kinetic energy.
+
+// Zeroes up to 8 ranges of 64-bit words in one launch (blockIdx.y = range).
+struct ZeroRanges {
+  unsigned long long *p[8];
+  long long n[8];
+};
+__global__ void k_zero(ZeroRanges z) {
+  const int r = blockIdx.y;
+  unsigned long long *p = z.p[r];
+  const long long n = z.n[r];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+// Reference minimalDifference(c1, c2) = c2 - c1 folded to the minimal image.
+__device__ inline void min_diff(const double *x, int c1, int c2, const Box &bx, double *d) {
+  double d2;
+  for (int k = 0; k < 3; ++k) d[k] = x[3 * c2 + k] - x[3 * c1 + k];
+  d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (d2 > bx.D) {
+    for (int k = 0; k < 3; ++k) {
+      if (d[k] < -bx.H2[k])
+        d[k] += bx.H[k];
+      else if (d[k] > bx.H2[k])
+        d[k] -= bx.H[k];
+    }
+    d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (d2 > bx.D)
+      for (int k = 0; k < 3; ++k) d[k] -= bx.L[k] * rint(bx.Lr[k] * d[k]);
+  }
+}
+
+// Harmonic bonds (BondSystemForce.h:97-118) and angles without Urey-Bradley
+// (AngleSystemForce.h:95-147), FP64.  Threads [0, nb) bonds, [nb, nb+na) angles.
+template <bool MIXED>
+__global__ void k_bonded(int nb, int na, const int2 *__restrict__ bonds, const double2 *__restrict__ bpar,
+                         const int4 *__restrict__ angles, const double2 *__restrict__ apar,
+                         const double *__restrict__ x, const int *__restrict__ slot, Box box,
+                         typename AccT<MIXED>::type *__restrict__ facc, double *__restrict__ epart) {
+  __shared__ double s_e[2][kPairBlock / kWave];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double eb = 0.0, ea = 0.0;
+  if (t < nb) {
+    const int2 b = bonds[t];
+    const double2 p = bpar[t];
+    double r12[3];
+    min_diff(x, b.y, b.x, box, r12);  // atom1 - atom2
+    const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+    const double dpot = 2.0 * p.x * (r - p.y);
+    const double s1 = slot[b.x], s2 = slot[b.y];
+    for (int k = 0; k < 3; ++k) {
+      const double f1 = r12[k] * (-dpot / r);
+      AccT<MIXED>::add(&facc[3 * (int)s1 + k], f1);
+      AccT<MIXED>::add(&facc[3 * (int)s2 + k], -f1);
+    }
+    eb = p.x * (r - p.y) * (r - p.y);
+  } else if (t < nb + na) {
+    const int4 a = angles[t - nb];
+    const double2 p = apar[t - nb];
+    double r12[3], r32[3];
+    min_diff(x, a.y, a.x, box, r12);  // atom1 - atom2
+    min_diff(x, a.y, a.z, box, r32);  // atom3 - atom2
+    const double d12 = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+    const double d32 = sqrt(r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2]);
+    const double cx = r12[1] * r32[2] - r12[2] * r32[1], cy = r12[2] * r32[0] - r12[0] * r32[2],
+                 cz = r12[0] * r32[1] - r12[1] * r32[0];
+    const double theta = atan2(sqrt(cx * cx + cy * cy + cz * cz), r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2]);
+    const double st = sin(theta), ct = cos(theta);
+    const double dpot = 2.0 * p.x * (theta - p.y);
+    double f1[3], f3[3];
+    for (int k = 0; k < 3; ++k) {
+      const double dr12 = r12[k] / d12, dr32 = r32[k] / d32;
+      const double dt1 = (dr12 * ct - dr32) / (st * d12);
+      const double dt3 = (dr32 * ct - dr12) / (st * d32);
+      f1[k] = -dt1 * dpot;
+      f3[k] = -dt3 * dpot;
+    }
+    const int s1 = slot[a.x], s2 = slot[a.y], s3 = slot[a.z];
+    for (int k = 0; k < 3; ++k) {
+      AccT<MIXED>::add(&facc[3 * s1 + k], f1[k]);
+      AccT<MIXED>::add(&facc[3 * s2 + k], -f1[k] - f3[k]);
+      AccT<MIXED>::add(&facc[3 * s3 + k], f3[k]);
+    }
+    ea = p.x * (theta - p.y) * (theta - p.y);
+  }
+  eb = wave_sum(eb);
+  ea = wave_sum(ea);
+  if ((threadIdx.x & 63) == 0) {
+    s_e[0][threadIdx.x >> 6] = eb;
+    s_e[1][threadIdx.x >> 6] = ea;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0, a = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
+      b += s_e[0][w];
+      a += s_e[1][w];
+    }
+    epart[2 * blockIdx.x] = b;
+    epart[2 * blockIdx.x + 1] = a;
+  }
+}
+
+// Kick (v += F kick h / m) and drift (x += v h) of every atom, FP64
+// (LeapfrogIntegrator.cpp:33-87), then the sorted copies the pair kernel
+// reads are refreshed in place and the displacement since the last list build
+// is folded into dmax (float bits of the largest squared displacement).
+template <class R>
+__global__ void k_kick_drift(int n, double kick_h, double h, const int *__restrict__ perm,
+                             const double *__restrict__ fout, const double *__restrict__ minv, double *__restrict__ v,
+                             double *__restrict__ x, const double *__restrict__ xbuild, Box box,
+                             typename Real4<R>::type *__restrict__ xs, double4 *__restrict__ xs64,
+                             unsigned int *__restrict__ dmax) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  float d2f = 0.f;
+  if (s < n) {
+    const int a = perm[s];
+    double w[3];
+    double d2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      double vk = v[3 * a + k] + fout[3 * a + k] * kick_h * minv[a];
+      v[3 * a + k] = vk;
+      const double xk = x[3 * a + k] + vk * h;
+      x[3 * a + k] = xk;
+      double wk = xk - box.L[k] * floor(xk * box.Linv[k]);
+      if (wk >= box.L[k]) wk -= box.L[k];
+      if (wk < 0.0) wk = 0.0;
+      w[k] = wk;
+      double dd = xk - xbuild[3 * a + k];
+      dd -= box.L[k] * rint(dd * box.Linv[k]);
+      d2 += dd * dd;
+    }
+    d2f = (float)d2;
+    const double4 old = xs64[s];
+    xs64[s] = make_double4(w[0], w[1], w[2], old.w);
+    typename Real4<R>::type r = xs[s];
+    r.x = (R)w[0];
+    r.y = (R)w[1];
+    r.z = (R)w[2];
+    xs[s] = r;
+  }
+  // wave max, one atomic per wave
+  for (int m = 32; m >= 1; m >>= 1) d2f = fmaxf(d2f, __shfl_xor(d2f, m));
+  if ((threadIdx.x & 63) == 0) atomicMax(dmax, __float_as_uint(d2f));
+}
+
+__global__ void k_half_kick(int n, double kick_h, const double *__restrict__ fout, const double *__restrict__ minv,
+                            double *__restrict__ v) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  for (int k = 0; k < 3; ++k) v[3 * a + k] += fout[3 * a + k] * kick_h * minv[a];
+}
+
+// Kinetic energy 0.5 sum m v^2 into per-block partials.
+__global__ void k_kinetic(int n, const double *__restrict__ v, const double *__restrict__ minv,
+                          double *__restrict__ epart) {
+  __shared__ double s_e[kPairBlock / kWave];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (a < n) e = 0.5 * (v[3 * a] * v[3 * a] + v[3 * a + 1] * v[3 * a + 1] + v[3 * a + 2] * v[3 * a + 2]) / minv[a];
+  e = wave_sum(e);
+  if ((threadIdx.x & 63) == 0) s_e[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_e[w];
+    epart[blockIdx.x] = t;
   }
 }
 

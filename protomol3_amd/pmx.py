@@ -46,10 +46,18 @@ class SysDesc(C.Structure):
                 ("pme_alpha", C.c_double), ("pme_terms", C.c_int), ("precision", C.c_int)]
 
 
+class MdDesc(C.Structure):
+    _fields_ = [("nbonds", C.c_int), ("nangles", C.c_int), ("bonds", C.POINTER(C.c_int)),
+                ("bond_par", C.POINTER(C.c_double)), ("angles", C.POINTER(C.c_int)),
+                ("angle_par", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
+                ("timestep_fs", C.c_double), ("skin", C.c_double)]
+
+
 EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_create", "pmx_sys_destroy",
             "pmx_sys_uncache", "pmx_sys_evaluate", "pmx_sys_set_positions", "pmx_sys_compute",
             "pmx_sys_get_forces", "pmx_sys_sync", "pmx_sys_alpha", "pmx_sys_pair_stats", "pmx_pair_hash",
-            "pmx_sys_cells", "pmx_sys_stage_times", "pmx_sys_timing_begin", "pmx_sys_timing_end"]
+            "pmx_sys_cells", "pmx_sys_stage_times", "pmx_sys_timing_begin", "pmx_sys_timing_end",
+            "pmx_md_init", "pmx_md_set_state", "pmx_md_run", "pmx_md_get_state", "pmx_md_stats"]
 
 _lib = None
 
@@ -84,6 +92,11 @@ def load() -> C.CDLL:
     L.pmx_sys_stage_times.argtypes = [vp, C.POINTER(C.c_float)]
     L.pmx_sys_timing_begin.argtypes = [vp]
     L.pmx_sys_timing_end.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_int)]
+    L.pmx_md_init.argtypes = [vp, C.POINTER(MdDesc)]
+    L.pmx_md_set_state.argtypes = [vp, dp, dp]
+    L.pmx_md_run.argtypes = [vp, C.c_int]
+    L.pmx_md_get_state.argtypes = [vp, dp, dp, dp]
+    L.pmx_md_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     _lib = L
     return L
 
@@ -232,6 +245,46 @@ class System:
         ms = (C.c_float * 8)()
         self.ctx.check(self.ctx.lib.pmx_sys_stage_times(self.h, ms), "pmx_sys_stage_times")
         return dict(zip(STAGES, [float(v) for v in ms]))
+
+    # -- device-resident MD -------------------------------------------------
+    def md_init(self, s: MolSystem, timestep_fs: float, skin: float = 1.0):
+        self._md_keep = [np.ascontiguousarray(s.bonds, np.int32).reshape(-1, 2),
+                         np.ascontiguousarray(s.bond_par, np.float64).reshape(-1, 2),
+                         np.ascontiguousarray(s.angles, np.int32).reshape(-1, 3),
+                         np.ascontiguousarray(s.angle_par, np.float64).reshape(-1, 2),
+                         np.ascontiguousarray(s.mass, np.float64)]
+        b, bp, a, ap, m = self._md_keep
+        d = MdDesc()
+        d.nbonds, d.nangles = b.shape[0], a.shape[0]
+        d.bonds = b.ctypes.data_as(C.POINTER(C.c_int))
+        d.bond_par = _dp(bp)
+        d.angles = a.ctypes.data_as(C.POINTER(C.c_int))
+        d.angle_par = _dp(ap)
+        d.mass = _dp(m)
+        d.timestep_fs = timestep_fs
+        d.skin = skin
+        self.ctx.check(self.ctx.lib.pmx_md_init(self.h, C.byref(d)), "pmx_md_init")
+
+    def md_set_state(self, x: np.ndarray, v: np.ndarray):
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        self.ctx.check(self.ctx.lib.pmx_md_set_state(self.h, _dp(x), _dp(v)), "pmx_md_set_state")
+
+    def md_run(self, nsteps: int):
+        self.ctx.check(self.ctx.lib.pmx_md_run(self.h, nsteps), "pmx_md_run")
+
+    def md_get_state(self):
+        """(x, v, energies dict) after the last run."""
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        e = np.zeros(5)
+        self.ctx.check(self.ctx.lib.pmx_md_get_state(self.h, _dp(x), _dp(v), _dp(e)), "pmx_md_get_state")
+        return x, v, dict(zip(("coulomb", "lj", "bond", "angle", "kinetic"), e.tolist()))
+
+    def md_stats(self):
+        st, rb, rl = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        self.ctx.check(self.ctx.lib.pmx_md_stats(self.h, C.byref(st), C.byref(rb), C.byref(rl)), "pmx_md_stats")
+        return dict(steps=st.value, rebuilds=rb.value, rollbacks=rl.value)
 
     def timing_begin(self):
         self.ctx.check(self.ctx.lib.pmx_sys_timing_begin(self.h), "pmx_sys_timing_begin")

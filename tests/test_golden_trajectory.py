@@ -1,0 +1,56 @@
+"""The oracle's pair path, bonded terms and leapfrog integrator against the
+reference's own end-to-end regression test test/tests/water_CHARMM_PERIODIC_L
+(423 waters, 28 A periodic box, Coulomb C1 6.5 A, LJ C2 0.1-6.5 A, bonds,
+angles, 200 leapfrog steps of 0.5 fs): final positions, velocities, the
+step-200 forces and the t = 100 fs energies in test/tests/expected/, which the
+reference's harness compares with epsilon 1e-5 (test/test.py:14)."""
+import os
+
+import numpy as np
+
+from golden import fixtures as FX
+from oracle import md
+from oracle import oracle as O
+from protomol3_amd import charmm
+
+W = FX.W423
+
+
+def load_golden():
+    def xyz(lines):
+        return np.array([[float(t) for t in ln.split()[1:4]] for ln in lines])
+
+    pos = xyz(open(os.path.join(W, "expected.pos")).read().splitlines()[1:424])
+    vel = xyz(open(os.path.join(W, "expected.vel")).read().splitlines()[1:424])
+    fl = open(os.path.join(W, "expected.forces")).read().splitlines()
+    f0, f200 = xyz(fl[2:425]), xyz(fl[426:849])
+    e = [float(t) for t in open(os.path.join(W, "expected.energy")).read().splitlines()[1].split()]
+    energies = dict(kinetic=e[2], bond=e[5], angle=e[6], lj=e[9], coulomb=e[10])
+    return pos, vel, f0, f200, energies
+
+
+def water423():
+    s = charmm.build_system(os.path.join(W, "water_423.psf"), os.path.join(W, "water_423.pos.pdb"),
+                            os.path.join(W, "par_all27_prot_lipid.inp"), box=(28.0, 28.0, 28.0))
+    # velfile is read in PDB units and divided by PDBVELSCALINGFACTOR
+    # (protomol/module/IOModule.cpp:126); momentum removal touches only
+    # non-water atoms (topology/TopologyUtilities.cpp:405-428), so none here.
+    v0 = charmm.read_pdb(os.path.join(W, "water_423.vel.pdb")) / md.PDBVELSCALINGFACTOR
+    return s, v0
+
+
+SPEC = dict(cellsize=6.5, lj=(O.SW_C2, 0.1, 6.5), coul=(O.SW_C1, 0.0, 6.5))
+
+
+def test_oracle_reproduces_reference_golden_trajectory():
+    s, v0 = water423()
+    sp = O.make_spec(**SPEC)
+    pos, vel, f0, f200, en = load_golden()
+    f_init, _ = md.total_forces(s, sp, s.pos)
+    assert np.abs(f_init - f0).max() < 1e-9
+    x, v, f, e = md.leapfrog_run(s, sp, s.pos, v0, 0.5, 200)
+    assert np.abs(x - pos).max() < 1e-9
+    assert np.abs(v - vel).max() < 1e-9
+    assert np.abs(f - f200).max() < 1e-8
+    for k, ref in en.items():
+        assert abs(e[k] - ref) <= 6e-8 * abs(ref) + 1e-5, k

@@ -1,0 +1,86 @@
+"""GPU device-resident MD (leapfrog + bonded + nonbonded, all on the device)
+against the oracle integrator and the reference's golden trajectory."""
+import numpy as np
+import pytest
+
+from oracle import md
+from oracle import oracle as O
+from protomol3_amd import pmx, systems
+from test_golden_trajectory import SPEC, load_golden, water423
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pmx.Context(0)
+    yield c
+    c.close()
+
+
+def run_gpu_md(ctx, s, spec, v0, dt, nsteps, precision, skin, chunks=1):
+    sy = pmx.System(ctx, s, pmx.ForceSpec(spec["cellsize"], lj=spec.get("lj"), coul=spec.get("coul"),
+                                          pme=spec.get("pme")), precision)
+    try:
+        sy.md_init(s, dt, skin=skin)
+        sy.md_set_state(s.pos, v0)
+        for _ in range(chunks):
+            sy.md_run(nsteps // chunks)
+        x, v, e = sy.md_get_state()
+        f, _, _ = sy.get_forces()
+        stats = sy.md_stats()
+    finally:
+        sy.close()
+    return x, v, f, e, stats
+
+
+@pytest.mark.parametrize("skin", [0.0, 1.0])
+def test_gpu_md_reproduces_reference_golden_trajectory(ctx, skin):
+    """water_CHARMM_PERIODIC_L: 200 steps of 0.5 fs on the GPU (FP64 form),
+    compared with the reference's expected outputs at its harness epsilon."""
+    s, v0 = water423()
+    pos, vel, f0, f200, en = load_golden()
+    x, v, f, e, stats = run_gpu_md(ctx, s, SPEC, v0, 0.5, 200, pmx.FP64, skin)
+    print(f"skin={skin} stats={stats} dpos={np.abs(x - pos).max():.2e} dvel={np.abs(v - vel).max():.2e} "
+          f"dforce={np.abs(f - f200).max():.2e} energies={e}", flush=True)
+    assert stats["steps"] == 200
+    assert np.abs(x - pos).max() < 1e-8
+    assert np.abs(v - vel).max() < 1e-8
+    assert np.abs(f - f200).max() < 1e-7
+    for k, ref in en.items():
+        assert abs(e[k] - ref) <= 6e-8 * abs(ref) + 1e-5, k
+
+
+def test_gpu_md_pme_matches_oracle_integrator(ctx):
+    s = systems.water_box(seed=21).with_velocities(300.0, seed=5)
+    spec = dict(cellsize=5.0, lj=(O.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 32, 32), order=4, alpha=0.32))
+    ospec = O.make_spec(5.0, lj=spec["lj"], pme=spec["pme"])
+    xr, vr, fr, er = md.leapfrog_run(s, ospec, s.pos, s.vel, 1.0, 10)
+    x, v, f, e, stats = run_gpu_md(ctx, s, spec, s.vel, 1.0, 10, pmx.FP64, 1.0)
+    print(f"fp64 dx={np.abs(x - xr).max():.2e} dv={np.abs(v - vr).max():.2e} stats={stats}", flush=True)
+    assert np.abs(x - xr).max() < 1e-9
+    assert np.abs(v - vr).max() < 1e-8
+    for k in er:
+        assert abs(e[k] - er[k]) <= 1e-9 * max(1.0, abs(er[k])), k
+    # mixed form: same trajectory within FP32 pair-arithmetic error
+    xm, vm, fm, em, _ = run_gpu_md(ctx, s, spec, s.vel, 1.0, 10, pmx.MIXED, 1.0)
+    print(f"mixed dx={np.abs(xm - xr).max():.2e} dv={np.abs(vm - vr).max():.2e}", flush=True)
+    assert np.abs(xm - xr).max() < 1e-5
+    assert np.abs(vm - vr).max() < 1e-4
+
+
+def test_gpu_md_skin_reuse_is_exact_and_rolls_back(ctx):
+    """A tiny skin forces rollbacks; the trajectory must equal the one that
+    rebuilds the list every step (FP64 form: up to atomics ordering)."""
+    s = systems.water_box(seed=22).with_velocities(600.0, seed=6)
+    spec = dict(cellsize=5.0, lj=(O.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 32, 32), order=4, alpha=0.32))
+    x0, v0, _, e0, s0 = run_gpu_md(ctx, s, spec, s.vel, 1.0, 60, pmx.FP64, 0.0)
+    x1, v1, _, e1, s1 = run_gpu_md(ctx, s, spec, s.vel, 1.0, 60, pmx.FP64, 0.05, chunks=2)
+    x2, v2, _, e2, s2 = run_gpu_md(ctx, s, spec, s.vel, 1.0, 60, pmx.FP64, 1.0)
+    print(f"skin0 {s0} skin0.05 {s1} skin1 {s2} dx={np.abs(x1 - x0).max():.2e} {np.abs(x2 - x0).max():.2e}",
+          flush=True)
+    assert s1["rollbacks"] > 0
+    assert s2["rebuilds"] < s0["rebuilds"] // 2
+    for x, v in ((x1, v1), (x2, v2)):
+        assert np.abs(x - x0).max() < 1e-10
+        assert np.abs(v - v0).max() < 1e-9
