@@ -196,7 +196,7 @@ def main():
         "pair_interactions_per_s": float(f"{mixed['npairs'] / (mixed['ms_per_step'] * 1e-3):.4e}"),
         "stage_ms_mixed": {k: round(v, 5) for k, v in mixed["stages"].items()},
         "stage_ms_fp64": {k: round(v, 5) for k, v in fp64["stages"].items()},
-        "roofline": {"bound": "valu", "kernel": "k_pair<mixed,PME,C2,Cutoff> + k_border",
+        "roofline": {"bound": "valu", "kernel": "k_pair<mixed,PME,C2,Cutoff>",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
                      "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": int(mixed["npairs"]),

@@ -126,12 +126,15 @@ unsigned long long pmx_pair_hash(int i, int j);
 int pmx_sys_cells(pmx_sys *sys, int *cell_of_atom, int *dims);
 
 /* Device milliseconds of the last evaluate with PMX_EVAL_TIMING, in the
- * order: sort, list, pair, spread, fft+convolve, gather, correction,
- * finalize.  ms must hold 8 floats. */
+ * order: sort (+ MD integrate), list (+ accumulator zeroing), pair (the
+ * cluster-pair kernel alone), spread, fft+convolve, gather, fixups (FP64
+ * re-test of pairs at the cutoff, exclusion correction, bonded terms),
+ * finalize.  ms must hold PMX_STAGES floats. */
+enum { PMX_STAGES = 8 };
 int pmx_sys_stage_times(const pmx_sys *sys, float *ms);
 /* Per-stage device times averaged over every pmx_sys_compute call made with
  * PMX_EVAL_TIMING since pmx_sys_timing_begin (HIP events on the system's
- * stream, read after one synchronisation). avg_ms holds 8 floats. */
+ * stream, read after one synchronisation). avg_ms holds PMX_STAGES floats. */
 int pmx_sys_timing_begin(pmx_sys *sys);
 int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls);
 
@@ -171,8 +174,9 @@ int pmx_md_set_state(pmx_sys *sys, const double *x, const double *v);
  * forces) x (nsteps-1), half kick.  Returns after queueing and checking the
  * chunks (synchronous). */
 int pmx_md_run(pmx_sys *sys, int nsteps);
-/* State and energies after the last run (energies[PMX_MD_E_COUNT]; any
- * pointer may be NULL). */
+/* Current positions, velocities and energies (energies[PMX_MD_E_COUNT]:
+ * potential terms at x, kinetic energy of v; right after pmx_md_set_state
+ * the forces are computed first).  Any pointer may be NULL. */
 int pmx_md_get_state(pmx_sys *sys, double *x, double *v, double *energies);
 /* Counters: list rebuilds and rolled-back chunks since pmx_md_init. */
 int pmx_md_stats(const pmx_sys *sys, long long *steps, long long *rebuilds, long long *rollbacks);

@@ -31,7 +31,14 @@ namespace {
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  bool listed = false;  // registered in Sys::all (freed by pmx_sys_destroy)
 };
+
+// Timed stages: sort (+ integrate), list (+ accumulator zeroing), pair,
+// spread, fft (+ convolution), gather, fixups (boundary pairs, exclusion
+// correction, bonded terms), finalize.
+constexpr int kStages = 8;
+static_assert(kStages == PMX_STAGES, "stage count of the C ABI");
 
 struct Sys {
   pmx_ctx *ctx = nullptr;
@@ -62,14 +69,14 @@ struct Sys {
   DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
       ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
       pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec,
-      greal, theta, twx, twy, twz, exclmax, border, nborder, border_acc;
+      greal, theta, twx, twy, twz, border, nborder, border_acc;
   int maxborder = 65536;
   // results
   long long last_np = 0;
   unsigned long long last_hash = 0;
-  float stage_ms[8] = {0};
+  float stage_ms[kStages] = {0};
   hipEvent_t *ev = nullptr;                    // event set of the current timed call
-  std::vector<std::vector<hipEvent_t>> evsets;  // one set of 9 per timed call since timing_begin
+  std::vector<std::vector<hipEvent_t>> evsets;  // one set of kStages + 1 per timed call since timing_begin
   size_t evused = 0;
   bool timing_active = false;                   // MD steps record events (timing_begin .. timing_end)
   bool have_positions = false;
@@ -110,7 +117,8 @@ int dalloc(Sys *s, DevBuf &b, size_t bytes) {
     return fail(s->ctx, PMX_ERR_OOM, "hipMalloc failed (" + std::to_string(bytes) + " bytes)");
   }
   b.bytes = bytes;
-  s->all.push_back(&b);
+  if (!b.listed) s->all.push_back(&b);
+  b.listed = true;
   return PMX_OK;
 }
 
@@ -356,7 +364,11 @@ int setup(Sys *s) {
   // List parameters
   ListParams &lp = s->lp;
   const double margin = 4.0 * Lmax * std::ldexp(1.0, -23) + 1e-4;
-  for (int k = 0; k < 3; ++k) lp.L[k] = (float)bx.L[k];
+  for (int k = 0; k < 3; ++k) {
+    lp.L[k] = (float)bx.L[k];
+    lp.Linv[k] = (float)(1.0 / bx.L[k]);
+  }
+  lp.skin = (float)s->md.skin;
   lp.rlist2 = (float)((rcmax + margin + s->md.skin) * (rcmax + margin + s->md.skin));
   lp.ncl = s->ncl;
   lp.nsuper = s->nsuper;
@@ -391,7 +403,7 @@ int setup(Sys *s) {
       (rc = A(s->npairs, 8 * (size_t)s->ntiles_cap)) || (rc = A(s->hash, 8 * (size_t)s->ntiles_cap)) ||
       (rc = A(s->eptr_d, 4 * (N + 1))) || (rc = A(s->elist_d, 8 * s->elist.size())) ||
       (rc = A(s->etab_d, 16 * s->etab.size())) || (rc = A(s->eout, 64)) || (rc = A(s->np_out, 8)) ||
-      (rc = A(s->hash_out, 8)) || (rc = A(s->exclmax, 16)) || (rc = A(s->border, 8 * (size_t)s->maxborder)) ||
+      (rc = A(s->hash_out, 8)) || (rc = A(s->border, 8 * (size_t)s->maxborder)) ||
       (rc = A(s->nborder, 8)) || (rc = A(s->border_acc, 32)))
     return rc;
   s->corr_blocks = std::max(1, (int)nblk((long long)s->etab.size(), 256));
@@ -517,49 +529,114 @@ int setup(Sys *s) {
   return PMX_OK;
 }
 
-template <bool MIXED>
-int run_pair(Sys *s, bool stats) {
+// Launches of the kernels templated on the force kinds: the pair kernel
+// (phase 0) and the fused fixup pass (phase 1: boundary pairs, exclusion
+// correction, bonded terms).
+template <bool MIXED, int C, int LJ, int CS>
+void launch_pair_phase(Sys *s, bool stats, int phase) {
   hipStream_t st = s->ctx->stream;
   using R4 = typename Real4<typename std::conditional<MIXED, float, double>::type>::type;
-  const unsigned grid = (unsigned)std::max(1, s->ctx->cus * 8);
-  PairOut out{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash),
-              P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow)};
   auto *facc = P<typename AccT<MIXED>::type>(s->facc);
-  const int si = stats ? 1 : 0;
-  // border kernel grid: upper bound from the capacity; threads past the count exit
-  const unsigned bgrid = nblk(s->maxborder, 256);
-#define PMX_LAUNCH_PAIR(C, LJ, CS)                                                                                  \
-  do {                                                                                                             \
-    hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, si,                 \
-                       P<int2>(s->tiles), P<int>(s->ntiles), P<int>(s->tilectr), P<int4>(s->entries),               \
-                       P<int>(s->count), s->cap, P<unsigned long long>(s->masks), P<R4>(s->xs), P<int>(s->ts),      \
-                       P<int>(s->permp), P<double>(s->ljtab), facc, out);                                           \
-    hipLaunchKernelGGL((k_border<MIXED, C, LJ, CS>), dim3(bgrid), dim3(256), 0, st, s->pp, si, P<int2>(s->border), \
-                       P<int>(s->nborder), s->maxborder, P<unsigned long long>(s->masks), P<R4>(s->xs),           \
-                       P<int>(s->ts), P<int>(s->permp), P<double>(s->xorig), P<double>(s->ljtab),                \
-                       P<int>(s->eptr_d), P<int2>(s->elist_d), facc, P<unsigned long long>(s->border_acc),         \
-                       P<unsigned long long>(s->border_acc) + 2);                                                  \
-  } while (0)
+  if (phase == 0) {
+    // as many blocks as are resident at once (each wave's first tile is
+    // assigned statically)
+    static int per_cu = 0;
+    if (!per_cu) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair<MIXED, C, LJ, CS>, kPairBlock, 0) !=
+              hipSuccess ||
+          per_cu < 1)
+        per_cu = 1;
+    }
+    const unsigned grid = (unsigned)std::max(1, s->ctx->cus * per_cu);
+    PairOut out{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash),
+                P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow)};
+    hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, stats ? 1 : 0,
+                       P<int2>(s->tiles), P<int>(s->ntiles), P<int>(s->tilectr), P<int4>(s->entries),
+                       P<int>(s->count), s->cap, P<unsigned long long>(s->masks), P<R4>(s->xs), P<int>(s->ts),
+                       P<int>(s->permp), P<double>(s->ljtab), facc, out);
+    return;
+  }
+  FixupArgs<MIXED> a{};
+  a.border_blocks = 4;
+  a.pp = s->pp;
+  a.stats = stats ? 1 : 0;
+  a.border = P<int2>(s->border);
+  a.nborder = P<int>(s->nborder);
+  a.maxborder = s->maxborder;
+  a.xs = P<R4>(s->xs);
+  a.ts = P<int>(s->ts);
+  a.perm = P<int>(s->permp);
+  a.xorig = P<double>(s->xorig);
+  a.ljtab = P<double>(s->ljtab);
+  a.eptr = P<int>(s->eptr_d);
+  a.elist = P<int2>(s->elist_d);
+  a.border_e = P<unsigned long long>(s->border_acc);
+  a.border_stat = P<unsigned long long>(s->border_acc) + 2;
+  const bool corr = s->pme_on && (s->d.pme_terms & PMX_PME_CORR) && !s->etab.empty();
+  a.corr_blocks = corr ? s->corr_blocks : 0;
+  a.ne = (int)s->etab.size();
+  a.etab = P<int4>(s->etab_d);
+  a.q = P<double>(s->q_d);
+  a.slot = P<int>(s->slot);
+  a.alpha = s->alpha;
+  a.csf = s->d.csf;
+  a.corr_e = P<double>(s->corr_e);
+  int bonded_blocks = 0;
+  if (s->md.on && s->md.nb + s->md.na > 0) {
+    bonded_blocks = (int)nblk(s->md.nb + s->md.na, 256);
+    a.nbond = s->md.nb;
+    a.nangle = s->md.na;
+    a.bonds = P<int2>(s->md.bonds);
+    a.bpar = P<double2>(s->md.bpar);
+    a.angles = P<int4>(s->md.angles);
+    a.apar = P<double2>(s->md.apar);
+    a.bond_e = P<double>(s->md.bond_e);
+  }
+  a.facc = facc;
+  hipLaunchKernelGGL((k_fixups<MIXED, C, LJ, CS>), dim3(a.border_blocks + a.corr_blocks + bonded_blocks), dim3(256), 0,
+                     st, a);
+}
+
+template <bool MIXED>
+void run_pair_phase(Sys *s, bool stats, int phase) {
   const int lj = s->d.lj_switch, cs = s->d.coul_switch;
   if (s->pp.coul == COUL_PME) {
     if (lj == SW_C2 && cs == SW_CUTOFF)
-      PMX_LAUNCH_PAIR(COUL_PME, SW_C2, SW_CUTOFF);
+      launch_pair_phase<MIXED, COUL_PME, SW_C2, SW_CUTOFF>(s, stats, phase);
     else if (lj == SW_NONE)
-      PMX_LAUNCH_PAIR(COUL_PME, SW_NONE, SW_ANY);
+      launch_pair_phase<MIXED, COUL_PME, SW_NONE, SW_ANY>(s, stats, phase);
     else
-      PMX_LAUNCH_PAIR(COUL_PME, SW_ANY, SW_ANY);
+      launch_pair_phase<MIXED, COUL_PME, SW_ANY, SW_ANY>(s, stats, phase);
   } else if (s->pp.coul == COUL_CUTOFF) {
     if (lj == SW_C2 && cs == SW_C1)
-      PMX_LAUNCH_PAIR(COUL_CUTOFF, SW_C2, SW_C1);
+      launch_pair_phase<MIXED, COUL_CUTOFF, SW_C2, SW_C1>(s, stats, phase);
     else if (lj == SW_NONE)
-      PMX_LAUNCH_PAIR(COUL_CUTOFF, SW_NONE, SW_ANY);
+      launch_pair_phase<MIXED, COUL_CUTOFF, SW_NONE, SW_ANY>(s, stats, phase);
     else
-      PMX_LAUNCH_PAIR(COUL_CUTOFF, SW_ANY, SW_ANY);
+      launch_pair_phase<MIXED, COUL_CUTOFF, SW_ANY, SW_ANY>(s, stats, phase);
   } else {
-    PMX_LAUNCH_PAIR(COUL_NONE, SW_ANY, SW_ANY);
+    launch_pair_phase<MIXED, COUL_NONE, SW_ANY, SW_ANY>(s, stats, phase);
   }
-#undef PMX_LAUNCH_PAIR
-  return PMX_OK;
+}
+
+// Spreading and gathering for the stencil order ORDER (validated: even, 2..8).
+template <bool MIXED, int ORDER>
+void launch_spread(Sys *s, hipStream_t st) {
+  constexpr int P3 = ORDER * ORDER * ORDER;
+  constexpr int apw = P3 >= 64 ? 1 : 64 / P3;
+  const int waves = (int)nblk(s->n, apw);
+  hipLaunchKernelGGL((k_spread<MIXED, ORDER>), dim3(nblk((long long)waves * 64, 256)), dim3(256), 0, st, s->n,
+                     s->pme, P<double>(s->xorig), P<int>(s->permp), P<double>(s->q_d), P<double>(s->grid64),
+                     P<unsigned long long>(s->gridfx));
+}
+
+template <bool MIXED, int ORDER>
+void launch_gather(Sys *s, hipStream_t st) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  const int waves = (int)nblk(s->n, 64 / pme_tpa(ORDER));
+  hipLaunchKernelGGL((k_gather<MIXED, ORDER>), dim3(nblk((long long)waves * 64, 256)), dim3(256), 0, st, s->n, waves,
+                     s->pme, P<double>(s->xorig), P<int>(s->permp), P<double>(s->q_d), P<T>(s->greal),
+                     P<typename AccT<MIXED>::type>(s->facc));
 }
 
 template <bool MIXED>
@@ -570,9 +647,12 @@ int run_pme(Sys *s, bool timing) {
   auto *facc = P<typename AccT<MIXED>::type>(s->facc);
   if (s->d.pme_terms & PMX_PME_RECIP) {
     // the charge grid was zeroed by the k_zero launch at the start of the pass
-    hipLaunchKernelGGL((k_spread<MIXED>), dim3(nblk(s->n, 128)), dim3(128), 0, st, s->n, pm, P<double>(s->xorig),
-                       P<int>(s->permp), P<double>(s->q_d), P<double>(s->grid64),
-                       P<unsigned long long>(s->gridfx));
+    switch (pm.order) {
+      case 2: launch_spread<MIXED, 2>(s, st); break;
+      case 4: launch_spread<MIXED, 4>(s, st); break;
+      case 6: launch_spread<MIXED, 6>(s, st); break;
+      default: launch_spread<MIXED, 8>(s, st); break;
+    }
     if (timing) hipEventRecord(s->ev[4], st);
     const size_t cx = sizeof(Cx<T>);
     const int zb = (pm.nx * pm.ny + s->zl - 1) / s->zl;
@@ -588,17 +668,17 @@ int run_pme(Sys *s, bool timing) {
     hipLaunchKernelGGL((k_fft_z_c2r<T>), dim3(zb), dim3(128), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
                        P<Cx<T>>(s->twz), P<Cx<T>>(s->spec), P<T>(s->greal));
     if (timing) hipEventRecord(s->ev[5], st);
-    hipLaunchKernelGGL((k_gather<MIXED>), dim3(nblk(s->n, 128)), dim3(128), 0, st, s->n, pm, P<double>(s->xorig),
-                       P<int>(s->permp), P<double>(s->q_d), P<T>(s->greal), facc);
+    switch (pm.order) {
+      case 2: launch_gather<MIXED, 2>(s, st); break;
+      case 4: launch_gather<MIXED, 4>(s, st); break;
+      case 6: launch_gather<MIXED, 6>(s, st); break;
+      default: launch_gather<MIXED, 8>(s, st); break;
+    }
   } else if (timing) {
     hipEventRecord(s->ev[4], st);
     hipEventRecord(s->ev[5], st);
   }
   if (timing) hipEventRecord(s->ev[6], st);
-  if ((s->d.pme_terms & PMX_PME_CORR) && !s->etab.empty())
-    hipLaunchKernelGGL((k_correction<MIXED>), dim3(s->corr_blocks), dim3(256), 0, st, (int)s->etab.size(),
-                       P<int4>(s->etab_d), P<double>(s->xorig), P<double>(s->q_d), P<int>(s->slot), s->box, s->alpha,
-                       s->d.csf, facc, P<double>(s->corr_e));
   return PMX_OK;
 }
 
@@ -616,7 +696,7 @@ void reduce_energies(Sys *s, bool stats) {
 void timing_begin_call(Sys *s, bool timing) {
   if (!timing) return;
   if (s->evused == s->evsets.size()) {
-    std::vector<hipEvent_t> set(9);
+    std::vector<hipEvent_t> set(kStages + 1);
     for (auto &e : set) hipEventCreate(&e);
     s->evsets.push_back(set);
   }
@@ -638,40 +718,39 @@ int rebuild(Sys *s, bool timing) {
                      P<uint32_t>(s->scansum));
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, st, P<uint32_t>(s->scansum), s->nscanblocks);
   hipLaunchKernelGGL(k_scan_final, dim3(s->nscanblocks), dim3(kScanBlock), 0, st, P<uint32_t>(s->hist), s->nbins,
-                     P<uint32_t>(s->scansum), P<uint32_t>(s->start));
-  HIPCHK(hipMemcpyAsync(s->cursor.p, s->start.p, 4 * s->nbins, hipMemcpyDeviceToDevice, st));
+                     P<uint32_t>(s->scansum), P<uint32_t>(s->start), P<uint32_t>(s->cursor));
   hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<uint32_t>(s->key), P<uint32_t>(s->cursor),
                      P<int>(s->perm));
   hipLaunchKernelGGL(k_binsort, dim3(nblk(s->nbins, 256)), dim3(256), 0, st, s->nbins, P<uint32_t>(s->start), n,
                      P<int>(s->perm));
+  double *xbuild = s->md.on ? P<double>(s->md.xbuild) : nullptr;
   if (s->mixed)
     hipLaunchKernelGGL(k_fill_sorted<float>, dim3(nblk(npad, 256)), dim3(256), 0, st, n, npad, P<int>(s->perm),
                        P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<float4>(s->xs),
-                       P<double4>(s->xs64), P<int>(s->ts), P<int>(s->slot), P<int>(s->permp));
+                       P<double4>(s->xs64), P<int>(s->ts), P<int>(s->slot), P<int>(s->permp),
+                       P<double>(s->xorig), xbuild);
   else
     hipLaunchKernelGGL(k_fill_sorted<double>, dim3(nblk(npad, 256)), dim3(256), 0, st, n, npad, P<int>(s->perm),
                        P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<double4>(s->xs),
-                       P<double4>(s->xs64), P<int>(s->ts), P<int>(s->slot), P<int>(s->permp));
+                       P<double4>(s->xs64), P<int>(s->ts), P<int>(s->slot), P<int>(s->permp),
+                       P<double>(s->xorig), xbuild);
   if (timing) hipEventRecord(s->ev[1], st);
   hipLaunchKernelGGL(k_cluster_bbox, dim3(nblk(s->ncl, 256)), dim3(256), 0, st, s->ncl, n, P<double4>(s->xs64),
                      P<float4>(s->ctr), P<float4>(s->half));
-  // exclmax words: [1] unused list radius slot, [2] mask work counter
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, P<uint32_t>(s->exclmax), 0u, 4ll);
-  hipLaunchKernelGGL(k_build_list, dim3(nblk((long long)s->nsuper * 64, 256)), dim3(256), 0, st, s->lp,
-                     P<float>(s->exclmax) + 1, P<float4>(s->ctr), P<float4>(s->half), P<int4>(s->entries),
-                     P<int>(s->count), P<int>(s->maskwork), P<int>(s->exclmax) + 2, s->maxmaskwork,
-                     P<int>(s->overflow));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, P<uint32_t>(s->nmaskwork), 0u, 1ll);
+  hipLaunchKernelGGL(k_build_list, dim3(s->nsuper), dim3(kListBlock), 0, st, s->lp,
+                     P<float4>(s->ctr), P<float4>(s->half), P<int4>(s->entries), P<int>(s->count),
+                     P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork, P<int>(s->overflow));
   if (!s->etab.empty())
     hipLaunchKernelGGL(k_mark_exclusions, dim3(nblk((long long)s->etab.size(), 256)), dim3(256), 0, st,
                        (int)s->etab.size(), P<int4>(s->etab_d), P<int>(s->slot), P<int4>(s->entries),
-                       P<int>(s->count), s->cap, P<int>(s->maskwork), P<int>(s->exclmax) + 2, s->maxmaskwork,
+                       P<int>(s->count), s->cap, P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork,
                        P<int>(s->overflow));
   hipLaunchKernelGGL(k_build_masks, dim3(s->ctx->cus * 4), dim3(256), 0, st, P<int>(s->maskwork),
-                     P<int>(s->exclmax) + 2, s->maxmaskwork, P<int4>(s->entries), s->cap, n, P<int>(s->permp),
+                     P<int>(s->nmaskwork), s->maxmaskwork, P<int4>(s->entries), s->cap, n, P<int>(s->permp),
                      P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned long long>(s->masks));
   hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int2>(s->tiles),
                      P<int>(s->ntiles));
-  if (s->md.on) HIPCHK(hipMemcpyAsync(s->md.xbuild.p, s->xorig.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
   if (timing) hipEventRecord(s->ev[2], st);
   return PMX_OK;
 }
@@ -700,29 +779,24 @@ int forces(Sys *s, unsigned flags, bool energies) {
     z.n[nr++] = G;
   }
   hipLaunchKernelGGL(k_zero, dim3(std::max(1u, nblk(3ll * npad, 256)), nr), dim3(256), 0, st, z);
-  int rc = s->mixed ? run_pair<true>(s, stats) : run_pair<false>(s, stats);
-  if (rc) return rc;
+  if (timing) hipEventRecord(s->ev[2], st);
+  if (s->mixed)
+    run_pair_phase<true>(s, stats, 0);
+  else
+    run_pair_phase<false>(s, stats, 0);
   if (timing) hipEventRecord(s->ev[3], st);
   if (s->pme_on) {
-    rc = s->mixed ? run_pme<true>(s, timing) : run_pme<false>(s, timing);
+    int rc = s->mixed ? run_pme<true>(s, timing) : run_pme<false>(s, timing);
     if (rc) return rc;
   } else if (timing) {
     hipEventRecord(s->ev[4], st);
     hipEventRecord(s->ev[5], st);
     hipEventRecord(s->ev[6], st);
   }
-  if (s->md.on && s->md.nb + s->md.na > 0) {
-    const int nt = s->md.nb + s->md.na;
-    if (s->mixed)
-      hipLaunchKernelGGL(k_bonded<true>, dim3(nblk(nt, 256)), dim3(256), 0, st, s->md.nb, s->md.na,
-                         P<int2>(s->md.bonds), P<double2>(s->md.bpar), P<int4>(s->md.angles), P<double2>(s->md.apar),
-                         P<double>(s->xorig), P<int>(s->slot), s->box, P<unsigned long long>(s->facc),
-                         P<double>(s->md.bond_e));
-    else
-      hipLaunchKernelGGL(k_bonded<false>, dim3(nblk(nt, 256)), dim3(256), 0, st, s->md.nb, s->md.na,
-                         P<int2>(s->md.bonds), P<double2>(s->md.bpar), P<int4>(s->md.angles), P<double2>(s->md.apar),
-                         P<double>(s->xorig), P<int>(s->slot), s->box, P<double>(s->facc), P<double>(s->md.bond_e));
-  }
+  if (s->mixed)
+    run_pair_phase<true>(s, stats, 1);
+  else
+    run_pair_phase<false>(s, stats, 1);
   if (timing) hipEventRecord(s->ev[7], st);
   if (s->mixed)
     hipLaunchKernelGGL(k_unsort_forces<true>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
@@ -731,7 +805,7 @@ int forces(Sys *s, unsigned flags, bool energies) {
     hipLaunchKernelGGL(k_unsort_forces<false>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
                        P<double>(s->facc), P<double>(s->fout));
   if (energies) reduce_energies(s, stats);
-  if (timing) hipEventRecord(s->ev[8], st);
+  if (timing) hipEventRecord(s->ev[kStages], st);
   HIPCHK(hipGetLastError());
   return PMX_OK;
 }
@@ -760,7 +834,7 @@ int finish(Sys *s, unsigned flags, double *f_aos, double *energies) {
   HIPCHK(hipStreamSynchronize(st));
   if (ovf) return fail(s->ctx, PMX_ERR_OVERFLOW, "cluster pair list capacity exceeded");
   if ((flags & PMX_EVAL_TIMING) && s->ev)
-    for (int k = 0; k < 8; ++k) hipEventElapsedTime(&s->stage_ms[k], s->ev[k], s->ev[k + 1]);
+    for (int k = 0; k < kStages; ++k) hipEventElapsedTime(&s->stage_ms[k], s->ev[k], s->ev[k + 1]);
   if (f_aos)
     for (size_t k = 0; k < f.size(); ++k) f_aos[k] += f[k];
   if (energies) {
@@ -858,12 +932,8 @@ int pmx_sys_destroy(pmx_sys *sys) {
   Sys *s = &sys->s;
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
-  std::set<void *> freed;
   for (DevBuf *b : s->all)
-    if (b->p && !freed.count(b->p)) {
-      freed.insert(b->p);
-      hipFree(b->p);
-    }
+    if (b->p) hipFree(b->p);
   for (auto &set : s->evsets)
     for (auto &e : set) hipEventDestroy(e);
   delete sys;
@@ -963,15 +1033,15 @@ int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls) {
   if (!sys || !avg_ms || !ncalls) return PMX_ERR_ARG;
   Sys *s = &sys->s;
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
-  for (int k = 0; k < 8; ++k) avg_ms[k] = 0.f;
+  for (int k = 0; k < kStages; ++k) avg_ms[k] = 0.f;
   const size_t m = s->evused;
   for (size_t c = 0; c < m; ++c)
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < kStages; ++k) {
       float t = 0.f;
       HIPCHK(hipEventElapsedTime(&t, s->evsets[c][k], s->evsets[c][k + 1]));
       avg_ms[k] += t;
     }
-  for (int k = 0; k < 8; ++k) avg_ms[k] = m ? avg_ms[k] / (float)m : 0.f;
+  for (int k = 0; k < kStages; ++k) avg_ms[k] = m ? avg_ms[k] / (float)m : 0.f;
   *ncalls = (int)m;
   s->evused = 0;
   s->timing_active = false;
@@ -980,7 +1050,7 @@ int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls) {
 
 int pmx_sys_stage_times(const pmx_sys *sys, float *ms) {
   if (!sys || !ms) return PMX_ERR_ARG;
-  for (int k = 0; k < 8; ++k) ms[k] = sys->s.stage_ms[k];
+  for (int k = 0; k < kStages; ++k) ms[k] = sys->s.stage_ms[k];
   return PMX_OK;
 }
 
@@ -1062,6 +1132,30 @@ int pmx_md_set_state(pmx_sys *sys, const double *x, const double *v) {
 
 namespace {
 
+// StandardIntegrator::initializeForces: forces at the current positions with
+// a freshly built list.  A list capacity overflow grows the buffers and
+// recomputes, so the forces the first step kicks with are complete.
+int md_ensure_forces(Sys *s) {
+  Sys::Md &m = s->md;
+  if (m.forces_valid) return PMX_OK;
+  hipStream_t st = s->ctx->stream;
+  for (int tries = 0;; ++tries) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
+    int rc = rebuild(s, false);
+    if (rc) return rc;
+    ++m.rebuilds;
+    if ((rc = forces(s, 0, false))) return rc;
+    int ovf = 0;
+    HIPCHK(hipMemcpyAsync(&ovf, s->overflow.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (!ovf) break;
+    if (tries >= 6) return fail(s->ctx, PMX_ERR_OVERFLOW, "pair list capacity exceeded");
+    if ((rc = grow_and_retry(s))) return rc;
+  }
+  m.forces_valid = true;
+  return PMX_OK;
+}
+
 int md_check_overflow(Sys *s) {
   int ovf = 0;
   HIPCHK(hipMemcpy(&ovf, s->overflow.p, 4, hipMemcpyDeviceToHost));
@@ -1110,14 +1204,8 @@ int pmx_md_run(pmx_sys *sys, int nsteps) {
   hipSetDevice(s->ctx->device);
   hipStream_t st = s->ctx->stream;
   const int n = s->n;
+  if (int rc = md_ensure_forces(s)) return rc;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
-  if (!m.forces_valid) {  // StandardIntegrator::initializeForces
-    int rc = rebuild(s, false);
-    if (rc) return rc;
-    ++m.rebuilds;
-    if ((rc = forces(s, 0, false))) return rc;
-    m.forces_valid = true;
-  }
   const float lim2 = (float)(0.25 * m.skin * m.skin);
   int overflow_retries = 0;
   int done = 0;
@@ -1189,14 +1277,16 @@ int pmx_md_get_state(pmx_sys *sys, double *x, double *v, double *energies) {
   if (!sys) return PMX_ERR_ARG;
   Sys *s = &sys->s;
   Sys::Md &m = s->md;
-  if (!m.on) return fail(s->ctx, PMX_ERR_ARG, "pmx_md_init first");
+  if (!m.on || !s->have_positions) return fail(s->ctx, PMX_ERR_ARG, "pmx_md_init and pmx_md_set_state first");
   hipSetDevice(s->ctx->device);
   hipStream_t st = s->ctx->stream;
   const int n = s->n;
   if (x) HIPCHK(hipMemcpyAsync(x, s->xorig.p, 24 * (size_t)n, hipMemcpyDeviceToHost, st));
   if (v) HIPCHK(hipMemcpyAsync(v, m.v.p, 24 * (size_t)n, hipMemcpyDeviceToHost, st));
   if (energies) {
-    // energies of the last force evaluation (positions x) and kinetic energy of v
+    // potential energies at the current positions (computed first if the
+    // state was just set) and kinetic energy of v
+    if (int rc = md_ensure_forces(s)) return rc;
     reduce_energies(s, false);
     hipLaunchKernelGGL(k_kinetic, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<double>(m.v), P<double>(m.minv),
                        P<double>(m.ke_e));

@@ -210,8 +210,9 @@ __global__ void k_scan_sums(uint32_t *sums, int nb) {
     __syncthreads();
   }
 }
+// Exclusive scan result to out and to out2 (the scatter cursors).
 __global__ void k_scan_final(const uint32_t *__restrict__ in, long long m, const uint32_t *__restrict__ sums,
-                             uint32_t *__restrict__ out) {
+                             uint32_t *__restrict__ out, uint32_t *__restrict__ out2) {
   __shared__ uint32_t s[kScanBlock];
   long long i = (long long)blockIdx.x * kScanBlock + threadIdx.x;
   uint32_t v = i < m ? in[i] : 0u;
@@ -223,7 +224,11 @@ __global__ void k_scan_final(const uint32_t *__restrict__ in, long long m, const
     s[threadIdx.x] += t;
     __syncthreads();
   }
-  if (i < m) out[i] = sums[blockIdx.x] + s[threadIdx.x] - v;
+  if (i < m) {
+    const uint32_t r = sums[blockIdx.x] + s[threadIdx.x] - v;
+    out[i] = r;
+    out2[i] = r;
+  }
 }
 
 __global__ void k_scatter(int n, const uint32_t *__restrict__ key, uint32_t *__restrict__ cursor,
@@ -249,16 +254,23 @@ __global__ void k_binsort(long long nbins, const uint32_t *__restrict__ start, i
 }
 
 // Sorted SoA: wrapped coordinates + charge, type, inverse permutation.
+// xbuild (MD only, else null) receives the positions the list is built from.
 template <class R>
 __global__ void k_fill_sorted(int n, int npad, const int *__restrict__ perm, const double *__restrict__ xwrap,
                               const double *__restrict__ q, const int *__restrict__ type,
                               typename Real4<R>::type *__restrict__ xs, double4 *__restrict__ xs64,
-                              int *__restrict__ ts, int *__restrict__ slot, int *__restrict__ permp) {
+                              int *__restrict__ ts, int *__restrict__ slot, int *__restrict__ permp,
+                              const double *__restrict__ xorig, double *__restrict__ xbuild) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= npad) return;
   typename Real4<R>::type v;
   if (s < n) {
     int a = perm[s];
+    if (xbuild) {
+      xbuild[3 * a] = xorig[3 * a];
+      xbuild[3 * a + 1] = xorig[3 * a + 1];
+      xbuild[3 * a + 2] = xorig[3 * a + 2];
+    }
     slot[a] = s;
     permp[s] = a;
     v.x = (R)xwrap[3 * a];
@@ -322,50 +334,55 @@ __device__ inline float bbox_dist2(float4 ca, float4 ha, float4 cb, float4 hb, c
   return d2;
 }
 
+// Image shift of a cluster pair: per axis s in {-1, 0, 1} with the j-center
+// + s L nearest the i-center, coded (sx+1) | (sy+1) << 2 | (sz+1) << 4.
+// bad is set when j-atoms shifted by s could be more than L/2 from an
+// i-atom on some axis, allowing each atom to move by up to skin/2 before the
+// next build (then the shift is not the minimum image of every pair and the
+// pair kernel takes the per-pair image instead).
+constexpr int kShiftPerPair = 64;
+__device__ inline int entry_shift(float4 ci, float4 hi, float4 cj, float4 hj, const float *L, const float *Linv,
+                                  float skin, bool &bad) {
+  const float dc[3] = {cj.x - ci.x, cj.y - ci.y, cj.z - ci.z};
+  const float hs[3] = {hi.x + hj.x, hi.y + hj.y, hi.z + hj.z};
+  int code = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float s = -rintf(dc[k] * Linv[k]);
+    const float d = dc[k] + s * L[k];
+    bad |= fabsf(d) + hs[k] + skin >= 0.5f * L[k] * 0.999f;
+    code |= ((int)s + 1) << (2 * k);
+  }
+  return code;
+}
+
 struct ListParams {
   float L[3];
+  float Linv[3];
+  float skin;        // MD pair-list buffer (0 outside MD)
   float rlist2;      // (rc_max + margin)^2
-  float rexcl2;      // (max excluded-pair distance + margin)^2: closer cluster pairs get explicit masks
   int ncl;           // clusters
   int nsuper;
   int cap;           // entries per supercluster
   int n;             // real atoms
 };
 
-__global__ void k_excl_maxdist(int ne, const int4 *__restrict__ etab, const double *__restrict__ xwrap, Box box,
-                               unsigned int *__restrict__ out) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  int4 t = etab[e];
-  double d2 = 0;
-  for (int k = 0; k < 3; ++k) {
-    double d = xwrap[3 * t.y + k] - xwrap[3 * t.x + k];
-    d -= box.L[k] * rint(d * box.Linv[k]);
-    d2 += d * d;
-  }
-  float r = (float)sqrt(d2);
-  atomicMax(out, __float_as_uint(r));
-}
-
-__global__ void k_set_rexcl(const unsigned int *__restrict__ in, float *__restrict__ out2) {
-  float r = __uint_as_float(*in) * 1.0001f + 1e-3f;
-  *out2 = r * r;
-}
-
-// The list kernel takes rexcl2 from device memory (computed each step).
-__global__ void __launch_bounds__(256) k_build_list(ListParams p, const float *__restrict__ rexcl2,
-                                                        const float4 *__restrict__ ctr,
-                                                        const float4 *__restrict__ half, int4 *__restrict__ entries,
-                                                        int *__restrict__ count, int *__restrict__ maskwork,
-                                                        int *__restrict__ nmaskwork, int maxmaskwork,
-                                                        int *__restrict__ overflow) {
-  // forward to the list builder with the device-resident exclusion radius
-  ListParams q = p;
-  q.rexcl2 = *rexcl2;
-  // body identical to k_build_list (inlined copy keeps one launch)
-  const int lane = threadIdx.x & 63;
-  const int S = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6);
-  if (S >= q.nsuper) return;
+// One block of kListBlock threads per supercluster S: scans the j-clusters
+// cj >= 8S (kListBlock per iteration, one per thread) and appends an entry for
+// every cj whose bounding box is within the list radius of at least one
+// i-cluster of S.  A block-wide prefix count keeps the entries sorted by cj.
+// Diagonal entries and entries with a partial cluster get a mask work slot
+// (explicit pair masks).
+constexpr int kListBlock = 256;
+__global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const float4 *__restrict__ ctr,
+                                                           const float4 *__restrict__ half,
+                                                           int4 *__restrict__ entries, int *__restrict__ count,
+                                                           int *__restrict__ maskwork, int *__restrict__ nmaskwork,
+                                                           int maxmaskwork, int *__restrict__ overflow) {
+  constexpr int NW = kListBlock / kWave;
+  __shared__ int s_cnt[NW];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int S = blockIdx.x;
   float4 ic[8], ih[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
@@ -373,11 +390,13 @@ __global__ void __launch_bounds__(256) k_build_list(ListParams p, const float *_
     ic[c] = ci < q.ncl ? ctr[ci] : make_float4(0, 0, 0, 0);
     ih[c] = ci < q.ncl ? half[ci] : make_float4(-1, -1, -1, 0);
   }
-  int total = 0;
-  for (int base = S * 8; base < q.ncl; base += kWave) {
-    int cj = base + lane;
+  int total = 0;  // block-uniform
+  for (int base = S * 8; base < q.ncl; base += kListBlock) {
+    const int cj = base + threadIdx.x;
     uint32_t imask = 0;
     bool need = false;
+    int shift = -1;        // image shift code of the entry (see entry_shift)
+    bool per_pair = false; // no single shift is the minimum image for every pair
     if (cj < q.ncl) {
       float4 jc = ctr[cj], jh = half[cj];
       if (jh.x >= 0.f) {
@@ -391,18 +410,38 @@ __global__ void __launch_bounds__(256) k_build_list(ListParams p, const float *_
             imask |= 1u << c;
             bool ipartial = (ci + 1) * kClusterSize > q.n;
             if (ci == cj || ipartial || jpartial) need = true;
+            const int code = entry_shift(ic[c], ih[c], jc, jh, q.L, q.Linv, q.skin, per_pair);
+            if (shift < 0) shift = code;
+            per_pair |= code != shift;
           }
         }
       }
     }
-    unsigned long long act = __ballot(imask != 0);
-    int rank = __popcll(act & ((1ull << lane) - 1ull));
+    const unsigned long long act = __ballot(imask != 0);
+    if (lane == 0) s_cnt[wv] = __popcll(act);
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const int c = s_cnt[w];
+      before += w < wv ? c : 0;
+      all += c;
+    }
+    __syncthreads();
+    const int e = total + before + __popcll(act & ((1ull << lane) - 1ull));
+    // mask work slots for this wave's entries that need explicit masks, one
+    // atomic per wave
+    const unsigned long long nw = __ballot(imask != 0 && need && e < q.cap);
+    int wbase = 0;
+    if (nw) {
+      if (lane == 0) wbase = atomicAdd(nmaskwork, (int)__popcll(nw));
+      wbase = __builtin_amdgcn_readfirstlane(__shfl(wbase, 0));
+    }
     if (imask != 0) {
-      int e = total + rank;
       if (e < q.cap) {
         int moff = -1;
         if (need) {
-          int w = atomicAdd(nmaskwork, 1);
+          const int w = wbase + __popcll(nw & ((1ull << lane) - 1ull));
           if (w < maxmaskwork) {
             maskwork[w] = S * q.cap + e;
             moff = -2;
@@ -410,14 +449,14 @@ __global__ void __launch_bounds__(256) k_build_list(ListParams p, const float *_
             atomicExch(overflow, 1);
           }
         }
-        entries[(size_t)S * q.cap + e] = make_int4(cj, (int)imask, moff, 0);
+        entries[(size_t)S * q.cap + e] = make_int4(cj, (int)imask, moff, per_pair ? kShiftPerPair : shift);
       } else {
         atomicExch(overflow, 1);
       }
     }
-    total += __popcll(act);
+    total += all;
   }
-  if (lane == 0) count[S] = total < q.cap ? total : q.cap;
+  if (threadIdx.x == 0) count[S] = total < q.cap ? total : q.cap;
 }
 
 // Flags the list entry that holds each excluded pair (FULL or MODIFIED) so
@@ -429,33 +468,44 @@ __global__ void k_mark_exclusions(int ne, const int4 *__restrict__ etab, const i
                                   int *__restrict__ maskwork, int *__restrict__ nmaskwork, int maxmaskwork,
                                   int *__restrict__ overflow) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  const int4 t = etab[e];
-  const int c1 = slot[t.x] / kClusterSize, c2 = slot[t.y] / kClusterSize;
-  if (c1 == c2) return;  // diagonal entries always carry masks
-  const int ci = c1 < c2 ? c1 : c2, cj = c1 < c2 ? c2 : c1;
-  const int S = ci / 8;
-  int lo = 0, hi = count[S] - 1;
-  while (lo <= hi) {
-    const int mid = (lo + hi) >> 1;
-    const int v = entries[(size_t)S * cap + mid].x;
-    if (v == cj) {
-      int *z = &entries[(size_t)S * cap + mid].z;
-      if (atomicCAS(z, -1, -3) == -1) {
-        const int w = atomicAdd(nmaskwork, 1);
-        if (w < maxmaskwork) {
-          maskwork[w] = S * cap + mid;
-          atomicExch(z, -2);
-        } else {
-          atomicExch(overflow, 1);
+  const int lane = threadIdx.x & 63;
+  int hit = -1;  // entry index (S * cap + k) this thread claimed
+  if (e < ne) {
+    const int4 t = etab[e];
+    const int c1 = slot[t.x] / kClusterSize, c2 = slot[t.y] / kClusterSize;
+    if (c1 != c2) {  // diagonal entries always carry masks
+      const int ci = c1 < c2 ? c1 : c2, cj = c1 < c2 ? c2 : c1;
+      const int S = ci / 8;
+      int lo = 0, hi = count[S] - 1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = entries[(size_t)S * cap + mid].x;
+        if (v == cj) {
+          if (atomicCAS(&entries[(size_t)S * cap + mid].z, -1, -3) == -1) hit = S * cap + mid;
+          break;
         }
+        if (v < cj)
+          lo = mid + 1;
+        else
+          hi = mid - 1;
       }
-      return;
     }
-    if (v < cj)
-      lo = mid + 1;
-    else
-      hi = mid - 1;
+  }
+  // one mask work slot per claimed entry, one counter atomic per wave
+  const unsigned long long got = __ballot(hit >= 0);
+  int wbase = 0;
+  if (got) {
+    if (lane == 0) wbase = atomicAdd(nmaskwork, (int)__popcll(got));
+    wbase = __builtin_amdgcn_readfirstlane(__shfl(wbase, 0));
+  }
+  if (hit >= 0) {
+    const int w = wbase + __popcll(got & ((1ull << lane) - 1ull));
+    if (w < maxmaskwork) {
+      maskwork[w] = hit;
+      atomicExch(&entries[hit].z, -2);
+    } else {
+      atomicExch(overflow, 1);
+    }
   }
 }
 
@@ -717,6 +767,47 @@ struct PairOut {
   int *overflow;
 };
 
+// Queue entry of the compacted pair kernel: one in-cutoff pair awaiting its
+// force.  info packs the LJ type pair (bits 0..7), MODIFIED (bit 8), in LJ
+// range (bit 9), in Coulomb range (bit 10) and its force slot (bits 16..26).
+template <class R>
+struct PairQ {
+  R r2, qq;
+  int info, pad;
+};
+
+constexpr int kGroup = 2;    // list entries whose pairs share one queue flush
+constexpr int kQueue = 128;  // queue ring per wave (holds < 64 + 64 pairs)
+
+// Minimum-image difference j - i in the kernel's arithmetic type: xj is
+// already moved to the image of its list entry, unless the entry needs the
+// per-pair image (per_pair, wave-uniform).  Phase A and phase C of k_pair
+// call it with the same operands.
+template <class R, class R4>
+__device__ inline void pair_delta(const R4 &xi, const R4 &xj, R Lx, R Ly, R Lz, R Lix, R Liy, R Liz, int per_pair,
+                                  R &dx, R &dy, R &dz) {
+  dx = xj.x - xi.x;
+  dy = xj.y - xi.y;
+  dz = xj.z - xi.z;
+  if (per_pair) {
+    dx -= Lx * fast_rint(dx * Lix);
+    dy -= Ly * fast_rint(dy * Liy);
+    dz -= Lz * fast_rint(dz * Liz);
+  }
+}
+
+// Cluster-pair kernel with compaction.  For a group of kGroup list entries:
+//   A. every candidate lane (i = 8c+ti of the supercluster, j = lane tj of
+//      the entry's j-cluster) computes r^2 and its membership; members are
+//      appended to a per-wave LDS queue (ballot + mbcnt), and every full
+//      64-pair batch of the queue is evaluated at once, all lanes busy;
+//   B. the last partial batch is evaluated; each pair's scalar force lands in
+//      its slot of an LDS array (zero for non-members);
+//   C. the candidate lanes read their slot and accumulate F = f d into the
+//      i-forces (registers, whole tile) and j-forces (summed over the 8
+//      i-lanes with DPP, one atomic add per j-atom component and entry).
+// Only phase B does the expensive math (rsqrt, exp, erfc polynomial,
+// switches), on members only, which are about a quarter of the candidates.
 template <bool MIXED, int COUL, int LJSW, int CSW>
 __global__ void __launch_bounds__(kPairBlock) k_pair(
     PairParams pp, int stats, const int2 *__restrict__ tiles, const int *__restrict__ ntiles_p,
@@ -728,8 +819,12 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   using R = typename std::conditional<MIXED, float, double>::type;
   using R4 = typename Real4<R>::type;
   using Acc = AccT<MIXED>;
-  __shared__ R4 s_i[kPairBlock / kWave][kSuperSize];
+  constexpr int W = kPairBlock / kWave;
+  __shared__ R4 s_i[W][kSuperSize];
+  __shared__ int s_ty[W][kSuperSize];
   __shared__ R4 s_lj[kMaxTypes * kMaxTypes];  // {A, B, A14, B14}
+  __shared__ PairQ<R> s_q[W][kQueue];
+  __shared__ R s_fr[W][kGroup * 8 * kWave];
 
   const int nt = pp.ntypes;
   for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) {
@@ -753,21 +848,24 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   const R guard = (R)pp.guard;
   const R alpha = (R)pp.alpha, a2pi = (R)pp.a2pi, csf = (R)pp.csf;
   const int ntiles = *ntiles_p;
+  R4 *const si = s_i[wv];
+  PairQ<R> *const q = s_q[wv];
+  R *const fr_slot = s_fr[wv];
 
+  // The first tile of each wave is its global wave index (the grid holds
+  // the waves that fit on the device at once); later tiles come from the
+  // shared counter.
+  const int nwaves = gridDim.x * W;
+  int t = blockIdx.x * W + wv;
   while (true) {
-    int t = 0;
-    if (lane == 0) t = atomicAdd(tile_counter, 1);
-    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
     if (t >= ntiles) break;
     const int2 tile = tiles[t];
     const int S = __builtin_amdgcn_readfirstlane(tile.x);
     const int e0 = __builtin_amdgcn_readfirstlane(tile.y);
-    s_i[wv][lane] = xs[S * kSuperSize + lane];
-    int tyi[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int v = ts[S * kSuperSize + c * 8 + ti];
-      tyi[c] = v < 0 ? 0 : v;
+    si[lane] = xs[S * kSuperSize + lane];
+    {
+      const int v = ts[S * kSuperSize + lane];
+      s_ty[wv][lane] = v < 0 ? 0 : v;
     }
     __builtin_amdgcn_wave_barrier();
 
@@ -777,34 +875,83 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
     double elj = 0.0, eco = 0.0;
     int npair = 0;
     unsigned long long hsum = 0;
+    int qhead = 0, qtail = 0;  // wave-uniform queue positions
+
+    // Evaluates queue pairs [qhead, qhead + cnt) with lane l on pair qhead + l.
+    auto eval_batch = [&](int cnt, R &qel, R &qec) {
+      __builtin_amdgcn_wave_barrier();
+      if (lane < cnt) {
+        const PairQ<R> e = q[(qhead + lane) & (kQueue - 1)];
+        const int info = e.info;
+        const bool mod = (info >> 8) & 1, il = (info >> 9) & 1, ic = (info >> 10) & 1;
+        R A = 0, B = 0;
+        if (LJSW != SW_NONE) {
+          const R4 lj = s_lj[info & 0xff];
+          A = mod ? lj.z : lj.x;
+          B = mod ? lj.w : lj.y;
+        }
+        R e1, e2, f;
+        pair_eval<COUL, LJSW, CSW, R>(pp, sl, sc, alpha, a2pi, csf, e.r2, e.qq, A, B, il, ic, e1, e2, f);
+        qel += e1;
+        qec += e2;
+        fr_slot[info >> 16] = f;
+      }
+      qhead += cnt;
+    };
 
     const int eend = __builtin_amdgcn_readfirstlane(min(e0 + kTileEntries, count[S]));
-    for (int e = e0; e < eend; ++e) {
-      const int4 ent = entries[(size_t)S * cap + e];
-      const int cj = __builtin_amdgcn_readfirstlane(ent.x);
-      const int emask = __builtin_amdgcn_readfirstlane(ent.y);
-      const int moff = __builtin_amdgcn_readfirstlane(ent.z);
-      const int sj = cj * kClusterSize + tj;
-      const R4 xj = xs[sj];
-      const int tyj0 = ts[sj];
-      const int tyj = tyj0 < 0 ? 0 : tyj0;
-      R fj0 = 0, fj1 = 0, fj2 = 0;
-      R el = 0, ec = 0;
+    for (int eg = e0; eg < eend; eg += kGroup) {
+      const int ng = eend - eg < kGroup ? eend - eg : kGroup;
+      int cjv[kGroup], emv[kGroup], mov[kGroup], shv[kGroup], tyj0[kGroup];
+      R4 xj[kGroup];
+      unsigned gmask = 0;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        if (!(emask & (1 << c))) continue;
+      for (int g = 0; g < kGroup; ++g) {
+        int4 ent = make_int4(0, 0, -1, 0);
+        if (g < ng) ent = entries[(size_t)S * cap + eg + g];
+        cjv[g] = __builtin_amdgcn_readfirstlane(ent.x);
+        emv[g] = __builtin_amdgcn_readfirstlane(ent.y) & 0xff;
+        mov[g] = __builtin_amdgcn_readfirstlane(ent.z);
+        shv[g] = __builtin_amdgcn_readfirstlane(ent.w);
+        const int sj = cjv[g] * kClusterSize + tj;
+        xj[g] = xs[sj];
+        if (!(shv[g] & kShiftPerPair)) {  // j-cluster moved to the image of the entry
+          xj[g].x += Lx * (R)((shv[g] & 3) - 1);
+          xj[g].y += Ly * (R)(((shv[g] >> 2) & 3) - 1);
+          xj[g].z += Lz * (R)(((shv[g] >> 4) & 3) - 1);
+        }
+        tyj0[g] = ts[sj];
+        gmask |= (unsigned)emv[g] << (8 * g);
+      }
+
+      // ---- phase A: membership, queue, full batches
+      R qel = 0, qec = 0;
+      unsigned mrem = gmask;
+      while (mrem) {
+        const int gc = __builtin_ctz(mrem);
+        mrem &= mrem - 1;
+        const int g = gc >> 3, c = gc & 7;
+        R4 xjg = xj[0];
+        int tyjg = tyj0[0], mog = mov[0], cjg = cjv[0], shg = shv[0];
+#pragma unroll
+        for (int h = 1; h < kGroup; ++h)
+          if (g == h) {
+            xjg = xj[h];
+            tyjg = tyj0[h];
+            mog = mov[h];
+            cjg = cjv[h];
+            shg = shv[h];
+          }
         bool cand = true, modified = false;
-        if (moff >= 0) {
-          const unsigned long long m = masks[2 * ((size_t)moff + c)];
-          const unsigned long long mm = masks[2 * ((size_t)moff + c) + 1];
+        if (mog >= 0) {
+          const unsigned long long m = masks[2 * ((size_t)mog + c)];
+          const unsigned long long mm = masks[2 * ((size_t)mog + c) + 1];
           cand = (m >> lane) & 1ull;
           modified = (mm >> lane) & 1ull;
         }
-        const R4 xi = s_i[wv][c * 8 + ti];
-        R dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        dx -= Lx * fast_rint(dx * Lix);
-        dy -= Ly * fast_rint(dy * Liy);
-        dz -= Lz * fast_rint(dz * Liz);
+        const R4 xi = si[c * 8 + ti];
+        R dx, dy, dz;
+        pair_delta<R, R4>(xi, xjg, Lx, Ly, Lz, Lix, Liy, Liz, shg & kShiftPerPair, dx, dy, dz);
         const R r2 = dx * dx + dy * dy + dz * dz;
         bool border = false;
         if (LJSW != SW_NONE) border |= fabs(r2 - sl.rc2) <= guard;
@@ -813,6 +960,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
         const bool ok = cand && !border;
         const bool in_lj = LJSW != SW_NONE && ok && r2 <= sl.rc2;
         const bool in_c = COUL != COUL_NONE && ok && r2 <= sc.rc2;
+        const int sj = cjg * kClusterSize + tj;
         if (border) {
           const int b = atomicAdd(out.nborder, 1);
           if (b < out.maxborder)
@@ -820,40 +968,68 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
           else
             atomicExch(out.overflow, 1);
         }
-        if (stats && (in_lj || in_c)) {
+        const bool in = in_lj || in_c;
+        if (stats && in) {
           const int ai = perm[S * kSuperSize + c * 8 + ti], aj = perm[sj];
           ++npair;
           hsum += pair_hash(ai < aj ? ai : aj, ai < aj ? aj : ai);
         }
-        R A = 0, B = 0;
-        if (LJSW != SW_NONE) {
-          const R4 lj = s_lj[tyi[c] * nt + tyj];
-          A = modified ? lj.z : lj.x;
-          B = modified ? lj.w : lj.y;
+        const int slot = gc * kWave + lane;
+        fr_slot[slot] = (R)0;
+        const unsigned long long bm = __ballot(in);
+        if (bm) {
+          if (in) {
+            const unsigned below =
+                __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
+            PairQ<R> e;
+            e.r2 = r2;
+            e.qq = modified ? xi.w * xjg.w * csf : xi.w * xjg.w;
+            e.info = (s_ty[wv][c * 8 + ti] * nt + (tyjg < 0 ? 0 : tyjg)) | ((int)modified << 8) |
+                     ((int)in_lj << 9) | ((int)in_c << 10) | (slot << 16);
+            e.pad = 0;
+            q[(qtail + (int)below) & (kQueue - 1)] = e;
+          }
+          qtail += __popcll(bm);
+          while (qtail - qhead >= kWave) eval_batch(kWave, qel, qec);
         }
-        R qq = xi.w * xj.w;
-        if (modified) qq *= csf;
-        R e1, e2, fr;
-        pair_eval<COUL, LJSW, CSW, R>(pp, sl, sc, alpha, a2pi, csf, r2, qq, A, B, in_lj, in_c, e1, e2, fr);
-        el += e1;
-        ec += e2;
-        const R fx = dx * fr, fy = dy * fr, fz = dz * fr;
-        fi[c][0] -= fx;
-        fi[c][1] -= fy;
-        fi[c][2] -= fz;
-        fj0 += fx;
-        fj1 += fy;
-        fj2 += fz;
       }
-      fj0 = group8_sum(fj0);
-      fj1 = group8_sum(fj1);
-      fj2 = group8_sum(fj2);
-      if (ti < 3 && tyj0 >= 0) {
-        const R v = ti == 0 ? fj0 : (ti == 1 ? fj1 : fj2);
-        Acc::add(&facc[3 * sj + ti], v);
+      // ---- phase B: the rest of the queue
+      if (qtail > qhead) eval_batch(qtail - qhead, qel, qec);
+      elj += (double)qel;
+      eco += (double)qec;
+      __builtin_amdgcn_wave_barrier();
+
+      // ---- phase C: forces from the slots
+#pragma unroll
+      for (int g = 0; g < kGroup; ++g) {
+        if (g >= ng) break;
+        const int em = emv[g];
+        const int sj = cjv[g] * kClusterSize + tj;
+        R fj0 = 0, fj1 = 0, fj2 = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          if (!(em & (1 << c))) continue;
+          const R4 xi = si[c * 8 + ti];
+          R dx, dy, dz;
+          pair_delta<R, R4>(xi, xj[g], Lx, Ly, Lz, Lix, Liy, Liz, shv[g] & kShiftPerPair, dx, dy, dz);
+          const R f = fr_slot[(g * 8 + c) * kWave + lane];
+          const R fx = dx * f, fy = dy * f, fz = dz * f;
+          fi[c][0] -= fx;
+          fi[c][1] -= fy;
+          fi[c][2] -= fz;
+          fj0 += fx;
+          fj1 += fy;
+          fj2 += fz;
+        }
+        fj0 = group8_sum(fj0);
+        fj1 = group8_sum(fj1);
+        fj2 = group8_sum(fj2);
+        if (ti < 3 && tyj0[g] >= 0) {
+          const R v = ti == 0 ? fj0 : (ti == 1 ? fj1 : fj2);
+          Acc::add(&facc[3 * sj + ti], v);
+        }
       }
-      elj += (double)el;
-      eco += (double)ec;
+      __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
@@ -869,11 +1045,11 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
         my1 = fi[c][1];
         my2 = fi[c][2];
       }
-    const int si = S * kSuperSize + tj * 8 + ti;
-    if (ts[si] >= 0) {
-      Acc::add(&facc[3 * si], my0);
-      Acc::add(&facc[3 * si + 1], my1);
-      Acc::add(&facc[3 * si + 2], my2);
+    const int sidx = S * kSuperSize + tj * 8 + ti;
+    if (ts[sidx] >= 0) {
+      Acc::add(&facc[3 * sidx], my0);
+      Acc::add(&facc[3 * sidx + 1], my1);
+      Acc::add(&facc[3 * sidx + 2], my2);
     }
     elj = wave_sum(elj);
     eco = wave_sum(eco);
@@ -891,81 +1067,70 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
       }
     }
     __builtin_amdgcn_wave_barrier();
+    int tn = 0;
+    if (lane == 0) tn = nwaves + atomicAdd(tile_counter, 1);
+    t = __builtin_amdgcn_readfirstlane(__shfl(tn, 0));
   }
 }
 
 // Boundary pairs: exact FP64 membership on the original coordinates
 // (reference operation order), then the same pair evaluation.  Energies go
 // to int64 fixed point (2^-32) so the total is independent of record order.
+// Blocks [0, nblocks) of the fixup launch walk the records with a grid stride.
 template <bool MIXED, int COUL, int LJSW, int CSW>
-__global__ void k_border(PairParams pp, int stats, const int2 *__restrict__ border, const int *__restrict__ nborder,
-                         int maxborder, const unsigned long long *__restrict__ masks_unused,
-                         const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
-                         const int *__restrict__ ts, const int *__restrict__ perm, const double *__restrict__ xorig,
-                         const double *__restrict__ ljtab, const int *__restrict__ eptr, const int2 *__restrict__ elist,
-                         typename AccT<MIXED>::type *__restrict__ facc, unsigned long long *__restrict__ eacc,
-                         unsigned long long *__restrict__ stat) {
+__device__ void border_pairs(int blk, int nblocks, const PairParams &pp, int stats, const int2 *__restrict__ border,
+                             const int *__restrict__ nborder, int maxborder,
+                             const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
+                             const int *__restrict__ ts, const int *__restrict__ perm, const double *__restrict__ xorig,
+                             const double *__restrict__ ljtab, const int *__restrict__ eptr,
+                             const int2 *__restrict__ elist, typename AccT<MIXED>::type *__restrict__ facc,
+                             unsigned long long *__restrict__ eacc, unsigned long long *__restrict__ stat) {
   using R = typename std::conditional<MIXED, float, double>::type;
   const int nb = min(*nborder, maxborder);
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  const int2 rec = border[b];
-  const int si = rec.x, sj = rec.y;
-  const int ai = perm[si], aj = perm[sj];
-  const int lo = ai < aj ? ai : aj, hi = ai < aj ? aj : ai;
-  const double d2 = exact_d2(xorig, lo, hi, pp.box);
-  const bool in_lj = LJSW != SW_NONE && d2 <= pp.lj.rc2;
-  const bool in_c = COUL != COUL_NONE && d2 <= pp.co.rc2;
-  if (!(in_lj || in_c)) return;
-  // exclusion class (recorded pairs already passed the FULL mask)
-  int cls = 0;
-  {
-    int l = eptr[ai], h = eptr[ai + 1] - 1;
-    while (l <= h) {
-      const int mid = (l + h) >> 1;
-      const int v = elist[mid].x;
-      if (v == aj) {
-        cls = elist[mid].y;
-        break;
-      }
-      if (v < aj)
-        l = mid + 1;
-      else
-        h = mid - 1;
+  for (int b = blk * blockDim.x + threadIdx.x; b < nb; b += nblocks * blockDim.x) {
+    const int2 rec = border[b];
+    const int si = rec.x, sj = rec.y;
+    const int ai = perm[si], aj = perm[sj];
+    const int lo = ai < aj ? ai : aj, hi = ai < aj ? aj : ai;
+    const double d2 = exact_d2(xorig, lo, hi, pp.box);
+    const bool in_lj = LJSW != SW_NONE && d2 <= pp.lj.rc2;
+    const bool in_c = COUL != COUL_NONE && d2 <= pp.co.rc2;
+    if (!(in_lj || in_c)) continue;
+    // exclusion class (recorded pairs already passed the FULL mask)
+    const int cls = excl_class(eptr, elist, ai, aj);
+    const bool modified = cls == 2;
+    SwR<R> sl, sc;
+    sl.load(pp.lj);
+    sc.load(pp.co);
+    const auto xi = xs[si], xj = xs[sj];
+    R dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+    dx -= (R)pp.box.L[0] * fast_rint(dx * (R)pp.box.Linv[0]);
+    dy -= (R)pp.box.L[1] * fast_rint(dy * (R)pp.box.Linv[1]);
+    dz -= (R)pp.box.L[2] * fast_rint(dz * (R)pp.box.Linv[2]);
+    R r2 = dx * dx + dy * dy + dz * dz;
+    // evaluate at the cutoff side the exact test chose
+    R A = 0, B = 0;
+    if (LJSW != SW_NONE) {
+      const int k = 4 * (ts[si] * pp.ntypes + ts[sj]);
+      A = (R)ljtab[k + (modified ? 2 : 0)];
+      B = (R)ljtab[k + (modified ? 3 : 1)];
     }
-  }
-  const bool modified = cls == 2;
-  SwR<R> sl, sc;
-  sl.load(pp.lj);
-  sc.load(pp.co);
-  const auto xi = xs[si], xj = xs[sj];
-  R dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-  dx -= (R)pp.box.L[0] * fast_rint(dx * (R)pp.box.Linv[0]);
-  dy -= (R)pp.box.L[1] * fast_rint(dy * (R)pp.box.Linv[1]);
-  dz -= (R)pp.box.L[2] * fast_rint(dz * (R)pp.box.Linv[2]);
-  R r2 = dx * dx + dy * dy + dz * dz;
-  // evaluate at the cutoff side the exact test chose
-  R A = 0, B = 0;
-  if (LJSW != SW_NONE) {
-    const int k = 4 * (ts[si] * pp.ntypes + ts[sj]);
-    A = (R)ljtab[k + (modified ? 2 : 0)];
-    B = (R)ljtab[k + (modified ? 3 : 1)];
-  }
-  R qq = xi.w * xj.w;
-  if (modified) qq *= (R)pp.csf;
-  R e1, e2, fr;
-  pair_eval<COUL, LJSW, CSW, R>(pp, sl, sc, (R)pp.alpha, (R)pp.a2pi, (R)pp.csf, r2, qq, A, B, in_lj, in_c, e1, e2, fr);
-  AccT<MIXED>::add(&facc[3 * si], -dx * fr);
-  AccT<MIXED>::add(&facc[3 * si + 1], -dy * fr);
-  AccT<MIXED>::add(&facc[3 * si + 2], -dz * fr);
-  AccT<MIXED>::add(&facc[3 * sj], dx * fr);
-  AccT<MIXED>::add(&facc[3 * sj + 1], dy * fr);
-  AccT<MIXED>::add(&facc[3 * sj + 2], dz * fr);
-  atomicAdd(&eacc[0], (unsigned long long)__double2ll_rn((double)e2 * kForceScale));
-  atomicAdd(&eacc[1], (unsigned long long)__double2ll_rn((double)e1 * kForceScale));
-  if (stats) {
-    atomicAdd(&stat[0], 1ull);
-    atomicAdd(&stat[1], pair_hash(lo, hi));
+    R qq = xi.w * xj.w;
+    if (modified) qq *= (R)pp.csf;
+    R e1, e2, fr;
+    pair_eval<COUL, LJSW, CSW, R>(pp, sl, sc, (R)pp.alpha, (R)pp.a2pi, (R)pp.csf, r2, qq, A, B, in_lj, in_c, e1, e2, fr);
+    AccT<MIXED>::add(&facc[3 * si], -dx * fr);
+    AccT<MIXED>::add(&facc[3 * si + 1], -dy * fr);
+    AccT<MIXED>::add(&facc[3 * si + 2], -dz * fr);
+    AccT<MIXED>::add(&facc[3 * sj], dx * fr);
+    AccT<MIXED>::add(&facc[3 * sj + 1], dy * fr);
+    AccT<MIXED>::add(&facc[3 * sj + 2], dz * fr);
+    atomicAdd(&eacc[0], (unsigned long long)__double2ll_rn((double)e2 * kForceScale));
+    atomicAdd(&eacc[1], (unsigned long long)__double2ll_rn((double)e1 * kForceScale));
+    if (stats) {
+      atomicAdd(&stat[0], 1ull);
+      atomicAdd(&stat[1], pair_hash(lo, hi));
+    }
   }
 }
 
@@ -979,25 +1144,54 @@ struct PmeParams {
   int offx, offy, offz;   // -(p-1)/2 + K (Grid.h:509-511)
 };
 
-// BSpline::set (mdl/interface/force/nonbonded/BSpline.cpp:55-105), order <= 8.
-template <class R>
-__device__ inline void bspline_w(int order, R w, R *th, R *dth) {
-  th[order - 1] = 0;
+// BSpline::set (mdl/interface/force/nonbonded/BSpline.cpp:55-105) for a
+// compile-time order P (2..8): weights th[0..P-1] and derivatives dth.
+template <int P, class R>
+__device__ inline void bspline_w(R w, R *th, R *dth) {
+  th[P - 1] = 0;
   th[1] = w;
   th[0] = (R)1 - w;
-  for (int k = 2; k < order - 1; k++) {
-    R div = (R)1 / (R)k;
+#pragma unroll
+  for (int k = 2; k < P - 1; k++) {
+    const R div = (R)1 / (R)k;
     th[k] = div * w * th[k - 1];
+#pragma unroll
     for (int j = 1; j < k; j++) th[k - j] = div * ((w + (R)j) * th[k - j - 1] + ((R)(k + 1 - j) - w) * th[k - j]);
     th[0] = div * ((R)1 - w) * th[0];
   }
   dth[0] = -th[0];
-  for (int j = 1; j <= order - 1; j++) dth[j] = th[j - 1] - th[j];
-  R div = (R)1 / (R)(order - 1);
-  th[order - 1] = div * w * th[order - 2];
-  for (int j = 1; j < order - 1; j++)
-    th[order - 1 - j] = div * ((w + (R)j) * th[order - j - 2] + ((R)(order - j) - w) * th[order - 1 - j]);
+#pragma unroll
+  for (int j = 1; j <= P - 1; j++) dth[j] = th[j - 1] - th[j];
+  const R div = (R)1 / (R)(P - 1);
+  th[P - 1] = div * w * th[P - 2];
+#pragma unroll
+  for (int j = 1; j < P - 1; j++)
+    th[P - 1 - j] = div * ((w + (R)j) * th[P - j - 2] + ((R)(P - j) - w) * th[P - 1 - j]);
   th[0] = div * ((R)1 - w) * th[0];
+}
+
+// Element m of a register array as a sum of 0/1-weighted terms.  A select
+// chain would be folded into a dynamically indexed load, which moves the
+// array out of registers into LDS or scratch.
+template <int P, class R>
+__device__ inline R pick(const R *v, int m) {
+  R r = (R)0;
+#pragma unroll
+  for (int t = 0; t < P; ++t) r += (R)(m == t) * v[t];
+  return r;
+}
+
+// Threads per atom in spreading and gathering: one per (i, j) stencil row,
+// rounded up to a power of two (order 2: 4, order 4: 16, orders 6 and 8: 64).
+__host__ __device__ constexpr int pme_tpa(int order) {
+  return order * order <= 4 ? 4 : order * order <= 16 ? 16 : 64;
+}
+
+// Grid index (base + offset) mod K for base + offset in [0, 3K).
+__device__ inline int wrap_grid(int v, int K) {
+  v -= v >= K ? K : 0;
+  v -= v >= K ? K : 0;
+  return v;
 }
 
 // Grid coordinate of one axis: u = x K/L wrapped to [0, K) (Grid.h:111-120).
@@ -1010,41 +1204,47 @@ __device__ inline void grid_coord(double x, double hr, int K, int off, int &base
   w = u - iu;
 }
 
-template <bool MIXED>
-__global__ void k_spread(int n, PmeParams p, const double *__restrict__ xorig, const int *__restrict__ perm,
-                         const double *__restrict__ q, double *__restrict__ grid64,
-                         unsigned long long *__restrict__ gridfx) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+// Charge spreading (Grid::anterpolateCharge, Grid.h:107-154).  The P^3
+// stencil points of one atom go to consecutive lanes (z fastest), so each
+// atomic instruction of a wave updates P^2 short contiguous rows of one
+// atom's stencil; orders 2 (8 points) pack 8 atoms into a wave, orders 6 and
+// 8 loop over their points in steps of 64.  Atoms run in sorted order.
+template <bool MIXED, int P>
+__global__ void __launch_bounds__(256) k_spread(int n, PmeParams p, const double *__restrict__ xorig,
+                                                const int *__restrict__ perm, const double *__restrict__ q,
+                                                double *__restrict__ grid64, unsigned long long *__restrict__ gridfx) {
   using R = typename std::conditional<MIXED, float, double>::type;
+  constexpr int P3 = P * P * P;
+  constexpr int LPA = P3 >= kWave ? kWave : P3;  // lanes per atom
+  constexpr int APW = kWave / LPA;               // atoms per wave
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int wave = gt >> 6, lane = gt & 63;
+  const int s = wave * APW + lane / LPA;
+  if (s >= n) return;
+  const int l0 = lane % LPA;
   const int a = perm[s];
   int b[3];
   double w[3];
   grid_coord(xorig[3 * a], p.hxr, p.nx, p.offx, b[0], w[0]);
   grid_coord(xorig[3 * a + 1], p.hyr, p.ny, p.offy, b[1], w[1]);
   grid_coord(xorig[3 * a + 2], p.hzr, p.nz, p.offz, b[2], w[2]);
-  R tx[8], ty[8], tz[8], d[8];
-  bspline_w<R>(p.order, (R)w[0], tx, d);
-  bspline_w<R>(p.order, (R)w[1], ty, d);
-  bspline_w<R>(p.order, (R)w[2], tz, d);
+  R tx[P], ty[P], tz[P], d[P];
+  bspline_w<P, R>((R)w[0], tx, d);
+  bspline_w<P, R>((R)w[1], ty, d);
+  bspline_w<P, R>((R)w[2], tz, d);
   const R qa = (R)q[a];
-  for (int i = 0; i < p.order; i++) {
-    const R aq = qa * tx[i];
-    const int i1 = (i + b[0]) % p.nx;
-    for (int j = 0; j < p.order; j++) {
-      const R ab = aq * ty[j];
-      const int j1 = (j + b[1]) % p.ny;
-      const size_t row = ((size_t)i1 * p.ny + j1) * p.nz;
-      for (int k = 0; k < p.order; k++) {
-        const int k1 = (k + b[2]) % p.nz;
-        const R v = ab * tz[k];
-        if (MIXED) {
-          long long fx = __double2ll_rn((double)v * kGridScale);
-          atomicAdd(&gridfx[row + k1], (unsigned long long)fx);
-        } else {
-          unsafeAtomicAdd(&grid64[row + k1], (double)v);
-        }
-      }
+#pragma unroll
+  for (int pt0 = 0; pt0 < P3; pt0 += LPA) {
+    const int pt = pt0 + l0;
+    const int i = pt / (P * P), j = (pt / P) % P, kk = pt % P;
+    const R v = ((qa * pick<P>(tx, i)) * pick<P>(ty, j)) * pick<P>(tz, kk);
+    const size_t g = ((size_t)wrap_grid(i + b[0], p.nx) * p.ny + wrap_grid(j + b[1], p.ny)) * p.nz +
+                     wrap_grid(kk + b[2], p.nz);
+    if (MIXED) {
+      long long fx = __double2ll_rn((double)v * kGridScale);
+      atomicAdd(&gridfx[g], (unsigned long long)fx);
+    } else {
+      unsafeAtomicAdd(&grid64[g], (double)v);
     }
   }
 }
@@ -1060,13 +1260,13 @@ __device__ inline Cx<T> cmul(Cx<T> a, Cx<T> b) {
 }
 
 // Forward DFT of small radix R in registers, e^{-2 pi i rk/R}.
-template <class T>
-__device__ inline void dft_small(Cx<T> *v, int R) {
-  if (R == 2) {
+template <int R, class T>
+__device__ inline void dft_small(Cx<T> *v) {
+  if constexpr (R == 2) {
     Cx<T> a = v[0], b = v[1];
     v[0] = Cx<T>{a.x + b.x, a.y + b.y};
     v[1] = Cx<T>{a.x - b.x, a.y - b.y};
-  } else if (R == 4) {
+  } else if constexpr (R == 4) {
     Cx<T> a = v[0], b = v[1], c = v[2], d = v[3];
     Cx<T> s0{a.x + c.x, a.y + c.y}, s1{a.x - c.x, a.y - c.y};
     Cx<T> s2{b.x + d.x, b.y + d.y}, s3{b.x - d.x, b.y - d.y};
@@ -1075,7 +1275,7 @@ __device__ inline void dft_small(Cx<T> *v, int R) {
     // -i * s3 = (s3.y, -s3.x)
     v[1] = Cx<T>{s1.x + s3.y, s1.y - s3.x};
     v[3] = Cx<T>{s1.x - s3.y, s1.y + s3.x};
-  } else if (R == 3) {
+  } else if constexpr (R == 3) {
     const T c = (T)-0.5, s = (T)0.86602540378443864676;
     Cx<T> a = v[0], b = v[1], d = v[2];
     Cx<T> t{b.x + d.x, b.y + d.y}, u{b.x - d.x, b.y - d.y};
@@ -1084,7 +1284,8 @@ __device__ inline void dft_small(Cx<T> *v, int R) {
     // -i s u = (s u.y, -s u.x)
     v[1] = Cx<T>{m.x + s * u.y, m.y - s * u.x};
     v[2] = Cx<T>{m.x - s * u.y, m.y + s * u.x};
-  } else {  // R == 5
+  } else {
+    static_assert(R == 5, "radix 2, 3, 4 or 5");
     const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
     const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
     Cx<T> a = v[0];
@@ -1109,6 +1310,32 @@ struct FftPlan {
   int rad[16];
 };
 
+// One Stockham pass of radix R over `nb` lines of length n (src -> dst):
+// butterfly j reads src[j + r n/R], twiddles by tw[k r n/(Ns R)] (k = j mod
+// Ns; the index stays below n), writes dst[(j - k) R + k + r Ns].
+template <int R, class T>
+__device__ inline void fft_pass(const Cx<T> *__restrict__ src, Cx<T> *__restrict__ dst, int nb, int n, int Ns,
+                                const Cx<T> *__restrict__ tw) {
+  const int nbut = n / R;
+  const int tstep = n / (Ns * R);
+  for (int t = threadIdx.x; t < nbut * nb; t += blockDim.x) {
+    const int line = t / nbut, j = t - line * nbut;
+    const int k = j % Ns;
+    const Cx<T> *s = src + line * n;
+    Cx<T> *d = dst + line * n;
+    Cx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      v[r] = s[j + r * nbut];
+      if (r > 0 && k > 0) v[r] = cmul(v[r], tw[k * r * tstep]);
+    }
+    dft_small<R, T>(v);
+    const int idxD = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[idxD + r * Ns] = v[r];
+  }
+}
+
 // Batched forward Stockham FFT of `nb` lines of length plan.n in LDS
 // (a -> b ping-pong); twiddle table tw[m] = e^{-2 pi i m/n}.  Returns the
 // buffer holding the result.  All threads of the block participate.
@@ -1118,22 +1345,14 @@ __device__ Cx<T> *lds_fft(Cx<T> *a, Cx<T> *b, int nb, const FftPlan &pl, const C
   int Ns = 1;
   for (int s = 0; s < pl.nrad; ++s) {
     const int R = pl.rad[s];
-    const int nbut = n / R;
-    const int tstep = n / (Ns * R);
-    for (int t = threadIdx.x; t < nbut * nb; t += blockDim.x) {
-      const int line = t / nbut, j = t - line * nbut;
-      const Cx<T> *src = a + line * n;
-      Cx<T> *dst = b + line * n;
-      const int k = j % Ns;
-      Cx<T> v[5];
-      for (int r = 0; r < R; ++r) {
-        v[r] = src[j + r * nbut];
-        if (r > 0 && k > 0) v[r] = cmul(v[r], tw[(k * r * tstep) % n]);
-      }
-      dft_small(v, R);
-      const int idxD = (j / Ns) * Ns * R + k;
-      for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[r];
-    }
+    if (R == 4)
+      fft_pass<4, T>(a, b, nb, n, Ns, tw);
+    else if (R == 2)
+      fft_pass<2, T>(a, b, nb, n, Ns, tw);
+    else if (R == 3)
+      fft_pass<3, T>(a, b, nb, n, Ns, tw);
+    else
+      fft_pass<5, T>(a, b, nb, n, Ns, tw);
     __syncthreads();
     Cx<T> *tmp = a;
     a = b;
@@ -1275,55 +1494,71 @@ __global__ void k_fft_z_c2r(PmeParams p, FftPlan pl, int lines_per_block, const 
   for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) greal[(size_t)l0 * p.nz + t] = r[t].x;
 }
 
-// Force interpolation (Grid.h:159-189) into the sorted accumulators.
-template <bool MIXED>
-__global__ void k_gather(int n, PmeParams p, const double *__restrict__ xorig, const int *__restrict__ perm,
-                         const double *__restrict__ q,
-                         const typename std::conditional<MIXED, float, double>::type *__restrict__ greal,
-                         typename AccT<MIXED>::type *__restrict__ facc) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+// Force interpolation (Grid.h:159-189) into the sorted accumulators.  Lane
+// groups as in k_spread: each lane sums one (i, j) stencil row, the group
+// reduces with cross-lane adds and its first lane adds the atom's force.
+template <bool MIXED, int P>
+__global__ void __launch_bounds__(256) k_gather(int n, int waves, PmeParams p, const double *__restrict__ xorig,
+                                                const int *__restrict__ perm, const double *__restrict__ q,
+                                                const typename std::conditional<MIXED, float, double>::type *__restrict__ greal,
+                                                typename AccT<MIXED>::type *__restrict__ facc) {
   using R = typename std::conditional<MIXED, float, double>::type;
-  const int a = perm[s];
-  int b[3];
-  double w[3];
-  grid_coord(xorig[3 * a], p.hxr, p.nx, p.offx, b[0], w[0]);
-  grid_coord(xorig[3 * a + 1], p.hyr, p.ny, p.offy, b[1], w[1]);
-  grid_coord(xorig[3 * a + 2], p.hzr, p.nz, p.offz, b[2], w[2]);
-  R tx[8], ty[8], tz[8], dx[8], dy[8], dz[8];
-  bspline_w<R>(p.order, (R)w[0], tx, dx);
-  bspline_w<R>(p.order, (R)w[1], ty, dy);
-  bspline_w<R>(p.order, (R)w[2], tz, dz);
+  constexpr int TPA = pme_tpa(P);
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int wave = gt >> 6, lane = gt & 63;
+  if (wave >= waves) return;  // whole waves only: the group reduction below needs every lane
+  const int r = lane % TPA;
+  const int s = (lane / TPA) * waves + wave;
   R fx = 0, fy = 0, fz = 0;
-  for (int i = 0; i < p.order; i++) {
-    const int i1 = (i + b[0]) % p.nx;
-    for (int j = 0; j < p.order; j++) {
-      const int j1 = (j + b[1]) % p.ny;
-      const R xij = dx[i] * ty[j], yij = tx[i] * dy[j], zij = tx[i] * ty[j];
-      const size_t row = ((size_t)i1 * p.ny + j1) * p.nz;
-      for (int k = 0; k < p.order; k++) {
-        const int k1 = (k + b[2]) % p.nz;
-        const R term = greal[row + k1];
-        fx -= term * xij * tz[k];
-        fy -= term * yij * tz[k];
-        fz -= term * zij * dz[k];
-      }
+  int a = -1;
+  if (s < n) a = perm[s];
+  if (s < n && r < P * P) {
+    const int i = r / P, j = r - (r / P) * P;
+    int b[3];
+    double w[3];
+    grid_coord(xorig[3 * a], p.hxr, p.nx, p.offx, b[0], w[0]);
+    grid_coord(xorig[3 * a + 1], p.hyr, p.ny, p.offy, b[1], w[1]);
+    grid_coord(xorig[3 * a + 2], p.hzr, p.nz, p.offz, b[2], w[2]);
+    R tx[P], ty[P], tz[P], dx[P], dy[P], dz[P];
+    bspline_w<P, R>((R)w[0], tx, dx);
+    bspline_w<P, R>((R)w[1], ty, dy);
+    bspline_w<P, R>((R)w[2], tz, dz);
+    const R txi = pick<P>(tx, i), tyj = pick<P>(ty, j);
+    const R xij = pick<P>(dx, i) * tyj, yij = txi * pick<P>(dy, j), zij = txi * tyj;
+    const int i1 = wrap_grid(i + b[0], p.nx), j1 = wrap_grid(j + b[1], p.ny);
+    const size_t row = ((size_t)i1 * p.ny + j1) * p.nz;
+#pragma unroll
+    for (int kk = 0; kk < P; kk++) {
+      const int k1 = wrap_grid(kk + b[2], p.nz);
+      const R term = greal[row + k1];
+      fx -= term * xij * tz[kk];
+      fy -= term * yij * tz[kk];
+      fz -= term * zij * dz[kk];
     }
   }
-  const R qa = (R)q[a];
-  AccT<MIXED>::add(&facc[3 * s], fx * (R)p.hxr * qa);
-  AccT<MIXED>::add(&facc[3 * s + 1], fy * (R)p.hyr * qa);
-  AccT<MIXED>::add(&facc[3 * s + 2], fz * (R)p.hzr * qa);
+#pragma unroll
+  for (int m = 1; m < TPA; m <<= 1) {
+    fx += __shfl_xor(fx, m);
+    fy += __shfl_xor(fy, m);
+    fz += __shfl_xor(fz, m);
+  }
+  if (r == 0 && s < n) {
+    const R qa = (R)q[a];
+    AccT<MIXED>::add(&facc[3 * s], fx * (R)p.hxr * qa);
+    AccT<MIXED>::add(&facc[3 * s + 1], fy * (R)p.hyr * qa);
+    AccT<MIXED>::add(&facc[3 * s + 2], fz * (R)p.hzr * qa);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // 5. Exclusion correction (NonbondedPMEwaldSystemForce.h:874-911), FP64, exact erf.
+// One block of 256 entries (block blk of the correction range).
 template <bool MIXED>
-__global__ void k_correction(int ne, const int4 *__restrict__ etab, const double *__restrict__ xorig,
-                             const double *__restrict__ q, const int *__restrict__ slot, Box box, double alpha,
-                             double csf, typename AccT<MIXED>::type *__restrict__ facc, double *__restrict__ epart) {
+__device__ void correction_block(int blk, int ne, const int4 *__restrict__ etab, const double *__restrict__ xorig,
+                                 const double *__restrict__ q, const int *__restrict__ slot, Box box, double alpha,
+                                 double csf, typename AccT<MIXED>::type *__restrict__ facc, double *__restrict__ epart) {
   __shared__ double s_e[kPairBlock / kWave];
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = blk * blockDim.x + threadIdx.x;
   double en = 0.0;
   if (e < ne) {
     const int4 t = etab[e];
@@ -1362,7 +1597,7 @@ __global__ void k_correction(int ne, const int4 *__restrict__ etab, const double
   if (threadIdx.x == 0) {
     double tot = 0;
     for (int w = 0; w < (int)(blockDim.x / kWave); ++w) tot += s_e[w];
-    epart[blockIdx.x] = tot;
+    epart[blk] = tot;
   }
 }
 
@@ -1481,12 +1716,12 @@ __device__ inline void min_diff(const double *x, int c1, int c2, const Box &bx, 
 // Harmonic bonds (BondSystemForce.h:97-118) and angles without Urey-Bradley
 // (AngleSystemForce.h:95-147), FP64.  Threads [0, nb) bonds, [nb, nb+na) angles.
 template <bool MIXED>
-__global__ void k_bonded(int nb, int na, const int2 *__restrict__ bonds, const double2 *__restrict__ bpar,
-                         const int4 *__restrict__ angles, const double2 *__restrict__ apar,
-                         const double *__restrict__ x, const int *__restrict__ slot, Box box,
-                         typename AccT<MIXED>::type *__restrict__ facc, double *__restrict__ epart) {
+__device__ void bonded_block(int blk, int nb, int na, const int2 *__restrict__ bonds, const double2 *__restrict__ bpar,
+                             const int4 *__restrict__ angles, const double2 *__restrict__ apar,
+                             const double *__restrict__ x, const int *__restrict__ slot, Box box,
+                             typename AccT<MIXED>::type *__restrict__ facc, double *__restrict__ epart) {
   __shared__ double s_e[2][kPairBlock / kWave];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blk * blockDim.x + threadIdx.x;
   double eb = 0.0, ea = 0.0;
   if (t < nb) {
     const int2 b = bonds[t];
@@ -1544,47 +1779,97 @@ __global__ void k_bonded(int nb, int na, const int2 *__restrict__ bonds, const d
       b += s_e[0][w];
       a += s_e[1][w];
     }
-    epart[2 * blockIdx.x] = b;
-    epart[2 * blockIdx.x + 1] = a;
+    epart[2 * blk] = b;
+    epart[2 * blk + 1] = a;
   }
 }
 
+// The small FP64 passes that follow the pair and PME kernels, in one launch
+// of 256-thread blocks: [0, border) boundary pairs, then exclusion
+// correction blocks, then bonded-term blocks.
+template <bool MIXED>
+struct FixupArgs {
+  using R4 = typename Real4<typename std::conditional<MIXED, float, double>::type>::type;
+  int border_blocks, corr_blocks;
+  // boundary pairs
+  PairParams pp;
+  int stats;
+  const int2 *border;
+  const int *nborder;
+  int maxborder;
+  const R4 *xs;
+  const int *ts, *perm;
+  const double *xorig, *ljtab;
+  const int *eptr;
+  const int2 *elist;
+  unsigned long long *border_e, *border_stat;
+  // correction
+  int ne;
+  const int4 *etab;
+  const double *q;
+  const int *slot;
+  double alpha, csf;
+  double *corr_e;
+  // bonded
+  int nbond, nangle;
+  const int2 *bonds;
+  const double2 *bpar;
+  const int4 *angles;
+  const double2 *apar;
+  double *bond_e;
+  typename AccT<MIXED>::type *facc;
+};
+
+template <bool MIXED, int COUL, int LJSW, int CSW>
+__global__ void __launch_bounds__(256) k_fixups(FixupArgs<MIXED> a) {
+  int b = blockIdx.x;
+  if (b < a.border_blocks) {
+    border_pairs<MIXED, COUL, LJSW, CSW>(b, a.border_blocks, a.pp, a.stats, a.border, a.nborder, a.maxborder, a.xs,
+                                         a.ts, a.perm, a.xorig, a.ljtab, a.eptr, a.elist, a.facc, a.border_e,
+                                         a.border_stat);
+    return;
+  }
+  b -= a.border_blocks;
+  if (b < a.corr_blocks) {
+    correction_block<MIXED>(b, a.ne, a.etab, a.xorig, a.q, a.slot, a.pp.box, a.alpha, a.csf, a.facc, a.corr_e);
+    return;
+  }
+  b -= a.corr_blocks;
+  bonded_block<MIXED>(b, a.nbond, a.nangle, a.bonds, a.bpar, a.angles, a.apar, a.xorig, a.slot, a.pp.box, a.facc,
+                      a.bond_e);
+}
+
 // Kick (v += F kick h / m) and drift (x += v h) of every atom, FP64
-// (LeapfrogIntegrator.cpp:33-87), then the sorted copies the pair kernel
-// reads are refreshed in place and the displacement since the last list build
-// is folded into dmax (float bits of the largest squared displacement).
+// (LeapfrogIntegrator.cpp:33-87).  The pair kernel's sorted coordinates then
+// follow each atom continuously from its wrapped position at the last list
+// build (xs64), since list entries carry a fixed image shift, and the largest
+// squared displacement since that build is folded into dmax (float bits).
 template <class R>
 __global__ void k_kick_drift(int n, double kick_h, double h, const int *__restrict__ perm,
                              const double *__restrict__ fout, const double *__restrict__ minv, double *__restrict__ v,
                              double *__restrict__ x, const double *__restrict__ xbuild, Box box,
-                             typename Real4<R>::type *__restrict__ xs, double4 *__restrict__ xs64,
+                             typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64,
                              unsigned int *__restrict__ dmax) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   float d2f = 0.f;
   if (s < n) {
     const int a = perm[s];
-    double w[3];
+    double dd[3];
     double d2 = 0;
     for (int k = 0; k < 3; ++k) {
       double vk = v[3 * a + k] + fout[3 * a + k] * kick_h * minv[a];
       v[3 * a + k] = vk;
       const double xk = x[3 * a + k] + vk * h;
       x[3 * a + k] = xk;
-      double wk = xk - box.L[k] * floor(xk * box.Linv[k]);
-      if (wk >= box.L[k]) wk -= box.L[k];
-      if (wk < 0.0) wk = 0.0;
-      w[k] = wk;
-      double dd = xk - xbuild[3 * a + k];
-      dd -= box.L[k] * rint(dd * box.Linv[k]);
-      d2 += dd * dd;
+      dd[k] = xk - xbuild[3 * a + k];
+      d2 += dd[k] * dd[k];
     }
     d2f = (float)d2;
-    const double4 old = xs64[s];
-    xs64[s] = make_double4(w[0], w[1], w[2], old.w);
+    const double4 b = xs64[s];
     typename Real4<R>::type r = xs[s];
-    r.x = (R)w[0];
-    r.y = (R)w[1];
-    r.z = (R)w[2];
+    r.x = (R)(b.x + dd[0]);
+    r.y = (R)(b.y + dd[1]);
+    r.z = (R)(b.z + dd[2]);
     xs[s] = r;
   }
   // wave max, one atomic per wave

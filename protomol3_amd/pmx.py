@@ -22,7 +22,7 @@ SW_NONE, SW_CUTOFF, SW_C1, SW_C2 = 0, 1, 2, 3
 COUL_NONE, COUL_CUTOFF, COUL_PME = 0, 1, 2
 PME_REAL, PME_RECIP, PME_CORR = 1, 2, 4
 EVAL_PAIRSTATS, EVAL_TIMING = 1, 2
-STAGES = ("sort", "list", "pair", "spread", "fft", "gather", "correction", "finalize")
+STAGES = ("sort", "list", "pair", "spread", "fft", "gather", "fixups", "finalize")
 
 ERRORS = {-1: "PMX_ERR_ARG", -2: "PMX_ERR_OOM", -3: "PMX_ERR_HIP", -4: "PMX_ERR_UNSUPPORTED",
           -5: "PMX_ERR_NODEV", -6: "PMX_ERR_OVERFLOW"}
@@ -242,7 +242,7 @@ class System:
         return out, tuple(int(v) for v in dims)
 
     def stage_times(self) -> dict:
-        ms = (C.c_float * 8)()
+        ms = (C.c_float * len(STAGES))()
         self.ctx.check(self.ctx.lib.pmx_sys_stage_times(self.h, ms), "pmx_sys_stage_times")
         return dict(zip(STAGES, [float(v) for v in ms]))
 
@@ -291,7 +291,7 @@ class System:
 
     def timing_end(self):
         """(stage -> average ms, number of timed calls)."""
-        ms = (C.c_float * 8)()
+        ms = (C.c_float * len(STAGES))()
         n = C.c_int()
         self.ctx.check(self.ctx.lib.pmx_sys_timing_end(self.h, ms, C.byref(n)), "pmx_sys_timing_end")
         return dict(zip(STAGES, [float(v) for v in ms])), n.value

@@ -84,3 +84,28 @@ def test_gpu_md_skin_reuse_is_exact_and_rolls_back(ctx):
     for x, v in ((x1, v1), (x2, v2)):
         assert np.abs(x - x0).max() < 1e-10
         assert np.abs(v - v0).max() < 1e-9
+
+
+def test_gpu_md_state_energies_before_any_step(ctx):
+    """Energies read right after md_set_state are those of the given
+    positions (forces computed on demand), not of an earlier evaluation."""
+    s = systems.water_box(seed=23).with_velocities(300.0, seed=7)
+    spec = dict(cellsize=5.0, lj=(O.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 32, 32), order=4, alpha=0.32))
+    ospec = O.make_spec(5.0, lj=spec["lj"], pme=spec["pme"])
+    _, eref = md.total_forces(s, ospec, s.pos)
+    sy = pmx.System(ctx, s, pmx.ForceSpec(5.0, lj=spec["lj"], pme=spec["pme"]), pmx.FP64)
+    try:
+        sy.md_init(s, 1.0, skin=0.5)
+        sy.md_set_state(s.pos, s.vel)
+        _, _, e = sy.md_get_state()
+        x2 = s.pos + 0.01
+        sy.md_set_state(x2, s.vel)
+        _, _, e2 = sy.md_get_state()
+    finally:
+        sy.close()
+    _, eref2 = md.total_forces(s, ospec, x2)
+    print(f"energies {e} oracle {eref}", flush=True)
+    for k in eref:
+        assert abs(e[k] - eref[k]) <= 1e-9 * max(1.0, abs(eref[k])), k
+        assert abs(e2[k] - eref2[k]) <= 1e-9 * max(1.0, abs(eref2[k])), k
+    assert e["kinetic"] > 0
