@@ -94,6 +94,60 @@ def run_form(ctx, s, spec, precision, steps, warmup, skin, drift_steps):
                 x=x, drift_kcal_mol_ns=slope, etot_first=etot[0], etot_last=etot[-1])
 
 
+def serving(nreq: int, slots: int, seed: int = 7, log=print):
+    """Serving stream: requests of log-uniform size in [1,000, 100,000] atoms
+    (water + bead-polymer solute at 0.0334 atoms/A^3, per-request seeds),
+    half single force evaluations and half 100-step Langevin trajectories
+    (300 K, 1/ps), mixed precision, LJ C2 7-9 A + PME ~1 A grid.  First all
+    requests at once (capacity), then Poisson arrivals at 75% of that
+    capacity; latency = submit to result, per request."""
+    from protomol3_amd import pmx, serve, systems
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(100000), nreq)).astype(int)
+    kinds = rng.integers(0, 2, nreq)
+    reqs = []
+    for i, (n, kd) in enumerate(zip(sizes, kinds)):
+        s = systems.replica_box(int(n), 1000 + i).with_velocities(300.0, seed=5000 + i)
+        reqs.append((s, serve.request_spec(s), int(kd)))
+    natoms = int(sum(s.n for s, _, _ in reqs))
+    log(f"serving: {nreq} requests, {natoms} atoms in total, {slots} slots")
+
+    def submit(srv, s, spec, kd):
+        if kd == 0:
+            return srv.submit_forces(s, spec, pmx.MIXED)
+        return srv.submit_trajectory(s, spec, pmx.MIXED, 100, skin=1.0, gamma=1.0, seed=17)
+
+    srv = serve.Server(0, slots=slots)
+    try:
+        t0 = time.perf_counter()
+        ts = [submit(srv, *r) for r in reqs]
+        for t in ts:
+            srv.wait(t)
+        makespan = time.perf_counter() - t0
+        rate = 0.75 * nreq / makespan
+        gaps = rng.exponential(1.0 / rate, nreq)
+        t0 = time.perf_counter()
+        ts, due = [], t0
+        for r, g in zip(reqs, gaps):
+            due += g
+            now = time.perf_counter()
+            if due > now:
+                time.sleep(due - now)
+            ts.append(submit(srv, *r))
+        lat = np.array([srv.wait(t).done_ms for t in ts])
+        wall = time.perf_counter() - t0
+    finally:
+        srv.close()
+    traj_steps = 100 * sum(1 for _, _, kd in reqs if kd == 1)
+    return {"requests": nreq, "slots": slots, "atoms_total": natoms,
+            "capacity_req_per_s": round(nreq / makespan, 3), "arrival_rate_req_per_s": round(rate, 3),
+            "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
+            "aggregate_ns_per_day": round(traj_steps * 1e-6 / wall * 86400.0, 3),
+            "workload": "log-uniform 1k-100k atoms, 50% force evaluations / 50% 100-step Langevin 1 fs, mixed, "
+                        "Poisson arrivals at 75% of measured capacity"}
+
+
 def cpu_baseline(s, budget_s: float = 20.0):
     """The oracle (FP64 C++ restatement of the reference path + leapfrog) on
     the host's cores: whole MD steps of the same workload until ~budget_s."""
@@ -125,6 +179,8 @@ def main():
     ap.add_argument("--drift-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--serve-requests", type=int, default=64, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--serve-slots", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
@@ -164,6 +220,7 @@ def main():
     pair_ms = mixed["stages"]["pair"]
     achieved = FLOP_PER_PAIR * mixed["npairs"] / (pair_ms * 1e-3) / 1e12
     cpu = None if a.no_cpu else cpu_baseline(s, a.cpu_budget)
+    serve_out = serving(a.serve_requests, a.serve_slots, log=log) if a.serve_requests > 0 else None
     out = {
         "metric": "ns/day (26k-atom PME) mixed vs FP64 GPU at <=1e-5 force err; p50/p99 latency",
         "value": round(ns_day(mixed["ms_per_step"]), 3),
@@ -202,6 +259,7 @@ def main():
                      "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": int(mixed["npairs"]),
                      "avg_launch_ms": round(pair_ms, 5)},
         "cpu_baseline": cpu,
+        "serving": serve_out,
     }
     print(json.dumps(out), flush=True)
 

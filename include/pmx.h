@@ -109,6 +109,13 @@ int pmx_sys_compute(pmx_sys *sys, unsigned flags);
 int pmx_sys_get_forces(pmx_sys *sys, double *f_aos, double *energies);
 int pmx_sys_sync(pmx_sys *sys);
 
+/* Device bytes pmx_sys_create would allocate for desc, computed on the host
+ * without allocating (kind 1 adds the device MD state of pmx_md_init, with
+ * about one bond and one angle per atom); 0 for an invalid descriptor. */
+unsigned long long pmx_estimate_bytes(const pmx_sys_desc *desc, int kind);
+/* Device memory held by the system (all its buffers, bytes). */
+int pmx_sys_device_bytes(const pmx_sys *sys, unsigned long long *bytes);
+
 /* PME splitting parameter in use (after pmx_sys_create). */
 int pmx_sys_alpha(const pmx_sys *sys, double *alpha);
 
@@ -161,7 +168,17 @@ typedef struct {
   const double *mass;        /* [n] amu */
   double timestep_fs;
   double skin;               /* pair-list buffer in A (0: rebuild every step) */
+  int integrator;            /* PMX_MD_LEAPFROG or PMX_MD_LANGEVIN */
+  double temperature;        /* Langevin: K */
+  double gamma;              /* Langevin: friction in 1/ps (the reference's -gamma) */
+  unsigned long long seed;   /* Langevin: noise stream (Philox4x32-10 key) */
 } pmx_md_desc;
+/* Integrators: LeapfrogIntegrator (integrator/leapfrog/LeapfrogIntegrator.cpp:
+ * 33-104) and LangevinLeapfrogIntegrator (integrator/base/
+ * LangevinLeapfrogIntegrator.cpp:44-71), the latter with its noise drawn from
+ * a counter-based generator keyed by (seed, atom, half-kick index) instead of
+ * the reference's serial drand48 stream. */
+enum { PMX_MD_LEAPFROG = 0, PMX_MD_LANGEVIN = 1 };
 
 /* Energy slots of pmx_md_get_state. */
 enum { PMX_MD_E_COULOMB = 0, PMX_MD_E_LJ = 1, PMX_MD_E_BOND = 2, PMX_MD_E_ANGLE = 3, PMX_MD_E_KINETIC = 4,
@@ -178,6 +195,10 @@ int pmx_md_run(pmx_sys *sys, int nsteps);
  * potential terms at x, kinetic energy of v; right after pmx_md_set_state
  * the forces are computed first).  Any pointer may be NULL. */
 int pmx_md_get_state(pmx_sys *sys, double *x, double *v, double *energies);
+/* The Langevin noise of one half kick, computed on the host: g[3*i..3*i+2]
+ * for atoms i < n, the numbers the device draws for (seed, half kick `half`).
+ * No GPU needed. */
+int pmx_langevin_noise(unsigned long long seed, unsigned long long half, int n, double *g);
 /* Counters: list rebuilds and rolled-back chunks since pmx_md_init. */
 int pmx_md_stats(const pmx_sys *sys, long long *steps, long long *rebuilds, long long *rollbacks);
 

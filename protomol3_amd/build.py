@@ -49,6 +49,24 @@ def build_host(verbose: bool = False) -> str:
     return HOST_EXE
 
 
+SERVE_LIB = os.path.join(HERE, "lib", "libpmx_serve.so")
+
+
+def build_serve(verbose: bool = False) -> str:
+    """The request loop (host C++ over the C ABI), linked to libpmx.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-pthread", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(HERE, "host", "pmx_serve.cpp"), "-L" + os.path.dirname(LIB), "-lpmx",
+           "-Wl,-rpath,$ORIGIN", "-o", SERVE_LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + r.stderr[-4000:])
+    os.replace(SERVE_LIB + ".tmp", SERVE_LIB)
+    if verbose:
+        print("built", SERVE_LIB)
+    return SERVE_LIB
+
+
 if __name__ == "__main__":
     build_lib(verbose=True)
     build_host(verbose=True)
+    build_serve(verbose=True)

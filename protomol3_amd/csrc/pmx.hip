@@ -44,7 +44,7 @@ struct Sys {
   pmx_ctx *ctx = nullptr;
   pmx_sys_desc d{};
   bool mixed = false;
-  int n = 0, npad = 0, ncl = 0, nsuper = 0;
+  int n = 0, npad = 0, ncl = 0, nsuper = 0, ncols = 0;  // nsuper: upper bound (slots = 64 nsuper)
   // host copies
   std::vector<double> q, lj;
   std::vector<int> type;
@@ -67,7 +67,7 @@ struct Sys {
   // device buffers
   std::vector<DevBuf *> all;
   DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
-      ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
+      sc_info, ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
       pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec,
       greal, theta, twx, twy, twz, border, nborder, border_acc;
   int maxborder = 65536;
@@ -81,6 +81,8 @@ struct Sys {
   bool timing_active = false;                   // MD steps record events (timing_begin .. timing_end)
   bool have_positions = false;
   bool cached = false;
+  bool dry = false;                    // size only: allocations are counted, nothing runs
+  unsigned long long dry_bytes = 0;
   double eout_h[4] = {0, 0, 0, 0};
   // device-resident MD state
   struct Md {
@@ -91,10 +93,14 @@ struct Sys {
     int chunk = 16;
     long long steps = 0, rebuilds = 0, rollbacks = 0;
     bool forces_valid = false;
+    int integrator = PMX_MD_LEAPFROG;
+    double vdt = 1, fdt = 0, ndt = 0, fc = 0;  // Langevin half-kick constants
+    unsigned long long seed = 0;
   } md;
 };
 constexpr int kMaxChunk = 128;
 constexpr double kInvTimeFactor = 0.02045482828087295;  // protomol/base/PMConstants.cpp:54
+constexpr double kBoltzmann = 0.001987191;              // protomol/base/PMConstants.cpp:70
 
 int fail(pmx_ctx *c, int code, const std::string &m) {
   if (c) c->err = m;
@@ -109,6 +115,10 @@ int fail(pmx_ctx *c, int code, const std::string &m) {
 
 int dalloc(Sys *s, DevBuf &b, size_t bytes) {
   if (bytes == 0) bytes = 8;
+  if (s->dry) {
+    s->dry_bytes += bytes;
+    return PMX_OK;
+  }
   if (b.p && b.bytes >= bytes) return PMX_OK;
   if (b.p) hipFree(b.p);
   b.p = nullptr;
@@ -242,9 +252,6 @@ int setup(Sys *s) {
   const pmx_sys_desc &d = s->d;
   s->mixed = d.precision == PMX_MIXED;
   s->n = d.n;
-  s->npad = (d.n + kSuperSize - 1) / kSuperSize * kSuperSize;
-  s->ncl = s->npad / kClusterSize;
-  s->nsuper = s->npad / kSuperSize;
   s->q.assign(d.charge, d.charge + d.n);
   s->type.assign(d.type, d.type + d.n);
   s->lj.assign(d.lj, d.lj + (size_t)d.ntypes * d.ntypes * 4);
@@ -299,28 +306,37 @@ int setup(Sys *s) {
     bx.D = std::min(dx, std::min(dy, dz));
   }
 
-  // Binning parameters: fine grid ~ one atom per cell; reference cell grid
+  // Binning parameters: x-y columns about one supercluster (64 atoms) wide
+  // and deep, z bins of about two atoms per column; reference cell grid
   // (CubicCellManager.cpp:46-65, ArrayCellListStructure.cpp:40-42).
   BinParams &bp = s->bin;
   bp.box = bx;
   const double vol = bx.L[0] * bx.L[1] * bx.L[2];
-  const double edge = std::cbrt(vol / std::max(1, d.n));
-  int nfmax = 1;
+  const double rho = std::max(1, d.n) / vol;
+  const double a_sc = std::cbrt(kSuperSize / rho);
+  bp.ncx = (int)std::max(1L, std::min(1024L, std::lround(bx.L[0] / a_sc)));
+  bp.ncy = (int)std::max(1L, std::min(1024L, std::lround(bx.L[1] / a_sc)));
+  {
+    const double area = (bx.L[0] / bp.ncx) * (bx.L[1] / bp.ncy);
+    const long long nzb = std::lround(bx.L[2] * rho * area / 2.0);
+    bp.nzb = (int)std::max(1LL, std::min(1LL << 16, nzb));
+  }
+  bp.cxinv = bp.ncx / bx.L[0];
+  bp.cyinv = bp.ncy / bx.L[1];
+  bp.zinv = bp.nzb / bx.L[2];
+  s->ncols = bp.ncx * bp.ncy;
+  // each column's last supercluster may be partly padding
+  s->nsuper = (d.n + kSuperSize - 1) / kSuperSize + s->ncols;
+  s->npad = s->nsuper * kSuperSize;
+  s->ncl = s->npad / kClusterSize;
   for (int k = 0; k < 3; ++k) {
-    int nf = (int)std::lround(bx.L[k] / edge);
-    nf = std::max(1, std::min(128, nf));
-    bp.nf[k] = nf;
-    bp.finv[k] = nf / bx.L[k];
-    nfmax = std::max(nfmax, nf);
     double H = bx.H[k];
     double e = H / std::max(1.0, std::floor(H / d.cellsize));
     bp.rc_inv[k] = 1.0 / e;
     bp.rc_n[k] = std::max(1, (int)std::floor(H / e + 1.0e-14));
     bp.delta[k] = bx.o[k] - bx.mn[k];
   }
-  int p2 = 1;
-  while (p2 < nfmax) p2 <<= 1;
-  s->nbins = (long long)p2 * p2 * p2;
+  s->nbins = (long long)s->ncols * bp.nzb;
   s->nscanblocks = (int)((s->nbins + kScanBlock - 1) / kScanBlock);
 
   // Pair parameters
@@ -372,7 +388,6 @@ int setup(Sys *s) {
   lp.rlist2 = (float)((rcmax + margin + s->md.skin) * (rcmax + margin + s->md.skin));
   lp.ncl = s->ncl;
   lp.nsuper = s->nsuper;
-  lp.n = d.n;
   {
     // generous per-supercluster capacity from the cluster density
     const double rho_cl = s->ncl / vol;
@@ -393,7 +408,8 @@ int setup(Sys *s) {
       (rc = A(s->start, 4 * s->nbins)) || (rc = A(s->cursor, 4 * s->nbins)) ||
       (rc = A(s->scansum, 4 * (size_t)s->nscanblocks)) || (rc = A(s->perm, 4 * N)) || (rc = A(s->permp, 4 * NP)) ||
       (rc = A(s->slot, 4 * N)) || (rc = A(s->refcell, 4 * N)) || (rc = A(s->xs, 32 * NP)) ||
-      (rc = A(s->xs64, 32 * NP)) || (rc = A(s->ts, 4 * NP)) || (rc = A(s->ctr, 16 * (size_t)s->ncl)) ||
+      (rc = A(s->xs64, 32 * NP)) || (rc = A(s->ts, 4 * NP)) || (rc = A(s->sc_info, 8 * (size_t)s->nsuper)) ||
+      (rc = A(s->ctr, 16 * (size_t)s->ncl)) ||
       (rc = A(s->half, 16 * (size_t)s->ncl)) || (rc = A(s->entries, 16 * (size_t)s->nsuper * s->cap)) ||
       (rc = A(s->count, 4 * (size_t)s->nsuper)) || (rc = A(s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
       (rc = A(s->nmaskwork, 8)) || (rc = A(s->masks, 16 * 8 * (size_t)s->maxmaskwork)) ||
@@ -408,6 +424,16 @@ int setup(Sys *s) {
     return rc;
   s->corr_blocks = std::max(1, (int)nblk((long long)s->etab.size(), 256));
   if ((rc = A(s->corr_e, 8 * (size_t)s->corr_blocks))) return rc;
+  if (s->dry) {
+    // PME buffers as allocated below (grid, half spectrum, real potential,
+    // influence table, twiddles, energy partials bounded by ny * nzh)
+    if (s->pme_on) {
+      const size_t nx = d.pme_grid[0], ny = d.pme_grid[1], nz = d.pme_grid[2], nzh = nz / 2 + 1;
+      const size_t G = nx * ny * nz, GH = nx * ny * nzh, cxb = s->mixed ? 8 : 16, rb = s->mixed ? 4 : 8;
+      s->dry_bytes += 8 * G + cxb * GH + rb * G + rb * GH + cxb * (nx + ny + nz) + 8 * ny * nzh + 8 * 5;
+    }
+    return PMX_OK;
+  }
 
   hipStream_t st = s->ctx->stream;
   HIPCHK(hipMemcpyAsync(s->q_d.p, s->q.data(), 8 * N, hipMemcpyHostToDevice, st));
@@ -624,8 +650,8 @@ template <bool MIXED, int ORDER>
 void launch_spread(Sys *s, hipStream_t st) {
   constexpr int P3 = ORDER * ORDER * ORDER;
   constexpr int apw = P3 >= 64 ? 1 : 64 / P3;
-  const int waves = (int)nblk(s->n, apw);
-  hipLaunchKernelGGL((k_spread<MIXED, ORDER>), dim3(nblk((long long)waves * 64, 256)), dim3(256), 0, st, s->n,
+  const int waves = (int)nblk(s->npad, apw);
+  hipLaunchKernelGGL((k_spread<MIXED, ORDER>), dim3(nblk((long long)waves * 64, 256)), dim3(256), 0, st, s->npad,
                      s->pme, P<double>(s->xorig), P<int>(s->permp), P<double>(s->q_d), P<double>(s->grid64),
                      P<unsigned long long>(s->gridfx));
 }
@@ -633,8 +659,9 @@ void launch_spread(Sys *s, hipStream_t st) {
 template <bool MIXED, int ORDER>
 void launch_gather(Sys *s, hipStream_t st) {
   using T = typename std::conditional<MIXED, float, double>::type;
-  const int waves = (int)nblk(s->n, 64 / pme_tpa(ORDER));
-  hipLaunchKernelGGL((k_gather<MIXED, ORDER>), dim3(nblk((long long)waves * 64, 256)), dim3(256), 0, st, s->n, waves,
+  const int waves = (int)nblk(s->npad, 64 / pme_tpa(ORDER));
+  hipLaunchKernelGGL((k_gather<MIXED, ORDER>), dim3(nblk((long long)waves * 64, 256)), dim3(256), 0, st, s->npad,
+                     waves,
                      s->pme, P<double>(s->xorig), P<int>(s->permp), P<double>(s->q_d), P<T>(s->greal),
                      P<typename AccT<MIXED>::type>(s->facc));
 }
@@ -723,19 +750,22 @@ int rebuild(Sys *s, bool timing) {
                      P<int>(s->perm));
   hipLaunchKernelGGL(k_binsort, dim3(nblk(s->nbins, 256)), dim3(256), 0, st, s->nbins, P<uint32_t>(s->start), n,
                      P<int>(s->perm));
+  hipLaunchKernelGGL(k_columns, dim3(1), dim3(1024), 0, st, s->ncols, s->bin.nzb, n, P<uint32_t>(s->start),
+                     s->nsuper, P<int2>(s->sc_info));
+  hipLaunchKernelGGL(k_split, dim3(nblk((long long)s->nsuper * 64, 256)), dim3(256), 0, st, s->nsuper,
+                     P<int2>(s->sc_info), P<int>(s->perm), P<double>(s->xwrap), P<int>(s->permp), P<int>(s->slot));
   double *xbuild = s->md.on ? P<double>(s->md.xbuild) : nullptr;
   if (s->mixed)
-    hipLaunchKernelGGL(k_fill_sorted<float>, dim3(nblk(npad, 256)), dim3(256), 0, st, n, npad, P<int>(s->perm),
+    hipLaunchKernelGGL(k_fill_sorted<float>, dim3(nblk(npad, 256)), dim3(256), 0, st, npad, P<int>(s->permp),
                        P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<float4>(s->xs),
-                       P<double4>(s->xs64), P<int>(s->ts), P<int>(s->slot), P<int>(s->permp),
-                       P<double>(s->xorig), xbuild);
+                       P<double4>(s->xs64), P<int>(s->ts), P<double>(s->xorig), xbuild);
   else
-    hipLaunchKernelGGL(k_fill_sorted<double>, dim3(nblk(npad, 256)), dim3(256), 0, st, n, npad, P<int>(s->perm),
+    hipLaunchKernelGGL(k_fill_sorted<double>, dim3(nblk(npad, 256)), dim3(256), 0, st, npad, P<int>(s->permp),
                        P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<double4>(s->xs),
-                       P<double4>(s->xs64), P<int>(s->ts), P<int>(s->slot), P<int>(s->permp),
-                       P<double>(s->xorig), xbuild);
+                       P<double4>(s->xs64), P<int>(s->ts), P<double>(s->xorig), xbuild);
   if (timing) hipEventRecord(s->ev[1], st);
-  hipLaunchKernelGGL(k_cluster_bbox, dim3(nblk(s->ncl, 256)), dim3(256), 0, st, s->ncl, n, P<double4>(s->xs64),
+  hipLaunchKernelGGL(k_cluster_bbox, dim3(nblk(s->ncl, 256)), dim3(256), 0, st, s->ncl, P<int>(s->permp),
+                     P<double4>(s->xs64),
                      P<float4>(s->ctr), P<float4>(s->half));
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, P<uint32_t>(s->nmaskwork), 0u, 1ll);
   hipLaunchKernelGGL(k_build_list, dim3(s->nsuper), dim3(kListBlock), 0, st, s->lp,
@@ -747,7 +777,7 @@ int rebuild(Sys *s, bool timing) {
                        P<int>(s->count), s->cap, P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork,
                        P<int>(s->overflow));
   hipLaunchKernelGGL(k_build_masks, dim3(s->ctx->cus * 4), dim3(256), 0, st, P<int>(s->maskwork),
-                     P<int>(s->nmaskwork), s->maxmaskwork, P<int4>(s->entries), s->cap, n, P<int>(s->permp),
+                     P<int>(s->nmaskwork), s->maxmaskwork, P<int4>(s->entries), s->cap, P<int>(s->permp),
                      P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned long long>(s->masks));
   hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int2>(s->tiles),
                      P<int>(s->ntiles));
@@ -1064,6 +1094,9 @@ int pmx_md_init(pmx_sys *sys, const pmx_md_desc *d) {
   if (d->nbonds < 0 || d->nangles < 0 || !d->mass || !(d->timestep_fs > 0) || d->skin < 0 || d->skin > 4.0 ||
       (d->nbonds > 0 && (!d->bonds || !d->bond_par)) || (d->nangles > 0 && (!d->angles || !d->angle_par)))
     return fail(s->ctx, PMX_ERR_ARG, "bad MD descriptor");
+  if ((d->integrator != PMX_MD_LEAPFROG && d->integrator != PMX_MD_LANGEVIN) ||
+      (d->integrator == PMX_MD_LANGEVIN && (!(d->temperature >= 0) || !(d->gamma >= 0))))
+    return fail(s->ctx, PMX_ERR_ARG, "bad integrator parameters");
   for (int i = 0; i < s->n; ++i)
     if (!(d->mass[i] > 0)) return fail(s->ctx, PMX_ERR_ARG, "masses must be > 0");
   for (int b = 0; b < 2 * d->nbonds; ++b)
@@ -1083,6 +1116,17 @@ int pmx_md_init(pmx_sys *sys, const pmx_md_desc *d) {
   m.chunk = 16;
   m.steps = m.rebuilds = m.rollbacks = 0;
   m.forces_valid = false;
+  m.integrator = d->integrator;
+  m.seed = d->seed;
+  if (d->integrator == PMX_MD_LANGEVIN) {
+    // LangevinLeapfrogIntegrator.cpp:28 (gamma per internal time unit) and
+    // :46-51; gamma = 0 takes the limits (plain leapfrog half kick, no noise)
+    const double g = d->gamma / (1000.0 * kInvTimeFactor), dt = m.h;
+    m.vdt = std::exp(-0.5 * g * dt);
+    m.fdt = g > 0 ? (1.0 - std::exp(-0.5 * g * dt)) / g : 0.5 * dt;
+    m.ndt = g > 0 ? std::sqrt((1.0 - std::exp(-g * dt)) / (2.0 * g)) : 0.0;
+    m.fc = 2.0 * kBoltzmann * d->temperature * g;
+  }
   const size_t N = s->n;
   std::vector<int2> b(m.nb);
   std::vector<double2> bp(m.nb);
@@ -1165,20 +1209,38 @@ int md_check_overflow(Sys *s) {
 
 // One leapfrog step: kick + drift (+ optional list rebuild at the new
 // positions) + forces.  dslot: per-step displacement slot.
-int md_step(Sys *s, double kick, bool rebuild_here, int dslot) {
+// The kicks before the drift of global step `step`: the first step of a run
+// gets one half kick (kick = 0.5), later steps the previous step's closing
+// half kick and their own opening one (kick = 1).  Langevin half kicks are
+// numbered 2 step (opening) and 2 step + 1 (closing).
+Kick make_kick(const Sys::Md &m, double kick, long long step) {
+  Kick kk{};
+  kk.h_kick = kick * m.h;
+  kk.langevin = m.integrator == PMX_MD_LANGEVIN;
+  kk.nhalf = kick == 0.5 ? 1 : 2;
+  kk.half0 = (unsigned long long)(kick == 0.5 ? 2 * step : 2 * step - 1);
+  kk.seed = m.seed;
+  kk.vdt = m.vdt;
+  kk.fdt = m.fdt;
+  kk.ndt = m.ndt;
+  kk.fc = m.fc;
+  return kk;
+}
+
+int md_step(Sys *s, double kick, long long step, bool rebuild_here, int dslot) {
   hipStream_t st = s->ctx->stream;
   Sys::Md &m = s->md;
-  const int n = s->n;
   const bool timing = s->timing_active;
+  const Kick kk = make_kick(m, kick, step);
   // stage 0 of a timed MD step is integrate (+ sort when the list is rebuilt)
   timing_begin_call(s, timing);
   if (s->mixed)
-    hipLaunchKernelGGL(k_kick_drift<float>, dim3(nblk(n, 256)), dim3(256), 0, st, n, kick * m.h, m.h,
+    hipLaunchKernelGGL(k_kick_drift<float>, dim3(nblk(s->npad, 256)), dim3(256), 0, st, s->npad, kk, m.h,
                        P<int>(s->permp), P<double>(s->fout), P<double>(m.minv), P<double>(m.v), P<double>(s->xorig),
                        P<double>(m.xbuild), s->box, P<float4>(s->xs), P<double4>(s->xs64),
                        P<unsigned>(m.dmax) + dslot);
   else
-    hipLaunchKernelGGL(k_kick_drift<double>, dim3(nblk(n, 256)), dim3(256), 0, st, n, kick * m.h, m.h,
+    hipLaunchKernelGGL(k_kick_drift<double>, dim3(nblk(s->npad, 256)), dim3(256), 0, st, s->npad, kk, m.h,
                        P<int>(s->permp), P<double>(s->fout), P<double>(m.minv), P<double>(m.v), P<double>(s->xorig),
                        P<double>(m.xbuild), s->box, P<double4>(s->xs), P<double4>(s->xs64),
                        P<unsigned>(m.dmax) + dslot);
@@ -1219,7 +1281,7 @@ int pmx_md_run(pmx_sys *sys, int nsteps) {
     HIPCHK(hipMemcpyAsync(m.fbak.p, s->fout.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kMaxChunk), 0, st, P<uint32_t>(m.dmax), 0u, (long long)kMaxChunk);
     for (int k = 0; k < K; ++k) {
-      int rc = md_step(s, (first && k == 0) ? 0.5 : 1.0, k == 0 || m.skin == 0.0, k);
+      int rc = md_step(s, (first && k == 0) ? 0.5 : 1.0, m.steps + k, k == 0 || m.skin == 0.0, k);
       if (rc) return rc;
     }
     int ovf = 0;
@@ -1259,7 +1321,7 @@ int pmx_md_run(pmx_sys *sys, int nsteps) {
     HIPCHK(hipMemcpyAsync(m.v.p, m.vbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(s->fout.p, m.fbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
     for (int k = 0; k < bad; ++k) {
-      int rc = md_step(s, (first && k == 0) ? 0.5 : 1.0, k == 0, k);
+      int rc = md_step(s, (first && k == 0) ? 0.5 : 1.0, m.steps + k, k == 0, k);
       if (rc) return rc;
     }
     done += bad;
@@ -1267,7 +1329,10 @@ int pmx_md_run(pmx_sys *sys, int nsteps) {
     first = false;
     m.chunk = std::max(1, bad);
   }
-  hipLaunchKernelGGL(k_half_kick, dim3(nblk(n, 256)), dim3(256), 0, st, n, 0.5 * m.h, P<double>(s->fout),
+  // closing half kick of the last step (global index m.steps - 1)
+  Kick kk = make_kick(m, 0.5, m.steps);
+  kk.half0 = (unsigned long long)(2 * m.steps - 1);
+  hipLaunchKernelGGL(k_half_kick, dim3(nblk(n, 256)), dim3(256), 0, st, n, kk, P<double>(s->fout),
                      P<double>(m.minv), P<double>(m.v));
   HIPCHK(hipStreamSynchronize(st));
   return md_check_overflow(s);
@@ -1311,6 +1376,35 @@ int pmx_md_get_state(pmx_sys *sys, double *x, double *v, double *energies) {
     energies[PMX_MD_E_KINETIC] = ek;
   }
   HIPCHK(hipStreamSynchronize(st));
+  return PMX_OK;
+}
+
+unsigned long long pmx_estimate_bytes(const pmx_sys_desc *desc, int kind) {
+  if (!desc || validate(nullptr, desc) != PMX_OK) return 0;
+  pmx_sys tmp;
+  Sys *s = &tmp.s;
+  s->d = *desc;
+  s->dry = true;
+  if (setup(s) != PMX_OK) return 0;
+  unsigned long long b = s->dry_bytes;
+  // device MD state: velocities, masses, build/backup copies, bonded tables
+  // (bonds and angles about one per atom in the systems served)
+  if (kind == 1) b += 128ull * (unsigned long long)desc->n + 48ull * (unsigned long long)desc->n + 4096;
+  return b;
+}
+
+int pmx_sys_device_bytes(const pmx_sys *sys, unsigned long long *bytes) {
+  if (!sys || !bytes) return PMX_ERR_ARG;
+  unsigned long long b = 0;
+  for (const DevBuf *d : sys->s.all)
+    if (d->p) b += d->bytes;
+  *bytes = b;
+  return PMX_OK;
+}
+
+int pmx_langevin_noise(unsigned long long seed, unsigned long long half, int n, double *g) {
+  if (n < 0 || (n > 0 && !g)) return PMX_ERR_ARG;
+  for (int i = 0; i < n; ++i) langevin_gauss3(seed, half, (uint32_t)i, g + 3 * i);
   return PMX_OK;
 }
 

@@ -1,5 +1,6 @@
 // Shared definitions for the pmx HIP kernels and host code (gfx950).
 #pragma once
+#include <cmath>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -48,6 +49,45 @@ __host__ __device__ inline uint64_t pair_hash(uint32_t i, uint32_t j) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
+}
+
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ten rounds of
+// two 32x32 -> 64-bit products with the published multipliers and Weyl key
+// increments.
+__host__ __device__ inline void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+// Three standard normal numbers for (atom, half kick) of a Langevin run:
+// Philox block at counter (atom, half lo, half hi, 'LANG'), uniforms
+// (x + 1/2) 2^-32, Box-Muller on (u0, u1) and (u2, u3).
+__host__ __device__ inline void langevin_gauss3(uint64_t seed, uint64_t half, uint32_t atom, double g[3]) {
+  const uint32_t ctr[4] = {atom, (uint32_t)half, (uint32_t)(half >> 32), 0x4C414E47u};
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t x[4];
+  philox4x32_10(ctr, key, x);
+  const double s = 2.3283064365386963e-10;  // 2^-32
+  const double u0 = (x[0] + 0.5) * s, u1 = (x[1] + 0.5) * s, u2 = (x[2] + 0.5) * s, u3 = (x[3] + 0.5) * s;
+  const double twopi = 6.283185307179586;
+  const double r0 = sqrt(-2.0 * log(u0)), r1 = sqrt(-2.0 * log(u2));
+  g[0] = r0 * cos(twopi * u1);
+  g[1] = r0 * sin(twopi * u1);
+  g[2] = r1 * cos(twopi * u3);
 }
 
 }  // namespace pmx

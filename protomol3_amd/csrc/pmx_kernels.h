@@ -2,14 +2,16 @@
 //
 // Data layout in HBM (per system):
 //   xorig  double[3N]  positions as the caller hands them (AoS, original order)
-//   perm   int[Npad]   sorted slot -> original atom (-1 for padding slots)
+//   permp  int[Npad]   sorted slot -> original atom (-1 for padding slots)
 //   slot   int[N]      original atom -> sorted slot
-//   xs     R4[Npad]    wrapped coordinates + scaled charge, sorted (R = double
-//                      for the FP64 form, float for the mixed form), SoA-by-atom
+//   xs     R4[Npad]    coordinates + scaled charge, sorted (R = double for the
+//                      FP64 form, float for the mixed form)
 //   ts     int[Npad]   LJ type per sorted slot (-1 padding)
 //   facc   Acc[3Npad]  force accumulators in sorted order (double, or int64
 //                      fixed point 2^-32 for the mixed form)
-// Clusters are 8 consecutive sorted slots; superclusters 64 (one wavefront).
+// Superclusters are 64 slots (one wavefront): up to 64 atoms of one x-y
+// column of the box, consecutive in z, split by x, then y, then z into 8
+// clusters of 8 slots.  Padding slots fill the last supercluster of a column.
 #pragma once
 #include <type_traits>
 
@@ -30,18 +32,6 @@ template <>
 struct Real4<float> {
   using type = float4;
 };
-
-__device__ inline uint32_t expand_bits10(uint32_t v) {
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-__device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
-  return (expand_bits10(x) << 2) | (expand_bits10(y) << 1) | expand_bits10(z);
-}
 
 // Reference minimal image in FP64 with the reference's operation order and
 // no FMA contraction (PeriodicBoundaryConditions.h:72-110).  Returns d^2.
@@ -101,8 +91,8 @@ __device__ inline T wave_sum(T v) {
 // 1. Binning: wrapped coordinates, fine-grid Morton key, reference cell.
 struct BinParams {
   Box box;
-  int nf[3];          // fine grid for clustering
-  double finv[3];     // nf / L
+  int ncx, ncy, nzb;  // column grid in x-y, z bins per column (sort key)
+  double cxinv, cyinv, zinv;  // ncx / Lx, ncy / Ly, nzb / Lz
   int rc_n[3];        // reference cell grid
   double rc_inv[3];   // 1 / reference cell edge
   double delta[3];    // origin - min
@@ -113,7 +103,6 @@ __global__ void k_bin(int n, const double *__restrict__ xorig, BinParams p, uint
   int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
   double w[3];
-  int f[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     double x = xorig[3 * a + k];
@@ -121,10 +110,11 @@ __global__ void k_bin(int n, const double *__restrict__ xorig, BinParams p, uint
     if (v >= p.box.L[k]) v -= p.box.L[k];
     if (v < 0.0) v = 0.0;
     w[k] = v;
-    int c = (int)(v * p.finv[k]);
-    f[k] = c < 0 ? 0 : (c >= p.nf[k] ? p.nf[k] - 1 : c);
   }
-  key[a] = morton3(f[0], f[1], f[2]);
+  // key: x-y column, then z bin
+  const int cx = min((int)(w[0] * p.cxinv), p.ncx - 1), cy = min((int)(w[1] * p.cyinv), p.ncy - 1);
+  const int zb = min((int)(w[2] * p.zinv), p.nzb - 1);
+  key[a] = (uint32_t)((cx * p.ncy + cy) * p.nzb + zb);
   xwrap[3 * a] = w[0];
   xwrap[3 * a + 1] = w[1];
   xwrap[3 * a + 2] = w[2];
@@ -254,25 +244,102 @@ __global__ void k_binsort(long long nbins, const uint32_t *__restrict__ start, i
 }
 
 // Sorted SoA: wrapped coordinates + charge, type, inverse permutation.
+// Superclusters of each column: sc_info[sc] = (first sorted position, atom
+// count) for sc < total, (0, 0) beyond.  One block; columns in chunks of the
+// block size with a running offset.
+__global__ void k_columns(int ncols, int nzb, int n, const uint32_t *__restrict__ start, int nsuper_max,
+                          int2 *__restrict__ sc_info) {
+  __shared__ int s[1024];
+  int carry = 0;
+  for (int base = 0; base < ncols; base += blockDim.x) {
+    const int col = base + threadIdx.x;
+    int cs = 0, cnt = 0;
+    if (col < ncols) {
+      cs = (int)start[(size_t)col * nzb];
+      const int ce = col + 1 < ncols ? (int)start[(size_t)(col + 1) * nzb] : n;
+      cnt = ce - cs;
+    }
+    const int nsc = (cnt + kSuperSize - 1) / kSuperSize;
+    s[threadIdx.x] = nsc;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+      int v = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int first = carry + s[threadIdx.x] - nsc;
+    for (int q = 0; q < nsc && first + q < nsuper_max; ++q)
+      sc_info[first + q] = make_int2(cs + q * kSuperSize, min(kSuperSize, cnt - q * kSuperSize));
+    __syncthreads();
+    carry += s[blockDim.x - 1];
+    __syncthreads();
+  }
+  for (int sc = carry + threadIdx.x; sc < nsuper_max; sc += blockDim.x) sc_info[sc] = make_int2(0, 0);
+}
+
+// Ascending bitonic sort of (key, idx) within aligned segments of SEG lanes.
+template <int SEG>
+__device__ inline void wave_sort(double &key, int &idx) {
+  const int lane = threadIdx.x & 63, loc = lane & (SEG - 1);
+#pragma unroll
+  for (int kk = 2; kk <= SEG; kk <<= 1) {
+#pragma unroll
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      const double ok = __shfl_xor(key, j);
+      const int oi = __shfl_xor(idx, j);
+      const bool up = (loc & kk) == 0;
+      const bool lower = (lane & j) == 0;
+      const bool less = ok < key || (ok == key && oi < idx);
+      if ((lower == up) ? less : !less) {
+        key = ok;
+        idx = oi;
+      }
+    }
+  }
+}
+
+// One wave per supercluster: its atoms (lanes, padding last) sorted by x,
+// halves by y, quarters by z, so slots 8c..8c+7 form compact clusters.
+// Writes permp (slot -> atom, -1 padding) and slot (atom -> slot).
+__global__ void __launch_bounds__(256) k_split(int nsuper_max, const int2 *__restrict__ sc_info,
+                                               const int *__restrict__ order, const double *__restrict__ xwrap,
+                                               int *__restrict__ permp, int *__restrict__ slot) {
+  const int sc = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (sc >= nsuper_max) return;  // whole waves
+  const int2 info = sc_info[sc];
+  int a = lane < info.y ? order[info.x + lane] : -1;
+  int idx = a < 0 ? 0x7fffffff : a;
+  double key = a < 0 ? 1.0e300 : xwrap[3 * a];
+  wave_sort<64>(key, idx);
+  key = idx == 0x7fffffff ? 1.0e300 : xwrap[3 * idx + 1];
+  wave_sort<32>(key, idx);
+  key = idx == 0x7fffffff ? 1.0e300 : xwrap[3 * idx + 2];
+  wave_sort<16>(key, idx);
+  const int s = sc * kSuperSize + lane;
+  a = idx == 0x7fffffff ? -1 : idx;
+  permp[s] = a;
+  if (a >= 0) slot[a] = s;
+}
+
+// Sorted SoA: coordinates + charge and type per slot (padding: type -1).
 // xbuild (MD only, else null) receives the positions the list is built from.
 template <class R>
-__global__ void k_fill_sorted(int n, int npad, const int *__restrict__ perm, const double *__restrict__ xwrap,
+__global__ void k_fill_sorted(int npad, const int *__restrict__ permp, const double *__restrict__ xwrap,
                               const double *__restrict__ q, const int *__restrict__ type,
                               typename Real4<R>::type *__restrict__ xs, double4 *__restrict__ xs64,
-                              int *__restrict__ ts, int *__restrict__ slot, int *__restrict__ permp,
-                              const double *__restrict__ xorig, double *__restrict__ xbuild) {
+                              int *__restrict__ ts, const double *__restrict__ xorig, double *__restrict__ xbuild) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= npad) return;
   typename Real4<R>::type v;
-  if (s < n) {
-    int a = perm[s];
+  const int a = permp[s];
+  if (a >= 0) {
     if (xbuild) {
       xbuild[3 * a] = xorig[3 * a];
       xbuild[3 * a + 1] = xorig[3 * a + 1];
       xbuild[3 * a + 2] = xorig[3 * a + 2];
     }
-    slot[a] = s;
-    permp[s] = a;
     v.x = (R)xwrap[3 * a];
     v.y = (R)xwrap[3 * a + 1];
     v.z = (R)xwrap[3 * a + 2];
@@ -282,7 +349,6 @@ __global__ void k_fill_sorted(int n, int npad, const int *__restrict__ perm, con
   } else {
     v.x = v.y = v.z = v.w = (R)0;
     ts[s] = -1;
-    permp[s] = -1;
     xs64[s] = make_double4(0, 0, 0, 0);
   }
   xs[s] = v;
@@ -290,15 +356,17 @@ __global__ void k_fill_sorted(int n, int npad, const int *__restrict__ perm, con
 
 // ---------------------------------------------------------------------------
 // 2. Cluster bounding boxes (FP32; list-building only) and the cluster list.
-__global__ void k_cluster_bbox(int ncl, int n, const double4 *__restrict__ xs64, float4 *__restrict__ ctr,
-                               float4 *__restrict__ half) {
+// ctr.w = number of real atoms (padding slots skipped); empty clusters get
+// negative half extents.
+__global__ void k_cluster_bbox(int ncl, const int *__restrict__ permp, const double4 *__restrict__ xs64,
+                               float4 *__restrict__ ctr, float4 *__restrict__ half) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncl) return;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   int cnt = 0;
   for (int t = 0; t < kClusterSize; ++t) {
     int s = c * kClusterSize + t;
-    if (s >= n) break;
+    if (permp[s] < 0) continue;
     double4 p = xs64[s];
     lo[0] = fmin(lo[0], p.x);
     lo[1] = fmin(lo[1], p.y);
@@ -364,7 +432,6 @@ struct ListParams {
   int ncl;           // clusters
   int nsuper;
   int cap;           // entries per supercluster
-  int n;             // real atoms
 };
 
 // One block of kListBlock threads per supercluster S: scans the j-clusters
@@ -400,7 +467,7 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const f
     if (cj < q.ncl) {
       float4 jc = ctr[cj], jh = half[cj];
       if (jh.x >= 0.f) {
-        bool jpartial = (cj + 1) * kClusterSize > q.n;
+        const bool jpartial = jc.w < (float)kClusterSize;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
           int ci = S * 8 + c;
@@ -408,7 +475,7 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const f
           float d2 = bbox_dist2(ic[c], ih[c], jc, jh, q.L);
           if (d2 <= q.rlist2) {
             imask |= 1u << c;
-            bool ipartial = (ci + 1) * kClusterSize > q.n;
+            const bool ipartial = ic[c].w < (float)kClusterSize;
             if (ci == cj || ipartial || jpartial) need = true;
             const int code = entry_shift(ic[c], ih[c], jc, jh, q.L, q.Linv, q.skin, per_pair);
             if (shift < 0) shift = code;
@@ -525,10 +592,10 @@ __device__ inline int excl_class(const int *__restrict__ eptr, const int2 *__res
 }
 
 // One wave per entry needing explicit masks: bit (tj*8 + ti) of mask[ci] set
-// when i = slot 8ci+ti, j = slot 8cj+tj are real atoms, i precedes j within
+// when i = slot 8ci+ti, j = slot 8cj+tj are real atoms (perm >= 0), i precedes j within
 // a shared cluster, and the pair is not FULL-excluded; modmask marks MODIFIED.
 __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ maskwork, const int *__restrict__ nmaskwork,
-                                                     int maxmaskwork, int4 *__restrict__ entries, int cap, int n,
+                                                     int maxmaskwork, int4 *__restrict__ entries, int cap,
                                                      const int *__restrict__ perm, const int *__restrict__ eptr,
                                                      const int2 *__restrict__ elist,
                                                      unsigned long long *__restrict__ masks) {
@@ -543,13 +610,13 @@ __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ mas
   const int cj = ent.x;
   const int ti = lane & 7, tj = lane >> 3;
   const int sj = cj * kClusterSize + tj;
-  const int aj = sj < n ? perm[sj] : -1;
+  const int aj = perm[sj];
   for (int c = 0; c < 8; ++c) {
     unsigned long long m = 0, mm = 0;
     if (ent.y & (1 << c)) {
       const int ci = S * 8 + c;
       const int si = ci * kClusterSize + ti;
-      const int ai = si < n ? perm[si] : -1;
+      const int ai = perm[si];
       bool ok = ai >= 0 && aj >= 0 && (ci != cj || ti < tj);
       int cls = 0;
       if (ok) cls = excl_class(eptr, elist, ai, aj);
@@ -1208,7 +1275,8 @@ __device__ inline void grid_coord(double x, double hr, int K, int off, int &base
 // stencil points of one atom go to consecutive lanes (z fastest), so each
 // atomic instruction of a wave updates P^2 short contiguous rows of one
 // atom's stencil; orders 2 (8 points) pack 8 atoms into a wave, orders 6 and
-// 8 loop over their points in steps of 64.  Atoms run in sorted order.
+// 8 loop over their points in steps of 64.  Slots run in sorted order; n
+// counts slots (padding slots return at once).
 template <bool MIXED, int P>
 __global__ void __launch_bounds__(256) k_spread(int n, PmeParams p, const double *__restrict__ xorig,
                                                 const int *__restrict__ perm, const double *__restrict__ q,
@@ -1221,8 +1289,9 @@ __global__ void __launch_bounds__(256) k_spread(int n, PmeParams p, const double
   const int wave = gt >> 6, lane = gt & 63;
   const int s = wave * APW + lane / LPA;
   if (s >= n) return;
-  const int l0 = lane % LPA;
   const int a = perm[s];
+  if (a < 0) return;  // padding slot
+  const int l0 = lane % LPA;
   int b[3];
   double w[3];
   grid_coord(xorig[3 * a], p.hxr, p.nx, p.offx, b[0], w[0]);
@@ -1511,8 +1580,8 @@ __global__ void __launch_bounds__(256) k_gather(int n, int waves, PmeParams p, c
   const int s = (lane / TPA) * waves + wave;
   R fx = 0, fy = 0, fz = 0;
   int a = -1;
-  if (s < n) a = perm[s];
-  if (s < n && r < P * P) {
+  if (s < n) a = perm[s];  // n counts slots; padding slots have a < 0
+  if (a >= 0 && r < P * P) {
     const int i = r / P, j = r - (r / P) * P;
     int b[3];
     double w[3];
@@ -1542,7 +1611,7 @@ __global__ void __launch_bounds__(256) k_gather(int n, int waves, PmeParams p, c
     fy += __shfl_xor(fy, m);
     fz += __shfl_xor(fz, m);
   }
-  if (r == 0 && s < n) {
+  if (r == 0 && a >= 0) {
     const R qa = (R)q[a];
     AccT<MIXED>::add(&facc[3 * s], fx * (R)p.hxr * qa);
     AccT<MIXED>::add(&facc[3 * s + 1], fy * (R)p.hyr * qa);
@@ -1839,27 +1908,54 @@ __global__ void __launch_bounds__(256) k_fixups(FixupArgs<MIXED> a) {
                       a.bond_e);
 }
 
+// Velocity kicks of one leapfrog step.  Leapfrog (LeapfrogIntegrator.cpp):
+// v += F h_kick / m.  Langevin leapfrog (LangevinLeapfrogIntegrator.cpp:
+// 44-71): nhalf half kicks v = v vdt + F fdt / m + g sqrt(fc / m) ndt, g a
+// standard normal triple for (atom, global half-kick index).
+struct Kick {
+  double h_kick;
+  int langevin, nhalf;
+  unsigned long long half0, seed;
+  double vdt, fdt, ndt, fc;
+};
+
+__device__ inline void apply_kick(const Kick &kk, int a, const double f[3], double minv, double v[3]) {
+  if (!kk.langevin) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] += f[k] * kk.h_kick * minv;
+    return;
+  }
+  const double sfm = sqrt(kk.fc * minv) * kk.ndt;
+  for (int hh = 0; hh < kk.nhalf; ++hh) {
+    double g[3];
+    langevin_gauss3(kk.seed, kk.half0 + hh, (uint32_t)a, g);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = v[k] * kk.vdt + f[k] * kk.fdt * minv + g[k] * sfm;
+  }
+}
+
 // Kick (v += F kick h / m) and drift (x += v h) of every atom, FP64
 // (LeapfrogIntegrator.cpp:33-87).  The pair kernel's sorted coordinates then
 // follow each atom continuously from its wrapped position at the last list
 // build (xs64), since list entries carry a fixed image shift, and the largest
 // squared displacement since that build is folded into dmax (float bits).
 template <class R>
-__global__ void k_kick_drift(int n, double kick_h, double h, const int *__restrict__ perm,
+__global__ void k_kick_drift(int n, Kick kk, double h, const int *__restrict__ perm,
                              const double *__restrict__ fout, const double *__restrict__ minv, double *__restrict__ v,
                              double *__restrict__ x, const double *__restrict__ xbuild, Box box,
                              typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64,
                              unsigned int *__restrict__ dmax) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   float d2f = 0.f;
-  if (s < n) {
-    const int a = perm[s];
-    double dd[3];
+  const int a = s < n ? perm[s] : -1;  // n counts slots; padding slots have a < 0
+  if (a >= 0) {
+    double dd[3], vv[3] = {v[3 * a], v[3 * a + 1], v[3 * a + 2]};
+    const double ff[3] = {fout[3 * a], fout[3 * a + 1], fout[3 * a + 2]};
+    apply_kick(kk, a, ff, minv[a], vv);
     double d2 = 0;
     for (int k = 0; k < 3; ++k) {
-      double vk = v[3 * a + k] + fout[3 * a + k] * kick_h * minv[a];
-      v[3 * a + k] = vk;
-      const double xk = x[3 * a + k] + vk * h;
+      v[3 * a + k] = vv[k];
+      const double xk = x[3 * a + k] + vv[k] * h;
       x[3 * a + k] = xk;
       dd[k] = xk - xbuild[3 * a + k];
       d2 += dd[k] * dd[k];
@@ -1877,11 +1973,14 @@ __global__ void k_kick_drift(int n, double kick_h, double h, const int *__restri
   if ((threadIdx.x & 63) == 0) atomicMax(dmax, __float_as_uint(d2f));
 }
 
-__global__ void k_half_kick(int n, double kick_h, const double *__restrict__ fout, const double *__restrict__ minv,
+__global__ void k_half_kick(int n, Kick kk, const double *__restrict__ fout, const double *__restrict__ minv,
                             double *__restrict__ v) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
-  for (int k = 0; k < 3; ++k) v[3 * a + k] += fout[3 * a + k] * kick_h * minv[a];
+  double vv[3] = {v[3 * a], v[3 * a + 1], v[3 * a + 2]};
+  const double ff[3] = {fout[3 * a], fout[3 * a + 1], fout[3 * a + 2]};
+  apply_kick(kk, a, ff, minv[a], vv);
+  for (int k = 0; k < 3; ++k) v[3 * a + k] = vv[k];
 }
 
 // Kinetic energy 0.5 sum m v^2 into per-block partials.

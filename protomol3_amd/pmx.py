@@ -50,14 +50,19 @@ class MdDesc(C.Structure):
     _fields_ = [("nbonds", C.c_int), ("nangles", C.c_int), ("bonds", C.POINTER(C.c_int)),
                 ("bond_par", C.POINTER(C.c_double)), ("angles", C.POINTER(C.c_int)),
                 ("angle_par", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
-                ("timestep_fs", C.c_double), ("skin", C.c_double)]
+                ("timestep_fs", C.c_double), ("skin", C.c_double), ("integrator", C.c_int),
+                ("temperature", C.c_double), ("gamma", C.c_double), ("seed", C.c_ulonglong)]
+
+
+LEAPFROG, LANGEVIN = 0, 1
 
 
 EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_create", "pmx_sys_destroy",
             "pmx_sys_uncache", "pmx_sys_evaluate", "pmx_sys_set_positions", "pmx_sys_compute",
             "pmx_sys_get_forces", "pmx_sys_sync", "pmx_sys_alpha", "pmx_sys_pair_stats", "pmx_pair_hash",
             "pmx_sys_cells", "pmx_sys_stage_times", "pmx_sys_timing_begin", "pmx_sys_timing_end",
-            "pmx_md_init", "pmx_md_set_state", "pmx_md_run", "pmx_md_get_state", "pmx_md_stats"]
+            "pmx_md_init", "pmx_md_set_state", "pmx_md_run", "pmx_md_get_state", "pmx_md_stats",
+            "pmx_langevin_noise", "pmx_sys_device_bytes", "pmx_estimate_bytes"]
 
 _lib = None
 
@@ -97,12 +102,25 @@ def load() -> C.CDLL:
     L.pmx_md_run.argtypes = [vp, C.c_int]
     L.pmx_md_get_state.argtypes = [vp, dp, dp, dp]
     L.pmx_md_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.pmx_langevin_noise.argtypes = [C.c_ulonglong, C.c_ulonglong, C.c_int, C.POINTER(C.c_double)]
+    L.pmx_sys_device_bytes.argtypes = [vp, C.POINTER(C.c_ulonglong)]
+    L.pmx_estimate_bytes.argtypes = [C.POINTER(SysDesc), C.c_int]
+    L.pmx_estimate_bytes.restype = C.c_ulonglong
     _lib = L
     return L
 
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def langevin_noise(seed: int, half: int, n: int) -> np.ndarray:
+    """(n, 3) Langevin noise the device draws for half kick `half` (host)."""
+    g = np.zeros((n, 3))
+    rc = load().pmx_langevin_noise(seed, half, n, _dp(g))
+    if rc:
+        raise PmxError(rc, "pmx_langevin_noise")
+    return g
 
 
 def pair_hash_set(pairs: np.ndarray) -> int:
@@ -157,44 +175,81 @@ class ForceSpec:
         self.cellsize, self.lj, self.coul, self.pme = cellsize, lj, coul, pme
 
 
+def make_desc(s: MolSystem, spec: ForceSpec, precision: int = FP64):
+    """(pmx_sys_desc, arrays it points into) for a system and force spec."""
+    keep = [np.ascontiguousarray(s.charge, np.float64), np.ascontiguousarray(s.atype, np.int32),
+            np.ascontiguousarray(s.molecule, np.int32), np.ascontiguousarray(s.lj, np.float64),
+            np.ascontiguousarray(s.exclusions, np.int32).reshape(-1, 3)]
+    q, t, m, lj, ex = keep
+    d = SysDesc()
+    d.n, d.ntypes, d.nexcl = s.n, lj.shape[0], ex.shape[0]
+    d.charge = _dp(q)
+    d.type = t.ctypes.data_as(C.POINTER(C.c_int))
+    d.molecule = m.ctypes.data_as(C.POINTER(C.c_int))
+    d.lj = _dp(lj)
+    d.excl = ex.ctypes.data_as(C.POINTER(C.c_int))
+    for k in range(3):
+        d.box[k] = float(s.box[k])
+        d.origin[k] = float(s.origin[k])
+    d.csf = float(s.csf)
+    d.cellsize = float(spec.cellsize)
+    if spec.lj is not None:
+        d.lj_switch, d.lj_on, d.lj_rc = spec.lj
+    if spec.coul is not None:
+        d.coul_kind = COUL_CUTOFF
+        d.coul_switch, d.coul_on, d.coul_rc = spec.coul
+    if spec.pme is not None:
+        p = spec.pme
+        d.coul_kind = COUL_PME
+        d.coul_switch = p.get("switch", SW_CUTOFF)
+        d.coul_rc = p["rc"]
+        for k in range(3):
+            d.pme_grid[k] = p["grid"][k]
+        d.pme_order = p.get("order", 4)
+        d.pme_accuracy = p.get("accuracy", 1e-6)
+        d.pme_alpha = p.get("alpha", -1.0)
+        d.pme_terms = p.get("terms", PME_REAL | PME_RECIP | PME_CORR)
+    d.precision = precision
+    return d, keep
+
+
+def make_md_desc(s: MolSystem, timestep_fs: float, skin: float = 1.0, integrator: int = LEAPFROG,
+                 temperature: float = 300.0, gamma: float = 1.0, seed: int = 1234):
+    """(pmx_md_desc, arrays it points into)."""
+    keep = [np.ascontiguousarray(s.bonds, np.int32).reshape(-1, 2),
+            np.ascontiguousarray(s.bond_par, np.float64).reshape(-1, 2),
+            np.ascontiguousarray(s.angles, np.int32).reshape(-1, 3),
+            np.ascontiguousarray(s.angle_par, np.float64).reshape(-1, 2),
+            np.ascontiguousarray(s.mass, np.float64)]
+    b, bp, a, ap, m = keep
+    d = MdDesc()
+    d.nbonds, d.nangles = b.shape[0], a.shape[0]
+    d.bonds = b.ctypes.data_as(C.POINTER(C.c_int))
+    d.bond_par = _dp(bp)
+    d.angles = a.ctypes.data_as(C.POINTER(C.c_int))
+    d.angle_par = _dp(ap)
+    d.mass = _dp(m)
+    d.timestep_fs = timestep_fs
+    d.skin = skin
+    d.integrator = integrator
+    d.temperature = temperature
+    d.gamma = gamma
+    d.seed = seed
+    return d, keep
+
+
+def estimate_bytes(s: MolSystem, spec: ForceSpec, precision: int = FP64, trajectory: bool = False) -> int:
+    """Device bytes the engine would allocate for the system (host-side)."""
+    d, keep = make_desc(s, spec, precision)
+    return int(load().pmx_estimate_bytes(C.byref(d), 1 if trajectory else 0))
+
+
 class System:
     """A resident device system (pmx_sys)."""
 
     def __init__(self, ctx: Context, s: MolSystem, spec: ForceSpec, precision: int = FP64):
         self.ctx, self.n = ctx, s.n
-        self._keep = [np.ascontiguousarray(s.charge, np.float64), np.ascontiguousarray(s.atype, np.int32),
-                      np.ascontiguousarray(s.molecule, np.int32), np.ascontiguousarray(s.lj, np.float64),
-                      np.ascontiguousarray(s.exclusions, np.int32).reshape(-1, 3)]
-        q, t, m, lj, ex = self._keep
-        d = SysDesc()
-        d.n, d.ntypes, d.nexcl = s.n, lj.shape[0], ex.shape[0]
-        d.charge = _dp(q)
-        d.type = t.ctypes.data_as(C.POINTER(C.c_int))
-        d.molecule = m.ctypes.data_as(C.POINTER(C.c_int))
-        d.lj = _dp(lj)
-        d.excl = ex.ctypes.data_as(C.POINTER(C.c_int))
-        for k in range(3):
-            d.box[k] = float(s.box[k])
-            d.origin[k] = float(s.origin[k])
-        d.csf = float(s.csf)
-        d.cellsize = float(spec.cellsize)
-        if spec.lj is not None:
-            d.lj_switch, d.lj_on, d.lj_rc = spec.lj
-        if spec.coul is not None:
-            d.coul_kind = COUL_CUTOFF
-            d.coul_switch, d.coul_on, d.coul_rc = spec.coul
-        if spec.pme is not None:
-            p = spec.pme
-            d.coul_kind = COUL_PME
-            d.coul_switch = p.get("switch", SW_CUTOFF)
-            d.coul_rc = p["rc"]
-            for k in range(3):
-                d.pme_grid[k] = p["grid"][k]
-            d.pme_order = p.get("order", 4)
-            d.pme_accuracy = p.get("accuracy", 1e-6)
-            d.pme_alpha = p.get("alpha", -1.0)
-            d.pme_terms = p.get("terms", PME_REAL | PME_RECIP | PME_CORR)
-        d.precision = precision
+        d, self._keep = make_desc(s, spec, precision)
         self.desc = d
         self.h = C.c_void_p()
         ctx.check(ctx.lib.pmx_sys_create(ctx.h, C.byref(d), C.byref(self.h)), "pmx_sys_create")
@@ -228,6 +283,11 @@ class System:
         self.ctx.check(self.ctx.lib.pmx_sys_alpha(self.h, C.byref(a)), "pmx_sys_alpha")
         return a.value
 
+    def device_bytes(self) -> int:
+        b = C.c_ulonglong()
+        self.ctx.check(self.ctx.lib.pmx_sys_device_bytes(self.h, C.byref(b)), "pmx_sys_device_bytes")
+        return b.value
+
     def pair_stats(self):
         n = C.c_longlong()
         h = C.c_ulonglong()
@@ -247,22 +307,11 @@ class System:
         return dict(zip(STAGES, [float(v) for v in ms]))
 
     # -- device-resident MD -------------------------------------------------
-    def md_init(self, s: MolSystem, timestep_fs: float, skin: float = 1.0):
-        self._md_keep = [np.ascontiguousarray(s.bonds, np.int32).reshape(-1, 2),
-                         np.ascontiguousarray(s.bond_par, np.float64).reshape(-1, 2),
-                         np.ascontiguousarray(s.angles, np.int32).reshape(-1, 3),
-                         np.ascontiguousarray(s.angle_par, np.float64).reshape(-1, 2),
-                         np.ascontiguousarray(s.mass, np.float64)]
-        b, bp, a, ap, m = self._md_keep
-        d = MdDesc()
-        d.nbonds, d.nangles = b.shape[0], a.shape[0]
-        d.bonds = b.ctypes.data_as(C.POINTER(C.c_int))
-        d.bond_par = _dp(bp)
-        d.angles = a.ctypes.data_as(C.POINTER(C.c_int))
-        d.angle_par = _dp(ap)
-        d.mass = _dp(m)
-        d.timestep_fs = timestep_fs
-        d.skin = skin
+    def md_init(self, s: MolSystem, timestep_fs: float, skin: float = 1.0, integrator: int = LEAPFROG,
+                temperature: float = 300.0, gamma: float = 1.0, seed: int = 1234):
+        """Device MD set-up: leapfrog, or Langevin leapfrog at `temperature` (K)
+        with friction `gamma` (1/ps) and noise stream `seed`."""
+        d, self._md_keep = make_md_desc(s, timestep_fs, skin, integrator, temperature, gamma, seed)
         self.ctx.check(self.ctx.lib.pmx_md_init(self.h, C.byref(d)), "pmx_md_init")
 
     def md_set_state(self, x: np.ndarray, v: np.ndarray):

@@ -191,45 +191,106 @@ def water_box(n_side: int = 8, box: float = 24.8, seed: int = 0, jitter: float =
 
 
 def _polymer_walk(n_beads: int, box: float, rng, min_sep: float = 4.0) -> np.ndarray:
-    """Seeded self-avoiding random walk with fixed bond length, in the box."""
+    """Seeded self-avoiding random walk with fixed bond length, in the box:
+    a step is rejected when it comes within min_sep of any earlier bead other
+    than its predecessor (minimum image).  Earlier beads are kept in a cell
+    grid of edge >= min_sep, so each test looks at 27 cells."""
     pts = [rng.uniform(0.25 * box, 0.75 * box, 3)]
-    tries = 0
+    nc = int(box // min_sep)
+    grid = {} if nc >= 3 else None
+    cell = lambda p: tuple(int(c) % nc for c in np.floor((p - box * np.floor(p / box)) / (box / nc)))
+
+    def near(cand, last):
+        if grid is None:
+            prev = np.asarray(pts[:-1])
+            if not len(prev):
+                return False
+        else:
+            cx, cy, cz = cell(cand)
+            idx = [i for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)
+                   for i in grid.get(((cx + dx) % nc, (cy + dy) % nc, (cz + dz) % nc), ()) if i != last]
+            if not idx:
+                return False
+            prev = np.asarray([pts[i] for i in idx])
+        d = prev - cand
+        d -= box * np.rint(d / box)
+        return np.min(np.einsum("ij,ij->i", d, d)) < min_sep * min_sep
+
+    if grid is not None:
+        grid.setdefault(cell(pts[0]), []).append(0)
+    tries = streak = 0
     while len(pts) < n_beads:
         v = rng.standard_normal(3)
         cand = pts[-1] + BEAD_BOND * v / np.linalg.norm(v)
         tries += 1
-        if tries > 200000:
+        if tries > 2000000:
             raise RuntimeError("polymer walk failed")
-        prev = np.asarray(pts[:-1])
-        if len(prev):
-            d = prev - cand
-            d -= box * np.rint(d / box)
-            if np.min(np.einsum("ij,ij->i", d, d)) < min_sep * min_sep:
-                continue
+        if near(cand, len(pts) - 1):
+            streak += 1
+            if streak > 20000 and len(pts) > 8:
+                # trapped: step back 8 beads and continue from there
+                for _ in range(8):
+                    p = pts.pop()
+                    if grid is not None:
+                        grid[cell(p)].remove(len(pts))
+                streak = 0
+            continue
+        streak = 0
         pts.append(cand)
+        if grid is not None:
+            grid.setdefault(cell(cand), []).append(len(pts) - 1)
     return np.asarray(pts)
 
 
+def _clear_of(wat: np.ndarray, beads: np.ndarray, box: float, overlap: float) -> np.ndarray:
+    """Mask of water sites (S, 3 atoms, xyz) with no atom within `overlap` of
+    any bead (minimum image); water atoms are binned in cells of edge >=
+    overlap so each bead tests its 27 neighbouring cells."""
+    S = wat.shape[0]
+    keep = np.ones(S, bool)
+    nc = int(box // overlap)
+    if nc < 3:
+        for b in beads:
+            d = wat - b
+            d -= box * np.rint(d / box)
+            keep &= ~(np.einsum("sak,sak->sa", d, d).min(1) < overlap * overlap)
+        return keep
+    flat = wat.reshape(-1, 3)
+    cid = np.floor((flat - box * np.floor(flat / box)) / (box / nc)).astype(np.int64) % nc
+    key = (cid[:, 0] * nc + cid[:, 1]) * nc + cid[:, 2]
+    order = np.argsort(key, kind="stable")
+    start = np.searchsorted(key[order], np.arange(nc ** 3 + 1))
+    for b in beads:
+        c = np.floor((b - box * np.floor(b / box)) / (box / nc)).astype(np.int64) % nc
+        cells = [(((c[0] + dx) % nc) * nc + (c[1] + dy) % nc) * nc + (c[2] + dz) % nc
+                 for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)]
+        atoms = np.concatenate([order[start[q]:start[q + 1]] for q in cells])
+        if atoms.size == 0:
+            continue
+        d = flat[atoms] - b
+        d -= box * np.rint(d / box)
+        hit = np.einsum("ij,ij->i", d, d) < overlap * overlap
+        keep[atoms[hit] // 3] = False
+    return keep
+
+
 def polymer_water(n_total: int = 26000, box: float = 64.0, n_beads: int = 500, seed: int = 1,
-                  overlap: float = 2.4, jitter: float = 0.3) -> MolSystem:
+                  overlap: float = 2.4, jitter: float = 0.3, oversample: float = 1.25) -> MolSystem:
     """Solvated bead polymer: a seeded random-walk chain of n_beads (bond 3.8 A,
     LJ eps 0.2 / sigma 4.0, charges +-0.5 alternating) plus 3-site waters on a
     jittered lattice; waters with any atom within `overlap` A of a bead are
     removed and the remaining lattice sites are drawn in seeded order until the
-    system holds exactly n_total atoms."""
+    system holds exactly n_total atoms.  The lattice holds `oversample`
+    times the waters needed."""
     rng = np.random.default_rng(seed)
     n_water = (n_total - n_beads) // 3
     if n_beads + 3 * n_water != n_total:
         raise ValueError("n_total - n_beads must be a multiple of 3")
     beads = _polymer_walk(n_beads, box, rng)
-    n_side = int(math.ceil((n_water * 1.25) ** (1.0 / 3.0)))
+    n_side = int(math.ceil((n_water * oversample) ** (1.0 / 3.0)))
     o = _jittered_lattice(n_side, box / n_side, jitter, rng)
     wat = _water_atoms(o, rng)                     # (S, 3, 3)
-    keep = np.ones(o.shape[0], bool)
-    for b in beads:
-        d = wat - b
-        d -= box * np.rint(d / box)
-        keep &= ~(np.einsum("sak,sak->sa", d, d).min(1) < overlap * overlap)
+    keep = _clear_of(wat, beads, box, overlap)
     sites = np.flatnonzero(keep)
     if sites.size < n_water:
         raise RuntimeError("not enough lattice sites after overlap removal")
@@ -288,4 +349,10 @@ def replica_box(n_atoms_target: int, seed: int, density: float = 0.0334,
     n_water = max(1, (n_atoms_target - n_beads) // 3)
     n_total = n_beads + 3 * n_water
     box = (n_total / density) ** (1.0 / 3.0)
-    return polymer_water(n_total=n_total, box=box, n_beads=n_beads, seed=seed)
+    # a dense solute can clear too many lattice sites: denser lattices then
+    for over in (1.25, 1.6, 2.2, 3.0):
+        try:
+            return polymer_water(n_total=n_total, box=box, n_beads=n_beads, seed=seed, oversample=over)
+        except RuntimeError as err:
+            last = err
+    raise last

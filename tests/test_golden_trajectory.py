@@ -11,7 +11,7 @@ import numpy as np
 from golden import fixtures as FX
 from oracle import md
 from oracle import oracle as O
-from protomol3_amd import charmm
+from protomol3_amd import charmm, outputs
 
 W = FX.W423
 
@@ -42,7 +42,7 @@ def water423():
 SPEC = dict(cellsize=6.5, lj=(O.SW_C2, 0.1, 6.5), coul=(O.SW_C1, 0.0, 6.5))
 
 
-def test_oracle_reproduces_reference_golden_trajectory():
+def test_oracle_reproduces_reference_golden_trajectory(tmp_path):
     s, v0 = water423()
     sp = O.make_spec(**SPEC)
     pos, vel, f0, f200, en = load_golden()
@@ -54,3 +54,28 @@ def test_oracle_reproduces_reference_golden_trajectory():
     assert np.abs(f - f200).max() < 1e-8
     for k, ref in en.items():
         assert abs(e[k] - ref) <= 6e-8 * abs(ref) + 1e-5, k
+    check_outputs(s, f_init, f, e, tmp_path)
+
+
+def check_outputs(s, f_init, f_end, e_end, tmp_path):
+    """The written force trajectory and energies line against the reference's
+    expected files, line by line."""
+    names = ["OT" if m > 10 else "HT" for m in s.mass]
+    path = str(tmp_path / "forces.xyz")
+    outputs.write_xyz_frame(path, f_init, names)
+    outputs.write_xyz_frame(path, f_end, names)
+    ours = open(path).read().splitlines()
+    ref = open(os.path.join(W, "expected.forces")).read().splitlines()
+    assert len(ours) == len(ref) and ours[0] == ref[0] and ours[1] == ref[1]
+    for a, b in zip(ours[1:], ref[1:]):
+        fa, fb = a.split("\t"), b.split("\t")
+        assert len(fa) == len(fb) and fa[0] == fb[0]
+        if len(fa) > 1:
+            assert len(fa[1]) == len(fb[1]) == 72
+            va, vb = np.array(fa[1].split(), float), np.array(fb[1].split(), float)
+            assert np.abs(va - vb).max() <= 1e-8 * max(1.0, np.abs(vb).max())
+    line = outputs.energies_line(100.0, e_end, dof=3 * len(s.mass) - 3, volume=float(np.prod(s.box)))
+    refl = open(os.path.join(W, "expected.energy")).read().splitlines()[1]
+    assert len(line) == len(refl)
+    for a, b in zip(line.split(), refl.split()):
+        assert abs(float(a) - float(b)) <= 6e-8 * abs(float(b)) + 1e-5
