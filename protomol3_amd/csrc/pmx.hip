@@ -22,7 +22,8 @@ using namespace pmx;
 struct pmx_ctx {
   int device = 0;
   int cus = 256;  // compute units of the device
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // main stream: sort, list, pair kernel, boundary pairs, finalize
+  hipStream_t stream2 = nullptr;  // PME, exclusion correction and bonded terms, concurrent with the pair kernel
   std::string err;
 };
 
@@ -35,9 +36,14 @@ struct DevBuf {
 };
 
 // Timed stages: sort (+ integrate), list (+ accumulator zeroing), pair,
-// spread, fft (+ convolution), gather, fixups (boundary pairs, exclusion
-// correction, bonded terms), finalize.
+// spread, fft (+ convolution), gather, fixups (exclusion correction, bonded
+// terms), finalize (boundary pairs, force unsort).  Events: 0 start, 1 sorted,
+// 2 list built and accumulators zeroed, 4 spread done = fork (main stream), 3
+// pair done (main stream), 5 fft, 6 gather, 7 correction + bonded done
+// (second stream), 8 end.  The second stream's stages run beside the pair
+// kernel.
 constexpr int kStages = 8;
+constexpr int kStageEv[kStages][2] = {{0, 1}, {1, 2}, {4, 3}, {2, 4}, {4, 5}, {5, 6}, {6, 7}, {7, 8}};
 static_assert(kStages == PMX_STAGES, "stage count of the C ABI");
 
 struct Sys {
@@ -67,7 +73,7 @@ struct Sys {
   // device buffers
   std::vector<DevBuf *> all;
   DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
-      sc_info, ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
+      perm2, sc_info, ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
       pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec,
       greal, theta, twx, twy, twz, border, nborder, border_acc;
   int maxborder = 65536;
@@ -76,6 +82,7 @@ struct Sys {
   unsigned long long last_hash = 0;
   float stage_ms[kStages] = {0};
   hipEvent_t *ev = nullptr;                    // event set of the current timed call
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;  // main stream <-> second stream per force pass
   std::vector<std::vector<hipEvent_t>> evsets;  // one set of kStages + 1 per timed call since timing_begin
   size_t evused = 0;
   bool timing_active = false;                   // MD steps record events (timing_begin .. timing_end)
@@ -406,7 +413,8 @@ int setup(Sys *s) {
   if ((rc = A(s->xorig, 24 * N)) || (rc = A(s->xwrap, 24 * N)) || (rc = A(s->q_d, 8 * N)) ||
       (rc = A(s->type_d, 4 * N)) || (rc = A(s->key, 4 * N)) || (rc = A(s->hist, 4 * s->nbins)) ||
       (rc = A(s->start, 4 * s->nbins)) || (rc = A(s->cursor, 4 * s->nbins)) ||
-      (rc = A(s->scansum, 4 * (size_t)s->nscanblocks)) || (rc = A(s->perm, 4 * N)) || (rc = A(s->permp, 4 * NP)) ||
+      (rc = A(s->scansum, 4 * (size_t)s->nscanblocks)) || (rc = A(s->perm, 4 * N)) || (rc = A(s->perm2, 4 * N)) ||
+      (rc = A(s->permp, 4 * NP)) ||
       (rc = A(s->slot, 4 * N)) || (rc = A(s->refcell, 4 * N)) || (rc = A(s->xs, 32 * NP)) ||
       (rc = A(s->xs64, 32 * NP)) || (rc = A(s->ts, 4 * NP)) || (rc = A(s->sc_info, 8 * (size_t)s->nsuper)) ||
       (rc = A(s->ctr, 16 * (size_t)s->ncl)) ||
@@ -436,6 +444,8 @@ int setup(Sys *s) {
   }
 
   hipStream_t st = s->ctx->stream;
+  if (!s->fork_ev) HIPCHK(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
+  if (!s->join_ev) HIPCHK(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
   HIPCHK(hipMemcpyAsync(s->q_d.p, s->q.data(), 8 * N, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(s->type_d.p, s->type.data(), 4 * N, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(s->ljtab.p, s->lj.data(), 8 * s->lj.size(), hipMemcpyHostToDevice, st));
@@ -556,11 +566,12 @@ int setup(Sys *s) {
 }
 
 // Launches of the kernels templated on the force kinds: the pair kernel
-// (phase 0) and the fused fixup pass (phase 1: boundary pairs, exclusion
-// correction, bonded terms).
+// (phase 0, main stream), the boundary pairs (phase 1, main stream) and the
+// exclusion correction with the bonded terms (phase 2, second stream); the
+// last two are one kernel with different block ranges.
 template <bool MIXED, int C, int LJ, int CS>
 void launch_pair_phase(Sys *s, bool stats, int phase) {
-  hipStream_t st = s->ctx->stream;
+  hipStream_t st = phase == 2 ? s->ctx->stream2 : s->ctx->stream;
   using R4 = typename Real4<typename std::conditional<MIXED, float, double>::type>::type;
   auto *facc = P<typename AccT<MIXED>::type>(s->facc);
   if (phase == 0) {
@@ -583,7 +594,7 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
     return;
   }
   FixupArgs<MIXED> a{};
-  a.border_blocks = 4;
+  a.border_blocks = phase == 1 ? 4 : 0;
   a.pp = s->pp;
   a.stats = stats ? 1 : 0;
   a.border = P<int2>(s->border);
@@ -599,7 +610,7 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
   a.border_e = P<unsigned long long>(s->border_acc);
   a.border_stat = P<unsigned long long>(s->border_acc) + 2;
   const bool corr = s->pme_on && (s->d.pme_terms & PMX_PME_CORR) && !s->etab.empty();
-  a.corr_blocks = corr ? s->corr_blocks : 0;
+  a.corr_blocks = corr && phase == 2 ? s->corr_blocks : 0;
   a.ne = (int)s->etab.size();
   a.etab = P<int4>(s->etab_d);
   a.q = P<double>(s->q_d);
@@ -608,7 +619,7 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
   a.csf = s->d.csf;
   a.corr_e = P<double>(s->corr_e);
   int bonded_blocks = 0;
-  if (s->md.on && s->md.nb + s->md.na > 0) {
+  if (phase == 2 && s->md.on && s->md.nb + s->md.na > 0) {
     bonded_blocks = (int)nblk(s->md.nb + s->md.na, 256);
     a.nbond = s->md.nb;
     a.nangle = s->md.na;
@@ -619,8 +630,9 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
     a.bond_e = P<double>(s->md.bond_e);
   }
   a.facc = facc;
-  hipLaunchKernelGGL((k_fixups<MIXED, C, LJ, CS>), dim3(a.border_blocks + a.corr_blocks + bonded_blocks), dim3(256), 0,
-                     st, a);
+  if (a.border_blocks + a.corr_blocks + bonded_blocks > 0)
+    hipLaunchKernelGGL((k_fixups<MIXED, C, LJ, CS>), dim3(a.border_blocks + a.corr_blocks + bonded_blocks), dim3(256),
+                       0, st, a);
 }
 
 template <bool MIXED>
@@ -666,21 +678,27 @@ void launch_gather(Sys *s, hipStream_t st) {
                      P<typename AccT<MIXED>::type>(s->facc));
 }
 
+// Charge spreading on the main stream (before the pair kernel: alone it
+// runs at full rate, next to the pair kernel its many small waves would
+// not); the charge grid was zeroed by the k_zero launch of the pass.
+template <bool MIXED>
+void run_spread(Sys *s) {
+  hipStream_t st = s->ctx->stream;
+  switch (s->pme.order) {
+    case 2: launch_spread<MIXED, 2>(s, st); break;
+    case 4: launch_spread<MIXED, 4>(s, st); break;
+    case 6: launch_spread<MIXED, 6>(s, st); break;
+    default: launch_spread<MIXED, 8>(s, st); break;
+  }
+}
+
+// FFT convolution and force gather on the second stream.
 template <bool MIXED>
 int run_pme(Sys *s, bool timing) {
   using T = typename std::conditional<MIXED, float, double>::type;
-  hipStream_t st = s->ctx->stream;
+  hipStream_t st = s->ctx->stream2;
   const PmeParams &pm = s->pme;
-  auto *facc = P<typename AccT<MIXED>::type>(s->facc);
   if (s->d.pme_terms & PMX_PME_RECIP) {
-    // the charge grid was zeroed by the k_zero launch at the start of the pass
-    switch (pm.order) {
-      case 2: launch_spread<MIXED, 2>(s, st); break;
-      case 4: launch_spread<MIXED, 4>(s, st); break;
-      case 6: launch_spread<MIXED, 6>(s, st); break;
-      default: launch_spread<MIXED, 8>(s, st); break;
-    }
-    if (timing) hipEventRecord(s->ev[4], st);
     const size_t cx = sizeof(Cx<T>);
     const int zb = (pm.nx * pm.ny + s->zl - 1) / s->zl;
     hipLaunchKernelGGL((k_fft_z_r2c<T, MIXED>), dim3(zb), dim3(128), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
@@ -702,7 +720,6 @@ int run_pme(Sys *s, bool timing) {
       default: launch_gather<MIXED, 8>(s, st); break;
     }
   } else if (timing) {
-    hipEventRecord(s->ev[4], st);
     hipEventRecord(s->ev[5], st);
   }
   if (timing) hipEventRecord(s->ev[6], st);
@@ -748,12 +765,12 @@ int rebuild(Sys *s, bool timing) {
                      P<uint32_t>(s->scansum), P<uint32_t>(s->start), P<uint32_t>(s->cursor));
   hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<uint32_t>(s->key), P<uint32_t>(s->cursor),
                      P<int>(s->perm));
-  hipLaunchKernelGGL(k_binsort, dim3(nblk(s->nbins, 256)), dim3(256), 0, st, s->nbins, P<uint32_t>(s->start), n,
-                     P<int>(s->perm));
+  hipLaunchKernelGGL(k_binrank, dim3(nblk(n, 256)), dim3(256), 0, st, n, s->nbins, P<uint32_t>(s->key),
+                     P<uint32_t>(s->start), P<int>(s->perm), P<int>(s->perm2));
   hipLaunchKernelGGL(k_columns, dim3(1), dim3(1024), 0, st, s->ncols, s->bin.nzb, n, P<uint32_t>(s->start),
                      s->nsuper, P<int2>(s->sc_info));
   hipLaunchKernelGGL(k_split, dim3(nblk((long long)s->nsuper * 64, 256)), dim3(256), 0, st, s->nsuper,
-                     P<int2>(s->sc_info), P<int>(s->perm), P<double>(s->xwrap), P<int>(s->permp), P<int>(s->slot));
+                     P<int2>(s->sc_info), P<int>(s->perm2), P<double>(s->xwrap), P<int>(s->permp), P<int>(s->slot));
   double *xbuild = s->md.on ? P<double>(s->md.xbuild) : nullptr;
   if (s->mixed)
     hipLaunchKernelGGL(k_fill_sorted<float>, dim3(nblk(npad, 256)), dim3(256), 0, st, npad, P<int>(s->permp),
@@ -810,6 +827,18 @@ int forces(Sys *s, unsigned flags, bool energies) {
   }
   hipLaunchKernelGGL(k_zero, dim3(std::max(1u, nblk(3ll * npad, 256)), nr), dim3(256), 0, st, z);
   if (timing) hipEventRecord(s->ev[2], st);
+  if (s->pme_on && (s->d.pme_terms & PMX_PME_RECIP)) {
+    if (s->mixed)
+      run_spread<true>(s);
+    else
+      run_spread<false>(s);
+  }
+  if (timing) hipEventRecord(s->ev[4], st);
+  // fork: FFT, gather, exclusion correction and bonded terms on the second
+  // stream beside the pair kernel; all add into the zeroed accumulators
+  hipStream_t st2 = s->ctx->stream2;
+  HIPCHK(hipEventRecord(s->fork_ev, st));
+  HIPCHK(hipStreamWaitEvent(st2, s->fork_ev, 0));
   if (s->mixed)
     run_pair_phase<true>(s, stats, 0);
   else
@@ -819,15 +848,21 @@ int forces(Sys *s, unsigned flags, bool energies) {
     int rc = s->mixed ? run_pme<true>(s, timing) : run_pme<false>(s, timing);
     if (rc) return rc;
   } else if (timing) {
-    hipEventRecord(s->ev[4], st);
-    hipEventRecord(s->ev[5], st);
-    hipEventRecord(s->ev[6], st);
+    hipEventRecord(s->ev[5], st2);
+    hipEventRecord(s->ev[6], st2);
   }
+  if (s->mixed)
+    run_pair_phase<true>(s, stats, 2);
+  else
+    run_pair_phase<false>(s, stats, 2);
+  if (timing) hipEventRecord(s->ev[7], st2);
+  HIPCHK(hipEventRecord(s->join_ev, st2));
+  // boundary pairs after the pair kernel, then join
   if (s->mixed)
     run_pair_phase<true>(s, stats, 1);
   else
     run_pair_phase<false>(s, stats, 1);
-  if (timing) hipEventRecord(s->ev[7], st);
+  HIPCHK(hipStreamWaitEvent(st, s->join_ev, 0));
   if (s->mixed)
     hipLaunchKernelGGL(k_unsort_forces<true>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
                        P<unsigned long long>(s->facc), P<double>(s->fout));
@@ -835,7 +870,7 @@ int forces(Sys *s, unsigned flags, bool energies) {
     hipLaunchKernelGGL(k_unsort_forces<false>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
                        P<double>(s->facc), P<double>(s->fout));
   if (energies) reduce_energies(s, stats);
-  if (timing) hipEventRecord(s->ev[kStages], st);
+  if (timing) hipEventRecord(s->ev[8], st);
   HIPCHK(hipGetLastError());
   return PMX_OK;
 }
@@ -864,7 +899,8 @@ int finish(Sys *s, unsigned flags, double *f_aos, double *energies) {
   HIPCHK(hipStreamSynchronize(st));
   if (ovf) return fail(s->ctx, PMX_ERR_OVERFLOW, "cluster pair list capacity exceeded");
   if ((flags & PMX_EVAL_TIMING) && s->ev)
-    for (int k = 0; k < kStages; ++k) hipEventElapsedTime(&s->stage_ms[k], s->ev[k], s->ev[k + 1]);
+    for (int k = 0; k < kStages; ++k)
+      hipEventElapsedTime(&s->stage_ms[k], s->ev[kStageEv[k][0]], s->ev[kStageEv[k][1]]);
   if (f_aos)
     for (size_t k = 0; k < f.size(); ++k) f_aos[k] += f[k];
   if (energies) {
@@ -925,6 +961,11 @@ int pmx_ctx_create(int device, pmx_ctx **out) {
     delete c;
     return PMX_ERR_HIP;
   }
+  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return PMX_ERR_HIP;
+  }
   *out = c;
   return PMX_OK;
 }
@@ -933,6 +974,7 @@ int pmx_ctx_destroy(pmx_ctx *ctx) {
   if (!ctx) return PMX_ERR_ARG;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) hipStreamDestroy(ctx->stream2);
   delete ctx;
   return PMX_OK;
 }
@@ -962,8 +1004,11 @@ int pmx_sys_destroy(pmx_sys *sys) {
   Sys *s = &sys->s;
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
+  hipStreamSynchronize(s->ctx->stream2);
   for (DevBuf *b : s->all)
     if (b->p) hipFree(b->p);
+  if (s->fork_ev) hipEventDestroy(s->fork_ev);
+  if (s->join_ev) hipEventDestroy(s->join_ev);
   for (auto &set : s->evsets)
     for (auto &e : set) hipEventDestroy(e);
   delete sys;
@@ -1068,7 +1113,7 @@ int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls) {
   for (size_t c = 0; c < m; ++c)
     for (int k = 0; k < kStages; ++k) {
       float t = 0.f;
-      HIPCHK(hipEventElapsedTime(&t, s->evsets[c][k], s->evsets[c][k + 1]));
+      HIPCHK(hipEventElapsedTime(&t, s->evsets[c][kStageEv[k][0]], s->evsets[c][kStageEv[k][1]]));
       avg_ms[k] += t;
     }
   for (int k = 0; k < kStages; ++k) avg_ms[k] = m ? avg_ms[k] / (float)m : 0.f;

@@ -227,20 +227,19 @@ __global__ void k_scatter(int n, const uint32_t *__restrict__ key, uint32_t *__r
   if (a < n) perm[atomicAdd(&cursor[key[a]], 1u)] = a;
 }
 
-// Orders each bin's slots by atom index (bins hold a handful of atoms).
-__global__ void k_binsort(long long nbins, const uint32_t *__restrict__ start, int n, int *__restrict__ perm) {
-  long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
-  int s0 = start[b];
-  int s1 = b + 1 < nbins ? (int)start[b + 1] : n;
-  for (int i = s0 + 1; i < s1; ++i) {
-    int v = perm[i], j = i - 1;
-    while (j >= s0 && perm[j] > v) {
-      perm[j + 1] = perm[j];
-      --j;
-    }
-    perm[j + 1] = v;
-  }
+// Orders each bin by atom index: position p of the scatter output moves to
+// its bin's start plus the number of smaller atom indices in the bin (bins
+// hold a handful of atoms; the result does not depend on the scatter order).
+__global__ void k_binrank(int n, long long nbins, const uint32_t *__restrict__ key,
+                          const uint32_t *__restrict__ start, const int *__restrict__ in, int *__restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int a = in[p];
+  const uint32_t b = key[a];
+  const int s0 = (int)start[b], s1 = b + 1 < (uint64_t)nbins ? (int)start[b + 1] : n;
+  int rank = 0;
+  for (int q = s0; q < s1; ++q) rank += in[q] < a;
+  out[s0 + rank] = a;
 }
 
 // Sorted SoA: wrapped coordinates + charge, type, inverse permutation.
@@ -458,14 +457,24 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const f
     ih[c] = ci < q.ncl ? half[ci] : make_float4(-1, -1, -1, 0);
   }
   int total = 0;  // block-uniform
+  // j-cluster boxes are loaded one iteration ahead
+  float4 jcn = make_float4(0, 0, 0, 0), jhn = make_float4(-1, -1, -1, 0);
+  if (S * 8 + (int)threadIdx.x < q.ncl) {
+    jcn = ctr[S * 8 + threadIdx.x];
+    jhn = half[S * 8 + threadIdx.x];
+  }
   for (int base = S * 8; base < q.ncl; base += kListBlock) {
     const int cj = base + threadIdx.x;
+    const float4 jc = jcn, jh = jhn;
+    if (cj + kListBlock < q.ncl) {
+      jcn = ctr[cj + kListBlock];
+      jhn = half[cj + kListBlock];
+    }
     uint32_t imask = 0;
     bool need = false;
     int shift = -1;        // image shift code of the entry (see entry_shift)
     bool per_pair = false; // no single shift is the minimum image for every pair
     if (cj < q.ncl) {
-      float4 jc = ctr[cj], jh = half[cj];
       if (jh.x >= 0.f) {
         const bool jpartial = jc.w < (float)kClusterSize;
 #pragma unroll
@@ -604,31 +613,55 @@ __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ mas
   if (nw > maxmaskwork) nw = maxmaskwork;
   const int nwaves = gridDim.x * (blockDim.x / kWave);
   for (int w = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6); w < nw; w += nwaves) {
-  const int eidx = maskwork[w];
-  const int S = eidx / cap;
-  int4 ent = entries[eidx];
-  const int cj = ent.x;
-  const int ti = lane & 7, tj = lane >> 3;
-  const int sj = cj * kClusterSize + tj;
-  const int aj = perm[sj];
-  for (int c = 0; c < 8; ++c) {
-    unsigned long long m = 0, mm = 0;
-    if (ent.y & (1 << c)) {
+    const int eidx = maskwork[w];
+    const int S = eidx / cap;
+    const int4 ent = entries[eidx];
+    const int cj = ent.x;
+    const int ti = lane & 7, tj = lane >> 3;
+    const int aj = perm[cj * kClusterSize + tj];
+    // the eight i-clusters' exclusion lookups advance together (one
+    // dependent load chain instead of eight)
+    int ai[8], lo[8], hi[8], cls[8];
+    bool ok[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
       const int ci = S * 8 + c;
-      const int si = ci * kClusterSize + ti;
-      const int ai = perm[si];
-      bool ok = ai >= 0 && aj >= 0 && (ci != cj || ti < tj);
-      int cls = 0;
-      if (ok) cls = excl_class(eptr, elist, ai, aj);
-      m = __ballot(ok && cls != 1);
-      mm = __ballot(ok && cls == 2);
+      ai[c] = (ent.y & (1 << c)) ? perm[ci * kClusterSize + ti] : -1;
+      ok[c] = ai[c] >= 0 && aj >= 0 && (ci != cj || ti < tj);
+      lo[c] = ok[c] ? eptr[ai[c]] : 0;
+      hi[c] = ok[c] ? eptr[ai[c] + 1] - 1 : -1;
+      cls[c] = 0;
     }
-    if (lane == 0) {
-      masks[2 * ((size_t)w * 8 + c)] = m;
-      masks[2 * ((size_t)w * 8 + c) + 1] = mm;
+    bool any = true;
+    while (any) {
+      any = false;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (lo[c] <= hi[c]) {
+          const int mid = (lo[c] + hi[c]) >> 1;
+          const int2 v = elist[mid];
+          if (v.x == aj) {
+            cls[c] = v.y;
+            hi[c] = lo[c] - 1;
+          } else if (v.x < aj) {
+            lo[c] = mid + 1;
+          } else {
+            hi[c] = mid - 1;
+          }
+          any = true;
+        }
+      }
     }
-  }
-  if (lane == 0) entries[eidx].z = w * 8;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const unsigned long long m = __ballot(ok[c] && cls[c] != 1);
+      const unsigned long long mm = __ballot(ok[c] && cls[c] == 2);
+      if (lane == 0) {
+        masks[2 * ((size_t)w * 8 + c)] = (ent.y & (1 << c)) ? m : 0ull;
+        masks[2 * ((size_t)w * 8 + c) + 1] = (ent.y & (1 << c)) ? mm : 0ull;
+      }
+    }
+    if (lane == 0) entries[eidx].z = w * 8;
   }
 }
 
