@@ -189,8 +189,18 @@ int pmx_md_init(pmx_sys *sys, const pmx_md_desc *desc);
 int pmx_md_set_state(pmx_sys *sys, const double *x, const double *v);
 /* LeapfrogIntegrator::run(nsteps): half kick + drift, forces, (kick + drift,
  * forces) x (nsteps-1), half kick.  Returns after queueing and checking the
- * chunks (synchronous). */
+ * chunks (synchronous).  Two chunks are in flight at a time, so the device
+ * does not wait for the host's check of a chunk. */
 int pmx_md_run(pmx_sys *sys, int nsteps);
+/* Batched replicas: pmx_md_run(systems[i], nsteps) for every i, with the
+ * systems' chunks interleaved.  Systems on the same context share its
+ * streams and one host thread; systems on different contexts run
+ * concurrently on the device, one host thread per context.  Each system's
+ * trajectory is the one pmx_md_run gives it alone.  No counterpart in the
+ * reference, which runs one system per process (ProtoMolApp.cpp); it serves
+ * many independent replicas of uneven size (Hybrid Monte Carlo proposals,
+ * replica ensembles).  Returns the first error of any system. */
+int pmx_md_run_many(pmx_sys *const *systems, int nsys, int nsteps);
 /* Current positions, velocities and energies (energies[PMX_MD_E_COUNT]:
  * potential terms at x, kinetic energy of v; right after pmx_md_set_state
  * the forces are computed first).  Any pointer may be NULL. */
@@ -199,7 +209,8 @@ int pmx_md_get_state(pmx_sys *sys, double *x, double *v, double *energies);
  * for atoms i < n, the numbers the device draws for (seed, half kick `half`).
  * No GPU needed. */
 int pmx_langevin_noise(unsigned long long seed, unsigned long long half, int n, double *g);
-/* Counters: list rebuilds and rolled-back chunks since pmx_md_init. */
+/* Counters: committed steps, list rebuilds of committed chunks and rolled-back
+ * chunks since pmx_md_init. */
 int pmx_md_stats(const pmx_sys *sys, long long *steps, long long *rebuilds, long long *rollbacks);
 
 #ifdef __cplusplus

@@ -7,9 +7,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <set>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <type_traits>
 #include <vector>
@@ -73,7 +75,7 @@ struct Sys {
   // device buffers
   std::vector<DevBuf *> all;
   DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
-      perm2, sc_info, ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
+      perm2, sc_info, sc_col, colsc, sctr, shalf, sorg, ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
       pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec,
       greal, theta, twx, twy, twz, border, nborder, border_acc;
   int maxborder = 65536;
@@ -103,6 +105,10 @@ struct Sys {
     int integrator = PMX_MD_LEAPFROG;
     double vdt = 1, fdt = 0, ndt = 0, fc = 0;  // Langevin half-kick constants
     unsigned long long seed = 0;
+    // two chunks in flight: backups, displacement records and their pinned
+    // host copies (per slot: kMaxChunk displacements + the overflow flag)
+    unsigned *host = nullptr;
+    hipEvent_t chunk_ev[2] = {nullptr, nullptr};
   } md;
 };
 constexpr int kMaxChunk = 128;
@@ -367,6 +373,10 @@ int setup(Sys *s) {
   pp.guard = 4.0 * (2.0 * std::sqrt(3.0) * rcmax * 3.0 * Lmax * eps + 3.0 * pp.rc2max * eps);
   pp.csf = d.csf;
   pp.ntypes = d.ntypes;
+  for (int k = 0; k < 3; ++k) {
+    pp.Lhi[k] = origin_round(bx.L[k]);
+    pp.Llo[k] = bx.L[k] - pp.Lhi[k];
+  }
 
   s->pme_on = d.coul_kind == COUL_PME;
   s->alpha = 0;
@@ -396,6 +406,17 @@ int setup(Sys *s) {
   lp.ncl = s->ncl;
   lp.nsuper = s->nsuper;
   {
+    // supercluster pruning radius: the list radius plus a slack that covers
+    // the FP32 rounding of the boxes; column reach from the column widths
+    // (atoms of columns kx + 1 apart are more than kx widths apart in x)
+    const double rl = rcmax + margin + s->md.skin + 0.02;
+    lp.rlist2_sc = (float)(rl * rl);
+    lp.ncx = bp.ncx;
+    lp.ncy = bp.ncy;
+    lp.kx = (int)std::floor(rl / (bx.L[0] / bp.ncx)) + 1;
+    lp.ky = (int)std::floor(rl / (bx.L[1] / bp.ncy)) + 1;
+  }
+  {
     // generous per-supercluster capacity from the cluster density
     const double rho_cl = s->ncl / vol;
     const double reach = 2.0 * rcmax + 24.0;
@@ -417,6 +438,9 @@ int setup(Sys *s) {
       (rc = A(s->permp, 4 * NP)) ||
       (rc = A(s->slot, 4 * N)) || (rc = A(s->refcell, 4 * N)) || (rc = A(s->xs, 32 * NP)) ||
       (rc = A(s->xs64, 32 * NP)) || (rc = A(s->ts, 4 * NP)) || (rc = A(s->sc_info, 8 * (size_t)s->nsuper)) ||
+      (rc = A(s->sc_col, 4 * (size_t)s->nsuper)) || (rc = A(s->colsc, 4 * ((size_t)s->ncols + 1))) ||
+      (rc = A(s->sctr, 16 * (size_t)s->nsuper)) || (rc = A(s->shalf, 16 * (size_t)s->nsuper)) ||
+      (rc = A(s->sorg, 16 * (size_t)s->nsuper)) ||
       (rc = A(s->ctr, 16 * (size_t)s->ncl)) ||
       (rc = A(s->half, 16 * (size_t)s->ncl)) || (rc = A(s->entries, 16 * (size_t)s->nsuper * s->cap)) ||
       (rc = A(s->count, 4 * (size_t)s->nsuper)) || (rc = A(s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
@@ -589,8 +613,8 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
                 P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow)};
     hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, stats ? 1 : 0,
                        P<int2>(s->tiles), P<int>(s->ntiles), P<int>(s->tilectr), P<int4>(s->entries),
-                       P<int>(s->count), s->cap, P<unsigned long long>(s->masks), P<R4>(s->xs), P<int>(s->ts),
-                       P<int>(s->permp), P<double>(s->ljtab), facc, out);
+                       P<int>(s->count), s->cap, P<unsigned long long>(s->masks), P<R4>(s->xs),
+                       P<float4>(s->sorg), P<int>(s->ts), P<int>(s->permp), P<double>(s->ljtab), facc, out);
     return;
   }
   FixupArgs<MIXED> a{};
@@ -601,6 +625,7 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
   a.nborder = P<int>(s->nborder);
   a.maxborder = s->maxborder;
   a.xs = P<R4>(s->xs);
+  a.sorg = P<float4>(s->sorg);
   a.ts = P<int>(s->ts);
   a.perm = P<int>(s->permp);
   a.xorig = P<double>(s->xorig);
@@ -768,25 +793,28 @@ int rebuild(Sys *s, bool timing) {
   hipLaunchKernelGGL(k_binrank, dim3(nblk(n, 256)), dim3(256), 0, st, n, s->nbins, P<uint32_t>(s->key),
                      P<uint32_t>(s->start), P<int>(s->perm), P<int>(s->perm2));
   hipLaunchKernelGGL(k_columns, dim3(1), dim3(1024), 0, st, s->ncols, s->bin.nzb, n, P<uint32_t>(s->start),
-                     s->nsuper, P<int2>(s->sc_info));
+                     s->nsuper, P<int2>(s->sc_info), P<int>(s->sc_col), P<int>(s->colsc));
   hipLaunchKernelGGL(k_split, dim3(nblk((long long)s->nsuper * 64, 256)), dim3(256), 0, st, s->nsuper,
                      P<int2>(s->sc_info), P<int>(s->perm2), P<double>(s->xwrap), P<int>(s->permp), P<int>(s->slot));
   double *xbuild = s->md.on ? P<double>(s->md.xbuild) : nullptr;
   if (s->mixed)
     hipLaunchKernelGGL(k_fill_sorted<float>, dim3(nblk(npad, 256)), dim3(256), 0, st, npad, P<int>(s->permp),
                        P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<float4>(s->xs),
-                       P<double4>(s->xs64), P<int>(s->ts), P<double>(s->xorig), xbuild);
+                       P<double4>(s->xs64), P<int>(s->ts), P<double>(s->xorig), xbuild,
+                       P<float4>(s->sorg));
   else
     hipLaunchKernelGGL(k_fill_sorted<double>, dim3(nblk(npad, 256)), dim3(256), 0, st, npad, P<int>(s->permp),
                        P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<double4>(s->xs),
-                       P<double4>(s->xs64), P<int>(s->ts), P<double>(s->xorig), xbuild);
+                       P<double4>(s->xs64), P<int>(s->ts), P<double>(s->xorig), xbuild,
+                       P<float4>(s->sorg));
   if (timing) hipEventRecord(s->ev[1], st);
   hipLaunchKernelGGL(k_cluster_bbox, dim3(nblk(s->ncl, 256)), dim3(256), 0, st, s->ncl, P<int>(s->permp),
-                     P<double4>(s->xs64),
-                     P<float4>(s->ctr), P<float4>(s->half));
+                     P<double4>(s->xs64), P<float4>(s->ctr), P<float4>(s->half), P<float4>(s->sctr),
+                     P<float4>(s->shalf));
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, P<uint32_t>(s->nmaskwork), 0u, 1ll);
-  hipLaunchKernelGGL(k_build_list, dim3(s->nsuper), dim3(kListBlock), 0, st, s->lp,
-                     P<float4>(s->ctr), P<float4>(s->half), P<int4>(s->entries), P<int>(s->count),
+  hipLaunchKernelGGL(k_build_list, dim3(s->nsuper), dim3(kListBlock), 0, st, s->lp, P<int>(s->sc_col),
+                     P<int>(s->colsc), P<float4>(s->sctr), P<float4>(s->shalf), P<float4>(s->ctr),
+                     P<float4>(s->half), P<int4>(s->entries), P<int>(s->count),
                      P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork, P<int>(s->overflow));
   if (!s->etab.empty())
     hipLaunchKernelGGL(k_mark_exclusions, dim3(nblk((long long)s->etab.size(), 256)), dim3(256), 0, st,
@@ -1009,6 +1037,9 @@ int pmx_sys_destroy(pmx_sys *sys) {
     if (b->p) hipFree(b->p);
   if (s->fork_ev) hipEventDestroy(s->fork_ev);
   if (s->join_ev) hipEventDestroy(s->join_ev);
+  for (auto &e : s->md.chunk_ev)
+    if (e) hipEventDestroy(e);
+  if (s->md.host) hipHostFree(s->md.host);
   for (auto &set : s->evsets)
     for (auto &e : set) hipEventDestroy(e);
   delete sys;
@@ -1192,10 +1223,14 @@ int pmx_md_init(pmx_sys *sys, const pmx_md_desc *d) {
   if ((rc = dalloc(s, m.bonds, 8 * b.size())) || (rc = dalloc(s, m.bpar, 16 * bp.size())) ||
       (rc = dalloc(s, m.angles, 16 * a.size())) || (rc = dalloc(s, m.apar, 16 * ap.size())) ||
       (rc = dalloc(s, m.minv, 8 * N)) || (rc = dalloc(s, m.v, 24 * N)) || (rc = dalloc(s, m.xbuild, 24 * N)) ||
-      (rc = dalloc(s, m.xbak, 24 * N)) || (rc = dalloc(s, m.vbak, 24 * N)) || (rc = dalloc(s, m.fbak, 24 * N)) ||
-      (rc = dalloc(s, m.dmax, 4 * kMaxChunk)) || (rc = dalloc(s, m.bond_e, 16 * (size_t)nbonded_blocks)) ||
+      (rc = dalloc(s, m.xbak, 2 * 24 * N)) || (rc = dalloc(s, m.vbak, 2 * 24 * N)) ||
+      (rc = dalloc(s, m.fbak, 2 * 24 * N)) || (rc = dalloc(s, m.dmax, 4 * 2 * kMaxChunk)) || (rc = dalloc(s, m.bond_e, 16 * (size_t)nbonded_blocks)) ||
       (rc = dalloc(s, m.ke_e, 8 * (size_t)nblk(N, 256))))
     return rc;
+  if (!m.host) {
+    HIPCHK(hipHostMalloc((void **)&m.host, 4 * 2 * (kMaxChunk + 1), hipHostMallocDefault));
+    for (auto &e : m.chunk_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   hipStream_t st = s->ctx->stream;
   if (m.nb) HIPCHK(hipMemcpyAsync(m.bonds.p, b.data(), 8 * b.size(), hipMemcpyHostToDevice, st));
   if (m.nb) HIPCHK(hipMemcpyAsync(m.bpar.p, bp.data(), 16 * bp.size(), hipMemcpyHostToDevice, st));
@@ -1283,16 +1318,15 @@ int md_step(Sys *s, double kick, long long step, bool rebuild_here, int dslot) {
     hipLaunchKernelGGL(k_kick_drift<float>, dim3(nblk(s->npad, 256)), dim3(256), 0, st, s->npad, kk, m.h,
                        P<int>(s->permp), P<double>(s->fout), P<double>(m.minv), P<double>(m.v), P<double>(s->xorig),
                        P<double>(m.xbuild), s->box, P<float4>(s->xs), P<double4>(s->xs64),
-                       P<unsigned>(m.dmax) + dslot);
+                       P<float4>(s->sorg), P<unsigned>(m.dmax) + dslot);
   else
     hipLaunchKernelGGL(k_kick_drift<double>, dim3(nblk(s->npad, 256)), dim3(256), 0, st, s->npad, kk, m.h,
                        P<int>(s->permp), P<double>(s->fout), P<double>(m.minv), P<double>(m.v), P<double>(s->xorig),
                        P<double>(m.xbuild), s->box, P<double4>(s->xs), P<double4>(s->xs64),
-                       P<unsigned>(m.dmax) + dslot);
+                       P<float4>(s->sorg), P<unsigned>(m.dmax) + dslot);
   if (rebuild_here) {
     int rc = rebuild(s, timing);
     if (rc) return rc;
-    ++m.rebuilds;
   } else if (timing) {
     hipEventRecord(s->ev[1], st);
     hipEventRecord(s->ev[2], st);
@@ -1300,87 +1334,258 @@ int md_step(Sys *s, double kick, long long step, bool rebuild_here, int dslot) {
   return forces(s, timing ? PMX_EVAL_TIMING : 0u, false);
 }
 
-}  // namespace
+// A chunk of K steps from the committed state: its state is backed up in
+// slot `slot`, step 0 rebuilds the list, and the per-step displacements and
+// the overflow flag are copied to the slot's pinned host record, after which
+// the slot's event fires.
+struct Chunk {
+  int K;
+  long long step0;
+  bool first;
+  int slot;
+};
 
-int pmx_md_run(pmx_sys *sys, int nsteps) {
-  if (!sys || nsteps < 0) return PMX_ERR_ARG;
-  Sys *s = &sys->s;
+int enqueue_chunk(Sys *s, const Chunk &c) {
+  Sys::Md &m = s->md;
+  hipStream_t st = s->ctx->stream;
+  const size_t n = s->n, off = c.slot * 3 * n;
+  HIPCHK(hipMemcpyAsync(P<double>(m.xbak) + off, s->xorig.p, 24 * n, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(P<double>(m.vbak) + off, m.v.p, 24 * n, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(P<double>(m.fbak) + off, s->fout.p, 24 * n, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kMaxChunk), 0, st, P<uint32_t>(m.dmax) + c.slot * kMaxChunk, 0u,
+                     (long long)kMaxChunk);
+  for (int k = 0; k < c.K; ++k) {
+    int rc = md_step(s, (c.first && k == 0) ? 0.5 : 1.0, c.step0 + k, k == 0 || m.skin == 0.0,
+                     c.slot * kMaxChunk + k);
+    if (rc) return rc;
+  }
+  unsigned *h = m.host + c.slot * (kMaxChunk + 1);
+  HIPCHK(hipMemcpyAsync(h, P<uint32_t>(m.dmax) + c.slot * kMaxChunk, 4 * (size_t)c.K, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(h + kMaxChunk, s->overflow.p, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(m.chunk_ev[c.slot], st));
+  return PMX_OK;
+}
+
+// Waits for everything queued (a speculative chunk included), then restores
+// the state backed up in slot `slot`.
+int restore_chunk(Sys *s, int slot) {
+  Sys::Md &m = s->md;
+  hipStream_t st = s->ctx->stream;
+  HIPCHK(hipStreamSynchronize(st));
+  const size_t n = s->n, off = slot * 3 * n;
+  HIPCHK(hipMemcpyAsync(s->xorig.p, P<double>(m.xbak) + off, 24 * n, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(m.v.p, P<double>(m.vbak) + off, 24 * n, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(s->fout.p, P<double>(m.fbak) + off, 24 * n, hipMemcpyDeviceToDevice, st));
+  return PMX_OK;
+}
+
+// Progress of one pmx_md_run on one system (see md_advance).
+struct MdRun {
+  int nsteps = 0;
+  int done = 0;     // committed steps of this run
+  int planned = 0;  // committed + queued
+  int next_slot = 0;
+  int overflow_retries = 0;
+  std::deque<Chunk> inflight;
+  bool finished = false;
+};
+
+int md_begin(Sys *s, MdRun &r, int nsteps) {
   Sys::Md &m = s->md;
   if (!m.on || !s->have_positions) return fail(s->ctx, PMX_ERR_ARG, "pmx_md_init and pmx_md_set_state first");
-  if (nsteps == 0) return PMX_OK;
+  r = MdRun{};
+  r.nsteps = nsteps;
   hipSetDevice(s->ctx->device);
+  if (nsteps == 0) {
+    r.finished = true;
+    return PMX_OK;
+  }
+  return md_ensure_forces(s);
+}
+
+// The steps run in chunks that each start with a list rebuild.  Two chunks
+// are in flight: while the device runs chunk c + 1 (queued on the assumption
+// that chunk c keeps every atom within half the skin of its build position),
+// the host reads chunk c's displacements.  If chunk c crossed the limit at
+// step b, the speculative chunk is discarded, the state of chunk c's start is
+// restored and its first b steps (which used a valid list) are replayed; the
+// trajectory is the same as with a rebuild at every step.  The device never
+// waits for the host between chunks.
+//
+// One call queues chunks up to two in flight, then settles the oldest one if
+// it has completed (waiting for it when `block`).  *progressed tells whether
+// anything was queued or settled.  After the last step the closing half kick
+// is queued and r.finished is set.
+int md_advance(Sys *s, MdRun &r, bool block, bool *progressed) {
+  Sys::Md &m = s->md;
   hipStream_t st = s->ctx->stream;
   const int n = s->n;
-  if (int rc = md_ensure_forces(s)) return rc;
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
   const float lim2 = (float)(0.25 * m.skin * m.skin);
-  int overflow_retries = 0;
-  int done = 0;
-  bool first = true;
-  std::vector<float> dm(kMaxChunk);
-  while (done < nsteps) {
-    int K = std::min(m.chunk, nsteps - done);
-    // snapshot of the committed state at the chunk start
-    HIPCHK(hipMemcpyAsync(m.xbak.p, s->xorig.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(m.vbak.p, m.v.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(m.fbak.p, s->fout.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kMaxChunk), 0, st, P<uint32_t>(m.dmax), 0u, (long long)kMaxChunk);
-    for (int k = 0; k < K; ++k) {
-      int rc = md_step(s, (first && k == 0) ? 0.5 : 1.0, m.steps + k, k == 0 || m.skin == 0.0, k);
-      if (rc) return rc;
+  *progressed = false;
+  if (r.finished) return PMX_OK;
+  while (r.inflight.size() < 2 && r.planned < r.nsteps) {
+    Chunk c{std::min(m.chunk, r.nsteps - r.planned), m.steps + (r.planned - r.done), r.planned == 0, r.next_slot};
+    if (int rc = enqueue_chunk(s, c)) return rc;
+    r.inflight.push_back(c);
+    r.planned += c.K;
+    r.next_slot ^= 1;
+    *progressed = true;
+  }
+  if (!r.inflight.empty()) {
+    const Chunk c = r.inflight.front();
+    if (block) {
+      HIPCHK(hipEventSynchronize(m.chunk_ev[c.slot]));
+    } else {
+      const hipError_t q = hipEventQuery(m.chunk_ev[c.slot]);
+      if (q == hipErrorNotReady) return PMX_OK;
+      if (q != hipSuccess) return fail(s->ctx, PMX_ERR_HIP, hipGetErrorString(q));
     }
-    int ovf = 0;
-    HIPCHK(hipMemcpyAsync(dm.data(), m.dmax.p, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&ovf, s->overflow.p, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (ovf) {
-      // a list or mask capacity was exceeded: grow, roll back, replay the chunk
-      if (++overflow_retries > 6) return fail(s->ctx, PMX_ERR_OVERFLOW, "pair list capacity exceeded during MD");
-      int rc = grow_and_retry(s);
-      if (rc) return rc;
-      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
-      HIPCHK(hipMemcpyAsync(s->xorig.p, m.xbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpyAsync(m.v.p, m.vbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpyAsync(s->fout.p, m.fbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
-      continue;
+    r.inflight.pop_front();
+    *progressed = true;
+    const unsigned *h = m.host + c.slot * (kMaxChunk + 1);
+    if (h[kMaxChunk]) {
+      // a list or mask capacity was exceeded: grow, roll back, replay
+      if (++r.overflow_retries > 6) return fail(s->ctx, PMX_ERR_OVERFLOW, "pair list capacity exceeded during MD");
+      if (int rc = restore_chunk(s, c.slot)) return rc;
+      if (int rc = grow_and_retry(s)) return rc;
+      r.inflight.clear();
+      r.planned = r.done;
+      return PMX_OK;
     }
     int bad = -1;
     float worst = 0.f;
-    for (int k = 1; k < K; ++k) {
-      worst = std::max(worst, dm[k]);
-      if (m.skin > 0 && dm[k] > lim2) {
+    for (int k = 1; k < c.K; ++k) {
+      const float d = __builtin_bit_cast(float, h[k]);
+      worst = std::max(worst, d);
+      if (m.skin > 0 && d > lim2) {
         bad = k;
         break;
       }
     }
     if (bad < 0) {
-      done += K;
-      first = false;
-      m.steps += K;
-      if (m.skin > 0 && K == m.chunk && worst < 0.25f * lim2) m.chunk = std::min(kMaxChunk, 2 * m.chunk);
-      continue;
+      r.done += c.K;
+      m.steps += c.K;
+      m.rebuilds += m.skin > 0 ? 1 : c.K;
+      if (m.skin > 0 && c.K == m.chunk && worst < 0.25f * lim2) m.chunk = std::min(kMaxChunk, 2 * m.chunk);
+    } else {
+      // roll back to the chunk's start and replay the steps that used a
+      // valid list (deterministic: the replay reproduces them exactly)
+      ++m.rollbacks;
+      if (int rc = restore_chunk(s, c.slot)) return rc;
+      r.inflight.clear();
+      for (int k = 0; k < bad; ++k) {
+        int rc = md_step(s, (c.first && k == 0) ? 0.5 : 1.0, c.step0 + k, k == 0, k);
+        if (rc) return rc;
+      }
+      r.done += bad;
+      m.steps += bad;
+      m.rebuilds += 1;
+      r.planned = r.done;
+      m.chunk = std::max(1, bad);
     }
-    // roll back and replay the steps that used a valid list
-    ++m.rollbacks;
-    HIPCHK(hipMemcpyAsync(s->xorig.p, m.xbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(m.v.p, m.vbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(s->fout.p, m.fbak.p, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
-    for (int k = 0; k < bad; ++k) {
-      int rc = md_step(s, (first && k == 0) ? 0.5 : 1.0, m.steps + k, k == 0, k);
-      if (rc) return rc;
-    }
-    done += bad;
-    m.steps += bad;
-    first = false;
-    m.chunk = std::max(1, bad);
   }
-  // closing half kick of the last step (global index m.steps - 1)
-  Kick kk = make_kick(m, 0.5, m.steps);
-  kk.half0 = (unsigned long long)(2 * m.steps - 1);
-  hipLaunchKernelGGL(k_half_kick, dim3(nblk(n, 256)), dim3(256), 0, st, n, kk, P<double>(s->fout),
-                     P<double>(m.minv), P<double>(m.v));
-  HIPCHK(hipStreamSynchronize(st));
+  if (r.done == r.nsteps && r.inflight.empty()) {
+    // closing half kick of the last step (global index m.steps - 1)
+    Kick kk = make_kick(m, 0.5, m.steps);
+    kk.half0 = (unsigned long long)(2 * m.steps - 1);
+    hipLaunchKernelGGL(k_half_kick, dim3(nblk(n, 256)), dim3(256), 0, st, n, kk, P<double>(s->fout),
+                       P<double>(m.minv), P<double>(m.v));
+    r.finished = true;
+  }
+  return PMX_OK;
+}
+
+int md_end(Sys *s) {
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
   return md_check_overflow(s);
+}
+
+}  // namespace
+
+int pmx_md_run(pmx_sys *sys, int nsteps) {
+  if (!sys || nsteps < 0) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  MdRun r;
+  if (int rc = md_begin(s, r, nsteps)) return rc;
+  if (r.finished) return PMX_OK;
+  bool progressed;
+  while (!r.finished)
+    if (int rc = md_advance(s, r, true, &progressed)) return rc;
+  return md_end(s);
+}
+
+int pmx_md_run_many(pmx_sys *const *systems, int nsys, int nsteps) {
+  if (!systems || nsys < 0 || nsteps < 0) return PMX_ERR_ARG;
+  for (int i = 0; i < nsys; ++i) {
+    if (!systems[i]) return PMX_ERR_ARG;
+    for (int j = 0; j < i; ++j)
+      if (systems[j] == systems[i]) return fail(systems[i]->s.ctx, PMX_ERR_ARG, "a system is listed twice");
+  }
+  // systems grouped by context; one host thread drives each group
+  std::vector<pmx_ctx *> ctxs;
+  std::vector<std::vector<Sys *>> groups;
+  for (int i = 0; i < nsys; ++i) {
+    Sys *s = &systems[i]->s;
+    size_t g = std::find(ctxs.begin(), ctxs.end(), s->ctx) - ctxs.begin();
+    if (g == ctxs.size()) {
+      ctxs.push_back(s->ctx);
+      groups.emplace_back();
+    }
+    groups[g].push_back(s);
+  }
+  std::vector<int> status(groups.size(), PMX_OK);
+  auto drive = [&](size_t g) {
+    std::vector<Sys *> &sys = groups[g];
+    std::vector<MdRun> runs(sys.size());
+    for (size_t i = 0; i < sys.size(); ++i)
+      if (int rc = md_begin(sys[i], runs[i], nsteps)) {
+        status[g] = rc;
+        return;
+      }
+    for (;;) {
+      bool any = false, progressed_any = false;
+      for (size_t i = 0; i < sys.size(); ++i) {
+        if (runs[i].finished) continue;
+        any = true;
+        bool p = false;
+        if (int rc = md_advance(sys[i], runs[i], false, &p)) {
+          status[g] = rc;
+          return;
+        }
+        progressed_any |= p;
+      }
+      if (!any) break;
+      if (!progressed_any) {
+        // nothing completed: wait for the oldest chunk of the first
+        // unfinished system
+        for (size_t i = 0; i < sys.size(); ++i)
+          if (!runs[i].finished) {
+            bool p = false;
+            if (int rc = md_advance(sys[i], runs[i], true, &p)) {
+              status[g] = rc;
+              return;
+            }
+            break;
+          }
+      }
+    }
+    for (Sys *s : sys)
+      if (int rc = md_end(s)) {
+        status[g] = rc;
+        return;
+      }
+  };
+  if (groups.size() == 1) {
+    drive(0);
+  } else {
+    std::vector<std::thread> threads;
+    for (size_t g = 0; g < groups.size(); ++g) threads.emplace_back(drive, g);
+    for (auto &t : threads) t.join();
+  }
+  for (int rc : status)
+    if (rc) return rc;
+  return PMX_OK;
 }
 
 int pmx_md_get_state(pmx_sys *sys, double *x, double *v, double *energies) {
@@ -1432,9 +1637,10 @@ unsigned long long pmx_estimate_bytes(const pmx_sys_desc *desc, int kind) {
   s->dry = true;
   if (setup(s) != PMX_OK) return 0;
   unsigned long long b = s->dry_bytes;
-  // device MD state: velocities, masses, build/backup copies, bonded tables
-  // (bonds and angles about one per atom in the systems served)
-  if (kind == 1) b += 128ull * (unsigned long long)desc->n + 48ull * (unsigned long long)desc->n + 4096;
+  // device MD state: velocities, masses, build positions, two chunk backups
+  // of x, v and f, bonded tables (bonds and angles about one per atom in the
+  // systems served)
+  if (kind == 1) b += 200ull * (unsigned long long)desc->n + 48ull * (unsigned long long)desc->n + 8192;
   return b;
 }
 

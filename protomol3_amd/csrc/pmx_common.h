@@ -40,7 +40,17 @@ struct PairParams {
   int ntypes;
   int coul;          // COUL_*
   int has_lj;
+  // box edges split as Lhi + Llo with Lhi a multiple of 1/16 A, so that image
+  // shifts of supercluster origins (also multiples of 1/16) add exactly
+  double Lhi[3], Llo[3];
 };
+
+// Sorted coordinates are stored relative to the origin of their supercluster
+// (the position of its first atom rounded to a multiple of kOriginQuantum),
+// so FP32 keeps about 1e-6 A of precision in any box size; origin
+// differences and multiples of Lhi are exact in FP32.
+constexpr double kOriginQuantum = 1.0 / 16.0;
+__host__ __device__ inline double origin_round(double x) { return rint(x * 16.0) * kOriginQuantum; }
 
 // Order-independent pair hash: splitmix64 of (i << 32 | j), i < j.
 __host__ __device__ inline uint64_t pair_hash(uint32_t i, uint32_t j) {

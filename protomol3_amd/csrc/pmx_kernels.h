@@ -242,12 +242,12 @@ __global__ void k_binrank(int n, long long nbins, const uint32_t *__restrict__ k
   out[s0 + rank] = a;
 }
 
-// Sorted SoA: wrapped coordinates + charge, type, inverse permutation.
 // Superclusters of each column: sc_info[sc] = (first sorted position, atom
-// count) for sc < total, (0, 0) beyond.  One block; columns in chunks of the
-// block size with a running offset.
+// count) and sc_col[sc] = its column for sc < total, (0, 0) and -1 beyond;
+// colsc[col] = first supercluster of column col, colsc[ncols] = total.  One
+// block; columns in chunks of the block size with a running offset.
 __global__ void k_columns(int ncols, int nzb, int n, const uint32_t *__restrict__ start, int nsuper_max,
-                          int2 *__restrict__ sc_info) {
+                          int2 *__restrict__ sc_info, int *__restrict__ sc_col, int *__restrict__ colsc) {
   __shared__ int s[1024];
   int carry = 0;
   for (int base = 0; base < ncols; base += blockDim.x) {
@@ -268,13 +268,20 @@ __global__ void k_columns(int ncols, int nzb, int n, const uint32_t *__restrict_
       __syncthreads();
     }
     const int first = carry + s[threadIdx.x] - nsc;
-    for (int q = 0; q < nsc && first + q < nsuper_max; ++q)
+    if (col < ncols) colsc[col] = first;
+    for (int q = 0; q < nsc && first + q < nsuper_max; ++q) {
       sc_info[first + q] = make_int2(cs + q * kSuperSize, min(kSuperSize, cnt - q * kSuperSize));
+      sc_col[first + q] = col;
+    }
     __syncthreads();
     carry += s[blockDim.x - 1];
     __syncthreads();
   }
-  for (int sc = carry + threadIdx.x; sc < nsuper_max; sc += blockDim.x) sc_info[sc] = make_int2(0, 0);
+  if (threadIdx.x == 0) colsc[ncols] = carry;
+  for (int sc = carry + threadIdx.x; sc < nsuper_max; sc += blockDim.x) {
+    sc_info[sc] = make_int2(0, 0);
+    sc_col[sc] = -1;
+  }
 }
 
 // Ascending bitonic sort of (key, idx) within aligned segments of SEG lanes.
@@ -322,15 +329,23 @@ __global__ void __launch_bounds__(256) k_split(int nsuper_max, const int2 *__res
   if (a >= 0) slot[a] = s;
 }
 
-// Sorted SoA: coordinates + charge and type per slot (padding: type -1).
+// Sorted SoA: coordinates relative to the supercluster origin + charge, and
+// type per slot (padding: type -1); sorg[S] = origin of supercluster S (the
+// wrapped position of its first atom rounded to 1/16 A; zero when empty).
 // xbuild (MD only, else null) receives the positions the list is built from.
 template <class R>
 __global__ void k_fill_sorted(int npad, const int *__restrict__ permp, const double *__restrict__ xwrap,
                               const double *__restrict__ q, const int *__restrict__ type,
                               typename Real4<R>::type *__restrict__ xs, double4 *__restrict__ xs64,
-                              int *__restrict__ ts, const double *__restrict__ xorig, double *__restrict__ xbuild) {
+                              int *__restrict__ ts, const double *__restrict__ xorig, double *__restrict__ xbuild,
+                              float4 *__restrict__ sorg) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= npad) return;
+  const int a0 = permp[s & ~(kSuperSize - 1)];
+  double o[3] = {0.0, 0.0, 0.0};
+  if (a0 >= 0)
+    for (int k = 0; k < 3; ++k) o[k] = origin_round(xwrap[3 * a0 + k]);
+  if ((s & (kSuperSize - 1)) == 0) sorg[s / kSuperSize] = make_float4((float)o[0], (float)o[1], (float)o[2], 0.f);
   typename Real4<R>::type v;
   const int a = permp[s];
   if (a >= 0) {
@@ -339,9 +354,9 @@ __global__ void k_fill_sorted(int npad, const int *__restrict__ permp, const dou
       xbuild[3 * a + 1] = xorig[3 * a + 1];
       xbuild[3 * a + 2] = xorig[3 * a + 2];
     }
-    v.x = (R)xwrap[3 * a];
-    v.y = (R)xwrap[3 * a + 1];
-    v.z = (R)xwrap[3 * a + 2];
+    v.x = (R)(xwrap[3 * a] - o[0]);
+    v.y = (R)(xwrap[3 * a + 1] - o[1]);
+    v.z = (R)(xwrap[3 * a + 2] - o[2]);
     v.w = (R)q[a];
     ts[s] = type[a];
     xs64[s] = make_double4(xwrap[3 * a], xwrap[3 * a + 1], xwrap[3 * a + 2], q[a]);
@@ -356,11 +371,15 @@ __global__ void k_fill_sorted(int npad, const int *__restrict__ permp, const dou
 // ---------------------------------------------------------------------------
 // 2. Cluster bounding boxes (FP32; list-building only) and the cluster list.
 // ctr.w = number of real atoms (padding slots skipped); empty clusters get
-// negative half extents.
+// negative half extents.  The eight clusters of a supercluster are eight
+// consecutive lanes; their union box, widened by kScSlack, is the
+// supercluster box (sctr, shalf) the list build prunes with.
+constexpr float kScSlack = 1e-3f;
 __global__ void k_cluster_bbox(int ncl, const int *__restrict__ permp, const double4 *__restrict__ xs64,
-                               float4 *__restrict__ ctr, float4 *__restrict__ half) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncl) return;
+                               float4 *__restrict__ ctr, float4 *__restrict__ half, float4 *__restrict__ sctr,
+                               float4 *__restrict__ shalf) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;  // ncl is a multiple of 8: whole lane groups leave together
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   int cnt = 0;
   for (int t = 0; t < kClusterSize; ++t) {
@@ -378,14 +397,38 @@ __global__ void k_cluster_bbox(int ncl, const int *__restrict__ permp, const dou
   if (cnt == 0) {
     ctr[c] = make_float4(0, 0, 0, 0);
     half[c] = make_float4(-1, -1, -1, 0);  // empty
-    return;
+  } else {
+    ctr[c] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])),
+                         (float)(0.5 * (lo[2] + hi[2])), (float)cnt);
+    // round the half extents up by a relative ulp margin so the FP32 box
+    // contains the FP64 atoms
+    half[c] = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.0000005f + 1e-6f,
+                          (float)(0.5 * (hi[1] - lo[1])) * 1.0000005f + 1e-6f,
+                          (float)(0.5 * (hi[2] - lo[2])) * 1.0000005f + 1e-6f, 0.f);
   }
-  ctr[c] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])), (float)(0.5 * (lo[2] + hi[2])),
-                       (float)cnt);
-  // round the half extents up by a relative ulp margin so the FP32 box contains the FP64 atoms
-  half[c] = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.0000005f + 1e-6f,
-                        (float)(0.5 * (hi[1] - lo[1])) * 1.0000005f + 1e-6f,
-                        (float)(0.5 * (hi[2] - lo[2])) * 1.0000005f + 1e-6f, 0.f);
+  // supercluster union over the 8-lane group
+  int scnt = cnt;
+#pragma unroll
+  for (int m = 1; m < kClusterSize; m <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fmin(lo[k], __shfl_xor(lo[k], m));
+      hi[k] = fmax(hi[k], __shfl_xor(hi[k], m));
+    }
+    scnt += __shfl_xor(scnt, m);
+  }
+  if ((c & (kClusterSize - 1)) == 0) {
+    const int S = c / kClusterSize;
+    if (scnt == 0) {
+      sctr[S] = make_float4(0, 0, 0, 0);
+      shalf[S] = make_float4(-1, -1, -1, 0);
+    } else {
+      sctr[S] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])),
+                            (float)(0.5 * (lo[2] + hi[2])), (float)scnt);
+      shalf[S] = make_float4((float)(0.5 * (hi[0] - lo[0])) + kScSlack, (float)(0.5 * (hi[1] - lo[1])) + kScSlack,
+                             (float)(0.5 * (hi[2] - lo[2])) + kScSlack, 0.f);
+    }
+  }
 }
 
 __device__ inline float bbox_dist2(float4 ca, float4 ha, float4 cb, float4 hb, const float *L) {
@@ -428,110 +471,193 @@ struct ListParams {
   float Linv[3];
   float skin;        // MD pair-list buffer (0 outside MD)
   float rlist2;      // (rc_max + margin)^2
+  float rlist2_sc;   // supercluster pruning radius^2 (rlist plus slack)
   int ncl;           // clusters
   int nsuper;
   int cap;           // entries per supercluster
+  int ncx, ncy;      // x-y column grid of the sort
+  int kx, ky;        // column reach: columns within +-kx, +-ky can hold a j-cluster in range
 };
 
-// One block of kListBlock threads per supercluster S: scans the j-clusters
-// cj >= 8S (kListBlock per iteration, one per thread) and appends an entry for
-// every cj whose bounding box is within the list radius of at least one
-// i-cluster of S.  A block-wide prefix count keeps the entries sorted by cj.
-// Diagonal entries and entries with a partial cluster get a mask work slot
-// (explicit pair masks).
+// Contiguous runs of the column window c-k..c+k on a ring of n, ascending:
+// returns the number of runs (1 or 2) in r.
+__device__ inline int col_runs(int c, int k, int n, int r[2][2]) {
+  if (2 * k + 1 >= n) {
+    r[0][0] = 0;
+    r[0][1] = n - 1;
+    return 1;
+  }
+  const int lo = c - k, hi = c + k;
+  if (lo < 0) {
+    r[0][0] = 0;
+    r[0][1] = hi;
+    r[1][0] = lo + n;
+    r[1][1] = n - 1;
+    return 2;
+  }
+  if (hi >= n) {
+    r[0][0] = 0;
+    r[0][1] = hi - n;
+    r[1][0] = lo;
+    r[1][1] = n - 1;
+    return 2;
+  }
+  r[0][0] = lo;
+  r[0][1] = hi;
+  return 1;
+}
+
+// Tests j-cluster cj (< 0: none) against the eight i-clusters of S and
+// appends a list entry when at least one is within the list radius.  Called
+// by every thread of the block with cj ascending over threads; a block-wide
+// prefix count keeps the entries sorted by cj.  Diagonal entries and entries
+// with a partial cluster get a mask work slot (explicit pair masks).
 constexpr int kListBlock = 256;
-__global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const float4 *__restrict__ ctr,
+__device__ inline void list_append(const ListParams &q, int S, const float4 (&ic)[8], const float4 (&ih)[8], int cj,
+                                   float4 jc, float4 jh, int4 *__restrict__ entries, int *__restrict__ maskwork,
+                                   int *__restrict__ nmaskwork, int maxmaskwork, int *__restrict__ overflow,
+                                   int *s_cnt, int &total) {
+  constexpr int NW = kListBlock / kWave;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t imask = 0;
+  bool need = false;
+  int shift = -1;         // image shift code of the entry (see entry_shift)
+  bool per_pair = false;  // no single shift is the minimum image for every pair
+  if (cj >= 0 && jh.x >= 0.f) {
+    const bool jpartial = jc.w < (float)kClusterSize;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int ci = S * 8 + c;
+      if (ih[c].x < 0.f || ci > cj) continue;
+      const float d2 = bbox_dist2(ic[c], ih[c], jc, jh, q.L);
+      if (d2 <= q.rlist2) {
+        imask |= 1u << c;
+        const bool ipartial = ic[c].w < (float)kClusterSize;
+        if (ci == cj || ipartial || jpartial) need = true;
+        const int code = entry_shift(ic[c], ih[c], jc, jh, q.L, q.Linv, q.skin, per_pair);
+        if (shift < 0) shift = code;
+        per_pair |= code != shift;
+      }
+    }
+  }
+  const unsigned long long act = __ballot(imask != 0);
+  if (lane == 0) s_cnt[wv] = __popcll(act);
+  __syncthreads();
+  int before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const int c = s_cnt[w];
+    before += w < wv ? c : 0;
+    all += c;
+  }
+  __syncthreads();
+  const int e = total + before + __popcll(act & ((1ull << lane) - 1ull));
+  // mask work slots for this wave's entries that need explicit masks, one
+  // atomic per wave
+  const unsigned long long nw = __ballot(imask != 0 && need && e < q.cap);
+  int wbase = 0;
+  if (nw) {
+    if (lane == 0) wbase = atomicAdd(nmaskwork, (int)__popcll(nw));
+    wbase = __builtin_amdgcn_readfirstlane(__shfl(wbase, 0));
+  }
+  if (imask != 0) {
+    if (e < q.cap) {
+      int moff = -1;
+      if (need) {
+        const int w = wbase + __popcll(nw & ((1ull << lane) - 1ull));
+        if (w < maxmaskwork) {
+          maskwork[w] = S * q.cap + e;
+          moff = -2;
+        } else {
+          atomicExch(overflow, 1);
+        }
+      }
+      entries[(size_t)S * q.cap + e] = make_int4(cj, (int)imask, moff, per_pair ? kShiftPerPair : shift);
+    } else {
+      atomicExch(overflow, 1);
+    }
+  }
+  total += all;
+}
+
+// Cluster-pair list of supercluster S (one block each).  Superclusters are
+// numbered column by column (x-major, then y, then z inside a column), so the
+// j-superclusters of the columns within the column reach of S's column form
+// at most four ascending index ranges per column row x.  Each range is
+// pruned by supercluster boxes (a conservative test: a cluster box lies in
+// its supercluster box), the survivors' clusters are tested against the
+// eight i-clusters of S, and entries are appended in ascending cj.  Only
+// cj >= 8S is listed, so each cluster pair appears once.  The work per S is
+// bounded by the neighbourhood, independent of the system size.
+__global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const int *__restrict__ sc_col,
+                                                           const int *__restrict__ colsc,
+                                                           const float4 *__restrict__ sctr,
+                                                           const float4 *__restrict__ shalf,
+                                                           const float4 *__restrict__ ctr,
                                                            const float4 *__restrict__ half,
                                                            int4 *__restrict__ entries, int *__restrict__ count,
                                                            int *__restrict__ maskwork, int *__restrict__ nmaskwork,
                                                            int maxmaskwork, int *__restrict__ overflow) {
   constexpr int NW = kListBlock / kWave;
+  constexpr int kScPerBatch = kListBlock / kClusterSize;
   __shared__ int s_cnt[NW];
+  __shared__ int s_sc[kListBlock];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int S = blockIdx.x;
+  const int col = sc_col[S];
+  if (col < 0) {  // past the last occupied supercluster: no atoms
+    if (threadIdx.x == 0) count[S] = 0;
+    return;
+  }
   float4 ic[8], ih[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    int ci = S * 8 + c;
-    ic[c] = ci < q.ncl ? ctr[ci] : make_float4(0, 0, 0, 0);
-    ih[c] = ci < q.ncl ? half[ci] : make_float4(-1, -1, -1, 0);
+    ic[c] = ctr[S * 8 + c];
+    ih[c] = half[S * 8 + c];
   }
+  const float4 Sc = sctr[S], Sh = shalf[S];
+  int xr[2][2], yr[2][2];
+  const int cx = col / q.ncy, cy = col - cx * q.ncy;
+  const int nxr = col_runs(cx, q.kx, q.ncx, xr), nyr = col_runs(cy, q.ky, q.ncy, yr);
   int total = 0;  // block-uniform
-  // j-cluster boxes are loaded one iteration ahead
-  float4 jcn = make_float4(0, 0, 0, 0), jhn = make_float4(-1, -1, -1, 0);
-  if (S * 8 + (int)threadIdx.x < q.ncl) {
-    jcn = ctr[S * 8 + threadIdx.x];
-    jhn = half[S * 8 + threadIdx.x];
-  }
-  for (int base = S * 8; base < q.ncl; base += kListBlock) {
-    const int cj = base + threadIdx.x;
-    const float4 jc = jcn, jh = jhn;
-    if (cj + kListBlock < q.ncl) {
-      jcn = ctr[cj + kListBlock];
-      jhn = half[cj + kListBlock];
-    }
-    uint32_t imask = 0;
-    bool need = false;
-    int shift = -1;        // image shift code of the entry (see entry_shift)
-    bool per_pair = false; // no single shift is the minimum image for every pair
-    if (cj < q.ncl) {
-      if (jh.x >= 0.f) {
-        const bool jpartial = jc.w < (float)kClusterSize;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          int ci = S * 8 + c;
-          if (ih[c].x < 0.f || ci > cj) continue;
-          float d2 = bbox_dist2(ic[c], ih[c], jc, jh, q.L);
-          if (d2 <= q.rlist2) {
-            imask |= 1u << c;
-            const bool ipartial = ic[c].w < (float)kClusterSize;
-            if (ci == cj || ipartial || jpartial) need = true;
-            const int code = entry_shift(ic[c], ih[c], jc, jh, q.L, q.Linv, q.skin, per_pair);
-            if (shift < 0) shift = code;
-            per_pair |= code != shift;
+  for (int rx = 0; rx < nxr; ++rx)
+    for (int x = xr[rx][0]; x <= xr[rx][1]; ++x)
+      for (int ry = 0; ry < nyr; ++ry) {
+        const int a = max(S, colsc[x * q.ncy + yr[ry][0]]);
+        const int b = colsc[x * q.ncy + yr[ry][1] + 1];
+        for (int base = a; base < b; base += kListBlock) {
+          const int sc = base + threadIdx.x;
+          bool pass = false;
+          if (sc < b) {
+            const float4 jh = shalf[sc];
+            pass = jh.x >= 0.f && bbox_dist2(Sc, Sh, sctr[sc], jh, q.L) <= q.rlist2_sc;
           }
+          const unsigned long long act = __ballot(pass);
+          if (lane == 0) s_cnt[wv] = __popcll(act);
+          __syncthreads();
+          int before = 0, npass = 0;
+#pragma unroll
+          for (int w = 0; w < NW; ++w) {
+            const int c = s_cnt[w];
+            before += w < wv ? c : 0;
+            npass += c;
+          }
+          if (pass) s_sc[before + __popcll(act & ((1ull << lane) - 1ull))] = sc;
+          __syncthreads();
+          for (int k0 = 0; k0 < npass; k0 += kScPerBatch) {
+            const int k = k0 + (int)(threadIdx.x / kClusterSize);
+            int cj = -1;
+            float4 jc = make_float4(0, 0, 0, 0), jh = make_float4(-1, -1, -1, 0);
+            if (k < npass) {
+              cj = s_sc[k] * kClusterSize + (int)(threadIdx.x & (kClusterSize - 1));
+              jc = ctr[cj];
+              jh = half[cj];
+            }
+            list_append(q, S, ic, ih, cj, jc, jh, entries, maskwork, nmaskwork, maxmaskwork, overflow, s_cnt, total);
+          }
+          __syncthreads();  // s_sc is rewritten by the next range
         }
       }
-    }
-    const unsigned long long act = __ballot(imask != 0);
-    if (lane == 0) s_cnt[wv] = __popcll(act);
-    __syncthreads();
-    int before = 0, all = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      const int c = s_cnt[w];
-      before += w < wv ? c : 0;
-      all += c;
-    }
-    __syncthreads();
-    const int e = total + before + __popcll(act & ((1ull << lane) - 1ull));
-    // mask work slots for this wave's entries that need explicit masks, one
-    // atomic per wave
-    const unsigned long long nw = __ballot(imask != 0 && need && e < q.cap);
-    int wbase = 0;
-    if (nw) {
-      if (lane == 0) wbase = atomicAdd(nmaskwork, (int)__popcll(nw));
-      wbase = __builtin_amdgcn_readfirstlane(__shfl(wbase, 0));
-    }
-    if (imask != 0) {
-      if (e < q.cap) {
-        int moff = -1;
-        if (need) {
-          const int w = wbase + __popcll(nw & ((1ull << lane) - 1ull));
-          if (w < maxmaskwork) {
-            maskwork[w] = S * q.cap + e;
-            moff = -2;
-          } else {
-            atomicExch(overflow, 1);
-          }
-        }
-        entries[(size_t)S * q.cap + e] = make_int4(cj, (int)imask, moff, per_pair ? kShiftPerPair : shift);
-      } else {
-        atomicExch(overflow, 1);
-      }
-    }
-    total += all;
-  }
   if (threadIdx.x == 0) count[S] = total < q.cap ? total : q.cap;
 }
 
@@ -879,20 +1005,41 @@ struct PairQ {
 constexpr int kGroup = 2;    // list entries whose pairs share one queue flush
 constexpr int kQueue = 128;  // queue ring per wave (holds < 64 + 64 pairs)
 
-// Minimum-image difference j - i in the kernel's arithmetic type: xj is
-// already moved to the image of its list entry, unless the entry needs the
-// per-pair image (per_pair, wave-uniform).  Phase A and phase C of k_pair
-// call it with the same operands.
+// Box edges in the kernel's type, split as hi + lo (see PairParams::Lhi).
+template <class R>
+struct EdgesR {
+  R hi[3], lo[3], inv[3];
+  __device__ void load(const PairParams &pp) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      hi[k] = (R)pp.Lhi[k];
+      lo[k] = (R)pp.Llo[k];
+      inv[k] = (R)pp.box.Linv[k];
+    }
+  }
+  // image k of a difference: d - k (hi + lo), hi exact
+  __device__ R image(R d, int ax) const {
+    const R k = fast_rint(d * inv[ax]);
+    return (d - k * hi[ax]) - k * lo[ax];
+  }
+};
+
+// Minimum-image difference j - i in the kernel's arithmetic type.  Sorted
+// coordinates are relative to supercluster origins; xj has already been
+// moved by the entry offset (origin of j's supercluster - origin of i's,
+// plus the entry's image shift unless the entry needs the per-pair image,
+// per_pair, wave-uniform).  Phase A and phase C of k_pair call it with the
+// same operands.
 template <class R, class R4>
-__device__ inline void pair_delta(const R4 &xi, const R4 &xj, R Lx, R Ly, R Lz, R Lix, R Liy, R Liz, int per_pair,
-                                  R &dx, R &dy, R &dz) {
+__device__ inline void pair_delta(const R4 &xi, const R4 &xj, const EdgesR<R> &E, int per_pair, R &dx, R &dy,
+                                  R &dz) {
   dx = xj.x - xi.x;
   dy = xj.y - xi.y;
   dz = xj.z - xi.z;
   if (per_pair) {
-    dx -= Lx * fast_rint(dx * Lix);
-    dy -= Ly * fast_rint(dy * Liy);
-    dz -= Lz * fast_rint(dz * Liz);
+    dx = E.image(dx, 0);
+    dy = E.image(dy, 1);
+    dz = E.image(dz, 2);
   }
 }
 
@@ -914,8 +1061,8 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
     int *__restrict__ tile_counter, const int4 *__restrict__ entries, const int *__restrict__ count, int cap,
     const unsigned long long *__restrict__ masks,
     const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
-    const int *__restrict__ ts, const int *__restrict__ perm, const double *__restrict__ ljtab,
-    typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
+    const float4 *__restrict__ sorg, const int *__restrict__ ts, const int *__restrict__ perm,
+    const double *__restrict__ ljtab, typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
   using R = typename std::conditional<MIXED, float, double>::type;
   using R4 = typename Real4<R>::type;
   using Acc = AccT<MIXED>;
@@ -940,8 +1087,8 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int ti = lane & 7, tj = lane >> 3;
-  const R Lx = (R)pp.box.L[0], Ly = (R)pp.box.L[1], Lz = (R)pp.box.L[2];
-  const R Lix = (R)pp.box.Linv[0], Liy = (R)pp.box.Linv[1], Liz = (R)pp.box.Linv[2];
+  EdgesR<R> E;
+  E.load(pp);
   SwR<R> sl, sc;
   sl.load(pp.lj);
   sc.load(pp.co);
@@ -962,6 +1109,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
     const int2 tile = tiles[t];
     const int S = __builtin_amdgcn_readfirstlane(tile.x);
     const int e0 = __builtin_amdgcn_readfirstlane(tile.y);
+    const float4 oS = sorg[S];
     si[lane] = xs[S * kSuperSize + lane];
     {
       const int v = ts[S * kSuperSize + lane];
@@ -1015,10 +1163,20 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
         shv[g] = __builtin_amdgcn_readfirstlane(ent.w);
         const int sj = cjv[g] * kClusterSize + tj;
         xj[g] = xs[sj];
-        if (!(shv[g] & kShiftPerPair)) {  // j-cluster moved to the image of the entry
-          xj[g].x += Lx * (R)((shv[g] & 3) - 1);
-          xj[g].y += Ly * (R)(((shv[g] >> 2) & 3) - 1);
-          xj[g].z += Lz * (R)(((shv[g] >> 4) & 3) - 1);
+        {
+          // j-cluster into the frame of S: origin difference (exact) plus
+          // the entry's image shift (hi part exact, lo part rounded once)
+          const float4 oJ = sorg[cjv[g] / (kSuperSize / kClusterSize)];
+          R ox = (R)(oJ.x - oS.x), oy = (R)(oJ.y - oS.y), oz = (R)(oJ.z - oS.z);
+          if (!(shv[g] & kShiftPerPair)) {
+            const R sx = (R)((shv[g] & 3) - 1), sy = (R)(((shv[g] >> 2) & 3) - 1), sz = (R)(((shv[g] >> 4) & 3) - 1);
+            ox = (ox + sx * E.hi[0]) + sx * E.lo[0];
+            oy = (oy + sy * E.hi[1]) + sy * E.lo[1];
+            oz = (oz + sz * E.hi[2]) + sz * E.lo[2];
+          }
+          xj[g].x += ox;
+          xj[g].y += oy;
+          xj[g].z += oz;
         }
         tyj0[g] = ts[sj];
         gmask |= (unsigned)emv[g] << (8 * g);
@@ -1051,7 +1209,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
         }
         const R4 xi = si[c * 8 + ti];
         R dx, dy, dz;
-        pair_delta<R, R4>(xi, xjg, Lx, Ly, Lz, Lix, Liy, Liz, shg & kShiftPerPair, dx, dy, dz);
+        pair_delta<R, R4>(xi, xjg, E, shg & kShiftPerPair, dx, dy, dz);
         const R r2 = dx * dx + dy * dy + dz * dz;
         bool border = false;
         if (LJSW != SW_NONE) border |= fabs(r2 - sl.rc2) <= guard;
@@ -1111,7 +1269,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
           if (!(em & (1 << c))) continue;
           const R4 xi = si[c * 8 + ti];
           R dx, dy, dz;
-          pair_delta<R, R4>(xi, xj[g], Lx, Ly, Lz, Lix, Liy, Liz, shv[g] & kShiftPerPair, dx, dy, dz);
+          pair_delta<R, R4>(xi, xj[g], E, shv[g] & kShiftPerPair, dx, dy, dz);
           const R f = fr_slot[(g * 8 + c) * kWave + lane];
           const R fx = dx * f, fy = dy * f, fz = dz * f;
           fi[c][0] -= fx;
@@ -1181,7 +1339,8 @@ template <bool MIXED, int COUL, int LJSW, int CSW>
 __device__ void border_pairs(int blk, int nblocks, const PairParams &pp, int stats, const int2 *__restrict__ border,
                              const int *__restrict__ nborder, int maxborder,
                              const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
-                             const int *__restrict__ ts, const int *__restrict__ perm, const double *__restrict__ xorig,
+                             const float4 *__restrict__ sorg, const int *__restrict__ ts, const int *__restrict__ perm,
+                             const double *__restrict__ xorig,
                              const double *__restrict__ ljtab, const int *__restrict__ eptr,
                              const int2 *__restrict__ elist, typename AccT<MIXED>::type *__restrict__ facc,
                              unsigned long long *__restrict__ eacc, unsigned long long *__restrict__ stat) {
@@ -1203,10 +1362,14 @@ __device__ void border_pairs(int blk, int nblocks, const PairParams &pp, int sta
     sl.load(pp.lj);
     sc.load(pp.co);
     const auto xi = xs[si], xj = xs[sj];
-    R dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-    dx -= (R)pp.box.L[0] * fast_rint(dx * (R)pp.box.Linv[0]);
-    dy -= (R)pp.box.L[1] * fast_rint(dy * (R)pp.box.Linv[1]);
-    dz -= (R)pp.box.L[2] * fast_rint(dz * (R)pp.box.Linv[2]);
+    const float4 oi = sorg[si / kSuperSize], oj = sorg[sj / kSuperSize];
+    EdgesR<R> E;
+    E.load(pp);
+    R dx = (xj.x - xi.x) + (R)(oj.x - oi.x), dy = (xj.y - xi.y) + (R)(oj.y - oi.y),
+      dz = (xj.z - xi.z) + (R)(oj.z - oi.z);
+    dx = E.image(dx, 0);
+    dy = E.image(dy, 1);
+    dz = E.image(dz, 2);
     R r2 = dx * dx + dy * dy + dz * dz;
     // evaluate at the cutoff side the exact test chose
     R A = 0, B = 0;
@@ -1900,6 +2063,7 @@ struct FixupArgs {
   const int *nborder;
   int maxborder;
   const R4 *xs;
+  const float4 *sorg;
   const int *ts, *perm;
   const double *xorig, *ljtab;
   const int *eptr;
@@ -1927,7 +2091,7 @@ __global__ void __launch_bounds__(256) k_fixups(FixupArgs<MIXED> a) {
   int b = blockIdx.x;
   if (b < a.border_blocks) {
     border_pairs<MIXED, COUL, LJSW, CSW>(b, a.border_blocks, a.pp, a.stats, a.border, a.nborder, a.maxborder, a.xs,
-                                         a.ts, a.perm, a.xorig, a.ljtab, a.eptr, a.elist, a.facc, a.border_e,
+                                         a.sorg, a.ts, a.perm, a.xorig, a.ljtab, a.eptr, a.elist, a.facc, a.border_e,
                                          a.border_stat);
     return;
   }
@@ -1977,7 +2141,7 @@ __global__ void k_kick_drift(int n, Kick kk, double h, const int *__restrict__ p
                              const double *__restrict__ fout, const double *__restrict__ minv, double *__restrict__ v,
                              double *__restrict__ x, const double *__restrict__ xbuild, Box box,
                              typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64,
-                             unsigned int *__restrict__ dmax) {
+                             const float4 *__restrict__ sorg, unsigned int *__restrict__ dmax) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   float d2f = 0.f;
   const int a = s < n ? perm[s] : -1;  // n counts slots; padding slots have a < 0
@@ -1995,10 +2159,11 @@ __global__ void k_kick_drift(int n, Kick kk, double h, const int *__restrict__ p
     }
     d2f = (float)d2;
     const double4 b = xs64[s];
+    const float4 o = sorg[s / kSuperSize];
     typename Real4<R>::type r = xs[s];
-    r.x = (R)(b.x + dd[0]);
-    r.y = (R)(b.y + dd[1]);
-    r.z = (R)(b.z + dd[2]);
+    r.x = (R)((b.x + dd[0]) - (double)o.x);
+    r.y = (R)((b.y + dd[1]) - (double)o.y);
+    r.z = (R)((b.z + dd[2]) - (double)o.z);
     xs[s] = r;
   }
   // wave max, one atomic per wave

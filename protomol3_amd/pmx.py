@@ -62,7 +62,7 @@ EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_crea
             "pmx_sys_get_forces", "pmx_sys_sync", "pmx_sys_alpha", "pmx_sys_pair_stats", "pmx_pair_hash",
             "pmx_sys_cells", "pmx_sys_stage_times", "pmx_sys_timing_begin", "pmx_sys_timing_end",
             "pmx_md_init", "pmx_md_set_state", "pmx_md_run", "pmx_md_get_state", "pmx_md_stats",
-            "pmx_langevin_noise", "pmx_sys_device_bytes", "pmx_estimate_bytes"]
+            "pmx_langevin_noise", "pmx_sys_device_bytes", "pmx_estimate_bytes", "pmx_md_run_many"]
 
 _lib = None
 
@@ -100,6 +100,7 @@ def load() -> C.CDLL:
     L.pmx_md_init.argtypes = [vp, C.POINTER(MdDesc)]
     L.pmx_md_set_state.argtypes = [vp, dp, dp]
     L.pmx_md_run.argtypes = [vp, C.c_int]
+    L.pmx_md_run_many.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
     L.pmx_md_get_state.argtypes = [vp, dp, dp, dp]
     L.pmx_md_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.pmx_langevin_noise.argtypes = [C.c_ulonglong, C.c_ulonglong, C.c_int, C.POINTER(C.c_double)]
@@ -236,6 +237,19 @@ def make_md_desc(s: MolSystem, timestep_fs: float, skin: float = 1.0, integrator
     d.gamma = gamma
     d.seed = seed
     return d, keep
+
+
+def run_many(systems_: list, nsteps: int) -> None:
+    """pmx_md_run_many: nsteps MD steps on every System (each after md_init
+    and md_set_state); systems on different Contexts run concurrently."""
+    if not systems_:
+        return
+    lib = systems_[0].ctx.lib
+    arr = (C.c_void_p * len(systems_))(*[sy.h for sy in systems_])
+    rc = lib.pmx_md_run_many(arr, len(systems_), nsteps)
+    if rc:
+        msgs = {sy.ctx.lib.pmx_last_error(sy.ctx.h).decode() for sy in systems_}
+        raise PmxError(rc, "pmx_md_run_many: " + "; ".join(m for m in msgs if m))
 
 
 def estimate_bytes(s: MolSystem, spec: ForceSpec, precision: int = FP64, trajectory: bool = False) -> int:

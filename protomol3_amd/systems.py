@@ -324,6 +324,37 @@ def big_water(n_water: int = 333000, box: float = 216.0, seed: int = 2, jitter: 
     return _assemble(f"water{3 * n_water}", [box] * 3, [_water_part(o, rng)], lj)
 
 
+def tiled(s: MolSystem, reps=(2, 2, 2)) -> MolSystem:
+    """The periodic system s repeated reps[k] times along axis k: the box
+    grows by reps, copy (i, j, k) is shifted by (i Lx, j Ly, k Lz) and gets its
+    own atom and molecule numbers, so each copy's atoms see the neighbourhood
+    they see in s (for cutoffs below half the small box).  Atom a of copy c is
+    atom c * s.n + a, copies ordered x-major."""
+    reps = tuple(int(r) for r in reps)
+    shifts = np.array([(i, j, k) for i in range(reps[0]) for j in range(reps[1]) for k in range(reps[2])],
+                      dtype=np.float64) * s.box
+    nc, n = len(shifts), s.n
+    nmol = int(s.molecule.max()) + 1 if n else 0
+    off = (np.arange(nc) * n)[:, None, None]
+
+    def rep_idx(a):
+        return (a[None, :, :] + off).reshape(-1, a.shape[1]).astype(np.int32)
+
+    excl = np.tile(s.exclusions, (nc, 1)).astype(np.int32)
+    excl[:, :2] = rep_idx(s.exclusions[:, :2])
+    return MolSystem(
+        name=f"{s.name}_x{reps[0]}{reps[1]}{reps[2]}",
+        pos=(s.pos[None, :, :] + shifts[:, None, :]).reshape(-1, 3),
+        charge=np.tile(s.charge, nc), atype=np.tile(s.atype, nc),
+        molecule=(s.molecule[None, :] + (np.arange(nc) * nmol)[:, None]).reshape(-1).astype(np.int32),
+        mass=np.tile(s.mass, nc), lj=s.lj, box=s.box * np.array(reps, dtype=np.float64),
+        origin=s.origin.copy(),
+        bonds=rep_idx(s.bonds), bond_par=np.tile(s.bond_par, (nc, 1)),
+        angles=rep_idx(s.angles), angle_par=np.tile(s.angle_par, (nc, 1)),
+        exclusions=excl, csf=s.csf,
+        vel=None if s.vel is None else np.tile(s.vel, (nc, 1)))
+
+
 def pme_grid_size(length: float, spacing: float = 1.0) -> int:
     """Smallest 2^a 3^b 5^c >= length / spacing (even, as the PME order
     check requires grid >= order)."""

@@ -13,7 +13,7 @@ from . import pmx
 
 
 def tune_skin(ctx, s, spec: pmx.ForceSpec, precision: int, x, v, timestep_fs: float = 1.0,
-              candidates=(0.4, 0.7, 1.0, 1.4), steps: int = 40) -> tuple[float, dict]:
+              candidates=(0.8, 1.2, 1.6, 2.0), steps: int = 200) -> tuple[float, dict]:
     """Returns (best skin, {skin: ms per step})."""
     times = {}
     for skin in candidates:
@@ -21,7 +21,7 @@ def tune_skin(ctx, s, spec: pmx.ForceSpec, precision: int, x, v, timestep_fs: fl
         try:
             sy.md_init(s, timestep_fs, skin=skin)
             sy.md_set_state(x, v)
-            sy.md_run(5)
+            sy.md_run(20)
             t0 = time.perf_counter()
             sy.md_run(steps)
             times[skin] = (time.perf_counter() - t0) * 1e3 / steps

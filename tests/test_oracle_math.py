@@ -161,3 +161,28 @@ def test_cell_enumerator_pair_set_equals_brute_force(seed):
     key = lambda arr: set(map(tuple, np.sort(arr, axis=1).tolist()))
     assert len(p) == len(b)
     assert key(p) == key(b)
+
+
+def test_periodic_replication_identity():
+    """Forces of a 2x2x2 tiling of a periodic box equal the tiled forces of
+    the box, energies are 8x: real space because the cutoff is below half the
+    small box, PME because the doubled grid at the same spacing sees only the
+    even-index structure factors (8x the small grid's).  This is the identity
+    the full-size large-box GPU test relies on; here both sides are the
+    oracle, with the large box's PME order and alpha."""
+    small = systems.water_box(n_side=6, box=20.4, seed=9)
+    big = systems.tiled(small, (2, 2, 2))
+    assert big.n == 8 * small.n
+    lj = (O.SW_C2, 8.0, 10.0)
+    fs, ecs, eljs = O.evaluate(small, O.make_spec(5.0, lj=lj, pme=dict(rc=10.0, grid=(24, 24, 24), order=6,
+                                                                          alpha=0.34)))
+    fb, ecb, eljb = O.evaluate(big, O.make_spec(5.0, lj=lj, pme=dict(rc=10.0, grid=(48, 48, 48), order=6,
+                                                                        alpha=0.34)), nthreads=4)
+    assert np.abs(fb - np.tile(fs, (8, 1))).max() <= 1e-9 * np.abs(fs).max()
+    assert abs(ecb - 8 * ecs) <= 1e-9 * big.n
+    assert abs(eljb - 8 * eljs) <= 1e-9 * max(1.0, abs(eljb))
+    # a mistake the identity must catch: the doubled grid at the small box's
+    # size (grid spacing halved) is a different approximation
+    fw, _, _ = O.evaluate(big, O.make_spec(5.0, lj=lj, pme=dict(rc=10.0, grid=(24, 24, 24), order=6,
+                                                                   alpha=0.34)), nthreads=4)
+    assert np.abs(fw - np.tile(fs, (8, 1))).max() > 1e-6 * np.abs(fs).max()
