@@ -148,6 +148,43 @@ def serving(nreq: int, slots: int, seed: int = 7, log=print):
                         "Poisson arrivals at 75% of measured capacity"}
 
 
+def replicas(nsys: int, steps: int, lanes: int, seed: int = 11, log=print):
+    """Batched replicas: nsys independent systems with atom counts drawn
+    log-uniform in [500, 20,000] (water + 5-10% bead-polymer solute at 0.0334
+    atoms/A^3, per-system seeds), LJ C2 7-9 A + PME ~1 A grid, mixed
+    precision, `steps` Langevin steps (300 K, 1/ps) each, all run by one
+    pmx_md_run_many call over `lanes` contexts."""
+    from protomol3_amd import pmx, serve, systems
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(500), np.log(20000), nsys)).astype(int)
+    t0 = time.perf_counter()
+    boxes = [systems.replica_box(int(n), 2000 + i).with_velocities(300.0, seed=3000 + i) for i, n in enumerate(sizes)]
+    natoms = int(sum(b.n for b in boxes))
+    log(f"replicas: {nsys} systems, {natoms} atoms in total, built in {time.perf_counter() - t0:.1f} s")
+    ctxs = [pmx.Context(0) for _ in range(lanes)]
+    sys_ = []
+    try:
+        for i, b in enumerate(boxes):
+            sy = pmx.System(ctxs[i % lanes], b, serve.request_spec(b), pmx.MIXED)
+            sys_.append(sy)
+            sy.md_init(b, TIMESTEP_FS, skin=1.0, integrator=pmx.LANGEVIN, temperature=300.0, gamma=1.0, seed=i)
+            sy.md_set_state(b.pos, b.vel)
+        pmx.run_many(sys_, 5)  # warm-up (first list builds, kernel loads)
+        t0 = time.perf_counter()
+        pmx.run_many(sys_, steps)
+        wall = time.perf_counter() - t0
+    finally:
+        for sy in sys_:
+            sy.close()
+        for c in ctxs:
+            c.close()
+    return {"systems": nsys, "atoms_total": natoms, "steps": steps, "lanes": lanes, "wall_s": round(wall, 3),
+            "aggregate_ns_per_day": round(nsys * steps * TIMESTEP_FS * 1e-6 / wall * 86400.0, 3),
+            "atom_steps_per_s": float(f"{natoms * steps / wall:.4e}"),
+            "workload": "log-uniform 500-20,000 atoms, water + bead polymer at 0.0334 atoms/A^3, rc 9 A, "
+                        "PME ~1 A grid, Langevin 300 K 1/ps, mixed, one pmx_md_run_many call"}
+
+
 def cpu_baseline(s, budget_s: float = 20.0):
     """The oracle (FP64 C++ restatement of the reference path + leapfrog) on
     the host's cores: whole MD steps of the same workload until ~budget_s."""
@@ -176,11 +213,14 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--skin", type=float, default=-1.0, help="pair-list skin in A (<0: autotune)")
-    ap.add_argument("--drift-steps", type=int, default=2000)
+    ap.add_argument("--drift-steps", type=int, default=10000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--serve-requests", type=int, default=64, help="requests in the serving stream (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--replicas", type=int, default=0, help="batched replicas to run after the main metric (0: skip)")
+    ap.add_argument("--replica-steps", type=int, default=500)
+    ap.add_argument("--replica-lanes", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
@@ -221,6 +261,7 @@ def main():
     achieved = FLOP_PER_PAIR * mixed["npairs"] / (pair_ms * 1e-3) / 1e12
     cpu = None if a.no_cpu else cpu_baseline(s, a.cpu_budget)
     serve_out = serving(a.serve_requests, a.serve_slots, log=log) if a.serve_requests > 0 else None
+    rep_out = replicas(a.replicas, a.replica_steps, a.replica_lanes, log=log) if a.replicas > 0 else None
     out = {
         "metric": "ns/day (26k-atom PME) mixed vs FP64 GPU at <=1e-5 force err; p50/p99 latency",
         "value": round(ns_day(mixed["ms_per_step"]), 3),
@@ -261,6 +302,8 @@ def main():
         "cpu_baseline": cpu,
         "serving": serve_out,
     }
+    if rep_out is not None:
+        out["replicas"] = rep_out
     print(json.dumps(out), flush=True)
 
 
