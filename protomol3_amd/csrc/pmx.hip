@@ -90,6 +90,7 @@ struct Sys {
   bool timing_active = false;                   // MD steps record events (timing_begin .. timing_end)
   bool have_positions = false;
   bool cached = false;
+  bool acc_clean = false;              // force accumulators, PME grid and pass counters are zero
   bool dry = false;                    // size only: allocations are counted, nothing runs
   unsigned long long dry_bytes = 0;
   double eout_h[4] = {0, 0, 0, 0};
@@ -263,6 +264,7 @@ int validate(pmx_ctx *c, const pmx_sys_desc *d) {
 
 int setup(Sys *s) {
   const pmx_sys_desc &d = s->d;
+  s->acc_clean = false;  // buffers are (re)allocated below
   s->mixed = d.precision == PMX_MIXED;
   s->n = d.n;
   s->q.assign(d.charge, d.charge + d.n);
@@ -610,7 +612,8 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
     }
     const unsigned grid = (unsigned)std::max(1, s->ctx->cus * per_cu);
     PairOut out{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash),
-                P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow)};
+                P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow),
+                P<unsigned long long>(s->border_acc)};
     hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, stats ? 1 : 0,
                        P<int2>(s->tiles), P<int>(s->ntiles), P<int>(s->tilectr), P<int4>(s->entries),
                        P<int>(s->count), s->cap, P<unsigned long long>(s->masks), P<R4>(s->xs),
@@ -685,6 +688,12 @@ void run_pair_phase(Sys *s, bool stats, int phase) {
 // Spreading and gathering for the stencil order ORDER (validated: even, 2..8).
 template <bool MIXED, int ORDER>
 void launch_spread(Sys *s, hipStream_t st) {
+  if (ORDER >= 4) {
+    hipLaunchKernelGGL((k_spread_patch<MIXED, ORDER>), dim3(nblk(s->npad, kSpreadSlots)), dim3(256), 0, st, s->npad,
+                       s->pme, P<double>(s->xorig), P<int>(s->permp), P<double>(s->q_d), P<double>(s->grid64),
+                       P<unsigned long long>(s->gridfx));
+    return;
+  }
   constexpr int P3 = ORDER * ORDER * ORDER;
   constexpr int apw = P3 >= 64 ? 1 : 64 / P3;
   const int waves = (int)nblk(s->npad, apw);
@@ -853,7 +862,12 @@ int forces(Sys *s, unsigned flags, bool energies) {
     z.p[nr] = s->mixed ? P<unsigned long long>(s->gridfx) : P<unsigned long long>(s->grid64);
     z.n[nr++] = G;
   }
-  hipLaunchKernelGGL(k_zero, dim3(std::max(1u, nblk(3ll * npad, 256)), nr), dim3(256), 0, st, z);
+  // A completed pass leaves the accumulators, the PME grid and the counters
+  // cleared (k_unsort_forces, k_fft_z_r2c, k_pair); only the first pass after
+  // (re)allocation or after an interrupted pass clears them here.
+  if (!s->acc_clean)
+    hipLaunchKernelGGL(k_zero, dim3(std::max(1u, nblk(3ll * npad, 256)), nr), dim3(256), 0, st, z);
+  s->acc_clean = false;
   if (timing) hipEventRecord(s->ev[2], st);
   if (s->pme_on && (s->d.pme_terms & PMX_PME_RECIP)) {
     if (s->mixed)
@@ -893,10 +907,12 @@ int forces(Sys *s, unsigned flags, bool energies) {
   HIPCHK(hipStreamWaitEvent(st, s->join_ev, 0));
   if (s->mixed)
     hipLaunchKernelGGL(k_unsort_forces<true>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
-                       P<unsigned long long>(s->facc), P<double>(s->fout));
+                       P<unsigned long long>(s->facc), P<double>(s->fout), P<int>(s->tilectr),
+                       P<int>(s->nborder));
   else
     hipLaunchKernelGGL(k_unsort_forces<false>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
-                       P<double>(s->facc), P<double>(s->fout));
+                       P<double>(s->facc), P<double>(s->fout), P<int>(s->tilectr), P<int>(s->nborder));
+  s->acc_clean = true;
   if (energies) reduce_energies(s, stats);
   if (timing) hipEventRecord(s->ev[8], st);
   HIPCHK(hipGetLastError());

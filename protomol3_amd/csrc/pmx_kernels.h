@@ -991,6 +991,7 @@ struct PairOut {
   int *nborder;
   int maxborder;
   int *overflow;
+  unsigned long long *border_acc;  // [4] boundary-pair energies and stats, cleared here
 };
 
 // Queue entry of the compacted pair kernel: one in-cutoff pair awaiting its
@@ -1095,6 +1096,8 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   const R guard = (R)pp.guard;
   const R alpha = (R)pp.alpha, a2pi = (R)pp.a2pi, csf = (R)pp.csf;
   const int ntiles = *ntiles_p;
+  // the boundary-pair sums are filled after this kernel (same stream)
+  if (blockIdx.x == 0 && threadIdx.x < 4) out.border_acc[threadIdx.x] = 0ull;
   R4 *const si = s_i[wv];
   PairQ<R> *const q = s_q[wv];
   R *const fr_slot = s_fr[wv];
@@ -1467,6 +1470,152 @@ __device__ inline void grid_coord(double x, double hr, int K, int off, int &base
   w = u - iu;
 }
 
+// Charge spreading through an LDS patch (Grid::anterpolateCharge,
+// Grid.h:107-154).  One block takes kSpreadSlots consecutive sorted slots
+// (half a supercluster: a compact group of atoms).  Wave 0 computes each
+// atom's grid base and fractional offsets and the box of grid points the
+// group's stencils cover (bases taken relative to the first atom's, on the
+// periodic grid); the block accumulates every stencil contribution into an
+// LDS copy of that box with LDS atomics, then adds the box's nonzero points
+// to the grid with one global atomic each.  The accumulator type is the
+// grid's (int64 fixed point 2^-40 in the mixed form, so the result does not
+// depend on the order of the adds; double in the FP64 form).  A group whose
+// box exceeds the patch capacity, or wraps onto itself, adds to the grid
+// directly.
+constexpr int kSpreadSlots = 32;
+template <int P>
+__host__ __device__ constexpr int spread_patch_max() {
+  return P <= 4 ? 2048 : 4096;
+}
+
+template <bool MIXED, int P>
+__global__ void __launch_bounds__(256) k_spread_patch(int n, PmeParams p, const double *__restrict__ xorig,
+                                                      const int *__restrict__ perm, const double *__restrict__ q,
+                                                      double *__restrict__ grid64,
+                                                      unsigned long long *__restrict__ gridfx) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using G = typename std::conditional<MIXED, unsigned long long, double>::type;
+  constexpr int P3 = P * P * P;
+  constexpr int PMAX = spread_patch_max<P>();
+  __shared__ G s_patch[PMAX];
+  __shared__ int s_rb[kSpreadSlots][3];  // base relative to the patch origin
+  __shared__ R s_w[kSpreadSlots][3];
+  __shared__ R s_q[kSpreadSlots];
+  __shared__ int s_org[3], s_ext[3], s_direct, s_any;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int s0 = blockIdx.x * kSpreadSlots;
+  const int K[3] = {p.nx, p.ny, p.nz};
+  if (wv == 0) {
+    const int t = lane;
+    const int s = s0 + t;
+    const int a = (t < kSpreadSlots && s < n) ? perm[s] : -1;
+    int b[3] = {0, 0, 0};
+    double w[3] = {0, 0, 0};
+    if (a >= 0) {
+      grid_coord(xorig[3 * a], p.hxr, p.nx, p.offx, b[0], w[0]);
+      grid_coord(xorig[3 * a + 1], p.hyr, p.ny, p.offy, b[1], w[1]);
+      grid_coord(xorig[3 * a + 2], p.hzr, p.nz, p.offz, b[2], w[2]);
+    }
+    const unsigned long long ok = __ballot(a >= 0);
+    if (ok) {
+      const int first = __builtin_ctzll(ok);
+      int lo[3], hi[3], db[3], ref[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        ref[k] = __shfl(b[k], first);
+        int d = b[k] - ref[k];  // on the periodic grid, nearest to the reference
+        d -= K[k] * (int)floorf(((float)d + 0.5f * (float)K[k]) / (float)K[k]);
+        db[k] = d;
+        lo[k] = a >= 0 ? d : 0x3fffffff;
+        hi[k] = a >= 0 ? d : -0x3fffffff;
+      }
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = min(lo[k], __shfl_xor(lo[k], m));
+          hi[k] = max(hi[k], __shfl_xor(hi[k], m));
+        }
+      const int ex = hi[0] - lo[0] + P, ey = hi[1] - lo[1] + P, ez = hi[2] - lo[2] + P;
+      const bool direct = ex > K[0] || ey > K[1] || ez > K[2] || (long long)ex * ey * ez > PMAX;
+      if (t < kSpreadSlots) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          // direct adds use the plain base; patch adds the base within the box
+          s_rb[t][k] = direct ? b[k] : db[k] - lo[k];
+          s_w[t][k] = (R)w[k];
+        }
+        s_q[t] = a >= 0 ? (R)q[a] : (R)0;
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s_org[k] = ref[k] + lo[k];
+        s_ext[0] = ex;
+        s_ext[1] = ey;
+        s_ext[2] = ez;
+        s_direct = direct ? 1 : 0;
+        s_any = 1;
+      }
+    } else if (lane == 0) {
+      s_any = 0;
+    }
+  }
+  __syncthreads();
+  if (!s_any) return;  // block-uniform: no atoms in this group
+  const bool direct = s_direct;
+  const int ex = s_ext[0], ey = s_ext[1], ez = s_ext[2];
+  const int vol = ex * ey * ez;
+  if (!direct)
+    for (int i = tid; i < vol; i += blockDim.x) s_patch[i] = (G)0;
+  __syncthreads();
+  for (int t = wv; t < kSpreadSlots; t += blockDim.x / kWave) {
+    const R qa = s_q[t];
+    if (qa == (R)0) continue;  // padding slot or uncharged atom: no contribution
+    R tx[P], ty[P], tz[P], dd[P];
+    bspline_w<P, R>(s_w[t][0], tx, dd);
+    bspline_w<P, R>(s_w[t][1], ty, dd);
+    bspline_w<P, R>(s_w[t][2], tz, dd);
+    const int rb0 = s_rb[t][0], rb1 = s_rb[t][1], rb2 = s_rb[t][2];
+#pragma unroll
+    for (int pt0 = 0; pt0 < P3; pt0 += kWave) {
+      const int pt = pt0 + lane;
+      if (P3 % kWave != 0 && pt >= P3) break;
+      const int i = pt / (P * P), j = (pt / P) % P, kk = pt % P;
+      const R v = ((qa * pick<P>(tx, i)) * pick<P>(ty, j)) * pick<P>(tz, kk);
+      G add;
+      if (MIXED)
+        add = (G)(unsigned long long)__double2ll_rn((double)v * kGridScale);
+      else
+        add = (G)v;
+      if (!direct) {
+        atomicAdd(&s_patch[((rb0 + i) * ey + (rb1 + j)) * ez + (rb2 + kk)], add);
+      } else {
+        const size_t g = ((size_t)wrap_grid(i + rb0, p.nx) * p.ny + wrap_grid(j + rb1, p.ny)) * p.nz +
+                         wrap_grid(kk + rb2, p.nz);
+        if (MIXED)
+          atomicAdd(&gridfx[g], (unsigned long long)add);
+        else
+          unsafeAtomicAdd(&grid64[g], (double)add);
+      }
+    }
+  }
+  if (direct) return;
+  __syncthreads();
+  const int ox = s_org[0], oy = s_org[1], oz = s_org[2];
+  for (int i = tid; i < vol; i += blockDim.x) {
+    const G v = s_patch[i];
+    if (v == (G)0) continue;
+    const int iz = i % ez, iy = (i / ez) % ey, ix = i / (ez * ey);
+    const int gx = (ox + ix) % p.nx, gy = (oy + iy) % p.ny, gz = (oz + iz) % p.nz;
+    const size_t g = ((size_t)(gx < 0 ? gx + p.nx : gx) * p.ny + (gy < 0 ? gy + p.ny : gy)) * p.nz +
+                     (gz < 0 ? gz + p.nz : gz);
+    if (MIXED)
+      atomicAdd(&gridfx[g], (unsigned long long)v);
+    else
+      unsafeAtomicAdd(&grid64[g], (double)v);
+  }
+}
+
 // Charge spreading (Grid::anterpolateCharge, Grid.h:107-154).  The P^3
 // stencil points of one atom go to consecutive lanes (z fastest), so each
 // atomic instruction of a wave updates P^2 short contiguous rows of one
@@ -1629,8 +1778,8 @@ __device__ Cx<T> *lds_fft(Cx<T> *a, Cx<T> *b, int nb, const FftPlan &pl, const C
 
 // z pass, real -> half complex: lines (i, j) contiguous along z.
 template <class T, bool FIXED>
-__global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int lines_per_block, const double *__restrict__ g64,
-                            const unsigned long long *__restrict__ gfx, const Cx<T> *__restrict__ tw,
+__global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int lines_per_block, double *__restrict__ g64,
+                            unsigned long long *__restrict__ gfx, const Cx<T> *__restrict__ tw,
                             Cx<T> *__restrict__ spec) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
@@ -1641,10 +1790,14 @@ __global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int lines_per_block, const 
   for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) {
     const size_t gi = (size_t)l0 * p.nz + t;
     T v;
-    if (FIXED)
+    // each grid point is read once here and cleared for the next spread
+    if (FIXED) {
       v = (T)((double)(long long)gfx[gi] * (1.0 / kGridScale));
-    else
+      gfx[gi] = 0ull;
+    } else {
       v = (T)g64[gi];
+      g64[gi] = 0.0;
+    }
     a[t] = Cx<T>{v, (T)0};
   }
   __syncthreads();
@@ -1868,10 +2021,18 @@ __device__ void correction_block(int blk, int ne, const int4 *__restrict__ etab,
 
 // ---------------------------------------------------------------------------
 // 6. Finalize: sorted accumulators -> caller's AoS order, FP64.
+// The accumulators are cleared as they are read, and so are the pair
+// kernel's tile counter and the boundary-pair count, so the next force pass
+// starts from zero without a separate clearing launch (padding slots are
+// never accumulated into).
 template <bool MIXED>
-__global__ void k_unsort_forces(int n, const int *__restrict__ slot, const typename AccT<MIXED>::type *__restrict__ facc,
-                                double *__restrict__ fout) {
+__global__ void k_unsort_forces(int n, const int *__restrict__ slot, typename AccT<MIXED>::type *__restrict__ facc,
+                                double *__restrict__ fout, int *__restrict__ tilectr, int *__restrict__ nborder) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a == 0) {
+    *tilectr = 0;
+    *nborder = 0;
+  }
   if (a >= n) return;
   const int s = slot[a];
   for (int k = 0; k < 3; ++k) {
@@ -1879,6 +2040,7 @@ __global__ void k_unsort_forces(int n, const int *__restrict__ slot, const typen
       fout[3 * a + k] = (double)(long long)facc[3 * s + k] * (1.0 / kForceScale);
     else
       fout[3 * a + k] = ((const double *)facc)[3 * s + k];
+    facc[3 * s + k] = 0;
   }
 }
 
