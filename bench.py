@@ -33,6 +33,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 69          # algorithmic FLOPs per in-cutoff pair (LJ C2 + PME real, counted once)
+# HBM bytes per k_pair launch (mixed form, skin 0.7 A): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
+# scales from a plain 256 MiB copy (profiles/k_pair_traffic_polymer26000.md)
+PAIR_TRAFFIC_BYTES = 57.7e6
+PAIR_TRAFFIC_SKIN = 0.7
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
 TIMESTEP_FS = 1.0
@@ -92,6 +96,21 @@ def run_form(ctx, s, spec, precision, steps, warmup, skin, drift_steps):
     return dict(ms_per_step=(t1 - t0) * 1e3 / steps, stages=stages, ncalls=ncalls, npairs=npairs,
                 rebuilds=st1["rebuilds"] - st0["rebuilds"], rollbacks=st1["rollbacks"] - st0["rollbacks"],
                 x=x, drift_kcal_mol_ns=slope, etot_first=etot[0], etot_last=etot[-1])
+
+
+def time_pair(ctx, s, spec, x, skin, reps: int = 200) -> float:
+    """Average device ms of one k_pair launch (mixed form) on the list an MD
+    step with this skin builds at positions x: the kernel alone, launched
+    back to back between two HIP events on its stream."""
+    from protomol3_amd import pmx
+    sy = pmx.System(ctx, s, spec, pmx.MIXED)
+    try:
+        sy.md_init(s, TIMESTEP_FS, skin=skin)
+        sy.md_set_state(x, s.vel)
+        sy.time_pair(10)
+        return sy.time_pair(reps)
+    finally:
+        sy.close()
 
 
 def serving(nreq: int, slots: int, seed: int = 7, log=print):
@@ -219,6 +238,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64, help="requests in the serving stream (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--replicas", type=int, default=0, help="batched replicas to run after the main metric (0: skip)")
+    ap.add_argument("--pair-only", action="store_true",
+                    help="only time the pair kernel alone (at --skin, default 0.7 A) and print it")
     ap.add_argument("--replica-steps", type=int, default=500)
     ap.add_argument("--replica-lanes", type=int, default=4)
     a = ap.parse_args()
@@ -231,6 +252,13 @@ def main():
     s, spec = workload(ctx)
     log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
     log(f"{s.n} atoms, steps={a.steps} warmup={a.warmup}")
+    if a.pair_only:
+        skin = a.skin if a.skin >= 0 else 0.7
+        ms = time_pair(ctx, s, spec, s.pos, skin)
+        print(json.dumps({"kernel": "k_pair<mixed,PME,C2,Cutoff>", "skin_A": skin, "avg_launch_ms": round(ms, 5)}),
+              flush=True)
+        ctx.close()
+        return
     skins = {}
     if a.skin < 0:
         skin_m, times_m = tune.tune_skin(ctx, s, spec, pmx.MIXED, s.pos, s.vel)
@@ -251,13 +279,14 @@ def main():
         sy = pmx.System(ctx, s, spec, prec)
         forms[prec] = sy.evaluate(mixed["x"])
         sy.close()
+    pair_launch_ms = time_pair(ctx, s, spec, mixed["x"], skin_m)
     ctx.close()
     f64, fm = forms[pmx.FP64][0], forms[pmx.MIXED][0]
     e64, em = forms[pmx.FP64], forms[pmx.MIXED]
     rel_e = {"coulomb": float(f"{abs(em[1] - e64[1]) / abs(e64[1]):.3e}"),
              "lj": float(f"{abs(em[2] - e64[2]) / abs(e64[2]):.3e}")}
     err = float(np.sqrt(np.mean(np.sum((fm - f64) ** 2, 1))) / np.sqrt(np.mean(np.sum(f64 ** 2, 1))))
-    pair_ms = mixed["stages"]["pair"]
+    pair_ms = pair_launch_ms
     achieved = FLOP_PER_PAIR * mixed["npairs"] / (pair_ms * 1e-3) / 1e12
     cpu = None if a.no_cpu else cpu_baseline(s, a.cpu_budget)
     serve_out = serving(a.serve_requests, a.serve_slots, log=log) if a.serve_requests > 0 else None
@@ -296,9 +325,11 @@ def main():
         "stage_ms_fp64": {k: round(v, 5) for k, v in fp64["stages"].items()},
         "roofline": {"bound": "valu", "kernel": "k_pair<mixed,PME,C2,Cutoff>",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": PAIR_TRAFFIC_BYTES,
+                     "traffic_unit": "bytes per launch (rocprofv3 PMC, skin %.1f A)" % PAIR_TRAFFIC_SKIN,
                      "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": int(mixed["npairs"]),
-                     "avg_launch_ms": round(pair_ms, 5)},
+                     "avg_launch_ms": round(pair_ms, 5),
+                     "avg_launch_ms_in_md_step": round(mixed["stages"]["pair"], 5)},
         "cpu_baseline": cpu,
         "serving": serve_out,
     }

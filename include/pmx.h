@@ -144,6 +144,12 @@ int pmx_sys_stage_times(const pmx_sys *sys, float *ms);
  * stream, read after one synchronisation). avg_ms holds PMX_STAGES floats. */
 int pmx_sys_timing_begin(pmx_sys *sys);
 int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls);
+/* Average device milliseconds of one launch of the cluster-pair kernel alone:
+ * one full evaluation at the last positions builds the list, then the kernel
+ * is launched `reps` times back to back between two HIP events on the
+ * system's main stream.  Measurement only, no counterpart in the reference;
+ * the forces of the last evaluation are kept. */
+int pmx_sys_time_pair(pmx_sys *sys, int reps, float *ms_per_launch);
 
 /* ---------------------------------------------------------------------
  * Device-resident MD (the caller of the hot path: StandardIntegrator /
@@ -172,6 +178,8 @@ typedef struct {
   double temperature;        /* Langevin: K */
   double gamma;              /* Langevin: friction in 1/ps (the reference's -gamma) */
   unsigned long long seed;   /* Langevin: noise stream (Philox4x32-10 key) */
+  int graphs;                /* 1: each chunk of steps is captured once as a HIP graph and replayed
+                                (one launch per chunk); 0: kernels are launched one by one */
 } pmx_md_desc;
 /* Integrators: LeapfrogIntegrator (integrator/leapfrog/LeapfrogIntegrator.cpp:
  * 33-104) and LangevinLeapfrogIntegrator (integrator/base/

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <set>
 #include <string>
 #include <thread>
@@ -67,7 +68,7 @@ struct Sys {
   PmeParams pme{};
   FftPlan plx{}, ply{}, plz{};
   bool pme_on = false;
-  int cap = 0, maxmaskwork = 0, ntiles_cap = 0;
+  int cap = 0, maxmaskwork = 0, ntiles_cap = 0, tile_target = 0;
   long long nbins = 0;
   int nscanblocks = 0;
   int recip_blocks = 0, corr_blocks = 0;
@@ -110,6 +111,11 @@ struct Sys {
     // host copies (per slot: kMaxChunk displacements + the overflow flag)
     unsigned *host = nullptr;
     hipEvent_t chunk_ev[2] = {nullptr, nullptr};
+    // chunks captured as HIP graphs, keyed by (steps, first, slot); the
+    // chunk's first step index is written to step_dev before each launch
+    bool graphs = false;
+    DevBuf step_dev;
+    std::map<long long, hipGraphExec_t> gexec;
   } md;
 };
 constexpr int kMaxChunk = 128;
@@ -127,12 +133,20 @@ int fail(pmx_ctx *c, int code, const std::string &m) {
     if (_e != hipSuccess) return fail(s->ctx, PMX_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
   } while (0)
 
+// Runtime calls that may synchronize the device (allocation, free, pinned
+// allocation, event and stream creation, synchronous copies) can invalidate a
+// stream capture that another thread has in progress, so they and the
+// captures themselves are serialized on this lock.
+std::recursive_mutex g_rt_mu;
+using RtLock = std::lock_guard<std::recursive_mutex>;
+
 int dalloc(Sys *s, DevBuf &b, size_t bytes) {
   if (bytes == 0) bytes = 8;
   if (s->dry) {
     s->dry_bytes += bytes;
     return PMX_OK;
   }
+  RtLock lk(g_rt_mu);
   if (b.p && b.bytes >= bytes) return PMX_OK;
   if (b.p) hipFree(b.p);
   b.p = nullptr;
@@ -427,7 +441,8 @@ int setup(Sys *s) {
     s->cap = (s->cap + kTileEntries - 1) / kTileEntries * kTileEntries;
   }
   lp.cap = s->cap;
-  s->ntiles_cap = s->nsuper * (s->cap / kTileEntries);
+  s->tile_target = 16 * (s->ctx ? s->ctx->cus : 256);  // a tile per resident pair-kernel wave
+  s->ntiles_cap = s->nsuper * (s->cap / kTileEntries) + 2 * s->tile_target + s->nsuper;
   s->maxmaskwork = std::max(1024, s->nsuper * 128);
 
   int rc;
@@ -470,6 +485,8 @@ int setup(Sys *s) {
   }
 
   hipStream_t st = s->ctx->stream;
+  // the cell histogram starts cleared; each rebuild's scan clears it again
+  hipLaunchKernelGGL(k_fill_u32, dim3(nblk(s->nbins, 256)), dim3(256), 0, st, P<uint32_t>(s->hist), 0u, s->nbins);
   if (!s->fork_ev) HIPCHK(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
   if (!s->join_ev) HIPCHK(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
   HIPCHK(hipMemcpyAsync(s->q_d.p, s->q.data(), 8 * N, hipMemcpyHostToDevice, st));
@@ -773,6 +790,7 @@ void reduce_energies(Sys *s, bool stats) {
 
 void timing_begin_call(Sys *s, bool timing) {
   if (!timing) return;
+  RtLock lk(g_rt_mu);
   if (s->evused == s->evsets.size()) {
     std::vector<hipEvent_t> set(kStages + 1);
     for (auto &e : set) hipEventCreate(&e);
@@ -788,39 +806,39 @@ void timing_begin_call(Sys *s, bool timing) {
 int rebuild(Sys *s, bool timing) {
   hipStream_t st = s->ctx->stream;
   const int n = s->n, npad = s->npad;
+  // the histogram starts cleared (set-up, then each scan clears it)
   hipLaunchKernelGGL(k_bin, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<double>(s->xorig), s->bin, P<uint32_t>(s->key),
-                     P<double>(s->xwrap), P<int>(s->refcell));
-  hipLaunchKernelGGL(k_fill_u32, dim3(nblk(s->nbins, 256)), dim3(256), 0, st, P<uint32_t>(s->hist), 0u, s->nbins);
-  hipLaunchKernelGGL(k_hist, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<uint32_t>(s->key), P<uint32_t>(s->hist));
-  hipLaunchKernelGGL(k_scan_blocksums, dim3(s->nscanblocks), dim3(kScanBlock), 0, st, P<uint32_t>(s->hist), s->nbins,
-                     P<uint32_t>(s->scansum));
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, st, P<uint32_t>(s->scansum), s->nscanblocks);
-  hipLaunchKernelGGL(k_scan_final, dim3(s->nscanblocks), dim3(kScanBlock), 0, st, P<uint32_t>(s->hist), s->nbins,
-                     P<uint32_t>(s->scansum), P<uint32_t>(s->start), P<uint32_t>(s->cursor));
+                     P<double>(s->xwrap), P<int>(s->refcell), P<uint32_t>(s->hist));
+  if (s->nbins <= kScanSingleMax) {
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanBlock), 0, st, P<uint32_t>(s->hist), s->nbins,
+                       P<uint32_t>(s->start), P<uint32_t>(s->cursor));
+  } else {
+    hipLaunchKernelGGL(k_scan_blocksums, dim3(s->nscanblocks), dim3(kScanBlock), 0, st, P<uint32_t>(s->hist),
+                       s->nbins, P<uint32_t>(s->scansum));
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, st, P<uint32_t>(s->scansum), s->nscanblocks);
+    hipLaunchKernelGGL(k_scan_final, dim3(s->nscanblocks), dim3(kScanBlock), 0, st, P<uint32_t>(s->hist), s->nbins,
+                       P<uint32_t>(s->scansum), P<uint32_t>(s->start), P<uint32_t>(s->cursor));
+  }
   hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<uint32_t>(s->key), P<uint32_t>(s->cursor),
                      P<int>(s->perm));
   hipLaunchKernelGGL(k_binrank, dim3(nblk(n, 256)), dim3(256), 0, st, n, s->nbins, P<uint32_t>(s->key),
                      P<uint32_t>(s->start), P<int>(s->perm), P<int>(s->perm2));
   hipLaunchKernelGGL(k_columns, dim3(1), dim3(1024), 0, st, s->ncols, s->bin.nzb, n, P<uint32_t>(s->start),
                      s->nsuper, P<int2>(s->sc_info), P<int>(s->sc_col), P<int>(s->colsc));
-  hipLaunchKernelGGL(k_split, dim3(nblk((long long)s->nsuper * 64, 256)), dim3(256), 0, st, s->nsuper,
-                     P<int2>(s->sc_info), P<int>(s->perm2), P<double>(s->xwrap), P<int>(s->permp), P<int>(s->slot));
   double *xbuild = s->md.on ? P<double>(s->md.xbuild) : nullptr;
   if (s->mixed)
-    hipLaunchKernelGGL(k_fill_sorted<float>, dim3(nblk(npad, 256)), dim3(256), 0, st, npad, P<int>(s->permp),
-                       P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<float4>(s->xs),
-                       P<double4>(s->xs64), P<int>(s->ts), P<double>(s->xorig), xbuild,
-                       P<float4>(s->sorg));
+    hipLaunchKernelGGL(k_split_fill<float>, dim3(nblk((long long)s->nsuper * 64, 256)), dim3(256), 0, st, s->nsuper,
+                       P<int2>(s->sc_info), P<int>(s->perm2), P<double>(s->xwrap), P<int>(s->permp), P<int>(s->slot),
+                       P<double>(s->q_d), P<int>(s->type_d), P<float4>(s->xs), P<double4>(s->xs64), P<int>(s->ts),
+                       P<double>(s->xorig), xbuild, P<float4>(s->sorg), P<float4>(s->ctr), P<float4>(s->half),
+                       P<float4>(s->sctr), P<float4>(s->shalf), P<int>(s->nmaskwork));
   else
-    hipLaunchKernelGGL(k_fill_sorted<double>, dim3(nblk(npad, 256)), dim3(256), 0, st, npad, P<int>(s->permp),
-                       P<double>(s->xwrap), P<double>(s->q_d), P<int>(s->type_d), P<double4>(s->xs),
-                       P<double4>(s->xs64), P<int>(s->ts), P<double>(s->xorig), xbuild,
-                       P<float4>(s->sorg));
+    hipLaunchKernelGGL(k_split_fill<double>, dim3(nblk((long long)s->nsuper * 64, 256)), dim3(256), 0, st, s->nsuper,
+                       P<int2>(s->sc_info), P<int>(s->perm2), P<double>(s->xwrap), P<int>(s->permp), P<int>(s->slot),
+                       P<double>(s->q_d), P<int>(s->type_d), P<double4>(s->xs), P<double4>(s->xs64), P<int>(s->ts),
+                       P<double>(s->xorig), xbuild, P<float4>(s->sorg), P<float4>(s->ctr), P<float4>(s->half),
+                       P<float4>(s->sctr), P<float4>(s->shalf), P<int>(s->nmaskwork));
   if (timing) hipEventRecord(s->ev[1], st);
-  hipLaunchKernelGGL(k_cluster_bbox, dim3(nblk(s->ncl, 256)), dim3(256), 0, st, s->ncl, P<int>(s->permp),
-                     P<double4>(s->xs64), P<float4>(s->ctr), P<float4>(s->half), P<float4>(s->sctr),
-                     P<float4>(s->shalf));
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, P<uint32_t>(s->nmaskwork), 0u, 1ll);
   hipLaunchKernelGGL(k_build_list, dim3(s->nsuper), dim3(kListBlock), 0, st, s->lp, P<int>(s->sc_col),
                      P<int>(s->colsc), P<float4>(s->sctr), P<float4>(s->shalf), P<float4>(s->ctr),
                      P<float4>(s->half), P<int4>(s->entries), P<int>(s->count),
@@ -834,7 +852,7 @@ int rebuild(Sys *s, bool timing) {
                      P<int>(s->nmaskwork), s->maxmaskwork, P<int4>(s->entries), s->cap, P<int>(s->permp),
                      P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned long long>(s->masks));
   hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int2>(s->tiles),
-                     P<int>(s->ntiles));
+                     P<int>(s->ntiles), s->ntiles_cap, s->tile_target);
   if (timing) hipEventRecord(s->ev[2], st);
   return PMX_OK;
 }
@@ -875,12 +893,14 @@ int forces(Sys *s, unsigned flags, bool energies) {
     else
       run_spread<false>(s);
   }
-  if (timing) hipEventRecord(s->ev[4], st);
   // fork: FFT, gather, exclusion correction and bonded terms on the second
-  // stream beside the pair kernel; all add into the zeroed accumulators
+  // stream beside the pair kernel; all add into the zeroed accumulators.
+  // The pair stage's start event is recorded after the fork, right before
+  // the pair kernel.
   hipStream_t st2 = s->ctx->stream2;
   HIPCHK(hipEventRecord(s->fork_ev, st));
   HIPCHK(hipStreamWaitEvent(st2, s->fork_ev, 0));
+  if (timing) hipEventRecord(s->ev[4], st);
   if (s->mixed)
     run_pair_phase<true>(s, stats, 0);
   else
@@ -956,12 +976,18 @@ int finish(Sys *s, unsigned flags, double *f_aos, double *energies) {
   return PMX_OK;
 }
 
+void drop_graphs(Sys *s) {
+  for (auto &kv : s->md.gexec) hipGraphExecDestroy(kv.second);
+  s->md.gexec.clear();
+}
+
 int grow_and_retry(Sys *s) {
+  drop_graphs(s);  // captured chunks hold the old buffer addresses
   // double the list capacity and the mask pool, then rebuild buffers
   s->cap = std::min(s->ncl + kTileEntries, 2 * s->cap);
   s->cap = (s->cap + kTileEntries - 1) / kTileEntries * kTileEntries;
   s->lp.cap = s->cap;
-  s->ntiles_cap = s->nsuper * (s->cap / kTileEntries);
+  s->ntiles_cap = s->nsuper * (s->cap / kTileEntries) + 2 * s->tile_target + s->nsuper;
   s->maxmaskwork *= 2;
   s->maxborder *= 2;
   if (int rc0 = dalloc(s, s->border, 8 * (size_t)s->maxborder)) return rc0;
@@ -988,6 +1014,7 @@ unsigned long long pmx_pair_hash(int i, int j) {
 }
 
 int pmx_ctx_create(int device, pmx_ctx **out) {
+  RtLock lk(g_rt_mu);
   if (!out) return PMX_ERR_ARG;
   *out = nullptr;
   int ndev = 0;
@@ -1015,6 +1042,7 @@ int pmx_ctx_create(int device, pmx_ctx **out) {
 }
 
 int pmx_ctx_destroy(pmx_ctx *ctx) {
+  RtLock lk(g_rt_mu);
   if (!ctx) return PMX_ERR_ARG;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -1026,6 +1054,7 @@ int pmx_ctx_destroy(pmx_ctx *ctx) {
 const char *pmx_last_error(const pmx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int pmx_sys_create(pmx_ctx *ctx, const pmx_sys_desc *desc, pmx_sys **out) {
+  RtLock lk(g_rt_mu);
   if (!ctx || !out) return PMX_ERR_ARG;
   *out = nullptr;
   int rc = validate(ctx, desc);
@@ -1044,11 +1073,13 @@ int pmx_sys_create(pmx_ctx *ctx, const pmx_sys_desc *desc, pmx_sys **out) {
 }
 
 int pmx_sys_destroy(pmx_sys *sys) {
+  RtLock lk(g_rt_mu);
   if (!sys) return PMX_ERR_ARG;
   Sys *s = &sys->s;
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   hipStreamSynchronize(s->ctx->stream2);
+  drop_graphs(s);
   for (DevBuf *b : s->all)
     if (b->p) hipFree(b->p);
   if (s->fork_ev) hipEventDestroy(s->fork_ev);
@@ -1170,6 +1201,54 @@ int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls) {
   return PMX_OK;
 }
 
+int pmx_sys_time_pair(pmx_sys *sys, int reps, float *ms_per_launch) {
+  if (!sys || reps < 1 || reps > 4096 || !ms_per_launch) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  if (!s->have_positions) return fail(s->ctx, PMX_ERR_ARG, "no positions set");
+  hipSetDevice(s->ctx->device);
+  hipStream_t st = s->ctx->stream;
+  if (int rc = compute(s, 0)) return rc;
+  // each launch gets its own zeroed tile counter and boundary-pair count
+  int *ctrs = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  {
+    RtLock lk(g_rt_mu);
+    if (hipMalloc(&ctrs, 8 * (size_t)reps) != hipSuccess) return fail(s->ctx, PMX_ERR_OOM, "hipMalloc failed");
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      if (e0) hipEventDestroy(e0);
+      hipFree(ctrs);
+      return fail(s->ctx, PMX_ERR_HIP, "hipEventCreate failed");
+    }
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3(nblk(2ll * reps, 256)), dim3(256), 0, st, ctrs, 0, 2ll * reps);
+  void *const tc = s->tilectr.p, *const nb = s->nborder.p;
+  hipEventRecord(e0, st);
+  for (int r = 0; r < reps; ++r) {
+    s->tilectr.p = ctrs + r;
+    s->nborder.p = ctrs + reps + r;
+    if (s->mixed)
+      run_pair_phase<true>(s, false, 0);
+    else
+      run_pair_phase<false>(s, false, 0);
+  }
+  s->tilectr.p = tc;
+  s->nborder.p = nb;
+  hipEventRecord(e1, st);
+  const hipError_t e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) hipEventElapsedTime(&t, e0, e1);
+  {
+    RtLock lk(g_rt_mu);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(ctrs);
+  }
+  s->acc_clean = false;  // the repeated launches added into the accumulators
+  if (e != hipSuccess) return fail(s->ctx, PMX_ERR_HIP, hipGetErrorString(e));
+  *ms_per_launch = t / (float)reps;
+  return PMX_OK;
+}
+
 int pmx_sys_stage_times(const pmx_sys *sys, float *ms) {
   if (!sys || !ms) return PMX_ERR_ARG;
   for (int k = 0; k < kStages; ++k) ms[k] = sys->s.stage_ms[k];
@@ -1180,6 +1259,7 @@ int pmx_sys_stage_times(const pmx_sys *sys, float *ms) {
 // Device-resident MD
 
 int pmx_md_init(pmx_sys *sys, const pmx_md_desc *d) {
+  RtLock lk(g_rt_mu);
   if (!sys || !d) return PMX_ERR_ARG;
   Sys *s = &sys->s;
   hipSetDevice(s->ctx->device);
@@ -1200,7 +1280,9 @@ int pmx_md_init(pmx_sys *sys, const pmx_md_desc *d) {
     if (rcmax + d->skin > 0.5 * s->box.L[k])
       return fail(s->ctx, PMX_ERR_UNSUPPORTED, "cutoff + skin larger than half the box");
   Sys::Md &m = s->md;
+  drop_graphs(s);
   m.on = true;
+  m.graphs = d->graphs != 0;
   m.nb = d->nbonds;
   m.na = d->nangles;
   m.h = d->timestep_fs * kInvTimeFactor;
@@ -1240,7 +1322,8 @@ int pmx_md_init(pmx_sys *sys, const pmx_md_desc *d) {
       (rc = dalloc(s, m.angles, 16 * a.size())) || (rc = dalloc(s, m.apar, 16 * ap.size())) ||
       (rc = dalloc(s, m.minv, 8 * N)) || (rc = dalloc(s, m.v, 24 * N)) || (rc = dalloc(s, m.xbuild, 24 * N)) ||
       (rc = dalloc(s, m.xbak, 2 * 24 * N)) || (rc = dalloc(s, m.vbak, 2 * 24 * N)) ||
-      (rc = dalloc(s, m.fbak, 2 * 24 * N)) || (rc = dalloc(s, m.dmax, 4 * 2 * kMaxChunk)) || (rc = dalloc(s, m.bond_e, 16 * (size_t)nbonded_blocks)) ||
+      (rc = dalloc(s, m.fbak, 2 * 24 * N)) || (rc = dalloc(s, m.dmax, 4 * 2 * kMaxChunk)) ||
+      (rc = dalloc(s, m.step_dev, 8)) || (rc = dalloc(s, m.bond_e, 16 * (size_t)nbonded_blocks)) ||
       (rc = dalloc(s, m.ke_e, 8 * (size_t)nblk(N, 256))))
     return rc;
   if (!m.host) {
@@ -1298,7 +1381,8 @@ int md_ensure_forces(Sys *s) {
 
 int md_check_overflow(Sys *s) {
   int ovf = 0;
-  HIPCHK(hipMemcpy(&ovf, s->overflow.p, 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(&ovf, s->overflow.p, 4, hipMemcpyDeviceToHost, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
   if (ovf) return fail(s->ctx, PMX_ERR_OVERFLOW, "pair list capacity exceeded during MD");
   return PMX_OK;
 }
@@ -1309,7 +1393,8 @@ int md_check_overflow(Sys *s) {
 // gets one half kick (kick = 0.5), later steps the previous step's closing
 // half kick and their own opening one (kick = 1).  Langevin half kicks are
 // numbered 2 step (opening) and 2 step + 1 (closing).
-Kick make_kick(const Sys::Md &m, double kick, long long step) {
+// With step_dev, `step` is relative to the step index stored there.
+Kick make_kick(const Sys::Md &m, double kick, long long step, const unsigned long long *step_dev = nullptr) {
   Kick kk{};
   kk.h_kick = kick * m.h;
   kk.langevin = m.integrator == PMX_MD_LANGEVIN;
@@ -1320,14 +1405,16 @@ Kick make_kick(const Sys::Md &m, double kick, long long step) {
   kk.fdt = m.fdt;
   kk.ndt = m.ndt;
   kk.fc = m.fc;
+  kk.step_dev = step_dev;
   return kk;
 }
 
-int md_step(Sys *s, double kick, long long step, bool rebuild_here, int dslot) {
+int md_step(Sys *s, double kick, long long step, bool rebuild_here, int dslot,
+            const unsigned long long *step_dev = nullptr) {
   hipStream_t st = s->ctx->stream;
   Sys::Md &m = s->md;
   const bool timing = s->timing_active;
-  const Kick kk = make_kick(m, kick, step);
+  const Kick kk = make_kick(m, kick, step, step_dev);
   // stage 0 of a timed MD step is integrate (+ sort when the list is rebuilt)
   timing_begin_call(s, timing);
   if (s->mixed)
@@ -1361,7 +1448,8 @@ struct Chunk {
   int slot;
 };
 
-int enqueue_chunk(Sys *s, const Chunk &c) {
+// The work of a chunk.  rel: step indices relative to *step_dev (capture).
+int chunk_body(Sys *s, const Chunk &c, bool rel) {
   Sys::Md &m = s->md;
   hipStream_t st = s->ctx->stream;
   const size_t n = s->n, off = c.slot * 3 * n;
@@ -1371,14 +1459,66 @@ int enqueue_chunk(Sys *s, const Chunk &c) {
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kMaxChunk), 0, st, P<uint32_t>(m.dmax) + c.slot * kMaxChunk, 0u,
                      (long long)kMaxChunk);
+  const unsigned long long *sd = rel ? P<unsigned long long>(m.step_dev) : nullptr;
   for (int k = 0; k < c.K; ++k) {
-    int rc = md_step(s, (c.first && k == 0) ? 0.5 : 1.0, c.step0 + k, k == 0 || m.skin == 0.0,
-                     c.slot * kMaxChunk + k);
+    int rc = md_step(s, (c.first && k == 0) ? 0.5 : 1.0, rel ? k : c.step0 + k, k == 0 || m.skin == 0.0,
+                     c.slot * kMaxChunk + k, sd);
     if (rc) return rc;
   }
   unsigned *h = m.host + c.slot * (kMaxChunk + 1);
   HIPCHK(hipMemcpyAsync(h, P<uint32_t>(m.dmax) + c.slot * kMaxChunk, 4 * (size_t)c.K, hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(h + kMaxChunk, s->overflow.p, 4, hipMemcpyDeviceToHost, st));
+  return PMX_OK;
+}
+
+constexpr size_t kMaxGraphs = 64;
+
+// Queues a chunk: replays its captured graph (capturing it on first use), or
+// launches its kernels one by one when graphs are off or stage timing is on.
+int enqueue_chunk(Sys *s, const Chunk &c) {
+  Sys::Md &m = s->md;
+  hipStream_t st = s->ctx->stream;
+  hipGraphExec_t x = nullptr;
+  // only chunks whose length is a power of two are captured (the others are
+  // launched kernel by kernel), so a system holds at most 2 x 2 x 8 graphs
+  if (m.graphs && !s->timing_active && (c.K & (c.K - 1)) == 0) {
+    const long long key = ((long long)c.K << 2) | ((long long)c.first << 1) | c.slot;
+    auto it = m.gexec.find(key);
+    if (it != m.gexec.end()) {
+      x = it->second;
+    } else {
+      RtLock lk(g_rt_mu);
+      if (m.gexec.size() >= kMaxGraphs) {
+        // a launched graph may still run: wait before destroying
+        HIPCHK(hipStreamSynchronize(st));
+        drop_graphs(s);
+      }
+      hipGraph_t g = nullptr;
+      bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+      if (ok) {
+        const int rc = chunk_body(s, c, true);
+        ok = hipStreamEndCapture(st, &g) == hipSuccess && rc == PMX_OK && g;
+      }
+      if (ok) ok = hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess;
+      if (g) hipGraphDestroy(g);
+      if (ok) {
+        m.gexec.emplace(key, x);
+      } else {
+        // nothing captured runs: take this system off graphs and launch the
+        // chunk's kernels one by one instead
+        x = nullptr;
+        m.graphs = false;
+        (void)hipGetLastError();
+      }
+    }
+  }
+  if (x) {
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, P<unsigned long long>(m.step_dev),
+                       (unsigned long long)c.step0);
+    HIPCHK(hipGraphLaunch(x, st));
+  } else if (int rc = chunk_body(s, c, false)) {
+    return rc;
+  }
   HIPCHK(hipEventRecord(m.chunk_ev[c.slot], st));
   return PMX_OK;
 }

@@ -4,8 +4,10 @@
 //   xorig  double[3N]  positions as the caller hands them (AoS, original order)
 //   permp  int[Npad]   sorted slot -> original atom (-1 for padding slots)
 //   slot   int[N]      original atom -> sorted slot
-//   xs     R4[Npad]    coordinates + scaled charge, sorted (R = double for the
-//                      FP64 form, float for the mixed form)
+//   xs     R4[Npad]    coordinates relative to the slot's supercluster origin
+//                      + scaled charge, sorted (R = double for the FP64 form,
+//                      float for the mixed form)
+//   sorg   float4[Nsuper]  supercluster origins (multiples of 1/16 A)
 //   ts     int[Npad]   LJ type per sorted slot (-1 padding)
 //   facc   Acc[3Npad]  force accumulators in sorted order (double, or int64
 //                      fixed point 2^-32 for the mixed form)
@@ -99,7 +101,7 @@ struct BinParams {
 };
 
 __global__ void k_bin(int n, const double *__restrict__ xorig, BinParams p, uint32_t *__restrict__ key,
-                      double *__restrict__ xwrap, int *__restrict__ refcell) {
+                      double *__restrict__ xwrap, int *__restrict__ refcell, uint32_t *__restrict__ hist) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
   double w[3];
@@ -114,7 +116,9 @@ __global__ void k_bin(int n, const double *__restrict__ xorig, BinParams p, uint
   // key: x-y column, then z bin
   const int cx = min((int)(w[0] * p.cxinv), p.ncx - 1), cy = min((int)(w[1] * p.cyinv), p.ncy - 1);
   const int zb = min((int)(w[2] * p.zinv), p.nzb - 1);
-  key[a] = (uint32_t)((cx * p.ncy + cy) * p.nzb + zb);
+  const uint32_t kb = (uint32_t)((cx * p.ncy + cy) * p.nzb + zb);
+  key[a] = kb;
+  atomicAdd(&hist[kb], 1u);  // the scan clears hist after reading it
   xwrap[3 * a] = w[0];
   xwrap[3 * a + 1] = w[1];
   xwrap[3 * a + 2] = w[2];
@@ -158,11 +162,6 @@ __global__ void k_fill_f64(double *p, double v, long long n) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_hist(int n, const uint32_t *__restrict__ key, uint32_t *__restrict__ hist) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < n) atomicAdd(&hist[key[a]], 1u);
-}
-
 // Exclusive scan of m uint32 in 3 launches: per-block sums, scan of sums in
 // one block, then per-block rescan with the block offset.
 constexpr int kScanBlock = 1024;
@@ -201,7 +200,7 @@ __global__ void k_scan_sums(uint32_t *sums, int nb) {
   }
 }
 // Exclusive scan result to out and to out2 (the scatter cursors).
-__global__ void k_scan_final(const uint32_t *__restrict__ in, long long m, const uint32_t *__restrict__ sums,
+__global__ void k_scan_final(uint32_t *__restrict__ in, long long m, const uint32_t *__restrict__ sums,
                              uint32_t *__restrict__ out, uint32_t *__restrict__ out2) {
   __shared__ uint32_t s[kScanBlock];
   long long i = (long long)blockIdx.x * kScanBlock + threadIdx.x;
@@ -218,6 +217,36 @@ __global__ void k_scan_final(const uint32_t *__restrict__ in, long long m, const
     const uint32_t r = sums[blockIdx.x] + s[threadIdx.x] - v;
     out[i] = r;
     out2[i] = r;
+    in[i] = 0u;  // cleared for the next histogram
+  }
+}
+
+// Exclusive scan of m <= kScanSingleMax counts in one block: each thread
+// scans a contiguous chunk, chunk totals are scanned across the block.  The
+// counts are cleared for the next histogram.
+constexpr long long kScanSingleMax = 1 << 16;
+__global__ void k_scan_single(uint32_t *__restrict__ in, long long m, uint32_t *__restrict__ out,
+                              uint32_t *__restrict__ out2) {
+  __shared__ uint32_t s[kScanBlock];
+  const long long per = (m + kScanBlock - 1) / kScanBlock;
+  const long long b = threadIdx.x * per, e = min(m, b + per);
+  uint32_t t = 0;
+  for (long long i = b; i < e; ++i) t += in[i];
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    const uint32_t u = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += u;
+    __syncthreads();
+  }
+  uint32_t r = s[threadIdx.x] - t;
+  for (long long i = b; i < e; ++i) {
+    const uint32_t v = in[i];
+    out[i] = r;
+    out2[i] = r;
+    in[i] = 0u;
+    r += v;
   }
 }
 
@@ -305,14 +334,38 @@ __device__ inline void wave_sort(double &key, int &idx) {
   }
 }
 
-// One wave per supercluster: its atoms (lanes, padding last) sorted by x,
-// halves by y, quarters by z, so slots 8c..8c+7 form compact clusters.
-// Writes permp (slot -> atom, -1 padding) and slot (atom -> slot).
-__global__ void __launch_bounds__(256) k_split(int nsuper_max, const int2 *__restrict__ sc_info,
-                                               const int *__restrict__ order, const double *__restrict__ xwrap,
-                                               int *__restrict__ permp, int *__restrict__ slot) {
+
+// Supercluster boxes are widened by this slack (A) beyond the union of their
+// cluster boxes, so that FP32 rounding keeps every cluster box inside.
+constexpr float kScSlack = 1e-3f;
+
+// One wave per supercluster (the atoms of sc_info[sc], in z order within
+// their column):
+//  - split: its atoms (lanes, padding last) are sorted by x, halves by y,
+//    quarters by z, so slots 8c..8c+7 form compact clusters; permp (slot ->
+//    atom, -1 padding) and slot (atom -> slot) are written;
+//  - sorted SoA: coordinates relative to the supercluster origin (the
+//    position of the first slot's atom rounded to 1/16 A, in sorg) + charge,
+//    FP64 copy in xs64, LJ type (padding: type -1); xbuild (MD only, else
+//    null) receives the positions the list is built from;
+//  - bounding boxes (FP32, list building only): per cluster (8-lane groups;
+//    ctr.w = real atoms, empty clusters get negative half extents), and per
+//    supercluster (the whole wave, widened by kScSlack).
+// Thread 0 also clears the mask work counter of the list build that follows.
+template <class R>
+__global__ void __launch_bounds__(256) k_split_fill(int nsuper_max, const int2 *__restrict__ sc_info,
+                                                    const int *__restrict__ order, const double *__restrict__ xwrap,
+                                                    int *__restrict__ permp, int *__restrict__ slot,
+                                                    const double *__restrict__ q, const int *__restrict__ type,
+                                                    typename Real4<R>::type *__restrict__ xs,
+                                                    double4 *__restrict__ xs64, int *__restrict__ ts,
+                                                    const double *__restrict__ xorig, double *__restrict__ xbuild,
+                                                    float4 *__restrict__ sorg, float4 *__restrict__ ctr,
+                                                    float4 *__restrict__ half, float4 *__restrict__ sctr,
+                                                    float4 *__restrict__ shalf, int *__restrict__ nmaskwork) {
   const int sc = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nmaskwork = 0;
   if (sc >= nsuper_max) return;  // whole waves
   const int2 info = sc_info[sc];
   int a = lane < info.y ? order[info.x + lane] : -1;
@@ -327,87 +380,45 @@ __global__ void __launch_bounds__(256) k_split(int nsuper_max, const int2 *__res
   a = idx == 0x7fffffff ? -1 : idx;
   permp[s] = a;
   if (a >= 0) slot[a] = s;
-}
-
-// Sorted SoA: coordinates relative to the supercluster origin + charge, and
-// type per slot (padding: type -1); sorg[S] = origin of supercluster S (the
-// wrapped position of its first atom rounded to 1/16 A; zero when empty).
-// xbuild (MD only, else null) receives the positions the list is built from.
-template <class R>
-__global__ void k_fill_sorted(int npad, const int *__restrict__ permp, const double *__restrict__ xwrap,
-                              const double *__restrict__ q, const int *__restrict__ type,
-                              typename Real4<R>::type *__restrict__ xs, double4 *__restrict__ xs64,
-                              int *__restrict__ ts, const double *__restrict__ xorig, double *__restrict__ xbuild,
-                              float4 *__restrict__ sorg) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= npad) return;
-  const int a0 = permp[s & ~(kSuperSize - 1)];
-  double o[3] = {0.0, 0.0, 0.0};
-  if (a0 >= 0)
-    for (int k = 0; k < 3; ++k) o[k] = origin_round(xwrap[3 * a0 + k]);
-  if ((s & (kSuperSize - 1)) == 0) sorg[s / kSuperSize] = make_float4((float)o[0], (float)o[1], (float)o[2], 0.f);
+  // sorted SoA
+  double p[3] = {0.0, 0.0, 0.0};
+  if (a >= 0)
+    for (int k = 0; k < 3; ++k) p[k] = xwrap[3 * a + k];
+  const int a0 = __shfl(a, 0);
+  double o[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double p0 = __shfl(p[k], 0);
+    o[k] = a0 >= 0 ? origin_round(p0) : 0.0;
+  }
+  if (lane == 0) sorg[sc] = make_float4((float)o[0], (float)o[1], (float)o[2], 0.f);
   typename Real4<R>::type v;
-  const int a = permp[s];
   if (a >= 0) {
     if (xbuild) {
       xbuild[3 * a] = xorig[3 * a];
       xbuild[3 * a + 1] = xorig[3 * a + 1];
       xbuild[3 * a + 2] = xorig[3 * a + 2];
     }
-    v.x = (R)(xwrap[3 * a] - o[0]);
-    v.y = (R)(xwrap[3 * a + 1] - o[1]);
-    v.z = (R)(xwrap[3 * a + 2] - o[2]);
+    v.x = (R)(p[0] - o[0]);
+    v.y = (R)(p[1] - o[1]);
+    v.z = (R)(p[2] - o[2]);
     v.w = (R)q[a];
     ts[s] = type[a];
-    xs64[s] = make_double4(xwrap[3 * a], xwrap[3 * a + 1], xwrap[3 * a + 2], q[a]);
+    xs64[s] = make_double4(p[0], p[1], p[2], q[a]);
   } else {
     v.x = v.y = v.z = v.w = (R)0;
     ts[s] = -1;
     xs64[s] = make_double4(0, 0, 0, 0);
   }
   xs[s] = v;
-}
-
-// ---------------------------------------------------------------------------
-// 2. Cluster bounding boxes (FP32; list-building only) and the cluster list.
-// ctr.w = number of real atoms (padding slots skipped); empty clusters get
-// negative half extents.  The eight clusters of a supercluster are eight
-// consecutive lanes; their union box, widened by kScSlack, is the
-// supercluster box (sctr, shalf) the list build prunes with.
-constexpr float kScSlack = 1e-3f;
-__global__ void k_cluster_bbox(int ncl, const int *__restrict__ permp, const double4 *__restrict__ xs64,
-                               float4 *__restrict__ ctr, float4 *__restrict__ half, float4 *__restrict__ sctr,
-                               float4 *__restrict__ shalf) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncl) return;  // ncl is a multiple of 8: whole lane groups leave together
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  int cnt = 0;
-  for (int t = 0; t < kClusterSize; ++t) {
-    int s = c * kClusterSize + t;
-    if (permp[s] < 0) continue;
-    double4 p = xs64[s];
-    lo[0] = fmin(lo[0], p.x);
-    lo[1] = fmin(lo[1], p.y);
-    lo[2] = fmin(lo[2], p.z);
-    hi[0] = fmax(hi[0], p.x);
-    hi[1] = fmax(hi[1], p.y);
-    hi[2] = fmax(hi[2], p.z);
-    ++cnt;
+  // boxes: clusters (8-lane groups), then the supercluster (the wave)
+  double lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = a >= 0 ? p[k] : 1e300;
+    hi[k] = a >= 0 ? p[k] : -1e300;
   }
-  if (cnt == 0) {
-    ctr[c] = make_float4(0, 0, 0, 0);
-    half[c] = make_float4(-1, -1, -1, 0);  // empty
-  } else {
-    ctr[c] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])),
-                         (float)(0.5 * (lo[2] + hi[2])), (float)cnt);
-    // round the half extents up by a relative ulp margin so the FP32 box
-    // contains the FP64 atoms
-    half[c] = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.0000005f + 1e-6f,
-                          (float)(0.5 * (hi[1] - lo[1])) * 1.0000005f + 1e-6f,
-                          (float)(0.5 * (hi[2] - lo[2])) * 1.0000005f + 1e-6f, 0.f);
-  }
-  // supercluster union over the 8-lane group
-  int scnt = cnt;
+  int cnt = a >= 0 ? 1 : 0;
 #pragma unroll
   for (int m = 1; m < kClusterSize; m <<= 1) {
 #pragma unroll
@@ -415,21 +426,46 @@ __global__ void k_cluster_bbox(int ncl, const int *__restrict__ permp, const dou
       lo[k] = fmin(lo[k], __shfl_xor(lo[k], m));
       hi[k] = fmax(hi[k], __shfl_xor(hi[k], m));
     }
-    scnt += __shfl_xor(scnt, m);
+    cnt += __shfl_xor(cnt, m);
   }
-  if ((c & (kClusterSize - 1)) == 0) {
-    const int S = c / kClusterSize;
-    if (scnt == 0) {
-      sctr[S] = make_float4(0, 0, 0, 0);
-      shalf[S] = make_float4(-1, -1, -1, 0);
+  if ((lane & (kClusterSize - 1)) == 0) {
+    const int c = s / kClusterSize;
+    if (cnt == 0) {
+      ctr[c] = make_float4(0, 0, 0, 0);
+      half[c] = make_float4(-1, -1, -1, 0);  // empty
     } else {
-      sctr[S] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])),
-                            (float)(0.5 * (lo[2] + hi[2])), (float)scnt);
-      shalf[S] = make_float4((float)(0.5 * (hi[0] - lo[0])) + kScSlack, (float)(0.5 * (hi[1] - lo[1])) + kScSlack,
-                             (float)(0.5 * (hi[2] - lo[2])) + kScSlack, 0.f);
+      ctr[c] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])),
+                           (float)(0.5 * (lo[2] + hi[2])), (float)cnt);
+      half[c] = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.0000005f + 1e-6f,
+                            (float)(0.5 * (hi[1] - lo[1])) * 1.0000005f + 1e-6f,
+                            (float)(0.5 * (hi[2] - lo[2])) * 1.0000005f + 1e-6f, 0.f);
+    }
+  }
+#pragma unroll
+  for (int m = kClusterSize; m < kWave; m <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fmin(lo[k], __shfl_xor(lo[k], m));
+      hi[k] = fmax(hi[k], __shfl_xor(hi[k], m));
+    }
+    cnt += __shfl_xor(cnt, m);
+  }
+  if (lane == 0) {
+    if (cnt == 0) {
+      sctr[sc] = make_float4(0, 0, 0, 0);
+      shalf[sc] = make_float4(-1, -1, -1, 0);
+    } else {
+      sctr[sc] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])),
+                             (float)(0.5 * (lo[2] + hi[2])), (float)cnt);
+      shalf[sc] = make_float4((float)(0.5 * (hi[0] - lo[0])) + kScSlack, (float)(0.5 * (hi[1] - lo[1])) + kScSlack,
+                              (float)(0.5 * (hi[2] - lo[2])) + kScSlack, 0.f);
     }
   }
 }
+
+
+// ---------------------------------------------------------------------------
+// 2. The cluster list, from the cluster and supercluster boxes.
 
 __device__ inline float bbox_dist2(float4 ca, float4 ha, float4 cb, float4 hb, const float *L) {
   float d2 = 0.f;
@@ -791,15 +827,39 @@ __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ mas
   }
 }
 
-// Compact tile list: tiles of kTileEntries entries per supercluster.
+// Compact tile list: tiles of up to kTileEntries entries per supercluster.
 __global__ void k_make_tiles(int nsuper, const int *__restrict__ count, int2 *__restrict__ tiles,
-                             int *__restrict__ ntiles) {
-  // single block of 1024 threads; running offset over chunks
+                             int *__restrict__ ntiles, int ntiles_cap, int target) {
+  // single block of 1024 threads.  Tile length: the largest of 16, 8, 4 and
+  // 2 entries that gives at least `target` tiles (one per resident wave of
+  // the pair kernel, so small systems still spread over the device), as long
+  // as the smaller length's tiles fit in ntiles_cap.
   __shared__ int s[1024];
+  __shared__ int s_tot[4][1024 / kWave];
+  int tot[4] = {0, 0, 0, 0};
+  for (int S = threadIdx.x; S < nsuper; S += blockDim.x) {
+    const int c = count[S];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) tot[k] += (c + (kTileEntries >> k) - 1) / (kTileEntries >> k);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int v = wave_sum(tot[k]);
+    if ((threadIdx.x & 63) == 0) s_tot[k][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    tot[k] = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) tot[k] += s_tot[k][w];
+  }
+  int lg = 0;
+  while (lg < 3 && tot[lg] < target && tot[lg + 1] <= ntiles_cap) ++lg;
+  const int T = kTileEntries >> lg;
   int carry = 0;
   for (int base = 0; base < nsuper; base += 1024) {
     int S = base + threadIdx.x;
-    int t = S < nsuper ? (count[S] + kTileEntries - 1) / kTileEntries : 0;
+    int t = S < nsuper ? (count[S] + T - 1) / T : 0;
     s[threadIdx.x] = t;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -809,12 +869,15 @@ __global__ void k_make_tiles(int nsuper, const int *__restrict__ count, int2 *__
       __syncthreads();
     }
     int excl = carry + s[threadIdx.x] - t;
-    for (int k = 0; k < t; ++k) tiles[excl + k] = make_int2(S, k * kTileEntries);
+    for (int k = 0; k < t; ++k) tiles[excl + k] = make_int2(S, k * T);
     __syncthreads();
     carry += s[1023];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *ntiles = carry;
+  if (threadIdx.x == 0) {
+    ntiles[0] = carry;
+    ntiles[1] = T;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1095,7 +1158,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   sc.load(pp.co);
   const R guard = (R)pp.guard;
   const R alpha = (R)pp.alpha, a2pi = (R)pp.a2pi, csf = (R)pp.csf;
-  const int ntiles = *ntiles_p;
+  const int ntiles = ntiles_p[0], tlen = ntiles_p[1];
   // the boundary-pair sums are filled after this kernel (same stream)
   if (blockIdx.x == 0 && threadIdx.x < 4) out.border_acc[threadIdx.x] = 0ull;
   R4 *const si = s_i[wv];
@@ -1104,7 +1167,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
 
   // The first tile of each wave is its global wave index (the grid holds
   // the waves that fit on the device at once); later tiles come from the
-  // shared counter.
+  // shared counter, or round robin when the tiles are short.
   const int nwaves = gridDim.x * W;
   int t = blockIdx.x * W + wv;
   while (true) {
@@ -1150,7 +1213,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
       qhead += cnt;
     };
 
-    const int eend = __builtin_amdgcn_readfirstlane(min(e0 + kTileEntries, count[S]));
+    const int eend = __builtin_amdgcn_readfirstlane(min(e0 + tlen, count[S]));
     for (int eg = e0; eg < eend; eg += kGroup) {
       const int ng = eend - eg < kGroup ? eend - eg : kGroup;
       int cjv[kGroup], emv[kGroup], mov[kGroup], shv[kGroup], tyj0[kGroup];
@@ -1328,9 +1391,15 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
       }
     }
     __builtin_amdgcn_wave_barrier();
-    int tn = 0;
-    if (lane == 0) tn = nwaves + atomicAdd(tile_counter, 1);
-    t = __builtin_amdgcn_readfirstlane(__shfl(tn, 0));
+    if (tlen < kTileEntries) {
+      // short tiles (a small system split finely): round robin, since a
+      // counter hit once per short tile would serialize the waves
+      t += nwaves;
+    } else {
+      int tn = 0;
+      if (lane == 0) tn = nwaves + atomicAdd(tile_counter, 1);
+      t = __builtin_amdgcn_readfirstlane(__shfl(tn, 0));
+    }
   }
 }
 
@@ -2271,11 +2340,16 @@ __global__ void __launch_bounds__(256) k_fixups(FixupArgs<MIXED> a) {
 // v += F h_kick / m.  Langevin leapfrog (LangevinLeapfrogIntegrator.cpp:
 // 44-71): nhalf half kicks v = v vdt + F fdt / m + g sqrt(fc / m) ndt, g a
 // standard normal triple for (atom, global half-kick index).
+// One kick: h_kick = fraction of the step times h; Langevin half kicks
+// numbered half0, half0 + 1 (plus 2 x *step_dev when step_dev is set: a
+// captured chunk reads its first step index from device memory, half0 then
+// being relative to it, modulo 2^64).
 struct Kick {
   double h_kick;
   int langevin, nhalf;
   unsigned long long half0, seed;
   double vdt, fdt, ndt, fc;
+  const unsigned long long *step_dev;
 };
 
 __device__ inline void apply_kick(const Kick &kk, int a, const double f[3], double minv, double v[3]) {
@@ -2285,9 +2359,10 @@ __device__ inline void apply_kick(const Kick &kk, int a, const double f[3], doub
     return;
   }
   const double sfm = sqrt(kk.fc * minv) * kk.ndt;
+  const unsigned long long h0 = kk.half0 + (kk.step_dev ? 2ull * *kk.step_dev : 0ull);
   for (int hh = 0; hh < kk.nhalf; ++hh) {
     double g[3];
-    langevin_gauss3(kk.seed, kk.half0 + hh, (uint32_t)a, g);
+    langevin_gauss3(kk.seed, h0 + hh, (uint32_t)a, g);
 #pragma unroll
     for (int k = 0; k < 3; ++k) v[k] = v[k] * kk.vdt + f[k] * kk.fdt * minv + g[k] * sfm;
   }
@@ -2332,6 +2407,8 @@ __global__ void k_kick_drift(int n, Kick kk, double h, const int *__restrict__ p
   for (int m = 32; m >= 1; m >>= 1) d2f = fmaxf(d2f, __shfl_xor(d2f, m));
   if ((threadIdx.x & 63) == 0) atomicMax(dmax, __float_as_uint(d2f));
 }
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
 __global__ void k_half_kick(int n, Kick kk, const double *__restrict__ fout, const double *__restrict__ minv,
                             double *__restrict__ v) {

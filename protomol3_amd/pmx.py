@@ -51,7 +51,8 @@ class MdDesc(C.Structure):
                 ("bond_par", C.POINTER(C.c_double)), ("angles", C.POINTER(C.c_int)),
                 ("angle_par", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
                 ("timestep_fs", C.c_double), ("skin", C.c_double), ("integrator", C.c_int),
-                ("temperature", C.c_double), ("gamma", C.c_double), ("seed", C.c_ulonglong)]
+                ("temperature", C.c_double), ("gamma", C.c_double), ("seed", C.c_ulonglong),
+                ("graphs", C.c_int)]
 
 
 LEAPFROG, LANGEVIN = 0, 1
@@ -62,7 +63,8 @@ EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_crea
             "pmx_sys_get_forces", "pmx_sys_sync", "pmx_sys_alpha", "pmx_sys_pair_stats", "pmx_pair_hash",
             "pmx_sys_cells", "pmx_sys_stage_times", "pmx_sys_timing_begin", "pmx_sys_timing_end",
             "pmx_md_init", "pmx_md_set_state", "pmx_md_run", "pmx_md_get_state", "pmx_md_stats",
-            "pmx_langevin_noise", "pmx_sys_device_bytes", "pmx_estimate_bytes", "pmx_md_run_many"]
+            "pmx_langevin_noise", "pmx_sys_device_bytes", "pmx_estimate_bytes", "pmx_md_run_many",
+            "pmx_sys_time_pair"]
 
 _lib = None
 
@@ -101,6 +103,7 @@ def load() -> C.CDLL:
     L.pmx_md_set_state.argtypes = [vp, dp, dp]
     L.pmx_md_run.argtypes = [vp, C.c_int]
     L.pmx_md_run_many.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
+    L.pmx_sys_time_pair.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
     L.pmx_md_get_state.argtypes = [vp, dp, dp, dp]
     L.pmx_md_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.pmx_langevin_noise.argtypes = [C.c_ulonglong, C.c_ulonglong, C.c_int, C.POINTER(C.c_double)]
@@ -215,7 +218,7 @@ def make_desc(s: MolSystem, spec: ForceSpec, precision: int = FP64):
 
 
 def make_md_desc(s: MolSystem, timestep_fs: float, skin: float = 1.0, integrator: int = LEAPFROG,
-                 temperature: float = 300.0, gamma: float = 1.0, seed: int = 1234):
+                 temperature: float = 300.0, gamma: float = 1.0, seed: int = 1234, graphs: bool = False):
     """(pmx_md_desc, arrays it points into)."""
     keep = [np.ascontiguousarray(s.bonds, np.int32).reshape(-1, 2),
             np.ascontiguousarray(s.bond_par, np.float64).reshape(-1, 2),
@@ -236,6 +239,7 @@ def make_md_desc(s: MolSystem, timestep_fs: float, skin: float = 1.0, integrator
     d.temperature = temperature
     d.gamma = gamma
     d.seed = seed
+    d.graphs = 1 if graphs else 0
     return d, keep
 
 
@@ -322,10 +326,11 @@ class System:
 
     # -- device-resident MD -------------------------------------------------
     def md_init(self, s: MolSystem, timestep_fs: float, skin: float = 1.0, integrator: int = LEAPFROG,
-                temperature: float = 300.0, gamma: float = 1.0, seed: int = 1234):
+                temperature: float = 300.0, gamma: float = 1.0, seed: int = 1234, graphs: bool = False):
         """Device MD set-up: leapfrog, or Langevin leapfrog at `temperature` (K)
-        with friction `gamma` (1/ps) and noise stream `seed`."""
-        d, self._md_keep = make_md_desc(s, timestep_fs, skin, integrator, temperature, gamma, seed)
+        with friction `gamma` (1/ps) and noise stream `seed`; graphs: replay
+        each chunk of steps as a captured HIP graph."""
+        d, self._md_keep = make_md_desc(s, timestep_fs, skin, integrator, temperature, gamma, seed, graphs)
         self.ctx.check(self.ctx.lib.pmx_md_init(self.h, C.byref(d)), "pmx_md_init")
 
     def md_set_state(self, x: np.ndarray, v: np.ndarray):
@@ -335,6 +340,12 @@ class System:
 
     def md_run(self, nsteps: int):
         self.ctx.check(self.ctx.lib.pmx_md_run(self.h, nsteps), "pmx_md_run")
+
+    def time_pair(self, reps: int = 100) -> float:
+        """Average device ms of one pair-kernel launch at the last positions."""
+        ms = C.c_float()
+        self.ctx.check(self.ctx.lib.pmx_sys_time_pair(self.h, reps, C.byref(ms)), "pmx_sys_time_pair")
+        return float(ms.value)
 
     def md_get_state(self):
         """(x, v, energies dict) after the last run."""

@@ -13,7 +13,7 @@ from . import pmx
 
 
 def tune_skin(ctx, s, spec: pmx.ForceSpec, precision: int, x, v, timestep_fs: float = 1.0,
-              candidates=(0.8, 1.2, 1.6, 2.0), steps: int = 200) -> tuple[float, dict]:
+              candidates=(0.5, 0.7, 0.9, 1.2), steps: int = 200) -> tuple[float, dict]:
     """Returns (best skin, {skin: ms per step})."""
     times = {}
     for skin in candidates:

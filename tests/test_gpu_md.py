@@ -109,3 +109,39 @@ def test_gpu_md_state_energies_before_any_step(ctx):
         assert abs(e[k] - eref[k]) <= 1e-9 * max(1.0, abs(eref[k])), k
         assert abs(e2[k] - eref2[k]) <= 1e-9 * max(1.0, abs(eref2[k])), k
     assert e["kinetic"] > 0
+
+
+@pytest.mark.parametrize("precision", [pmx.MIXED, pmx.FP64])
+def test_gpu_md_graph_replay_matches_kernel_launches(ctx, precision):
+    """Chunks replayed from captured HIP graphs give the trajectory of
+    kernel-by-kernel launches: several runs with host work between them, each
+    of several chunks (the same graph replayed with other step indices, which
+    the Langevin noise reads from device memory), and a small skin that
+    forces rolled-back chunks (replayed without graphs)."""
+    s = systems.water_box(seed=31).with_velocities(300.0, seed=8)
+    spec = pmx.ForceSpec(5.0, lj=(pmx.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 32, 32), order=4, alpha=0.32))
+    out = {}
+    for graphs in (False, True):
+        sy = pmx.System(ctx, s, spec, precision)
+        try:
+            sy.md_init(s, 1.0, skin=0.3, integrator=pmx.LANGEVIN, temperature=300.0, gamma=5.0, seed=77,
+                       graphs=graphs)
+            sy.md_set_state(s.pos, s.vel)
+            states = []
+            for _ in range(4):
+                sy.md_run(45)
+                states.append(sy.md_get_state())
+            out[graphs] = (states, sy.md_stats())
+        finally:
+            sy.close()
+    (eager, st0), (graph, st1) = out[False], out[True]
+    print(f"prec={precision} stats eager {st0} graphs {st1}", flush=True)
+    if precision == pmx.MIXED:  # FP64 atomics may round differently and move a rollback
+        assert st0 == st1
+    assert st1["rollbacks"] > 0 and st1["steps"] == 180
+    tol = 0.0 if precision == pmx.MIXED else 1e-10
+    for (xa, va, ea), (xb, vb, eb) in zip(eager, graph):
+        assert np.abs(xa - xb).max() <= tol * max(1.0, np.abs(xa).max())
+        assert np.abs(va - vb).max() <= tol * max(1.0, np.abs(va).max())
+        for k in ea:
+            assert abs(ea[k] - eb[k]) <= 1e-9 * max(1.0, abs(ea[k])), k
