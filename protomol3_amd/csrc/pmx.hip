@@ -77,7 +77,7 @@ struct Sys {
   std::vector<DevBuf *> all;
   DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
       perm2, sc_info, sc_col, colsc, sctr, shalf, sorg, ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
-      pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec,
+      pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec, spec2,
       greal, theta, twx, twy, twz, border, nborder, border_acc;
   int maxborder = 65536;
   // results
@@ -479,7 +479,7 @@ int setup(Sys *s) {
     if (s->pme_on) {
       const size_t nx = d.pme_grid[0], ny = d.pme_grid[1], nz = d.pme_grid[2], nzh = nz / 2 + 1;
       const size_t G = nx * ny * nz, GH = nx * ny * nzh, cxb = s->mixed ? 8 : 16, rb = s->mixed ? 4 : 8;
-      s->dry_bytes += 8 * G + cxb * GH + rb * G + rb * GH + cxb * (nx + ny + nz) + 8 * ny * nzh + 8 * 5;
+      s->dry_bytes += 8 * G + 2 * cxb * GH + rb * G + rb * GH + cxb * (nx + ny + nz) + 8 * ny * nzh + 8 * 5;
     }
     return PMX_OK;
   }
@@ -516,7 +516,7 @@ int setup(Sys *s) {
     const size_t G = (size_t)pm.nx * pm.ny * pm.nz, GH = (size_t)pm.nx * pm.ny * pm.nzh;
     const size_t cxb = s->mixed ? 8 : 16;
     if ((rc = A(s->grid64, s->mixed ? 8 : 8 * G)) || (rc = A(s->gridfx, s->mixed ? 8 * G : 8)) ||
-        (rc = A(s->spec, cxb * GH)) || (rc = A(s->greal, (s->mixed ? 4 : 8) * G)) ||
+        (rc = A(s->spec, cxb * GH)) || (rc = A(s->spec2, cxb * GH)) || (rc = A(s->greal, (s->mixed ? 4 : 8) * G)) ||
         (rc = A(s->theta, (s->mixed ? 4 : 8) * GH)) || (rc = A(s->twx, cxb * pm.nx)) ||
         (rc = A(s->twy, cxb * pm.ny)) || (rc = A(s->twz, cxb * pm.nz)))
       return rc;
@@ -568,7 +568,7 @@ int setup(Sys *s) {
             double mk = k * Hr, m2 = mij + mk * mk;
             v = bxy * bzm[k] * exy * ez[k] / m2;
           }
-          th[((size_t)i * pm.ny + j) * pm.nzh + k] = v;
+          th[((size_t)k * pm.ny + j) * pm.nx + i] = v;  // the x pass's layout [k][j][i]
         }
       }
     }
@@ -592,15 +592,16 @@ int setup(Sys *s) {
     } else {
       HIPCHK(hipMemcpy(s->theta.p, th.data(), 8 * GH, hipMemcpyHostToDevice));
     }
-    // FFT blocking: two LDS buffers of lines_per_block * n complex <= 64 KiB
+    // FFT blocking: lines per block, two LDS buffers of lines * n complex
+    // within 24 KiB (so that FFT blocks fit beside the pair kernel's blocks,
+    // which leave about that much LDS free on each CU), at most 16 lines
+    // (the transposed segments)
     const int cpx = s->mixed ? 8 : 16;
-    const int budget = 61440 / (2 * cpx);  // 60 KiB of the 64 KiB workgroup LDS
-    // Many small blocks: ~128 butterflies per stage per block, >= 4 blocks per CU.
-    s->zl = std::max(1, std::min(std::min(pm.nx * pm.ny, budget / pm.nz), 512 / pm.nz));
-    const int tk = s->mixed ? 8 : 4;  // 64-byte row segments
-    s->tky = std::max(1, std::min(std::min(pm.nzh, budget / pm.ny), tk));
-    s->tkx = std::max(1, std::min(std::min(pm.nzh, budget / pm.nx), tk));
-    s->recip_blocks = pm.ny * ((pm.nzh + s->tkx - 1) / s->tkx);
+    const int budget = 24576 / (2 * cpx);
+    s->zl = std::max(1, std::min(16, budget / pm.nz));
+    s->tky = std::max(1, std::min(16, budget / pm.ny));
+    s->tkx = std::max(1, std::min(16, budget / pm.nx));
+    s->recip_blocks = pm.nzh * ((pm.ny + s->tkx - 1) / s->tkx);
     if ((rc = A(s->recip_e, 8 * (size_t)s->recip_blocks))) return rc;
   }
   HIPCHK(hipStreamSynchronize(st));
@@ -751,17 +752,19 @@ int run_pme(Sys *s, bool timing) {
   const PmeParams &pm = s->pme;
   if (s->d.pme_terms & PMX_PME_RECIP) {
     const size_t cx = sizeof(Cx<T>);
-    const int zb = (pm.nx * pm.ny + s->zl - 1) / s->zl;
-    hipLaunchKernelGGL((k_fft_z_r2c<T, MIXED>), dim3(zb), dim3(128), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
+    // z pass per (i, TY lines of j), y passes per (k, TI lines of i), x pass
+    // per (k, TJ lines of j)
+    const int zb = pm.nx * ((pm.ny + s->zl - 1) / s->zl);
+    const int yb = pm.nzh * ((pm.nx + s->tky - 1) / s->tky);
+    hipLaunchKernelGGL((k_fft_z_r2c<T, MIXED>), dim3(zb), dim3(256), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
                        P<double>(s->grid64), P<unsigned long long>(s->gridfx), P<Cx<T>>(s->twz), P<Cx<T>>(s->spec));
-    const int yb = pm.nx * ((pm.nzh + s->tky - 1) / s->tky);
-    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(128), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, false,
-                       P<Cx<T>>(s->twy), P<Cx<T>>(s->spec));
-    hipLaunchKernelGGL((k_fft_x_conv<T>), dim3(s->recip_blocks), dim3(128), 2 * s->tkx * pm.nx * cx, st, pm, s->plx,
-                       s->tkx, P<Cx<T>>(s->twx), P<T>(s->theta), P<Cx<T>>(s->spec), P<double>(s->recip_e));
-    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(128), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, true,
-                       P<Cx<T>>(s->twy), P<Cx<T>>(s->spec));
-    hipLaunchKernelGGL((k_fft_z_c2r<T>), dim3(zb), dim3(128), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
+    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(256), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, false,
+                       P<Cx<T>>(s->twy), P<Cx<T>>(s->spec), P<Cx<T>>(s->spec2));
+    hipLaunchKernelGGL((k_fft_x_conv<T>), dim3(s->recip_blocks), dim3(256), 2 * s->tkx * pm.nx * cx, st, pm, s->plx,
+                       s->tkx, P<Cx<T>>(s->twx), P<T>(s->theta), P<Cx<T>>(s->spec2), P<double>(s->recip_e));
+    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(256), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, true,
+                       P<Cx<T>>(s->twy), P<Cx<T>>(s->spec2), P<Cx<T>>(s->spec));
+    hipLaunchKernelGGL((k_fft_z_c2r<T>), dim3(zb), dim3(256), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
                        P<Cx<T>>(s->twz), P<Cx<T>>(s->spec), P<T>(s->greal));
     if (timing) hipEventRecord(s->ev[5], st);
     switch (pm.order) {

@@ -1845,107 +1845,120 @@ __device__ Cx<T> *lds_fft(Cx<T> *a, Cx<T> *b, int nb, const FftPlan &pl, const C
   return a;
 }
 
-// z pass, real -> half complex: lines (i, j) contiguous along z.
+// The 3-D FFT passes move the data between three layouts so that every pass
+// reads and writes whole lines: the charge grid and the real potential are
+// [i][j][k] (k fastest); the half spectrum is SA = [i][k][j] after the z pass
+// and SB = [k][j][i] after the y pass.  A block takes a few consecutive lines
+// (TY, TI or TJ), so the transposing reads and writes move segments of that
+// many consecutive elements.
+
+// z pass, real -> half complex, for the TY lines (i, j0 .. j0 + TY - 1):
+// the grid points are read once and cleared for the next spread; out: SA.
 template <class T, bool FIXED>
-__global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int lines_per_block, double *__restrict__ g64,
-                            unsigned long long *__restrict__ gfx, const Cx<T> *__restrict__ tw,
-                            Cx<T> *__restrict__ spec) {
+__global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int TY, double *__restrict__ g64,
+                            unsigned long long *__restrict__ gfx, const Cx<T> *__restrict__ tw, Cx<T> *__restrict__ sa) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
-  Cx<T> *b = a + lines_per_block * p.nz;
-  const int nlines = p.nx * p.ny;
-  const int l0 = blockIdx.x * lines_per_block;
-  const int nb = min(lines_per_block, nlines - l0);
+  Cx<T> *b = a + TY * p.nz;
+  const int jt = (p.ny + TY - 1) / TY;
+  const int i = blockIdx.x / jt, j0 = (blockIdx.x - i * jt) * TY;
+  const int nb = min(TY, p.ny - j0);
+  const size_t base = ((size_t)i * p.ny + j0) * p.nz;
   for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) {
-    const size_t gi = (size_t)l0 * p.nz + t;
     T v;
-    // each grid point is read once here and cleared for the next spread
     if (FIXED) {
-      v = (T)((double)(long long)gfx[gi] * (1.0 / kGridScale));
-      gfx[gi] = 0ull;
+      v = (T)((double)(long long)gfx[base + t] * (1.0 / kGridScale));
+      gfx[base + t] = 0ull;
     } else {
-      v = (T)g64[gi];
-      g64[gi] = 0.0;
+      v = (T)g64[base + t];
+      g64[base + t] = 0.0;
     }
     a[t] = Cx<T>{v, (T)0};
   }
   __syncthreads();
-  Cx<T> *r = lds_fft<T>(a, b, nb, pl, tw);
+  const Cx<T> *r = lds_fft<T>(a, b, nb, pl, tw);
   for (int t = threadIdx.x; t < nb * p.nzh; t += blockDim.x) {
-    const int line = t / p.nzh, k = t - line * p.nzh;
-    spec[(size_t)(l0 + line) * p.nzh + k] = r[line * p.nz + k];
+    const int k = t / nb, jj = t - k * nb;
+    sa[((size_t)i * p.nzh + k) * p.ny + j0 + jj] = r[jj * p.nz + k];
   }
 }
 
-// y pass (stride nzh) for a tile of TK consecutive k at fixed i.
-// inverse: conj-in / conj-out around the forward transform.
+// y pass for the TI lines (i0 .. i0 + TI - 1, k): forward SA -> SB; inverse
+// (conj-in / conj-out around the forward transform) SB -> SA.
 template <class T>
-__global__ void k_fft_y(PmeParams p, FftPlan pl, int TK, bool inverse, const Cx<T> *__restrict__ tw,
-                        Cx<T> *__restrict__ spec) {
+__global__ void k_fft_y(PmeParams p, FftPlan pl, int TI, bool inverse, const Cx<T> *__restrict__ tw,
+                        const Cx<T> *__restrict__ src, Cx<T> *__restrict__ dst) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
-  Cx<T> *b = a + TK * p.ny;
-  const int ktiles = (p.nzh + TK - 1) / TK;
-  const int i = blockIdx.x / ktiles;
-  const int k0 = (blockIdx.x % ktiles) * TK;
-  const int nk = min(TK, p.nzh - k0);
-  for (int t = threadIdx.x; t < nk * p.ny; t += blockDim.x) {
-    const int j = t / nk, kk = t - j * nk;
-    Cx<T> v = spec[((size_t)i * p.ny + j) * p.nzh + k0 + kk];
-    if (inverse) v.y = -v.y;
-    a[kk * p.ny + j] = v;
+  Cx<T> *b = a + TI * p.ny;
+  const int it = (p.nx + TI - 1) / TI;
+  const int k = blockIdx.x / it, i0 = (blockIdx.x - k * it) * TI;
+  const int nb = min(TI, p.nx - i0);
+  if (!inverse) {
+    for (int t = threadIdx.x; t < nb * p.ny; t += blockDim.x) {
+      const int ii = t / p.ny, j = t - ii * p.ny;
+      a[t] = src[((size_t)(i0 + ii) * p.nzh + k) * p.ny + j];
+    }
+  } else {
+    for (int t = threadIdx.x; t < nb * p.ny; t += blockDim.x) {
+      const int j = t / nb, ii = t - j * nb;
+      Cx<T> v = src[((size_t)k * p.ny + j) * p.nx + i0 + ii];
+      v.y = -v.y;
+      a[ii * p.ny + j] = v;
+    }
   }
   __syncthreads();
-  Cx<T> *r = lds_fft<T>(a, b, nk, pl, tw);
-  for (int t = threadIdx.x; t < nk * p.ny; t += blockDim.x) {
-    const int j = t / nk, kk = t - j * nk;
-    Cx<T> v = r[kk * p.ny + j];
-    if (inverse) v.y = -v.y;
-    spec[((size_t)i * p.ny + j) * p.nzh + k0 + kk] = v;
+  const Cx<T> *r = lds_fft<T>(a, b, nb, pl, tw);
+  if (!inverse) {
+    for (int t = threadIdx.x; t < nb * p.ny; t += blockDim.x) {
+      const int j = t / nb, ii = t - j * nb;
+      dst[((size_t)k * p.ny + j) * p.nx + i0 + ii] = r[ii * p.ny + j];
+    }
+  } else {
+    for (int t = threadIdx.x; t < nb * p.ny; t += blockDim.x) {
+      const int ii = t / p.ny, j = t - ii * p.ny;
+      Cx<T> v = r[t];
+      v.y = -v.y;
+      dst[((size_t)(i0 + ii) * p.nzh + k) * p.ny + j] = v;
+    }
   }
 }
 
-// x pass forward, influence function multiply with energy (Grid.h:237-341),
-// x pass inverse, for a tile of TK consecutive k at fixed j.
+// x pass for the TJ lines (k, j0 .. j0 + TJ - 1) of SB, in place: forward
+// FFT, influence function multiply with the energy (Grid.h:237-341; theta in
+// the SB layout), inverse FFT.
 template <class T>
-__global__ void k_fft_x_conv(PmeParams p, FftPlan pl, int TK, const Cx<T> *__restrict__ tw,
-                             const T *__restrict__ theta, Cx<T> *__restrict__ spec, double *__restrict__ epart) {
+__global__ void k_fft_x_conv(PmeParams p, FftPlan pl, int TJ, const Cx<T> *__restrict__ tw,
+                             const T *__restrict__ theta, Cx<T> *__restrict__ sb, double *__restrict__ epart) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
-  Cx<T> *b = a + TK * p.nx;
-  __shared__ double s_e[kPairBlock / kWave];
-  const int ktiles = (p.nzh + TK - 1) / TK;
-  const int j = blockIdx.x / ktiles;
-  const int k0 = (blockIdx.x % ktiles) * TK;
-  const int nk = min(TK, p.nzh - k0);
-  const size_t stride = (size_t)p.ny * p.nzh;
-  for (int t = threadIdx.x; t < nk * p.nx; t += blockDim.x) {
-    const int i = t / nk, kk = t - i * nk;
-    a[kk * p.nx + i] = spec[(size_t)i * stride + (size_t)j * p.nzh + k0 + kk];
-  }
+  Cx<T> *b = a + TJ * p.nx;
+  __shared__ double s_e[1024 / kWave];
+  const int jt = (p.ny + TJ - 1) / TJ;
+  const int k = blockIdx.x / jt, j0 = (blockIdx.x - k * jt) * TJ;
+  const int nb = min(TJ, p.ny - j0);
+  const size_t base = ((size_t)k * p.ny + j0) * p.nx;
+  for (int t = threadIdx.x; t < nb * p.nx; t += blockDim.x) a[t] = sb[base + t];
   __syncthreads();
-  Cx<T> *r = lds_fft<T>(a, b, nk, pl, tw);
+  Cx<T> *r = lds_fft<T>(a, b, nb, pl, tw);
+  const double wgt = (k > 0 && ((k != p.nz / 2) || (p.nz & 1))) ? 2.0 : 1.0;
   double e = 0.0;
-  for (int t = threadIdx.x; t < nk * p.nx; t += blockDim.x) {
-    const int kk = t / p.nx, i = t - kk * p.nx;
-    const int k = k0 + kk;
-    const T th = theta[((size_t)i * p.ny + j) * p.nzh + k];
-    Cx<T> v = r[kk * p.nx + i];
-    const double wgt = (k > 0 && ((k != p.nz / 2) || (p.nz & 1))) ? 2.0 : 1.0;
-    e += wgt * (double)th * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+  for (int t = threadIdx.x; t < nb * p.nx; t += blockDim.x) {
+    const T th = theta[base + t];
+    const Cx<T> v = r[t];
+    e += (double)th * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
     // conj for the inverse transform (conj-in / conj-out)
-    r[kk * p.nx + i] = Cx<T>{v.x * th, -(v.y * th)};
+    r[t] = Cx<T>{v.x * th, -(v.y * th)};
   }
   __syncthreads();
   Cx<T> *o = (r == a) ? b : a;
-  Cx<T> *r2 = lds_fft<T>(r, o, nk, pl, tw);
-  for (int t = threadIdx.x; t < nk * p.nx; t += blockDim.x) {
-    const int i = t / nk, kk = t - i * nk;
-    Cx<T> v = r2[kk * p.nx + i];
+  const Cx<T> *r2 = lds_fft<T>(r, o, nb, pl, tw);
+  for (int t = threadIdx.x; t < nb * p.nx; t += blockDim.x) {
+    Cx<T> v = r2[t];
     v.y = -v.y;
-    spec[(size_t)i * stride + (size_t)j * p.nzh + k0 + kk] = v;
+    sb[base + t] = v;
   }
-  e = wave_sum(e);
+  e = wave_sum(wgt * e);
   if ((threadIdx.x & 63) == 0) s_e[threadIdx.x >> 6] = e;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1955,30 +1968,32 @@ __global__ void k_fft_x_conv(PmeParams p, FftPlan pl, int TK, const Cx<T> *__res
   }
 }
 
-// z pass, half complex -> real (Hermitian completion along z), inverse sign.
+// z pass, half complex -> real for the TY lines (i, j0 .. j0 + TY - 1) of SA
+// (Hermitian completion along z, inverse sign); out: the real potential.
 template <class T>
-__global__ void k_fft_z_c2r(PmeParams p, FftPlan pl, int lines_per_block, const Cx<T> *__restrict__ tw,
-                            const Cx<T> *__restrict__ spec, T *__restrict__ greal) {
+__global__ void k_fft_z_c2r(PmeParams p, FftPlan pl, int TY, const Cx<T> *__restrict__ tw,
+                            const Cx<T> *__restrict__ sa, T *__restrict__ greal) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
-  Cx<T> *b = a + lines_per_block * p.nz;
-  const int nlines = p.nx * p.ny;
-  const int l0 = blockIdx.x * lines_per_block;
-  const int nb = min(lines_per_block, nlines - l0);
+  Cx<T> *b = a + TY * p.nz;
+  const int jt = (p.ny + TY - 1) / TY;
+  const int i = blockIdx.x / jt, j0 = (blockIdx.x - i * jt) * TY;
+  const int nb = min(TY, p.ny - j0);
   for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) {
-    const int line = t / p.nz, k = t - line * p.nz;
+    const int k = t / nb, jj = t - k * nb;
     Cx<T> v;
     if (k < p.nzh) {
-      v = spec[(size_t)(l0 + line) * p.nzh + k];
+      v = sa[((size_t)i * p.nzh + k) * p.ny + j0 + jj];
       v.y = -v.y;  // conj-in
     } else {
-      v = spec[(size_t)(l0 + line) * p.nzh + (p.nz - k)];  // conj(conj(X[nz-k]))
+      v = sa[((size_t)i * p.nzh + (p.nz - k)) * p.ny + j0 + jj];  // conj(conj(X[nz-k]))
     }
-    a[t] = v;
+    a[jj * p.nz + k] = v;
   }
   __syncthreads();
-  Cx<T> *r = lds_fft<T>(a, b, nb, pl, tw);
-  for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) greal[(size_t)l0 * p.nz + t] = r[t].x;
+  const Cx<T> *r = lds_fft<T>(a, b, nb, pl, tw);
+  const size_t base = ((size_t)i * p.ny + j0) * p.nz;
+  for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) greal[base + t] = r[t].x;
 }
 
 // Force interpolation (Grid.h:159-189) into the sorted accumulators.  Lane
