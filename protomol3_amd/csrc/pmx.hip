@@ -487,8 +487,11 @@ int setup(Sys *s) {
   hipStream_t st = s->ctx->stream;
   // the cell histogram starts cleared; each rebuild's scan clears it again
   hipLaunchKernelGGL(k_fill_u32, dim3(nblk(s->nbins, 256)), dim3(256), 0, st, P<uint32_t>(s->hist), 0u, s->nbins);
-  if (!s->fork_ev) HIPCHK(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
-  if (!s->join_ev) HIPCHK(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
+  {
+    RtLock lk(g_rt_mu);
+    if (!s->fork_ev) HIPCHK(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
+    if (!s->join_ev) HIPCHK(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
+  }
   HIPCHK(hipMemcpyAsync(s->q_d.p, s->q.data(), 8 * N, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(s->type_d.p, s->type.data(), 4 * N, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(s->ljtab.p, s->lj.data(), 8 * s->lj.size(), hipMemcpyHostToDevice, st));
@@ -572,25 +575,31 @@ int setup(Sys *s) {
         }
       }
     }
-    auto tw = [&](int nn, DevBuf &b) -> int {
-      std::vector<double> t(2 * nn);
+    // uploads on the system's stream; the host arrays live until the
+    // synchronisation at the end of the set-up
+    std::vector<double> tw64[3];
+    std::vector<float> tw32[3], thf;
+    auto tw = [&](int axis, int nn, DevBuf &b) -> hipError_t {
+      std::vector<double> &t = tw64[axis];
+      t.resize(2 * nn);
       for (int m = 0; m < nn; ++m) {
         t[2 * m] = std::cos(-2.0 * M_PI * m / nn);
         t[2 * m + 1] = std::sin(-2.0 * M_PI * m / nn);
       }
       if (s->mixed) {
-        std::vector<float> tf(t.begin(), t.end());
-        return hipMemcpy(b.p, tf.data(), 4 * tf.size(), hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+        tw32[axis].assign(t.begin(), t.end());
+        return hipMemcpyAsync(b.p, tw32[axis].data(), 4 * tw32[axis].size(), hipMemcpyHostToDevice, st);
       }
-      return hipMemcpy(b.p, t.data(), 8 * t.size(), hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+      return hipMemcpyAsync(b.p, t.data(), 8 * t.size(), hipMemcpyHostToDevice, st);
     };
-    if (tw(pm.nx, s->twx) || tw(pm.ny, s->twy) || tw(pm.nz, s->twz))
-      return fail(s->ctx, PMX_ERR_HIP, "twiddle upload failed");
+    HIPCHK(tw(0, pm.nx, s->twx));
+    HIPCHK(tw(1, pm.ny, s->twy));
+    HIPCHK(tw(2, pm.nz, s->twz));
     if (s->mixed) {
-      std::vector<float> thf(th.begin(), th.end());
-      HIPCHK(hipMemcpy(s->theta.p, thf.data(), 4 * GH, hipMemcpyHostToDevice));
+      thf.assign(th.begin(), th.end());
+      HIPCHK(hipMemcpyAsync(s->theta.p, thf.data(), 4 * GH, hipMemcpyHostToDevice, st));
     } else {
-      HIPCHK(hipMemcpy(s->theta.p, th.data(), 8 * GH, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpyAsync(s->theta.p, th.data(), 8 * GH, hipMemcpyHostToDevice, st));
     }
     // FFT blocking: lines per block, two LDS buffers of lines * n complex
     // within 24 KiB (so that FFT blocks fit beside the pair kernel's blocks,
@@ -603,6 +612,7 @@ int setup(Sys *s) {
     s->tkx = std::max(1, std::min(16, budget / pm.nx));
     s->recip_blocks = pm.nzh * ((pm.ny + s->tkx - 1) / s->tkx);
     if ((rc = A(s->recip_e, 8 * (size_t)s->recip_blocks))) return rc;
+    HIPCHK(hipStreamSynchronize(st));  // the host tables above are still alive
   }
   HIPCHK(hipStreamSynchronize(st));
   s->cached = true;
@@ -808,7 +818,7 @@ void timing_begin_call(Sys *s, bool timing) {
 // ForceGroup.cpp:57; the MD path rebuilds when the skin requires it).
 int rebuild(Sys *s, bool timing) {
   hipStream_t st = s->ctx->stream;
-  const int n = s->n, npad = s->npad;
+  const int n = s->n;
   // the histogram starts cleared (set-up, then each scan clears it)
   hipLaunchKernelGGL(k_bin, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<double>(s->xorig), s->bin, P<uint32_t>(s->key),
                      P<double>(s->xwrap), P<int>(s->refcell), P<uint32_t>(s->hist));
@@ -1057,7 +1067,6 @@ int pmx_ctx_destroy(pmx_ctx *ctx) {
 const char *pmx_last_error(const pmx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int pmx_sys_create(pmx_ctx *ctx, const pmx_sys_desc *desc, pmx_sys **out) {
-  RtLock lk(g_rt_mu);
   if (!ctx || !out) return PMX_ERR_ARG;
   *out = nullptr;
   int rc = validate(ctx, desc);
@@ -1076,12 +1085,12 @@ int pmx_sys_create(pmx_ctx *ctx, const pmx_sys_desc *desc, pmx_sys **out) {
 }
 
 int pmx_sys_destroy(pmx_sys *sys) {
-  RtLock lk(g_rt_mu);
   if (!sys) return PMX_ERR_ARG;
   Sys *s = &sys->s;
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   hipStreamSynchronize(s->ctx->stream2);
+  RtLock lk(g_rt_mu);
   drop_graphs(s);
   for (DevBuf *b : s->all)
     if (b->p) hipFree(b->p);
@@ -1262,7 +1271,6 @@ int pmx_sys_stage_times(const pmx_sys *sys, float *ms) {
 // Device-resident MD
 
 int pmx_md_init(pmx_sys *sys, const pmx_md_desc *d) {
-  RtLock lk(g_rt_mu);
   if (!sys || !d) return PMX_ERR_ARG;
   Sys *s = &sys->s;
   hipSetDevice(s->ctx->device);
@@ -1283,7 +1291,10 @@ int pmx_md_init(pmx_sys *sys, const pmx_md_desc *d) {
     if (rcmax + d->skin > 0.5 * s->box.L[k])
       return fail(s->ctx, PMX_ERR_UNSUPPORTED, "cutoff + skin larger than half the box");
   Sys::Md &m = s->md;
-  drop_graphs(s);
+  {
+    RtLock lk(g_rt_mu);
+    drop_graphs(s);
+  }
   m.on = true;
   m.graphs = d->graphs != 0;
   m.nb = d->nbonds;
@@ -1330,6 +1341,7 @@ int pmx_md_init(pmx_sys *sys, const pmx_md_desc *d) {
       (rc = dalloc(s, m.ke_e, 8 * (size_t)nblk(N, 256))))
     return rc;
   if (!m.host) {
+    RtLock lk(g_rt_mu);
     HIPCHK(hipHostMalloc((void **)&m.host, 4 * 2 * (kMaxChunk + 1), hipHostMallocDefault));
     for (auto &e : m.chunk_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
@@ -1423,12 +1435,12 @@ int md_step(Sys *s, double kick, long long step, bool rebuild_here, int dslot,
   if (s->mixed)
     hipLaunchKernelGGL(k_kick_drift<float>, dim3(nblk(s->npad, 256)), dim3(256), 0, st, s->npad, kk, m.h,
                        P<int>(s->permp), P<double>(s->fout), P<double>(m.minv), P<double>(m.v), P<double>(s->xorig),
-                       P<double>(m.xbuild), s->box, P<float4>(s->xs), P<double4>(s->xs64),
+                       P<double>(m.xbuild), P<float4>(s->xs), P<double4>(s->xs64),
                        P<float4>(s->sorg), P<unsigned>(m.dmax) + dslot);
   else
     hipLaunchKernelGGL(k_kick_drift<double>, dim3(nblk(s->npad, 256)), dim3(256), 0, st, s->npad, kk, m.h,
                        P<int>(s->permp), P<double>(s->fout), P<double>(m.minv), P<double>(m.v), P<double>(s->xorig),
-                       P<double>(m.xbuild), s->box, P<double4>(s->xs), P<double4>(s->xs64),
+                       P<double>(m.xbuild), P<double4>(s->xs), P<double4>(s->xs64),
                        P<float4>(s->sorg), P<unsigned>(m.dmax) + dslot);
   if (rebuild_here) {
     int rc = rebuild(s, timing);

@@ -90,7 +90,7 @@ __device__ inline T wave_sum(T v) {
 }
 
 // ---------------------------------------------------------------------------
-// 1. Binning: wrapped coordinates, fine-grid Morton key, reference cell.
+// 1. Binning: wrapped coordinates, column x z-bin sort key, reference cell.
 struct BinParams {
   Box box;
   int ncx, ncy, nzb;  // column grid in x-y, z bins per column (sort key)
@@ -891,9 +891,9 @@ __global__ void k_make_tiles(int nsuper, const int *__restrict__ count, int2 *__
 // are summed over the 8 i-lanes with DPP and added once per entry.
 //
 // Membership is decided exactly: a pair whose FP32/FP64 r^2 lies within
-// `guard` of a cutoff^2 is not evaluated here but recorded, and k_border
-// re-tests it in FP64 with the reference's operation order on the original
-// coordinates before evaluating it.
+// `guard` of a cutoff^2 is not evaluated here but recorded, and the
+// boundary-pair pass of k_fixups re-tests it in FP64 with the reference's
+// operation order on the original coordinates before evaluating it.
 
 __device__ inline float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ inline double fast_rsqrt(double x) { return 1.0 / sqrt(x); }
@@ -2391,7 +2391,7 @@ __device__ inline void apply_kick(const Kick &kk, int a, const double f[3], doub
 template <class R>
 __global__ void k_kick_drift(int n, Kick kk, double h, const int *__restrict__ perm,
                              const double *__restrict__ fout, const double *__restrict__ minv, double *__restrict__ v,
-                             double *__restrict__ x, const double *__restrict__ xbuild, Box box,
+                             double *__restrict__ x, const double *__restrict__ xbuild,
                              typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64,
                              const float4 *__restrict__ sorg, unsigned int *__restrict__ dmax) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;

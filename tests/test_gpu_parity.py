@@ -55,6 +55,16 @@ CASES = {
     "water1536_pme_order6_grid30": (lambda: systems.water_box(seed=6),
                                     dict(cellsize=5.0, lj=(O.SW_C2, 8.0, 10.0),
                                          pme=dict(rc=10.0, grid=(30, 30, 30), order=6, alpha=0.34))),
+    # three different grid edges: every FFT pass and layout transpose sees a
+    # distinct line length and count
+    "water1536_pme_grid24x30x40": (lambda: systems.water_box(seed=7),
+                                   dict(cellsize=5.0, lj=(O.SW_C2, 8.0, 10.0),
+                                        pme=dict(rc=10.0, grid=(24, 30, 40), order=4, alpha=0.32))),
+    # a grid of 8 points with order 6: each spreading group's stencil box
+    # wraps onto itself, so spreading takes the direct-atomics path
+    "water1536_pme_order6_grid8": (lambda: systems.water_box(seed=8),
+                                   dict(cellsize=5.0, lj=(O.SW_C2, 8.0, 10.0),
+                                        pme=dict(rc=10.0, grid=(8, 8, 8), order=6, alpha=0.34))),
 }
 
 
@@ -118,3 +128,21 @@ def test_alpha_matches_reference_bisection(ctx):
         assert sys_.alpha() == O.pme_alpha(12.0, 1e-6)
     finally:
         sys_.close()
+
+
+def test_time_pair_measures_and_keeps_results(ctx):
+    """pmx_sys_time_pair returns a positive per-launch time and leaves the
+    system's results as they were (the next evaluation is unchanged)."""
+    make, kw = CASES["water1536_ljc2_pme"]
+    s = make()
+    _, gspec = both_specs(**kw)
+    sy = pmx.System(ctx, s, gspec, pmx.MIXED)
+    try:
+        f1, ec1, elj1 = sy.evaluate(s.pos)
+        ms = sy.time_pair(5)
+        f2, ec2, elj2 = sy.evaluate(s.pos)
+    finally:
+        sy.close()
+    print(f"pair launch {ms:.4f} ms", flush=True)
+    assert ms > 0
+    assert np.array_equal(f1, f2) and ec1 == ec2 and elj1 == elj2
