@@ -37,6 +37,7 @@ FLOP_PER_PAIR = 69          # algorithmic FLOPs per in-cutoff pair (LJ C2 + PME 
 # scales from a plain 256 MiB copy (profiles/k_pair_traffic_polymer26000.md)
 PAIR_TRAFFIC_BYTES = 57.7e6
 PAIR_TRAFFIC_SKIN = 0.7
+LAUNCH_KNOBS = {1: "tiles_per_cu", 2: "fft_lines", 3: "pair_blocks_per_cu"}
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
 TIMESTEP_FS = 1.0
@@ -59,11 +60,12 @@ def ns_day(ms_per_step: float) -> float:
     return 86400.0 / (ms_per_step * 1e-3) * TIMESTEP_FS * 1e-6
 
 
-def run_form(ctx, s, spec, precision, steps, warmup, skin, drift_steps):
-    from protomol3_amd import pmx
+def run_form(ctx, s, spec, precision, steps, warmup, skin, drift_steps, tune_launch=False):
+    from protomol3_amd import pmx, tune
     sy = pmx.System(ctx, s, spec, precision)
     try:
         sy.md_init(s, TIMESTEP_FS, skin=skin)
+        launch = tune.tune_launch(sy, s.pos, s.vel) if tune_launch else None
         sy.md_set_state(s.pos, s.vel)
         sy.md_run(warmup)
         st0 = sy.md_stats()
@@ -97,7 +99,7 @@ def run_form(ctx, s, spec, precision, steps, warmup, skin, drift_steps):
         sy.close()
     return dict(ms_per_step=(t1 - t0) * 1e3 / steps, stages=stages, ncalls=ncalls, npairs=npairs,
                 rebuilds=st1["rebuilds"] - st0["rebuilds"], rollbacks=st1["rollbacks"] - st0["rollbacks"],
-                x=x, drift_kcal_mol_ns=slope, etot_first=etot[0], etot_last=etot[-1])
+                x=x, drift_kcal_mol_ns=slope, etot_first=etot[0], etot_last=etot[-1], launch=launch)
 
 
 def large(ctx, steps: int, skin: float, log=print):
@@ -135,14 +137,17 @@ def large(ctx, steps: int, skin: float, log=print):
     return out
 
 
-def time_pair(ctx, s, spec, x, skin, reps: int = 200) -> float:
+def time_pair(ctx, s, spec, x, skin, launch=None, reps: int = 200) -> float:
     """Average device ms of one k_pair launch (mixed form) on the list an MD
     step with this skin builds at positions x: the kernel alone, launched
-    back to back between two HIP events on its stream."""
+    back to back between two HIP events on its stream, with the launch
+    shapes the MD run used (launch = (chosen knobs, timings) or None)."""
     from protomol3_amd import pmx
     sy = pmx.System(ctx, s, spec, pmx.MIXED)
     try:
         sy.md_init(s, TIMESTEP_FS, skin=skin)
+        for k, v in (launch[0].items() if launch else ()):
+            sy.set_tuning(k, v)
         sy.md_set_state(x, s.vel)
         sy.time_pair(10)
         return sy.time_pair(reps)
@@ -270,6 +275,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--skin", type=float, default=-1.0, help="pair-list skin in A (<0: autotune)")
     ap.add_argument("--drift-steps", type=int, default=10000)
+    ap.add_argument("--no-tune-launch", action="store_true",
+                    help="keep the default launch shapes (no tile/blocks/FFT-lines search)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--serve-requests", type=int, default=64, help="requests in the serving stream (0: skip)")
@@ -312,10 +319,11 @@ def main():
         log(f"skin tuning (ms/step) mixed {times_m} fp64 {times_d}")
     else:
         skin_m = skin_d = a.skin
-    mixed = run_form(ctx, s, spec, pmx.MIXED, a.steps, a.warmup, skin_m, a.drift_steps)
+    tl = not a.no_tune_launch
+    mixed = run_form(ctx, s, spec, pmx.MIXED, a.steps, a.warmup, skin_m, a.drift_steps, tl)
     log(f"mixed {mixed['ms_per_step']:.4f} ms/step rebuilds {mixed['rebuilds']} rollbacks {mixed['rollbacks']} "
         f"stages {mixed['stages']}")
-    fp64 = run_form(ctx, s, spec, pmx.FP64, a.steps, a.warmup, skin_d, a.drift_steps)
+    fp64 = run_form(ctx, s, spec, pmx.FP64, a.steps, a.warmup, skin_d, a.drift_steps, tl)
     log(f"fp64 {fp64['ms_per_step']:.4f} ms/step rebuilds {fp64['rebuilds']} stages {fp64['stages']}")
     # force accuracy of the mixed form: nonbonded forces of both forms at the
     # same configuration (the end of the mixed run)
@@ -324,7 +332,7 @@ def main():
         sy = pmx.System(ctx, s, spec, prec)
         forms[prec] = sy.evaluate(mixed["x"])
         sy.close()
-    pair_launch_ms = time_pair(ctx, s, spec, mixed["x"], skin_m)
+    pair_launch_ms = time_pair(ctx, s, spec, mixed["x"], skin_m, mixed["launch"])
     ctx.close()
     f64, fm = forms[pmx.FP64][0], forms[pmx.MIXED][0]
     e64, em = forms[pmx.FP64], forms[pmx.MIXED]
@@ -363,6 +371,11 @@ def main():
         "nve_drift_kcal_mol_per_ns": {"mixed": round(mixed["drift_kcal_mol_ns"], 3),
                                       "fp64": round(fp64["drift_kcal_mol_ns"], 3),
                                       "steps": a.drift_steps},
+        "launch_tuning": {form: None if r["launch"] is None else
+                          {"chosen": {LAUNCH_KNOBS[k]: v for k, v in r["launch"][0].items()},
+                           "ms_per_step": {LAUNCH_KNOBS[k]: {str(val): round(ms, 5) for val, ms in t.items()}
+                                           for k, t in r["launch"][1].items()}}
+                          for form, r in (("mixed", mixed), ("fp64", fp64))},
         "pair_list": {"skin_A": {"mixed": skin_m, "fp64": skin_d}, "tuning_ms_per_step": skins,
                       "rebuilds_in_timed_steps": {"mixed": mixed["rebuilds"], "fp64": fp64["rebuilds"]},
                       "rollbacks_in_timed_steps": {"mixed": mixed["rollbacks"], "fp64": fp64["rollbacks"]}},

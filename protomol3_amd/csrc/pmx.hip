@@ -69,6 +69,10 @@ struct Sys {
   FftPlan plx{}, ply{}, plz{};
   bool pme_on = false;
   int cap = 0, maxmaskwork = 0, ntiles_cap = 0, tile_target = 0;
+  // launch shapes the autotuner may change (pmx_sys_set_tuning)
+  int tune_tiles_per_cu = 16;  // tile-count target per CU for short tiles (0: 16-entry tiles)
+  int tune_fft_lines = 16;     // most lines per FFT block
+  int tune_pair_blocks = 0;    // pair-kernel blocks per CU (0: as many as fit)
   long long nbins = 0;
   int nscanblocks = 0;
   int recip_blocks = 0, corr_blocks = 0;
@@ -276,6 +280,23 @@ int validate(pmx_ctx *c, const pmx_sys_desc *d) {
   return PMX_OK;
 }
 
+// FFT blocking: lines per block, two LDS buffers of lines * n complex
+// within 24 KiB (so that FFT blocks fit beside the pair kernel's blocks,
+// which leave about that much LDS free on each CU), at most
+// tune_fft_lines lines (the transposed segments); one energy partial per
+// x-pass block.
+int fft_blocking(Sys *s) {
+  const PmeParams &pm = s->pme;
+  const int cpx = s->mixed ? 8 : 16;
+  const int budget = 24576 / (2 * cpx);
+  const int lines = s->tune_fft_lines;
+  s->zl = std::max(1, std::min(lines, budget / pm.nz));
+  s->tky = std::max(1, std::min(lines, budget / pm.ny));
+  s->tkx = std::max(1, std::min(lines, budget / pm.nx));
+  s->recip_blocks = pm.nzh * ((pm.ny + s->tkx - 1) / s->tkx);
+  return dalloc(s, s->recip_e, 8 * (size_t)s->recip_blocks);
+}
+
 int setup(Sys *s) {
   const pmx_sys_desc &d = s->d;
   s->acc_clean = false;  // buffers are (re)allocated below
@@ -441,8 +462,13 @@ int setup(Sys *s) {
     s->cap = (s->cap + kTileEntries - 1) / kTileEntries * kTileEntries;
   }
   lp.cap = s->cap;
-  s->tile_target = 16 * (s->ctx ? s->ctx->cus : 256);  // a tile per resident pair-kernel wave
-  s->ntiles_cap = s->nsuper * (s->cap / kTileEntries) + 2 * s->tile_target + s->nsuper;
+  {
+    const int cus = s->ctx ? s->ctx->cus : 256;
+    // default: a tile per resident pair-kernel wave; capacity for the largest
+    // target the tuning accepts (64 per CU)
+    s->tile_target = s->tune_tiles_per_cu * cus;
+    s->ntiles_cap = s->nsuper * (s->cap / kTileEntries) + 2 * 64 * cus + s->nsuper;
+  }
   s->maxmaskwork = std::max(1024, s->nsuper * 128);
 
   int rc;
@@ -601,17 +627,7 @@ int setup(Sys *s) {
     } else {
       HIPCHK(hipMemcpyAsync(s->theta.p, th.data(), 8 * GH, hipMemcpyHostToDevice, st));
     }
-    // FFT blocking: lines per block, two LDS buffers of lines * n complex
-    // within 24 KiB (so that FFT blocks fit beside the pair kernel's blocks,
-    // which leave about that much LDS free on each CU), at most 16 lines
-    // (the transposed segments)
-    const int cpx = s->mixed ? 8 : 16;
-    const int budget = 24576 / (2 * cpx);
-    s->zl = std::max(1, std::min(16, budget / pm.nz));
-    s->tky = std::max(1, std::min(16, budget / pm.ny));
-    s->tkx = std::max(1, std::min(16, budget / pm.nx));
-    s->recip_blocks = pm.nzh * ((pm.ny + s->tkx - 1) / s->tkx);
-    if ((rc = A(s->recip_e, 8 * (size_t)s->recip_blocks))) return rc;
+    if ((rc = fft_blocking(s))) return rc;
     HIPCHK(hipStreamSynchronize(st));  // the host tables above are still alive
   }
   HIPCHK(hipStreamSynchronize(st));
@@ -638,7 +654,8 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
           per_cu < 1)
         per_cu = 1;
     }
-    const unsigned grid = (unsigned)std::max(1, s->ctx->cus * per_cu);
+    const int bpc = s->tune_pair_blocks > 0 ? std::min(s->tune_pair_blocks, per_cu) : per_cu;
+    const unsigned grid = (unsigned)std::max(1, s->ctx->cus * bpc);
     PairOut out{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash),
                 P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow),
                 P<unsigned long long>(s->border_acc)};
@@ -1000,7 +1017,7 @@ int grow_and_retry(Sys *s) {
   s->cap = std::min(s->ncl + kTileEntries, 2 * s->cap);
   s->cap = (s->cap + kTileEntries - 1) / kTileEntries * kTileEntries;
   s->lp.cap = s->cap;
-  s->ntiles_cap = s->nsuper * (s->cap / kTileEntries) + 2 * s->tile_target + s->nsuper;
+  s->ntiles_cap = s->nsuper * (s->cap / kTileEntries) + 2 * 64 * s->ctx->cus + s->nsuper;
   s->maxmaskwork *= 2;
   s->maxborder *= 2;
   if (int rc0 = dalloc(s, s->border, 8 * (size_t)s->maxborder)) return rc0;
@@ -1259,6 +1276,38 @@ int pmx_sys_time_pair(pmx_sys *sys, int reps, float *ms_per_launch) {
   if (e != hipSuccess) return fail(s->ctx, PMX_ERR_HIP, hipGetErrorString(e));
   *ms_per_launch = t / (float)reps;
   return PMX_OK;
+}
+
+int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value) {
+  if (!sys) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  switch (knob) {
+    case PMX_TUNE_TILES_PER_CU:
+      if (value < 0 || value > 64) return fail(s->ctx, PMX_ERR_ARG, "tiles per CU must be 0..64");
+      s->tune_tiles_per_cu = value;
+      s->tile_target = value * s->ctx->cus;
+      break;
+    case PMX_TUNE_FFT_LINES:
+      if (value < 1 || value > 64) return fail(s->ctx, PMX_ERR_ARG, "FFT lines per block must be 1..64");
+      s->tune_fft_lines = value;
+      break;
+    case PMX_TUNE_PAIR_BLOCKS_PER_CU:
+      if (value < 0 || value > 64) return fail(s->ctx, PMX_ERR_ARG, "pair blocks per CU must be 0..64");
+      s->tune_pair_blocks = value;
+      break;
+    default:
+      return fail(s->ctx, PMX_ERR_ARG, "unknown tuning knob");
+  }
+  // captured chunks hold the old launch shapes; the energy partials of the
+  // x pass may be reallocated
+  hipSetDevice(s->ctx->device);
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream2));
+  {
+    RtLock lk(g_rt_mu);
+    drop_graphs(s);
+  }
+  return s->pme_on ? fft_blocking(s) : PMX_OK;
 }
 
 int pmx_sys_stage_times(const pmx_sys *sys, float *ms) {

@@ -1058,12 +1058,21 @@ struct PairOut {
 };
 
 // Queue entry of the compacted pair kernel: one in-cutoff pair awaiting its
-// force.  info packs the LJ type pair (bits 0..7), MODIFIED (bit 8), in LJ
-// range (bit 9), in Coulomb range (bit 10) and its force slot (bits 16..26).
+// force.  info packs its force slot (bits 0..9: entry of the group, i-cluster
+// and lane, kGroup * 8 * 64 slots) and MODIFIED (bit 10); the evaluating lane
+// recovers the atoms' charges and types from the slot.
 template <class R>
 struct PairQ {
-  R r2, qq;
-  int info, pad;
+  R r2;
+  int info;
+};
+constexpr int kSlotBits = 10;
+
+// Charge and LJ type (clamped to 0 for padding) of a j-atom of the group.
+template <class R>
+struct JAtom {
+  R q;
+  int ty;
 };
 
 constexpr int kGroup = 2;    // list entries whose pairs share one queue flush
@@ -1135,7 +1144,8 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   __shared__ int s_ty[W][kSuperSize];
   __shared__ R4 s_lj[kMaxTypes * kMaxTypes];  // {A, B, A14, B14}
   __shared__ PairQ<R> s_q[W][kQueue];
-  __shared__ R s_fr[W][kGroup * 8 * kWave];
+  __shared__ __align__(16) R s_fr[W][kGroup * 8 * kWave];
+  __shared__ JAtom<R> s_j[W][kGroup * kClusterSize];
 
   const int nt = pp.ntypes;
   for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) {
@@ -1192,23 +1202,32 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
     int qhead = 0, qtail = 0;  // wave-uniform queue positions
 
     // Evaluates queue pairs [qhead, qhead + cnt) with lane l on pair qhead + l.
+    // A queued pair passed the membership test: it is in range of at least
+    // one of the two cutoffs, and not within the guard band of either.
     auto eval_batch = [&](int cnt, R &qel, R &qec) {
       __builtin_amdgcn_wave_barrier();
       if (lane < cnt) {
         const PairQ<R> e = q[(qhead + lane) & (kQueue - 1)];
-        const int info = e.info;
-        const bool mod = (info >> 8) & 1, il = (info >> 9) & 1, ic = (info >> 10) & 1;
+        const int slot = e.info & ((1 << kSlotBits) - 1);
+        const bool mod = (e.info >> kSlotBits) & 1;
+        const int ia = ((slot >> 6) & 7) * 8 + (slot & 7);  // c * 8 + ti
+        const int ja = ((slot >> 9) << 3) + ((slot >> 3) & 7);  // g * 8 + tj
+        const JAtom<R> jd = s_j[wv][ja];
+        R qq = si[ia].w * jd.q;
+        if (mod) qq *= csf;
+        const bool il = LJSW != SW_NONE && e.r2 <= sl.rc2;
+        const bool ic = COUL != COUL_NONE && e.r2 <= sc.rc2;
         R A = 0, B = 0;
         if (LJSW != SW_NONE) {
-          const R4 lj = s_lj[info & 0xff];
+          const R4 lj = s_lj[s_ty[wv][ia] * nt + jd.ty];
           A = mod ? lj.z : lj.x;
           B = mod ? lj.w : lj.y;
         }
         R e1, e2, f;
-        pair_eval<COUL, LJSW, CSW, R>(pp, sl, sc, alpha, a2pi, csf, e.r2, e.qq, A, B, il, ic, e1, e2, f);
+        pair_eval<COUL, LJSW, CSW, R>(pp, sl, sc, alpha, a2pi, csf, e.r2, qq, A, B, il, ic, e1, e2, f);
         qel += e1;
         qec += e2;
-        fr_slot[info >> 16] = f;
+        fr_slot[slot] = f;
       }
       qhead += cnt;
     };
@@ -1218,7 +1237,6 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
       const int ng = eend - eg < kGroup ? eend - eg : kGroup;
       int cjv[kGroup], emv[kGroup], mov[kGroup], shv[kGroup], tyj0[kGroup];
       R4 xj[kGroup];
-      unsigned gmask = 0;
 #pragma unroll
       for (int g = 0; g < kGroup; ++g) {
         int4 ent = make_int4(0, 0, -1, 0);
@@ -1245,76 +1263,70 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
           xj[g].z += oz;
         }
         tyj0[g] = ts[sj];
-        gmask |= (unsigned)emv[g] << (8 * g);
+        if (ti == 0) s_j[wv][g * kClusterSize + tj] = JAtom<R>{xj[g].w, tyj0[g] < 0 ? 0 : tyj0[g]};
       }
+      // every force slot of the group starts at zero (phase B fills the
+      // members' slots, phase C reads the slots of the entries' masks)
+#pragma unroll
+      for (int k = 0; k < kGroup * 8 / 4; ++k) {
+        R4 z;
+        z.x = z.y = z.z = z.w = (R)0;
+        reinterpret_cast<R4 *>(fr_slot)[k * kWave + lane] = z;
+      }
+      __builtin_amdgcn_wave_barrier();
 
       // ---- phase A: membership, queue, full batches
       R qel = 0, qec = 0;
-      unsigned mrem = gmask;
-      while (mrem) {
-        const int gc = __builtin_ctz(mrem);
-        mrem &= mrem - 1;
-        const int g = gc >> 3, c = gc & 7;
-        R4 xjg = xj[0];
-        int tyjg = tyj0[0], mog = mov[0], cjg = cjv[0], shg = shv[0];
 #pragma unroll
-        for (int h = 1; h < kGroup; ++h)
-          if (g == h) {
-            xjg = xj[h];
-            tyjg = tyj0[h];
-            mog = mov[h];
-            cjg = cjv[h];
-            shg = shv[h];
+      for (int g = 0; g < kGroup; ++g) {
+        const int sj = cjv[g] * kClusterSize + tj;
+        unsigned mrem = emv[g];
+        while (mrem) {
+          const int c = __builtin_ctz(mrem);
+          mrem &= mrem - 1;
+          bool cand = true, modified = false;
+          if (mov[g] >= 0) {
+            // exclusion masks: bit l of each word is lane l's candidate
+            cand = __builtin_amdgcn_inverse_ballot_w64(masks[2 * ((size_t)mov[g] + c)]);
+            modified = __builtin_amdgcn_inverse_ballot_w64(masks[2 * ((size_t)mov[g] + c) + 1]);
           }
-        bool cand = true, modified = false;
-        if (mog >= 0) {
-          const unsigned long long m = masks[2 * ((size_t)mog + c)];
-          const unsigned long long mm = masks[2 * ((size_t)mog + c) + 1];
-          cand = (m >> lane) & 1ull;
-          modified = (mm >> lane) & 1ull;
-        }
-        const R4 xi = si[c * 8 + ti];
-        R dx, dy, dz;
-        pair_delta<R, R4>(xi, xjg, E, shg & kShiftPerPair, dx, dy, dz);
-        const R r2 = dx * dx + dy * dy + dz * dz;
-        bool border = false;
-        if (LJSW != SW_NONE) border |= fabs(r2 - sl.rc2) <= guard;
-        if (COUL != COUL_NONE) border |= fabs(r2 - sc.rc2) <= guard;
-        border = border && cand;
-        const bool ok = cand && !border;
-        const bool in_lj = LJSW != SW_NONE && ok && r2 <= sl.rc2;
-        const bool in_c = COUL != COUL_NONE && ok && r2 <= sc.rc2;
-        const int sj = cjg * kClusterSize + tj;
-        if (border) {
-          const int b = atomicAdd(out.nborder, 1);
-          if (b < out.maxborder)
-            out.border[b] = make_int2(S * kSuperSize + c * 8 + ti, sj);
-          else
-            atomicExch(out.overflow, 1);
-        }
-        const bool in = in_lj || in_c;
-        if (stats && in) {
-          const int ai = perm[S * kSuperSize + c * 8 + ti], aj = perm[sj];
-          ++npair;
-          hsum += pair_hash(ai < aj ? ai : aj, ai < aj ? aj : ai);
-        }
-        const int slot = gc * kWave + lane;
-        fr_slot[slot] = (R)0;
-        const unsigned long long bm = __ballot(in);
-        if (bm) {
-          if (in) {
-            const unsigned below =
-                __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
-            PairQ<R> e;
-            e.r2 = r2;
-            e.qq = modified ? xi.w * xjg.w * csf : xi.w * xjg.w;
-            e.info = (s_ty[wv][c * 8 + ti] * nt + (tyjg < 0 ? 0 : tyjg)) | ((int)modified << 8) |
-                     ((int)in_lj << 9) | ((int)in_c << 10) | (slot << 16);
-            e.pad = 0;
-            q[(qtail + (int)below) & (kQueue - 1)] = e;
+          const R4 xi = si[c * 8 + ti];
+          R dx, dy, dz;
+          pair_delta<R, R4>(xi, xj[g], E, shv[g] & kShiftPerPair, dx, dy, dz);
+          const R r2 = dx * dx + dy * dy + dz * dz;
+          bool border = false;
+          if (LJSW != SW_NONE) border |= fabs(r2 - sl.rc2) <= guard;
+          if (COUL != COUL_NONE) border |= fabs(r2 - sc.rc2) <= guard;
+          border = border && cand;
+          const bool ok = cand && !border;
+          const bool in_lj = LJSW != SW_NONE && ok && r2 <= sl.rc2;
+          const bool in_c = COUL != COUL_NONE && ok && r2 <= sc.rc2;
+          const bool in = in_lj || in_c;
+          const unsigned long long bm = __builtin_amdgcn_ballot_w64(in);
+          if (border) {
+            const int b = atomicAdd(out.nborder, 1);
+            if (b < out.maxborder)
+              out.border[b] = make_int2(S * kSuperSize + c * 8 + ti, sj);
+            else
+              atomicExch(out.overflow, 1);
           }
-          qtail += __popcll(bm);
-          while (qtail - qhead >= kWave) eval_batch(kWave, qel, qec);
+          if (stats && in) {
+            const int ai = perm[S * kSuperSize + c * 8 + ti], aj = perm[sj];
+            ++npair;
+            hsum += pair_hash(ai < aj ? ai : aj, ai < aj ? aj : ai);
+          }
+          if (bm) {
+            if (in) {
+              const unsigned below =
+                  __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
+              PairQ<R> e;
+              e.r2 = r2;
+              e.info = ((g * 8 + c) * kWave + lane) | ((int)modified << kSlotBits);
+              q[(qtail + (int)below) & (kQueue - 1)] = e;
+            }
+            qtail += __popcll(bm);
+            while (qtail - qhead >= kWave) eval_batch(kWave, qel, qec);
+          }
         }
       }
       // ---- phase B: the rest of the queue

@@ -56,6 +56,7 @@ class MdDesc(C.Structure):
 
 
 LEAPFROG, LANGEVIN = 0, 1
+TUNE_TILES_PER_CU, TUNE_FFT_LINES, TUNE_PAIR_BLOCKS_PER_CU = 1, 2, 3
 
 
 EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_create", "pmx_sys_destroy",
@@ -64,7 +65,7 @@ EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_crea
             "pmx_sys_cells", "pmx_sys_stage_times", "pmx_sys_timing_begin", "pmx_sys_timing_end",
             "pmx_md_init", "pmx_md_set_state", "pmx_md_run", "pmx_md_get_state", "pmx_md_stats",
             "pmx_langevin_noise", "pmx_sys_device_bytes", "pmx_estimate_bytes", "pmx_md_run_many",
-            "pmx_sys_time_pair"]
+            "pmx_sys_time_pair", "pmx_sys_set_tuning"]
 
 _lib = None
 
@@ -104,6 +105,7 @@ def load() -> C.CDLL:
     L.pmx_md_run.argtypes = [vp, C.c_int]
     L.pmx_md_run_many.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
     L.pmx_sys_time_pair.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
+    L.pmx_sys_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.pmx_md_get_state.argtypes = [vp, dp, dp, dp]
     L.pmx_md_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.pmx_langevin_noise.argtypes = [C.c_ulonglong, C.c_ulonglong, C.c_int, C.POINTER(C.c_double)]
@@ -346,6 +348,11 @@ class System:
         ms = C.c_float()
         self.ctx.check(self.ctx.lib.pmx_sys_time_pair(self.h, reps, C.byref(ms)), "pmx_sys_time_pair")
         return float(ms.value)
+
+    def set_tuning(self, knob: int, value: int):
+        """Sets a launch shape (TUNE_TILES_PER_CU, TUNE_FFT_LINES or
+        TUNE_PAIR_BLOCKS_PER_CU); the resident state is kept."""
+        self.ctx.check(self.ctx.lib.pmx_sys_set_tuning(self.h, knob, value), "pmx_sys_set_tuning")
 
     def md_get_state(self):
         """(x, v, energies dict) after the last run."""

@@ -1,15 +1,41 @@
 """Autotuning of per-system launch parameters.
 
-Currently tunes the pair-list skin of the device-resident MD path: a larger
-skin rebuilds the cell sort and cluster-pair list less often but makes the
-pair kernel test more candidate pairs.  Each candidate runs the same short
-trajectory from the same state; the fastest is kept.
+Two searches, both on the device-resident MD path and both timing the same
+short trajectory from the same state for every candidate:
+
+* tune_skin: the pair-list skin.  A larger skin rebuilds the cell sort and
+  cluster-pair list less often but makes the pair kernel test more candidate
+  pairs.
+* tune_launch: the launch shapes of one system (pmx.TUNE_* knobs), one knob
+  at a time with the others held at their best so far: the pair-kernel tile
+  target (how finely a small system's list is cut so that every CU gets
+  work), the pair-kernel blocks per CU (fewer leave room for the PME chain
+  on the second stream) and the FFT lines per block (the PME grid blocking).
+
+Blocks per CU and FFT lines change no sum except for the order of the FP64
+form's atomics (the mixed form accumulates in fixed point); the tile length
+regroups the mixed form's FP32 partial sums of the i-forces (one per tile),
+a change at FP32 rounding (about 1e-7 relative).
 """
 from __future__ import annotations
 
 import time
 
 from . import pmx
+
+LAUNCH_CANDIDATES = (
+    (pmx.TUNE_TILES_PER_CU, (16, 8, 32, 0)),
+    (pmx.TUNE_PAIR_BLOCKS_PER_CU, (0, 6, 4)),
+    (pmx.TUNE_FFT_LINES, (16, 8, 4)),
+)
+
+
+def _time_run(sy, x, v, steps: int, warmup: int = 20) -> float:
+    sy.md_set_state(x, v)
+    sy.md_run(warmup)
+    t0 = time.perf_counter()
+    sy.md_run(steps)
+    return (time.perf_counter() - t0) * 1e3 / steps
 
 
 def tune_skin(ctx, s, spec: pmx.ForceSpec, precision: int, x, v, timestep_fs: float = 1.0,
@@ -20,12 +46,36 @@ def tune_skin(ctx, s, spec: pmx.ForceSpec, precision: int, x, v, timestep_fs: fl
         sy = pmx.System(ctx, s, spec, precision)
         try:
             sy.md_init(s, timestep_fs, skin=skin)
-            sy.md_set_state(x, v)
-            sy.md_run(20)
-            t0 = time.perf_counter()
-            sy.md_run(steps)
-            times[skin] = (time.perf_counter() - t0) * 1e3 / steps
+            times[skin] = _time_run(sy, x, v, steps)
         finally:
             sy.close()
     best = min(times, key=times.get)
+    return best, times
+
+
+def tune_launch(sy, x, v, steps: int = 400, candidates=LAUNCH_CANDIDATES, rounds: int = 2,
+                margin: float = 0.01) -> tuple[dict, dict]:
+    """Tunes the launch shapes of a System whose MD state is initialised
+    (md_init done).  Leaves the best shapes set on it and returns
+    ({knob: value}, {knob: {value: ms per step}}).  The first candidate of
+    each knob is its default; each time is the best of `rounds` runs, and a
+    knob leaves its default only for a candidate faster by more than
+    `margin` (run-to-run noise of a short timed run is about that)."""
+    best = {k: vals[0] for k, vals in candidates}
+    for k, val in best.items():
+        sy.set_tuning(k, val)
+    times: dict = {}
+    for _ in range(rounds):
+        for knob, vals in candidates:
+            t = times.setdefault(knob, {})
+            for val in vals:
+                sy.set_tuning(knob, val)
+                ms = _time_run(sy, x, v, steps)
+                t[val] = min(ms, t.get(val, ms))
+            sy.set_tuning(knob, best[knob])
+    for knob, vals in candidates:
+        t = times[knob]
+        fastest = min(t, key=t.get)
+        best[knob] = fastest if t[fastest] < (1.0 - margin) * t[vals[0]] else vals[0]
+        sy.set_tuning(knob, best[knob])
     return best, times
