@@ -73,6 +73,8 @@ struct Sys {
   int tune_tiles_per_cu = 16;  // tile-count target per CU for short tiles (0: 16-entry tiles)
   int tune_fft_lines = 16;     // most lines per FFT block
   int tune_pair_blocks = 0;    // pair-kernel blocks per CU (0: as many as fit)
+  int tune_schedule = 0;       // 0: balanced runs per wave; 1: fixed-length tiles (counter / round robin)
+  int pair_grid = 0;           // blocks of the pair-kernel launch
   long long nbins = 0;
   int nscanblocks = 0;
   int recip_blocks = 0, corr_blocks = 0;
@@ -80,7 +82,7 @@ struct Sys {
   // device buffers
   std::vector<DevBuf *> all;
   DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
-      perm2, sc_info, sc_col, colsc, sctr, shalf, sorg, ctr, half, entries, count, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
+      perm2, sc_info, sc_col, colsc, sctr, shalf, sorg, ctr, half, entries, count, lcost, tep, tcp, tpos, wstart, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
       pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec, spec2,
       greal, theta, twx, twy, twz, border, nborder, border_acc;
   int maxborder = 65536;
@@ -95,6 +97,7 @@ struct Sys {
   bool timing_active = false;                   // MD steps record events (timing_begin .. timing_end)
   bool have_positions = false;
   bool cached = false;
+  bool list_valid = false;             // entries, counts and work of a built list are in place
   bool acc_clean = false;              // force accumulators, PME grid and pass counters are zero
   bool dry = false;                    // size only: allocations are counted, nothing runs
   unsigned long long dry_bytes = 0;
@@ -297,9 +300,13 @@ int fft_blocking(Sys *s) {
   return dalloc(s, s->recip_e, 8 * (size_t)s->recip_blocks);
 }
 
+template <bool MIXED>
+void run_pair_phase(Sys *s, bool stats, int phase);
+
 int setup(Sys *s) {
   const pmx_sys_desc &d = s->d;
   s->acc_clean = false;  // buffers are (re)allocated below
+  s->list_valid = false;
   s->mixed = d.precision == PMX_MIXED;
   s->n = d.n;
   s->q.assign(d.charge, d.charge + d.n);
@@ -486,9 +493,13 @@ int setup(Sys *s) {
       (rc = A(s->sorg, 16 * (size_t)s->nsuper)) ||
       (rc = A(s->ctr, 16 * (size_t)s->ncl)) ||
       (rc = A(s->half, 16 * (size_t)s->ncl)) || (rc = A(s->entries, 16 * (size_t)s->nsuper * s->cap)) ||
-      (rc = A(s->count, 4 * (size_t)s->nsuper)) || (rc = A(s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
+      (rc = A(s->count, 4 * (size_t)s->nsuper)) || (rc = A(s->lcost, 4 * (size_t)s->nsuper)) ||
+      (rc = A(s->tep, 4 * ((size_t)s->nsuper + 1))) || (rc = A(s->tcp, 8 * ((size_t)s->nsuper + 1))) ||
+      (rc = A(s->tpos, 4 * ((size_t)kMaxPairWavesPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
+      (rc = A(s->wstart, 4 * ((size_t)kMaxPairWavesPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
+      (rc = A(s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
       (rc = A(s->nmaskwork, 8)) || (rc = A(s->masks, 16 * 8 * (size_t)s->maxmaskwork)) ||
-      (rc = A(s->overflow, 8)) || (rc = A(s->tiles, 8 * (size_t)s->ntiles_cap)) || (rc = A(s->ntiles, 8)) ||
+      (rc = A(s->overflow, 8)) || (rc = A(s->tiles, 16 * (size_t)s->ntiles_cap)) || (rc = A(s->ntiles, 8)) ||
       (rc = A(s->tilectr, 8)) || (rc = A(s->ljtab, 8 * s->lj.size())) || (rc = A(s->facc, 24 * NP)) ||
       (rc = A(s->fout, 24 * N)) || (rc = A(s->pair_e, 16 * (size_t)s->ntiles_cap)) ||
       (rc = A(s->npairs, 8 * (size_t)s->ntiles_cap)) || (rc = A(s->hash, 8 * (size_t)s->ntiles_cap)) ||
@@ -631,6 +642,10 @@ int setup(Sys *s) {
     HIPCHK(hipStreamSynchronize(st));  // the host tables above are still alive
   }
   HIPCHK(hipStreamSynchronize(st));
+  if (s->mixed)  // the pair-kernel grid (phase -1 sizes it)
+    run_pair_phase<true>(s, false, -1);
+  else
+    run_pair_phase<false>(s, false, -1);
   s->cached = true;
   return PMX_OK;
 }
@@ -644,24 +659,29 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
   hipStream_t st = phase == 2 ? s->ctx->stream2 : s->ctx->stream;
   using R4 = typename Real4<typename std::conditional<MIXED, float, double>::type>::type;
   auto *facc = P<typename AccT<MIXED>::type>(s->facc);
-  if (phase == 0) {
-    // as many blocks as are resident at once (each wave's first tile is
-    // assigned statically)
+  if (phase < 0) {
+    // as many blocks as are resident at once (the schedules assign each
+    // wave its first tile, or its whole run, statically)
     static int per_cu = 0;
     if (!per_cu) {
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair<MIXED, C, LJ, CS>, kPairBlock, 0) !=
               hipSuccess ||
           per_cu < 1)
         per_cu = 1;
+      per_cu = std::min(per_cu, kMaxPairWavesPerCu / (kPairBlock / kWave));
     }
     const int bpc = s->tune_pair_blocks > 0 ? std::min(s->tune_pair_blocks, per_cu) : per_cu;
-    const unsigned grid = (unsigned)std::max(1, s->ctx->cus * bpc);
+    s->pair_grid = std::max(1, s->ctx->cus * bpc);
+    return;
+  }
+  if (phase == 0) {
+    const unsigned grid = (unsigned)s->pair_grid;
     PairOut out{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash),
                 P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow),
                 P<unsigned long long>(s->border_acc)};
     hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, stats ? 1 : 0,
-                       P<int2>(s->tiles), P<int>(s->ntiles), P<int>(s->tilectr), P<int4>(s->entries),
-                       P<int>(s->count), s->cap, P<unsigned long long>(s->masks), P<R4>(s->xs),
+                       P<int4>(s->tiles), P<int>(s->ntiles), P<int>(s->wstart), P<int>(s->tilectr), P<int4>(s->entries),
+                       s->cap, P<unsigned long long>(s->masks), P<R4>(s->xs),
                        P<float4>(s->sorg), P<int>(s->ts), P<int>(s->permp), P<double>(s->ljtab), facc, out);
     return;
   }
@@ -833,6 +853,17 @@ void timing_begin_call(Sys *s, bool timing) {
 // Cell sort of the current positions and the cluster-pair list built from it
 // (the reference rebuilds its cell lists at every force evaluation,
 // ForceGroup.cpp:57; the MD path rebuilds when the skin requires it).
+// Work tiles of the current list for the current schedule and pair grid.
+void make_tiles(Sys *s, hipStream_t st) {
+  if (s->tune_schedule == 0)
+    hipLaunchKernelGGL(k_make_tiles_balanced, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count),
+                       P<int>(s->lcost), s->pair_grid * (kPairBlock / kWave), P<int>(s->tep), P<long long>(s->tcp),
+                       P<int>(s->tpos), P<int>(s->wstart), P<int4>(s->tiles), P<int>(s->ntiles), s->ntiles_cap);
+  else
+    hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int4>(s->tiles),
+                       P<int>(s->ntiles), s->ntiles_cap, s->tile_target);
+}
+
 int rebuild(Sys *s, bool timing) {
   hipStream_t st = s->ctx->stream;
   const int n = s->n;
@@ -871,7 +902,7 @@ int rebuild(Sys *s, bool timing) {
   if (timing) hipEventRecord(s->ev[1], st);
   hipLaunchKernelGGL(k_build_list, dim3(s->nsuper), dim3(kListBlock), 0, st, s->lp, P<int>(s->sc_col),
                      P<int>(s->colsc), P<float4>(s->sctr), P<float4>(s->shalf), P<float4>(s->ctr),
-                     P<float4>(s->half), P<int4>(s->entries), P<int>(s->count),
+                     P<float4>(s->half), P<int4>(s->entries), P<int>(s->count), P<int>(s->lcost),
                      P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork, P<int>(s->overflow));
   if (!s->etab.empty())
     hipLaunchKernelGGL(k_mark_exclusions, dim3(nblk((long long)s->etab.size(), 256)), dim3(256), 0, st,
@@ -881,8 +912,8 @@ int rebuild(Sys *s, bool timing) {
   hipLaunchKernelGGL(k_build_masks, dim3(s->ctx->cus * 4), dim3(256), 0, st, P<int>(s->maskwork),
                      P<int>(s->nmaskwork), s->maxmaskwork, P<int4>(s->entries), s->cap, P<int>(s->permp),
                      P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned long long>(s->masks));
-  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int2>(s->tiles),
-                     P<int>(s->ntiles), s->ntiles_cap, s->tile_target);
+  make_tiles(s, st);
+  s->list_valid = true;
   if (timing) hipEventRecord(s->ev[2], st);
   return PMX_OK;
 }
@@ -1013,6 +1044,7 @@ void drop_graphs(Sys *s) {
 
 int grow_and_retry(Sys *s) {
   drop_graphs(s);  // captured chunks hold the old buffer addresses
+  s->list_valid = false;
   // double the list capacity and the mask pool, then rebuild buffers
   s->cap = std::min(s->ncl + kTileEntries, 2 * s->cap);
   s->cap = (s->cap + kTileEntries - 1) / kTileEntries * kTileEntries;
@@ -1022,7 +1054,7 @@ int grow_and_retry(Sys *s) {
   s->maxborder *= 2;
   if (int rc0 = dalloc(s, s->border, 8 * (size_t)s->maxborder)) return rc0;
   int rc;
-  if ((rc = dalloc(s, s->entries, 16 * (size_t)s->nsuper * s->cap)) || (rc = dalloc(s, s->tiles, 8 * (size_t)s->ntiles_cap)) ||
+  if ((rc = dalloc(s, s->entries, 16 * (size_t)s->nsuper * s->cap)) || (rc = dalloc(s, s->tiles, 16 * (size_t)s->ntiles_cap)) ||
       (rc = dalloc(s, s->pair_e, 16 * (size_t)s->ntiles_cap)) || (rc = dalloc(s, s->npairs, 8 * (size_t)s->ntiles_cap)) ||
       (rc = dalloc(s, s->hash, 8 * (size_t)s->ntiles_cap)) || (rc = dalloc(s, s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
       (rc = dalloc(s, s->masks, 16 * 8 * (size_t)s->maxmaskwork)))
@@ -1295,6 +1327,10 @@ int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value) {
       if (value < 0 || value > 64) return fail(s->ctx, PMX_ERR_ARG, "pair blocks per CU must be 0..64");
       s->tune_pair_blocks = value;
       break;
+    case PMX_TUNE_PAIR_SCHEDULE:
+      if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "pair schedule must be 0 or 1");
+      s->tune_schedule = value;
+      break;
     default:
       return fail(s->ctx, PMX_ERR_ARG, "unknown tuning knob");
   }
@@ -1306,6 +1342,16 @@ int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value) {
   {
     RtLock lk(g_rt_mu);
     drop_graphs(s);
+  }
+  if (s->mixed)
+    run_pair_phase<true>(s, false, -1);
+  else
+    run_pair_phase<false>(s, false, -1);
+  // the tiles follow the schedule and the grid: cut the current list again
+  // (the next MD step may use it without a rebuild)
+  if (s->list_valid) {
+    make_tiles(s, s->ctx->stream);
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
   }
   return s->pme_on ? fft_blocking(s) : PMX_OK;
 }

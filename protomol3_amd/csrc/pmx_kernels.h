@@ -549,10 +549,13 @@ __device__ inline int col_runs(int c, int k, int n, int r[2][2]) {
 // prefix count keeps the entries sorted by cj.  Diagonal entries and entries
 // with a partial cluster get a mask work slot (explicit pair masks).
 constexpr int kListBlock = 256;
+// Work estimate of a list entry in units of one cluster pair (64 candidate
+// pairs): its loads, j-force sums and atomics cost about one more.
+constexpr int kEntryCost = 1;
 __device__ inline void list_append(const ListParams &q, int S, const float4 (&ic)[8], const float4 (&ih)[8], int cj,
                                    float4 jc, float4 jh, int4 *__restrict__ entries, int *__restrict__ maskwork,
                                    int *__restrict__ nmaskwork, int maxmaskwork, int *__restrict__ overflow,
-                                   int *s_cnt, int &total) {
+                                   int *s_cnt, int &total, int &cost) {
   constexpr int NW = kListBlock / kWave;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t imask = 0;
@@ -609,6 +612,7 @@ __device__ inline void list_append(const ListParams &q, int S, const float4 (&ic
         }
       }
       entries[(size_t)S * q.cap + e] = make_int4(cj, (int)imask, moff, per_pair ? kShiftPerPair : shift);
+      cost += kEntryCost + __popc(imask);
     } else {
       atomicExch(overflow, 1);
     }
@@ -632,6 +636,7 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
                                                            const float4 *__restrict__ ctr,
                                                            const float4 *__restrict__ half,
                                                            int4 *__restrict__ entries, int *__restrict__ count,
+                                                           int *__restrict__ cost_out,
                                                            int *__restrict__ maskwork, int *__restrict__ nmaskwork,
                                                            int maxmaskwork, int *__restrict__ overflow) {
   constexpr int NW = kListBlock / kWave;
@@ -642,7 +647,10 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
   const int S = blockIdx.x;
   const int col = sc_col[S];
   if (col < 0) {  // past the last occupied supercluster: no atoms
-    if (threadIdx.x == 0) count[S] = 0;
+    if (threadIdx.x == 0) {
+      count[S] = 0;
+      cost_out[S] = 0;
+    }
     return;
   }
   float4 ic[8], ih[8];
@@ -656,6 +664,7 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
   const int cx = col / q.ncy, cy = col - cx * q.ncy;
   const int nxr = col_runs(cx, q.kx, q.ncx, xr), nyr = col_runs(cy, q.ky, q.ncy, yr);
   int total = 0;  // block-uniform
+  int cost = 0;   // this thread's entries' work (kEntryCost + cluster pairs each)
   for (int rx = 0; rx < nxr; ++rx)
     for (int x = xr[rx][0]; x <= xr[rx][1]; ++x)
       for (int ry = 0; ry < nyr; ++ry) {
@@ -689,12 +698,21 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
               jc = ctr[cj];
               jh = half[cj];
             }
-            list_append(q, S, ic, ih, cj, jc, jh, entries, maskwork, nmaskwork, maxmaskwork, overflow, s_cnt, total);
+            list_append(q, S, ic, ih, cj, jc, jh, entries, maskwork, nmaskwork, maxmaskwork, overflow, s_cnt, total,
+                        cost);
           }
           __syncthreads();  // s_sc is rewritten by the next range
         }
       }
-  if (threadIdx.x == 0) count[S] = total < q.cap ? total : q.cap;
+  cost = wave_sum(cost);
+  if (lane == 0) s_cnt[wv] = cost;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    count[S] = total < q.cap ? total : q.cap;
+    int c = 0;
+    for (int w = 0; w < NW; ++w) c += s_cnt[w];
+    cost_out[S] = c;
+  }
 }
 
 // Flags the list entry that holds each excluded pair (FULL or MODIFIED) so
@@ -828,7 +846,7 @@ __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ mas
 }
 
 // Compact tile list: tiles of up to kTileEntries entries per supercluster.
-__global__ void k_make_tiles(int nsuper, const int *__restrict__ count, int2 *__restrict__ tiles,
+__global__ void k_make_tiles(int nsuper, const int *__restrict__ count, int4 *__restrict__ tiles,
                              int *__restrict__ ntiles, int ntiles_cap, int target) {
   // single block of 1024 threads.  Tile length: the largest of 16, 8, 4 and
   // 2 entries that gives at least `target` tiles (one per resident wave of
@@ -869,7 +887,7 @@ __global__ void k_make_tiles(int nsuper, const int *__restrict__ count, int2 *__
       __syncthreads();
     }
     int excl = carry + s[threadIdx.x] - t;
-    for (int k = 0; k < t; ++k) tiles[excl + k] = make_int2(S, k * T);
+    for (int k = 0; k < t; ++k) tiles[excl + k] = make_int4(S, k * T, min(k * T + T, count[S]), 0);
     __syncthreads();
     carry += s[1023];
     __syncthreads();
@@ -877,6 +895,120 @@ __global__ void k_make_tiles(int nsuper, const int *__restrict__ count, int2 *__
   if (threadIdx.x == 0) {
     ntiles[0] = carry;
     ntiles[1] = T;
+  }
+}
+
+// Largest index k in [0, n) with a[k] <= v (a ascending, a[0] <= v).
+template <class T>
+__device__ inline int last_le(const T *__restrict__ a, int n, T v) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Balanced static schedule: the list entries of all superclusters, in
+// supercluster order, are cut into nwaves contiguous runs of equal estimated
+// work (the per-supercluster work of k_build_list, spread evenly over its
+// entries), one run per resident wave of the pair kernel, cuts at even entry
+// offsets.  A run is split into one tile per supercluster it touches, so
+// there are at most nwaves + nsuper tiles.  Wave w processes tiles
+// [wstart[w], wstart[w + 1]).  Single block of 1024 threads; ep, cp (nsuper
+// + 1 each) and pos (nwaves + 1) are scratch.
+__global__ void k_make_tiles_balanced(int nsuper, const int *__restrict__ count, const int *__restrict__ cost,
+                                      int nwaves, int *__restrict__ ep, long long *__restrict__ cp,
+                                      int *__restrict__ pos, int *__restrict__ wstart, int4 *__restrict__ tiles,
+                                      int *__restrict__ ntiles, int ntiles_cap) {
+  __shared__ int s_e[1024];
+  __shared__ long long s_c[1024];
+  const int tid = threadIdx.x;
+  // exclusive prefixes of the entry counts (ep) and of the work (cp)
+  int ecarry = 0;
+  long long ccarry = 0;
+  for (int base = 0; base < nsuper; base += 1024) {
+    const int S = base + tid;
+    const int c = S < nsuper ? count[S] : 0;
+    const long long w = S < nsuper ? (long long)cost[S] : 0;
+    s_e[tid] = c;
+    s_c[tid] = w;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int ve = tid >= off ? s_e[tid - off] : 0;
+      const long long vc = tid >= off ? s_c[tid - off] : 0;
+      __syncthreads();
+      s_e[tid] += ve;
+      s_c[tid] += vc;
+      __syncthreads();
+    }
+    if (S < nsuper) {
+      ep[S] = ecarry + s_e[tid] - c;
+      cp[S] = ccarry + s_c[tid] - w;
+    }
+    ecarry += s_e[1023];
+    ccarry += s_c[1023];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    ep[nsuper] = ecarry;
+    cp[nsuper] = ccarry;
+  }
+  __syncthreads();
+  const int E = ecarry;
+  const long long C = ccarry;
+  // run boundaries in entry positions (ep[S] + offset)
+  for (int w = tid; w <= nwaves; w += 1024) {
+    int p = E;
+    if (w < nwaves && C > 0) {
+      const long long b = (long long)w * C / nwaves;  // < C
+      const int S = last_le<long long>(cp, nsuper, b);  // cp[S] <= b < cp[S + 1]: cost[S] > 0
+      const long long off = (b - cp[S]) * (long long)count[S] / (long long)max(cost[S], 1);
+      p = ep[S] + ((int)min(off, (long long)count[S]) & ~1);
+    }
+    pos[w] = p;
+  }
+  __syncthreads();
+  // tiles per run (superclusters touched), then their exclusive prefix
+  int carry = 0;
+  for (int base = 0; base < nwaves; base += 1024) {
+    const int w = base + tid;
+    int n = 0;
+    if (w < nwaves) {
+      const int p0 = min(pos[w], E), p1 = min(max(pos[w + 1], p0), E);
+      if (p1 > p0) n = last_le<int>(ep, nsuper, p1 - 1) - last_le<int>(ep, nsuper, p0) + 1;
+    }
+    s_e[tid] = n;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int v = tid >= off ? s_e[tid - off] : 0;
+      __syncthreads();
+      s_e[tid] += v;
+      __syncthreads();
+    }
+    if (w < nwaves) wstart[w] = carry + s_e[tid] - n;
+    carry += s_e[1023];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    wstart[nwaves] = min(carry, ntiles_cap);
+    ntiles[0] = min(carry, ntiles_cap);
+    ntiles[1] = 0;
+  }
+  __syncthreads();
+  for (int w = tid; w < nwaves; w += 1024) {
+    const int p0 = min(pos[w], E), p1 = min(max(pos[w + 1], p0), E);
+    if (p1 <= p0) continue;
+    const int S0 = last_le<int>(ep, nsuper, p0), S1 = last_le<int>(ep, nsuper, p1 - 1);
+    int t = wstart[w];
+    for (int S = S0; S <= S1 && t < ntiles_cap; ++S, ++t) {
+      const int e0 = S == S0 ? p0 - ep[S] : 0;
+      const int e1 = S == S1 ? p1 - ep[S] : count[S];
+      tiles[t] = make_int4(S, e0, e1, 0);
+    }
   }
 }
 
@@ -1130,8 +1262,8 @@ __device__ inline void pair_delta(const R4 &xi, const R4 &xj, const EdgesR<R> &E
 // switches), on members only, which are about a quarter of the candidates.
 template <bool MIXED, int COUL, int LJSW, int CSW>
 __global__ void __launch_bounds__(kPairBlock) k_pair(
-    PairParams pp, int stats, const int2 *__restrict__ tiles, const int *__restrict__ ntiles_p,
-    int *__restrict__ tile_counter, const int4 *__restrict__ entries, const int *__restrict__ count, int cap,
+    PairParams pp, int stats, const int4 *__restrict__ tiles, const int *__restrict__ ntiles_p,
+    const int *__restrict__ wstart, int *__restrict__ tile_counter, const int4 *__restrict__ entries, int cap,
     const unsigned long long *__restrict__ masks,
     const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
     const float4 *__restrict__ sorg, const int *__restrict__ ts, const int *__restrict__ perm,
@@ -1175,14 +1307,19 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   PairQ<R> *const q = s_q[wv];
   R *const fr_slot = s_fr[wv];
 
-  // The first tile of each wave is its global wave index (the grid holds
-  // the waves that fit on the device at once); later tiles come from the
-  // shared counter, or round robin when the tiles are short.
+  // Schedules (the grid holds the waves that fit on the device at once):
+  //  tlen == 0: balanced runs, wave w takes tiles [wstart[w], wstart[w+1]);
+  //  tlen == kTileEntries: the first tile of a wave is its global wave
+  //    index, later tiles come from the shared counter;
+  //  shorter tiles (a small system split finely): round robin, since a
+  //    counter hit once per short tile would serialize the waves.
   const int nwaves = gridDim.x * W;
-  int t = blockIdx.x * W + wv;
+  const int gw = blockIdx.x * W + wv;
+  int t = tlen == 0 ? wstart[gw] : gw;
+  const int tstop = tlen == 0 ? wstart[gw + 1] : ntiles;
   while (true) {
-    if (t >= ntiles) break;
-    const int2 tile = tiles[t];
+    if (t >= tstop) break;
+    const int4 tile = tiles[t];
     const int S = __builtin_amdgcn_readfirstlane(tile.x);
     const int e0 = __builtin_amdgcn_readfirstlane(tile.y);
     const float4 oS = sorg[S];
@@ -1232,7 +1369,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
       qhead += cnt;
     };
 
-    const int eend = __builtin_amdgcn_readfirstlane(min(e0 + tlen, count[S]));
+    const int eend = __builtin_amdgcn_readfirstlane(tile.z);
     for (int eg = e0; eg < eend; eg += kGroup) {
       const int ng = eend - eg < kGroup ? eend - eg : kGroup;
       int cjv[kGroup], emv[kGroup], mov[kGroup], shv[kGroup], tyj0[kGroup];
@@ -1277,6 +1414,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
 
       // ---- phase A: membership, queue, full batches
       R qel = 0, qec = 0;
+      unsigned used = 0;  // bit g * 8 + c: the cluster pair has a member
 #pragma unroll
       for (int g = 0; g < kGroup; ++g) {
         const int sj = cjv[g] * kClusterSize + tj;
@@ -1316,6 +1454,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
             hsum += pair_hash(ai < aj ? ai : aj, ai < aj ? aj : ai);
           }
           if (bm) {
+            used |= 1u << (g * 8 + c);
             if (in) {
               const unsigned below =
                   __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
@@ -1339,7 +1478,9 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
 #pragma unroll
       for (int g = 0; g < kGroup; ++g) {
         if (g >= ng) break;
-        const int em = emv[g];
+        // cluster pairs without members add nothing (their slots are zero)
+        const int em = (used >> (g * 8)) & 0xff;
+        if (em == 0) continue;
         const int sj = cjv[g] * kClusterSize + tj;
         R fj0 = 0, fj1 = 0, fj2 = 0;
 #pragma unroll
@@ -1403,9 +1544,9 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
       }
     }
     __builtin_amdgcn_wave_barrier();
-    if (tlen < kTileEntries) {
-      // short tiles (a small system split finely): round robin, since a
-      // counter hit once per short tile would serialize the waves
+    if (tlen == 0) {
+      ++t;
+    } else if (tlen < kTileEntries) {
       t += nwaves;
     } else {
       int tn = 0;

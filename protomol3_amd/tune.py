@@ -7,15 +7,16 @@ short trajectory from the same state for every candidate:
   cluster-pair list less often but makes the pair kernel test more candidate
   pairs.
 * tune_launch: the launch shapes of one system (pmx.TUNE_* knobs), one knob
-  at a time with the others held at their best so far: the pair-kernel tile
-  target (how finely a small system's list is cut so that every CU gets
-  work), the pair-kernel blocks per CU (fewer leave room for the PME chain
-  on the second stream) and the FFT lines per block (the PME grid blocking).
+  at a time with the others at their defaults: the pair-kernel schedule
+  (balanced runs of equal estimated work per wave, or fixed-length tiles
+  from a counter), the pair-kernel blocks per CU (fewer leave room for the
+  PME chain on the second stream) and the FFT lines per block (the PME grid
+  blocking).
 
-Blocks per CU and FFT lines change no sum except for the order of the FP64
-form's atomics (the mixed form accumulates in fixed point); the tile length
-regroups the mixed form's FP32 partial sums of the i-forces (one per tile),
-a change at FP32 rounding (about 1e-7 relative).
+FFT lines change no sum; the schedule and, under the balanced schedule, the
+blocks per CU regroup the mixed form's FP32 partial sums of the i-forces
+(one per tile), a change at FP32 rounding (about 1e-7 relative); the FP64
+form's atomics may add in another order.
 """
 from __future__ import annotations
 
@@ -24,7 +25,7 @@ import time
 from . import pmx
 
 LAUNCH_CANDIDATES = (
-    (pmx.TUNE_TILES_PER_CU, (16, 8, 32, 0)),
+    (pmx.TUNE_PAIR_SCHEDULE, (0, 1)),
     (pmx.TUNE_PAIR_BLOCKS_PER_CU, (0, 6, 4)),
     (pmx.TUNE_FFT_LINES, (16, 8, 4)),
 )

@@ -150,11 +150,13 @@ def test_gpu_md_graph_replay_matches_kernel_launches(ctx, precision):
 @pytest.mark.parametrize("precision", [pmx.MIXED, pmx.FP64])
 def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
     """The autotuner's launch shapes change how the work is cut, not what is
-    computed.  Pair blocks per CU and FFT lines per block leave every sum as
-    it was: a run whose shapes change between its parts follows the default
-    run (the mixed form to the last bit: fixed-point accumulation).  The tile
-    length regroups the FP32 partial sums of the i-forces (one per tile), so
-    forces at one configuration agree to FP32 rounding in the mixed form."""
+    computed.  Under the counter schedule, pair blocks per CU and FFT lines
+    per block leave every sum as it was: a run whose shapes change between
+    its parts follows the unchanged run (the mixed form to the last bit:
+    fixed-point accumulation).  The schedule, the tile length and (balanced
+    schedule) the blocks per CU regroup the FP32 partial sums of the
+    i-forces (one per tile), so forces at one configuration agree to FP32
+    rounding in the mixed form."""
     s = systems.water_box(seed=32).with_velocities(300.0, seed=9)
     spec = pmx.ForceSpec(5.0, lj=(pmx.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 30, 24), order=4, alpha=0.32))
     shapes = [{}, {pmx.TUNE_FFT_LINES: 1}, {pmx.TUNE_PAIR_BLOCKS_PER_CU: 1, pmx.TUNE_FFT_LINES: 64},
@@ -163,6 +165,7 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
     for tuned in (False, True):
         sy = pmx.System(ctx, s, spec, precision)
         try:
+            sy.set_tuning(pmx.TUNE_PAIR_SCHEDULE, 1)
             sy.md_init(s, 1.0, skin=0.5, graphs=tuned)
             sy.md_set_state(s.pos, s.vel)
             states = []
@@ -188,15 +191,18 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
     try:
         f0, ec0, elj0 = sy.evaluate(x)
         ftol = 1e-6 if precision == pmx.MIXED else 1e-12
-        for tiles in (0, 8, 64):
-            sy.set_tuning(pmx.TUNE_TILES_PER_CU, tiles)
+        cases = [((pmx.TUNE_PAIR_BLOCKS_PER_CU, 1),), ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 0), (pmx.TUNE_PAIR_SCHEDULE, 1))]
+        cases += [((pmx.TUNE_TILES_PER_CU, t),) for t in (0, 8, 64)]
+        for case in cases:
+            for k, v in case:
+                sy.set_tuning(k, v)
             f, ec, elj = sy.evaluate(x)
             rel = np.sqrt(np.mean(np.sum((f - f0) ** 2, 1)) / np.mean(np.sum(f0 ** 2, 1)))
-            print(f"prec={precision} tiles_per_cu={tiles} rms_rel {rel:.3e}", flush=True)
+            print(f"prec={precision} {case} rms_rel {rel:.3e}", flush=True)
             assert rel <= ftol
             assert abs(ec - ec0) <= 1e-9 * abs(ec0) and abs(elj - elj0) <= 1e-9 * max(1.0, abs(elj0))
         for knob, bad in ((pmx.TUNE_FFT_LINES, 0), (pmx.TUNE_TILES_PER_CU, 65), (pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),
-                          (99, 1)):
+                          (pmx.TUNE_PAIR_SCHEDULE, 2), (99, 1)):
             with pytest.raises(pmx.PmxError):
                 sy.set_tuning(knob, bad)
     finally:
