@@ -156,17 +156,27 @@ int pmx_sys_time_pair(pmx_sys *sys, int reps, float *ms_per_launch);
  *                        0: always 16 entries per tile);
  * PMX_TUNE_FFT_LINES     most lines per FFT block (1..64, default 16; the
  *                        LDS limit of 24 KiB per block also applies);
- * PMX_TUNE_PAIR_BLOCKS_PER_CU  pair-kernel blocks per CU (0: as many as fit;
- *                        fewer leave room for the PME chain beside it);
+ * PMX_TUNE_PAIR_BLOCKS_PER_CU  pair-kernel blocks per CU (0: as many as fit,
+ *                        -k: k fewer than fit, at least 1; fewer leave room
+ *                        for the PME chain beside it);
  * PMX_TUNE_PAIR_SCHEDULE 0 (default): the list is cut into one run of equal
  *                        estimated work per resident wave; 1: tiles of up to
  *                        16 entries taken from a counter (or round robin when
- *                        TILES_PER_CU asks for shorter tiles).
+ *                        TILES_PER_CU asks for shorter tiles);
+ * PMX_TUNE_PAIR_RUNS     balanced runs per resident wave (1..4, default 1:
+ *                        fully static); each wave takes one run, the rest
+ *                        are taken in order from one device-wide counter.
  * The resident state is kept.  Blocks per CU and FFT lines do not change
  * any sum (beyond the order of the FP64 form's atomics); the tile length
  * regroups the FP32 partial sums of the mixed form's i-forces (one per tile,
  * about 1e-7 relative). */
-enum { PMX_TUNE_TILES_PER_CU = 1, PMX_TUNE_FFT_LINES = 2, PMX_TUNE_PAIR_BLOCKS_PER_CU = 3, PMX_TUNE_PAIR_SCHEDULE = 4 };
+enum {
+  PMX_TUNE_TILES_PER_CU = 1,
+  PMX_TUNE_FFT_LINES = 2,
+  PMX_TUNE_PAIR_BLOCKS_PER_CU = 3,
+  PMX_TUNE_PAIR_SCHEDULE = 4,
+  PMX_TUNE_PAIR_RUNS = 5
+};
 int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value);
 
 /* ---------------------------------------------------------------------

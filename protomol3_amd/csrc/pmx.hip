@@ -74,6 +74,7 @@ struct Sys {
   int tune_fft_lines = 16;     // most lines per FFT block
   int tune_pair_blocks = 0;    // pair-kernel blocks per CU (0: as many as fit)
   int tune_schedule = 0;       // 0: balanced runs per wave; 1: fixed-length tiles (counter / round robin)
+  int tune_runs = 1;           // balanced runs per resident wave (1: static, more: the rest from a counter)
   int pair_grid = 0;           // blocks of the pair-kernel launch
   long long nbins = 0;
   int nscanblocks = 0;
@@ -82,7 +83,7 @@ struct Sys {
   // device buffers
   std::vector<DevBuf *> all;
   DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
-      perm2, sc_info, sc_col, colsc, sctr, shalf, sorg, ctr, half, entries, count, lcost, tep, tcp, tpos, wstart, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
+      perm2, sc_info, sc_col, colsc, sctr, shalf, sorg, ctr, half, entries, count, ecost, lcost, tep, tcp, tpos, wstart, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
       pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec, spec2,
       greal, theta, twx, twy, twz, border, nborder, border_acc;
   int maxborder = 65536;
@@ -447,6 +448,7 @@ int setup(Sys *s) {
   }
   lp.skin = (float)s->md.skin;
   lp.rlist2 = (float)((rcmax + margin + s->md.skin) * (rcmax + margin + s->md.skin));
+  lp.rc = (float)rcmax;
   lp.ncl = s->ncl;
   lp.nsuper = s->nsuper;
   {
@@ -494,12 +496,13 @@ int setup(Sys *s) {
       (rc = A(s->ctr, 16 * (size_t)s->ncl)) ||
       (rc = A(s->half, 16 * (size_t)s->ncl)) || (rc = A(s->entries, 16 * (size_t)s->nsuper * s->cap)) ||
       (rc = A(s->count, 4 * (size_t)s->nsuper)) || (rc = A(s->lcost, 4 * (size_t)s->nsuper)) ||
+      (rc = A(s->ecost, 4 * (size_t)s->nsuper * s->cap)) ||
       (rc = A(s->tep, 4 * ((size_t)s->nsuper + 1))) || (rc = A(s->tcp, 8 * ((size_t)s->nsuper + 1))) ||
-      (rc = A(s->tpos, 4 * ((size_t)kMaxPairWavesPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
-      (rc = A(s->wstart, 4 * ((size_t)kMaxPairWavesPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
+      (rc = A(s->tpos, 4 * ((size_t)kMaxPairRunsPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
+      (rc = A(s->wstart, 4 * ((size_t)kMaxPairRunsPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
       (rc = A(s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
       (rc = A(s->nmaskwork, 8)) || (rc = A(s->masks, 16 * 8 * (size_t)s->maxmaskwork)) ||
-      (rc = A(s->overflow, 8)) || (rc = A(s->tiles, 16 * (size_t)s->ntiles_cap)) || (rc = A(s->ntiles, 8)) ||
+      (rc = A(s->overflow, 8)) || (rc = A(s->tiles, 16 * (size_t)s->ntiles_cap)) || (rc = A(s->ntiles, 16)) ||
       (rc = A(s->tilectr, 8)) || (rc = A(s->ljtab, 8 * s->lj.size())) || (rc = A(s->facc, 24 * NP)) ||
       (rc = A(s->fout, 24 * N)) || (rc = A(s->pair_e, 16 * (size_t)s->ntiles_cap)) ||
       (rc = A(s->npairs, 8 * (size_t)s->ntiles_cap)) || (rc = A(s->hash, 8 * (size_t)s->ntiles_cap)) ||
@@ -670,7 +673,9 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
         per_cu = 1;
       per_cu = std::min(per_cu, kMaxPairWavesPerCu / (kPairBlock / kWave));
     }
-    const int bpc = s->tune_pair_blocks > 0 ? std::min(s->tune_pair_blocks, per_cu) : per_cu;
+    // knob > 0: blocks per CU; 0: as many as fit; < 0: that many fewer
+    const int bpc = s->tune_pair_blocks > 0 ? std::min(s->tune_pair_blocks, per_cu)
+                                            : std::max(1, per_cu + s->tune_pair_blocks);
     s->pair_grid = std::max(1, s->ctx->cus * bpc);
     return;
   }
@@ -681,7 +686,7 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
                 P<unsigned long long>(s->border_acc)};
     hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, stats ? 1 : 0,
                        P<int4>(s->tiles), P<int>(s->ntiles), P<int>(s->wstart), P<int>(s->tilectr), P<int4>(s->entries),
-                       s->cap, P<unsigned long long>(s->masks), P<R4>(s->xs),
+                       s->cap, P<unsigned short>(s->masks), P<R4>(s->xs),
                        P<float4>(s->sorg), P<int>(s->ts), P<int>(s->permp), P<double>(s->ljtab), facc, out);
     return;
   }
@@ -857,7 +862,8 @@ void timing_begin_call(Sys *s, bool timing) {
 void make_tiles(Sys *s, hipStream_t st) {
   if (s->tune_schedule == 0)
     hipLaunchKernelGGL(k_make_tiles_balanced, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count),
-                       P<int>(s->lcost), s->pair_grid * (kPairBlock / kWave), P<int>(s->tep), P<long long>(s->tcp),
+                       P<int>(s->lcost), P<int>(s->ecost), s->cap,
+                       s->pair_grid * (kPairBlock / kWave) * s->tune_runs, P<int>(s->tep), P<long long>(s->tcp),
                        P<int>(s->tpos), P<int>(s->wstart), P<int4>(s->tiles), P<int>(s->ntiles), s->ntiles_cap);
   else
     hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int4>(s->tiles),
@@ -902,7 +908,7 @@ int rebuild(Sys *s, bool timing) {
   if (timing) hipEventRecord(s->ev[1], st);
   hipLaunchKernelGGL(k_build_list, dim3(s->nsuper), dim3(kListBlock), 0, st, s->lp, P<int>(s->sc_col),
                      P<int>(s->colsc), P<float4>(s->sctr), P<float4>(s->shalf), P<float4>(s->ctr),
-                     P<float4>(s->half), P<int4>(s->entries), P<int>(s->count), P<int>(s->lcost),
+                     P<float4>(s->half), P<int4>(s->entries), P<int>(s->count), P<int>(s->ecost), P<int>(s->lcost),
                      P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork, P<int>(s->overflow));
   if (!s->etab.empty())
     hipLaunchKernelGGL(k_mark_exclusions, dim3(nblk((long long)s->etab.size(), 256)), dim3(256), 0, st,
@@ -911,7 +917,7 @@ int rebuild(Sys *s, bool timing) {
                        P<int>(s->overflow));
   hipLaunchKernelGGL(k_build_masks, dim3(s->ctx->cus * 4), dim3(256), 0, st, P<int>(s->maskwork),
                      P<int>(s->nmaskwork), s->maxmaskwork, P<int4>(s->entries), s->cap, P<int>(s->permp),
-                     P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned long long>(s->masks));
+                     P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned short>(s->masks));
   make_tiles(s, st);
   s->list_valid = true;
   if (timing) hipEventRecord(s->ev[2], st);
@@ -1054,7 +1060,8 @@ int grow_and_retry(Sys *s) {
   s->maxborder *= 2;
   if (int rc0 = dalloc(s, s->border, 8 * (size_t)s->maxborder)) return rc0;
   int rc;
-  if ((rc = dalloc(s, s->entries, 16 * (size_t)s->nsuper * s->cap)) || (rc = dalloc(s, s->tiles, 16 * (size_t)s->ntiles_cap)) ||
+  if ((rc = dalloc(s, s->entries, 16 * (size_t)s->nsuper * s->cap)) || (rc = dalloc(s, s->ecost, 4 * (size_t)s->nsuper * s->cap)) ||
+      (rc = dalloc(s, s->tiles, 16 * (size_t)s->ntiles_cap)) ||
       (rc = dalloc(s, s->pair_e, 16 * (size_t)s->ntiles_cap)) || (rc = dalloc(s, s->npairs, 8 * (size_t)s->ntiles_cap)) ||
       (rc = dalloc(s, s->hash, 8 * (size_t)s->ntiles_cap)) || (rc = dalloc(s, s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
       (rc = dalloc(s, s->masks, 16 * 8 * (size_t)s->maxmaskwork)))
@@ -1324,12 +1331,17 @@ int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value) {
       s->tune_fft_lines = value;
       break;
     case PMX_TUNE_PAIR_BLOCKS_PER_CU:
-      if (value < 0 || value > 64) return fail(s->ctx, PMX_ERR_ARG, "pair blocks per CU must be 0..64");
+      if (value < -7 || value > 64) return fail(s->ctx, PMX_ERR_ARG, "pair blocks per CU must be -7..64");
       s->tune_pair_blocks = value;
       break;
     case PMX_TUNE_PAIR_SCHEDULE:
       if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "pair schedule must be 0 or 1");
       s->tune_schedule = value;
+      break;
+    case PMX_TUNE_PAIR_RUNS:
+      if (value < 1 || value > kMaxPairRunsPerCu / kMaxPairWavesPerCu)
+        return fail(s->ctx, PMX_ERR_ARG, "pair runs per wave must be 1..4");
+      s->tune_runs = value;
       break;
     default:
       return fail(s->ctx, PMX_ERR_ARG, "unknown tuning knob");

@@ -10,6 +10,7 @@ constexpr int kClusterSize = 8;    // atoms per cluster (one 8-lane group of a w
 constexpr int kSuperSize = 64;     // atoms per supercluster (8 clusters, one wave)
 constexpr int kTileEntries = 16;   // list entries per work tile
 constexpr int kMaxPairWavesPerCu = 32;  // resident pair-kernel waves per CU, at most
+constexpr int kMaxPairRunsPerCu = 4 * kMaxPairWavesPerCu;  // balanced runs: up to 4 per wave
 constexpr int kWave = 64;
 constexpr int kPairBlock = 256;    // 4 waves
 constexpr double kForceScale = 4294967296.0;      // 2^32: int64 fixed-point forces

@@ -480,6 +480,20 @@ __device__ inline float bbox_dist2(float4 ca, float4 ha, float4 cb, float4 hb, c
   return d2;
 }
 
+// Largest squared distance between points of two boxes (minimum image of
+// the centers).
+__device__ inline float bbox_far2(float4 ca, float4 ha, float4 cb, float4 hb, const float *L) {
+  float d2 = 0.f;
+  float dc[3] = {fabsf(ca.x - cb.x), fabsf(ca.y - cb.y), fabsf(ca.z - cb.z)};
+  float hs[3] = {ha.x + hb.x, ha.y + hb.y, ha.z + hb.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float g = fminf(dc[k], L[k] - dc[k]) + hs[k];
+    d2 += g * g;
+  }
+  return d2;
+}
+
 // Image shift of a cluster pair: per axis s in {-1, 0, 1} with the j-center
 // + s L nearest the i-center, coded (sx+1) | (sy+1) << 2 | (sz+1) << 4.
 // bad is set when j-atoms shifted by s could be more than L/2 from an
@@ -508,6 +522,7 @@ struct ListParams {
   float skin;        // MD pair-list buffer (0 outside MD)
   float rlist2;      // (rc_max + margin)^2
   float rlist2_sc;   // supercluster pruning radius^2 (rlist plus slack)
+  float rc;          // largest cutoff (work estimate of an entry)
   int ncl;           // clusters
   int nsuper;
   int cap;           // entries per supercluster
@@ -549,16 +564,24 @@ __device__ inline int col_runs(int c, int k, int n, int r[2][2]) {
 // prefix count keeps the entries sorted by cj.  Diagonal entries and entries
 // with a partial cluster get a mask work slot (explicit pair masks).
 constexpr int kListBlock = 256;
-// Work estimate of a list entry in units of one cluster pair (64 candidate
-// pairs): its loads, j-force sums and atomics cost about one more.
-constexpr int kEntryCost = 1;
+// Work estimate of a list entry for the balanced pair schedule, in units of
+// about 35 ns of one pair-kernel wave (fitted to per-wave times): kCpCost per
+// cluster pair (its 64 candidates: membership and force passes), kMemCost
+// times the estimated member fraction of the cluster pair (evaluation; the
+// fraction is linear in the cutoff between the boxes' nearest and farthest
+// distances), kEntryCost per entry (its dependent loads, the last partial
+// batch, j-force sums and atomics), kTileCost on the first entry of a
+// supercluster (a wave whose run crosses into it loads its i-atoms and adds
+// their forces once more).
+constexpr int kCpCost = 8, kMemCost = 9, kEntryCost = 14, kTileCost = 80;
 __device__ inline void list_append(const ListParams &q, int S, const float4 (&ic)[8], const float4 (&ih)[8], int cj,
                                    float4 jc, float4 jh, int4 *__restrict__ entries, int *__restrict__ maskwork,
                                    int *__restrict__ nmaskwork, int maxmaskwork, int *__restrict__ overflow,
-                                   int *s_cnt, int &total, int &cost) {
+                                   int *s_cnt, int *s_cost, int *__restrict__ ecost, int &total, int &ctotal) {
   constexpr int NW = kListBlock / kWave;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t imask = 0;
+  int mc = kEntryCost;  // work estimate of this entry
   bool need = false;
   int shift = -1;         // image shift code of the entry (see entry_shift)
   bool per_pair = false;  // no single shift is the minimum image for every pair
@@ -571,6 +594,11 @@ __device__ inline void list_append(const ListParams &q, int S, const float4 (&ic
       const float d2 = bbox_dist2(ic[c], ih[c], jc, jh, q.L);
       if (d2 <= q.rlist2) {
         imask |= 1u << c;
+        {
+          const float dn = sqrtf(d2), df = sqrtf(bbox_far2(ic[c], ih[c], jc, jh, q.L));
+          const float frac = fminf(1.f, fmaxf(0.f, (q.rc - dn) / fmaxf(df - dn, 1e-3f)));
+          mc += kCpCost + (int)(kMemCost * frac + 0.5f);
+        }
         const bool ipartial = ic[c].w < (float)kClusterSize;
         if (ci == cj || ipartial || jpartial) need = true;
         const int code = entry_shift(ic[c], ih[c], jc, jh, q.L, q.Linv, q.skin, per_pair);
@@ -580,14 +608,24 @@ __device__ inline void list_append(const ListParams &q, int S, const float4 (&ic
     }
   }
   const unsigned long long act = __ballot(imask != 0);
+  // inclusive prefix of the work over this call's entries (lanes in order)
+  int wc = imask != 0 ? mc : 0;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int v = __shfl_up(wc, off);
+    if (lane >= off) wc += v;
+  }
   if (lane == 0) s_cnt[wv] = __popcll(act);
+  if (lane == kWave - 1) s_cost[wv] = wc;
   __syncthreads();
-  int before = 0, all = 0;
+  int before = 0, all = 0, cbefore = 0, call = 0;
 #pragma unroll
   for (int w = 0; w < NW; ++w) {
-    const int c = s_cnt[w];
+    const int c = s_cnt[w], k = s_cost[w];
     before += w < wv ? c : 0;
     all += c;
+    cbefore += w < wv ? k : 0;
+    call += k;
   }
   __syncthreads();
   const int e = total + before + __popcll(act & ((1ull << lane) - 1ull));
@@ -612,12 +650,13 @@ __device__ inline void list_append(const ListParams &q, int S, const float4 (&ic
         }
       }
       entries[(size_t)S * q.cap + e] = make_int4(cj, (int)imask, moff, per_pair ? kShiftPerPair : shift);
-      cost += kEntryCost + __popc(imask);
+      ecost[(size_t)S * q.cap + e] = kTileCost + ctotal + cbefore + wc;  // the tile cost counts from entry 0
     } else {
       atomicExch(overflow, 1);
     }
   }
   total += all;
+  ctotal += call;
 }
 
 // Cluster-pair list of supercluster S (one block each).  Superclusters are
@@ -636,12 +675,13 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
                                                            const float4 *__restrict__ ctr,
                                                            const float4 *__restrict__ half,
                                                            int4 *__restrict__ entries, int *__restrict__ count,
-                                                           int *__restrict__ cost_out,
+                                                           int *__restrict__ ecost, int *__restrict__ cost_out,
                                                            int *__restrict__ maskwork, int *__restrict__ nmaskwork,
                                                            int maxmaskwork, int *__restrict__ overflow) {
   constexpr int NW = kListBlock / kWave;
   constexpr int kScPerBatch = kListBlock / kClusterSize;
   __shared__ int s_cnt[NW];
+  __shared__ int s_cost[NW];
   __shared__ int s_sc[kListBlock];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int S = blockIdx.x;
@@ -663,8 +703,8 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
   int xr[2][2], yr[2][2];
   const int cx = col / q.ncy, cy = col - cx * q.ncy;
   const int nxr = col_runs(cx, q.kx, q.ncx, xr), nyr = col_runs(cy, q.ky, q.ncy, yr);
-  int total = 0;  // block-uniform
-  int cost = 0;   // this thread's entries' work (kEntryCost + cluster pairs each)
+  int total = 0;   // block-uniform
+  int ctotal = 0;  // block-uniform work of the entries so far
   for (int rx = 0; rx < nxr; ++rx)
     for (int x = xr[rx][0]; x <= xr[rx][1]; ++x)
       for (int ry = 0; ry < nyr; ++ry) {
@@ -698,20 +738,16 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
               jc = ctr[cj];
               jh = half[cj];
             }
-            list_append(q, S, ic, ih, cj, jc, jh, entries, maskwork, nmaskwork, maxmaskwork, overflow, s_cnt, total,
-                        cost);
+            list_append(q, S, ic, ih, cj, jc, jh, entries, maskwork, nmaskwork, maxmaskwork, overflow, s_cnt, s_cost,
+                        ecost, total, ctotal);
           }
           __syncthreads();  // s_sc is rewritten by the next range
         }
       }
-  cost = wave_sum(cost);
-  if (lane == 0) s_cnt[wv] = cost;
-  __syncthreads();
   if (threadIdx.x == 0) {
-    count[S] = total < q.cap ? total : q.cap;
-    int c = 0;
-    for (int w = 0; w < NW; ++w) c += s_cnt[w];
-    cost_out[S] = c;
+    const int n = total < q.cap ? total : q.cap;
+    count[S] = n;
+    cost_out[S] = n > 0 ? ecost[(size_t)S * q.cap + n - 1] : 0;
   }
 }
 
@@ -780,14 +816,17 @@ __device__ inline int excl_class(const int *__restrict__ eptr, const int2 *__res
   return 0;
 }
 
-// One wave per entry needing explicit masks: bit (tj*8 + ti) of mask[ci] set
-// when i = slot 8ci+ti, j = slot 8cj+tj are real atoms (perm >= 0), i precedes j within
-// a shared cluster, and the pair is not FULL-excluded; modmask marks MODIFIED.
+// One wave per entry needing explicit masks.  Mask slot w holds one 16-bit
+// word per lane l = tj*8 + ti of the pair kernel: bit c set when i = slot
+// 8ci+ti (ci = 8S + c, c in the entry's mask) and j = slot 8cj+tj are real
+// atoms (perm >= 0), i precedes j within a shared cluster, and the pair is
+// not FULL-excluded; bit 8 + c marks MODIFIED.  Each lane of the pair kernel
+// loads its own word once per entry.  The entry's mask field becomes w.
 __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ maskwork, const int *__restrict__ nmaskwork,
                                                      int maxmaskwork, int4 *__restrict__ entries, int cap,
                                                      const int *__restrict__ perm, const int *__restrict__ eptr,
                                                      const int2 *__restrict__ elist,
-                                                     unsigned long long *__restrict__ masks) {
+                                                     unsigned short *__restrict__ masks) {
   const int lane = threadIdx.x & 63;
   int nw = *nmaskwork;
   if (nw > maxmaskwork) nw = maxmaskwork;
@@ -832,16 +871,15 @@ __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ mas
         }
       }
     }
+    unsigned bits = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      const unsigned long long m = __ballot(ok[c] && cls[c] != 1);
-      const unsigned long long mm = __ballot(ok[c] && cls[c] == 2);
-      if (lane == 0) {
-        masks[2 * ((size_t)w * 8 + c)] = (ent.y & (1 << c)) ? m : 0ull;
-        masks[2 * ((size_t)w * 8 + c) + 1] = (ent.y & (1 << c)) ? mm : 0ull;
-      }
+      // ok[c] implies c is in the entry's mask
+      if (ok[c] && cls[c] != 1) bits |= 1u << c;
+      if (ok[c] && cls[c] == 2) bits |= 1u << (8 + c);
     }
-    if (lane == 0) entries[eidx].z = w * 8;
+    masks[(size_t)w * kWave + lane] = (unsigned short)bits;
+    if (lane == 0) entries[eidx].z = w;
   }
 }
 
@@ -912,16 +950,17 @@ __device__ inline int last_le(const T *__restrict__ a, int n, T v) {
   return lo;
 }
 
-// Balanced static schedule: the list entries of all superclusters, in
-// supercluster order, are cut into nwaves contiguous runs of equal estimated
-// work (the per-supercluster work of k_build_list, spread evenly over its
-// entries), one run per resident wave of the pair kernel, cuts at even entry
-// offsets.  A run is split into one tile per supercluster it touches, so
-// there are at most nwaves + nsuper tiles.  Wave w processes tiles
-// [wstart[w], wstart[w + 1]).  Single block of 1024 threads; ep, cp (nsuper
-// + 1 each) and pos (nwaves + 1) are scratch.
+// Balanced schedule: the list entries of all superclusters, in supercluster
+// order, are cut into nruns contiguous runs of equal estimated work (the
+// work estimates of k_build_list: per supercluster, and as an inclusive
+// prefix over its entries), cuts at even entry offsets; nruns is a multiple
+// of the pair kernel's resident waves.  A run is split into one tile per
+// supercluster it touches, so there are at most nruns + nsuper tiles.  Run r
+// is tiles [wstart[r], wstart[r + 1]).  Single block of 1024 threads; ep, cp
+// (nsuper + 1 each) and pos (nruns + 1) are scratch.
 __global__ void k_make_tiles_balanced(int nsuper, const int *__restrict__ count, const int *__restrict__ cost,
-                                      int nwaves, int *__restrict__ ep, long long *__restrict__ cp,
+                                      const int *__restrict__ ecost, int cap,
+                                      int nruns, int *__restrict__ ep, long long *__restrict__ cp,
                                       int *__restrict__ pos, int *__restrict__ wstart, int4 *__restrict__ tiles,
                                       int *__restrict__ ntiles, int ntiles_cap) {
   __shared__ int s_e[1024];
@@ -961,23 +1000,26 @@ __global__ void k_make_tiles_balanced(int nsuper, const int *__restrict__ count,
   const int E = ecarry;
   const long long C = ccarry;
   // run boundaries in entry positions (ep[S] + offset)
-  for (int w = tid; w <= nwaves; w += 1024) {
+  for (int w = tid; w <= nruns; w += 1024) {
     int p = E;
-    if (w < nwaves && C > 0) {
-      const long long b = (long long)w * C / nwaves;  // < C
-      const int S = last_le<long long>(cp, nsuper, b);  // cp[S] <= b < cp[S + 1]: cost[S] > 0
-      const long long off = (b - cp[S]) * (long long)count[S] / (long long)max(cost[S], 1);
-      p = ep[S] + ((int)min(off, (long long)count[S]) & ~1);
+    if (w < nruns && C > 0) {
+      const long long b = (long long)w * C / nruns;  // < C
+      const int S = last_le<long long>(cp, nsuper, b);  // cp[S] <= b < cp[S + 1]: count[S] > 0
+      // entries of S whose inclusive work prefix is at most b - cp[S]
+      const int tgt = (int)(b - cp[S]);
+      const int *ec = ecost + (size_t)S * cap;
+      const int n = ec[0] > tgt ? 0 : last_le<int>(ec, count[S], tgt) + 1;
+      p = ep[S] + (n & ~1);
     }
     pos[w] = p;
   }
   __syncthreads();
   // tiles per run (superclusters touched), then their exclusive prefix
   int carry = 0;
-  for (int base = 0; base < nwaves; base += 1024) {
+  for (int base = 0; base < nruns; base += 1024) {
     const int w = base + tid;
     int n = 0;
-    if (w < nwaves) {
+    if (w < nruns) {
       const int p0 = min(pos[w], E), p1 = min(max(pos[w + 1], p0), E);
       if (p1 > p0) n = last_le<int>(ep, nsuper, p1 - 1) - last_le<int>(ep, nsuper, p0) + 1;
     }
@@ -989,17 +1031,18 @@ __global__ void k_make_tiles_balanced(int nsuper, const int *__restrict__ count,
       s_e[tid] += v;
       __syncthreads();
     }
-    if (w < nwaves) wstart[w] = carry + s_e[tid] - n;
+    if (w < nruns) wstart[w] = carry + s_e[tid] - n;
     carry += s_e[1023];
     __syncthreads();
   }
   if (tid == 0) {
-    wstart[nwaves] = min(carry, ntiles_cap);
+    wstart[nruns] = min(carry, ntiles_cap);
     ntiles[0] = min(carry, ntiles_cap);
     ntiles[1] = 0;
+    ntiles[2] = nruns;
   }
   __syncthreads();
-  for (int w = tid; w < nwaves; w += 1024) {
+  for (int w = tid; w < nruns; w += 1024) {
     const int p0 = min(pos[w], E), p1 = min(max(pos[w + 1], p0), E);
     if (p1 <= p0) continue;
     const int S0 = last_le<int>(ep, nsuper, p0), S1 = last_le<int>(ep, nsuper, p1 - 1);
@@ -1264,7 +1307,7 @@ template <bool MIXED, int COUL, int LJSW, int CSW>
 __global__ void __launch_bounds__(kPairBlock) k_pair(
     PairParams pp, int stats, const int4 *__restrict__ tiles, const int *__restrict__ ntiles_p,
     const int *__restrict__ wstart, int *__restrict__ tile_counter, const int4 *__restrict__ entries, int cap,
-    const unsigned long long *__restrict__ masks,
+    const unsigned short *__restrict__ masks,
     const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
     const float4 *__restrict__ sorg, const int *__restrict__ ts, const int *__restrict__ perm,
     const double *__restrict__ ljtab, typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
@@ -1308,17 +1351,30 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   R *const fr_slot = s_fr[wv];
 
   // Schedules (the grid holds the waves that fit on the device at once):
-  //  tlen == 0: balanced runs, wave w takes tiles [wstart[w], wstart[w+1]);
+  //  tlen == 0: balanced runs, run r is tiles [wstart[r], wstart[r+1]);
+  //    wave w takes run w, then runs from the counter;
   //  tlen == kTileEntries: the first tile of a wave is its global wave
   //    index, later tiles come from the shared counter;
   //  shorter tiles (a small system split finely): round robin, since a
   //    counter hit once per short tile would serialize the waves.
   const int nwaves = gridDim.x * W;
   const int gw = blockIdx.x * W + wv;
+  const int nruns = ntiles_p[2];  // balanced: runs (wave w starts with run w)
   int t = tlen == 0 ? wstart[gw] : gw;
-  const int tstop = tlen == 0 ? wstart[gw + 1] : ntiles;
+  int tstop = tlen == 0 ? wstart[gw + 1] : ntiles;
   while (true) {
-    if (t >= tstop) break;
+    if (t >= tstop) {
+      // balanced runs beyond one per wave are taken from the counter in
+      // order, so that waves that started late or ran slow take fewer
+      if (tlen != 0 || nruns <= nwaves) break;
+      int rn = 0;
+      if (lane == 0) rn = nwaves + atomicAdd(tile_counter, 1);
+      const int r = __builtin_amdgcn_readfirstlane(__shfl(rn, 0));
+      if (r >= nruns) break;
+      t = wstart[r];
+      tstop = wstart[r + 1];
+      continue;
+    }
     const int4 tile = tiles[t];
     const int S = __builtin_amdgcn_readfirstlane(tile.x);
     const int e0 = __builtin_amdgcn_readfirstlane(tile.y);
@@ -1373,6 +1429,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
     for (int eg = e0; eg < eend; eg += kGroup) {
       const int ng = eend - eg < kGroup ? eend - eg : kGroup;
       int cjv[kGroup], emv[kGroup], mov[kGroup], shv[kGroup], tyj0[kGroup];
+      unsigned mbv[kGroup];  // this lane's exclusion bits (candidate per i-cluster, MODIFIED << 8)
       R4 xj[kGroup];
 #pragma unroll
       for (int g = 0; g < kGroup; ++g) {
@@ -1400,6 +1457,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
           xj[g].z += oz;
         }
         tyj0[g] = ts[sj];
+        mbv[g] = mov[g] >= 0 ? (unsigned)masks[(size_t)mov[g] * kWave + lane] : 0xffu;
         if (ti == 0) s_j[wv][g * kClusterSize + tj] = JAtom<R>{xj[g].w, tyj0[g] < 0 ? 0 : tyj0[g]};
       }
       // every force slot of the group starts at zero (phase B fills the
@@ -1424,9 +1482,8 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
           mrem &= mrem - 1;
           bool cand = true, modified = false;
           if (mov[g] >= 0) {
-            // exclusion masks: bit l of each word is lane l's candidate
-            cand = __builtin_amdgcn_inverse_ballot_w64(masks[2 * ((size_t)mov[g] + c)]);
-            modified = __builtin_amdgcn_inverse_ballot_w64(masks[2 * ((size_t)mov[g] + c) + 1]);
+            cand = (mbv[g] >> c) & 1u;
+            modified = (mbv[g] >> (8 + c)) & 1u;
           }
           const R4 xi = si[c * 8 + ti];
           R dx, dy, dz;

@@ -8,13 +8,14 @@ short trajectory from the same state for every candidate:
   pairs.
 * tune_launch: the launch shapes of one system (pmx.TUNE_* knobs), one knob
   at a time with the others at their defaults: the pair-kernel schedule
-  (balanced runs of equal estimated work per wave, or fixed-length tiles
-  from a counter), the pair-kernel blocks per CU (fewer leave room for the
-  PME chain on the second stream) and the FFT lines per block (the PME grid
+  (balanced runs of equal estimated work, or fixed-length tiles from a
+  counter), the balanced runs per wave (one static, the rest from a
+  counter), the pair-kernel blocks per CU (fewer leave room for the PME
+  chain on the second stream) and the FFT lines per block (the PME grid
   blocking).
 
 FFT lines change no sum; the schedule and, under the balanced schedule, the
-blocks per CU regroup the mixed form's FP32 partial sums of the i-forces
+runs and blocks per CU regroup the mixed form's FP32 partial sums of the i-forces
 (one per tile), a change at FP32 rounding (about 1e-7 relative); the FP64
 form's atomics may add in another order.
 """
@@ -26,7 +27,8 @@ from . import pmx
 
 LAUNCH_CANDIDATES = (
     (pmx.TUNE_PAIR_SCHEDULE, (0, 1)),
-    (pmx.TUNE_PAIR_BLOCKS_PER_CU, (0, 6, 4)),
+    (pmx.TUNE_PAIR_RUNS, (1, 2)),
+    (pmx.TUNE_PAIR_BLOCKS_PER_CU, (0, -1, -2)),
     (pmx.TUNE_FFT_LINES, (16, 8, 4)),
 )
 

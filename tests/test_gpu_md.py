@@ -191,7 +191,9 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
     try:
         f0, ec0, elj0 = sy.evaluate(x)
         ftol = 1e-6 if precision == pmx.MIXED else 1e-12
-        cases = [((pmx.TUNE_PAIR_BLOCKS_PER_CU, 1),), ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 0), (pmx.TUNE_PAIR_SCHEDULE, 1))]
+        cases = [((pmx.TUNE_PAIR_BLOCKS_PER_CU, 1),), ((pmx.TUNE_PAIR_RUNS, 1),), ((pmx.TUNE_PAIR_RUNS, 4),),
+                 ((pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),),
+                 ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 0), (pmx.TUNE_PAIR_SCHEDULE, 1))]
         cases += [((pmx.TUNE_TILES_PER_CU, t),) for t in (0, 8, 64)]
         for case in cases:
             for k, v in case:
@@ -201,8 +203,8 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
             print(f"prec={precision} {case} rms_rel {rel:.3e}", flush=True)
             assert rel <= ftol
             assert abs(ec - ec0) <= 1e-9 * abs(ec0) and abs(elj - elj0) <= 1e-9 * max(1.0, abs(elj0))
-        for knob, bad in ((pmx.TUNE_FFT_LINES, 0), (pmx.TUNE_TILES_PER_CU, 65), (pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),
-                          (pmx.TUNE_PAIR_SCHEDULE, 2), (99, 1)):
+        for knob, bad in ((pmx.TUNE_FFT_LINES, 0), (pmx.TUNE_TILES_PER_CU, 65), (pmx.TUNE_PAIR_BLOCKS_PER_CU, -8),
+                          (pmx.TUNE_PAIR_SCHEDULE, 2), (pmx.TUNE_PAIR_RUNS, 0), (pmx.TUNE_PAIR_RUNS, 5), (99, 1)):
             with pytest.raises(pmx.PmxError):
                 sy.set_tuning(knob, bad)
     finally:
