@@ -83,7 +83,7 @@ struct Sys {
   // device buffers
   std::vector<DevBuf *> all;
   DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
-      perm2, sc_info, sc_col, colsc, sctr, shalf, sorg, ctr, half, entries, count, ecost, lcost, tep, tcp, tpos, wstart, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
+      perm2, sc_info, sc_col, colsc, sctr, shalf, sorg, ctr, half, entries, count, ecost, lcost, tep, tcp, tpos, tposS, wstart, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
       pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec, spec2,
       greal, theta, twx, twy, twz, border, nborder, border_acc;
   int maxborder = 65536;
@@ -499,6 +499,7 @@ int setup(Sys *s) {
       (rc = A(s->ecost, 4 * (size_t)s->nsuper * s->cap)) ||
       (rc = A(s->tep, 4 * ((size_t)s->nsuper + 1))) || (rc = A(s->tcp, 8 * ((size_t)s->nsuper + 1))) ||
       (rc = A(s->tpos, 4 * ((size_t)kMaxPairRunsPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
+      (rc = A(s->tposS, 4 * ((size_t)kMaxPairRunsPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
       (rc = A(s->wstart, 4 * ((size_t)kMaxPairRunsPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
       (rc = A(s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
       (rc = A(s->nmaskwork, 8)) || (rc = A(s->masks, 16 * 8 * (size_t)s->maxmaskwork)) ||
@@ -860,14 +861,19 @@ void timing_begin_call(Sys *s, bool timing) {
 // ForceGroup.cpp:57; the MD path rebuilds when the skin requires it).
 // Work tiles of the current list for the current schedule and pair grid.
 void make_tiles(Sys *s, hipStream_t st) {
-  if (s->tune_schedule == 0)
-    hipLaunchKernelGGL(k_make_tiles_balanced, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count),
-                       P<int>(s->lcost), P<int>(s->ecost), s->cap,
-                       s->pair_grid * (kPairBlock / kWave) * s->tune_runs, P<int>(s->tep), P<long long>(s->tcp),
-                       P<int>(s->tpos), P<int>(s->wstart), P<int4>(s->tiles), P<int>(s->ntiles), s->ntiles_cap);
-  else
+  if (s->tune_schedule == 0) {
+    const int nruns = s->pair_grid * (kPairBlock / kWave) * s->tune_runs;
+    hipLaunchKernelGGL(k_runs_prefix, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int>(s->lcost),
+                       nruns, P<int>(s->tep), P<long long>(s->tcp), P<int>(s->tpos), P<int>(s->tposS));
+    hipLaunchKernelGGL(k_runs_cut, dim3(s->nsuper), dim3(256), 0, st, s->nsuper, P<int>(s->count), P<int>(s->ecost),
+                       s->cap, nruns, P<int>(s->tep), P<long long>(s->tcp), P<int>(s->tpos), P<int>(s->tposS));
+    hipLaunchKernelGGL(k_runs_tiles, dim3(1), dim3(1024), 0, st, P<int>(s->count), nruns, P<int>(s->tep),
+                       P<int>(s->tpos), P<int>(s->tposS), P<int>(s->wstart), P<int4>(s->tiles), P<int>(s->ntiles),
+                       s->ntiles_cap);
+  } else {
     hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int4>(s->tiles),
                        P<int>(s->ntiles), s->ntiles_cap, s->tile_target);
+  }
 }
 
 int rebuild(Sys *s, bool timing) {

@@ -88,6 +88,11 @@ __device__ inline T wave_sum(T v) {
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
   return v;
 }
+__device__ inline int wave_max(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m));
+  return v;
+}
 
 // ---------------------------------------------------------------------------
 // 1. Binning: wrapped coordinates, column x z-bin sort key, reference cell.
@@ -953,26 +958,31 @@ __device__ inline int last_le(const T *__restrict__ a, int n, T v) {
 // Balanced schedule: the list entries of all superclusters, in supercluster
 // order, are cut into nruns contiguous runs of equal estimated work (the
 // work estimates of k_build_list: per supercluster, and as an inclusive
-// prefix over its entries), cuts at even entry offsets; nruns is a multiple
-// of the pair kernel's resident waves.  A run is split into one tile per
-// supercluster it touches, so there are at most nruns + nsuper tiles.  Run r
-// is tiles [wstart[r], wstart[r + 1]).  Single block of 1024 threads; ep, cp
-// (nsuper + 1 each) and pos (nruns + 1) are scratch.
-__global__ void k_make_tiles_balanced(int nsuper, const int *__restrict__ count, const int *__restrict__ cost,
-                                      const int *__restrict__ ecost, int cap,
-                                      int nruns, int *__restrict__ ep, long long *__restrict__ cp,
-                                      int *__restrict__ pos, int *__restrict__ wstart, int4 *__restrict__ tiles,
-                                      int *__restrict__ ntiles, int ntiles_cap) {
+// prefix over its entries); nruns is a multiple of the pair kernel's
+// resident waves.  Run w starts at entry position pos[w] (ep[S] + offset,
+// offsets even) in supercluster posS[w], where the global work prefix first
+// exceeds b_w = floor(w C / nruns), C the total.  A run is split into one
+// tile per supercluster it touches (at most nruns + nsuper tiles); run r is
+// tiles [wstart[r], wstart[r + 1]).  Three passes, none searching:
+//   k_runs_prefix (one block): prefixes ep (entries) and cp (work) over the
+//     superclusters, run boundaries preset to the end;
+//   k_runs_cut (a block per supercluster): each entry places the run
+//     boundaries b_w that fall in its work interval;
+//   k_runs_tiles (one block): tiles per run, their prefix, the tiles.
+__global__ void k_runs_prefix(int nsuper, const int *__restrict__ count, const int *__restrict__ cost, int nruns,
+                              int *__restrict__ ep, long long *__restrict__ cp, int *__restrict__ pos,
+                              int *__restrict__ posS) {
   __shared__ int s_e[1024];
   __shared__ long long s_c[1024];
+  __shared__ int s_last[1024 / kWave];
   const int tid = threadIdx.x;
-  // exclusive prefixes of the entry counts (ep) and of the work (cp)
-  int ecarry = 0;
+  int ecarry = 0, last = -1;
   long long ccarry = 0;
   for (int base = 0; base < nsuper; base += 1024) {
     const int S = base + tid;
     const int c = S < nsuper ? count[S] : 0;
     const long long w = S < nsuper ? (long long)cost[S] : 0;
+    if (c > 0) last = S;
     s_e[tid] = c;
     s_c[tid] = w;
     __syncthreads();
@@ -992,47 +1002,79 @@ __global__ void k_make_tiles_balanced(int nsuper, const int *__restrict__ count,
     ccarry += s_c[1023];
     __syncthreads();
   }
+  last = wave_max(last);
+  if ((tid & 63) == 0) s_last[tid >> 6] = last;
+  __syncthreads();
+  int lastS = -1;
+  for (int k = 0; k < 1024 / kWave; ++k) lastS = max(lastS, s_last[k]);
   if (tid == 0) {
     ep[nsuper] = ecarry;
     cp[nsuper] = ccarry;
   }
-  __syncthreads();
-  const int E = ecarry;
-  const long long C = ccarry;
-  // run boundaries in entry positions (ep[S] + offset)
+  // every boundary starts at the end (k_runs_cut moves those that fall in
+  // the list); run 0 starts at the beginning
   for (int w = tid; w <= nruns; w += 1024) {
-    int p = E;
-    if (w < nruns && C > 0) {
-      const long long b = (long long)w * C / nruns;  // < C
-      const int S = last_le<long long>(cp, nsuper, b);  // cp[S] <= b < cp[S + 1]: count[S] > 0
-      // entries of S whose inclusive work prefix is at most b - cp[S]
-      const int tgt = (int)(b - cp[S]);
-      const int *ec = ecost + (size_t)S * cap;
-      const int n = ec[0] > tgt ? 0 : last_le<int>(ec, count[S], tgt) + 1;
-      p = ep[S] + (n & ~1);
-    }
-    pos[w] = p;
+    pos[w] = w == 0 ? 0 : ecarry;
+    posS[w] = w == 0 ? 0 : lastS + 1;
   }
-  __syncthreads();
-  // tiles per run (superclusters touched), then their exclusive prefix
+}
+
+__global__ void k_runs_cut(int nsuper, const int *__restrict__ count, const int *__restrict__ ecost, int cap,
+                           int nruns, const int *__restrict__ ep, const long long *__restrict__ cp,
+                           int *__restrict__ pos, int *__restrict__ posS) {
+  const int S = blockIdx.x;
+  const int n = count[S];
+  const long long C = cp[nsuper], base = cp[S];
+  if (C <= 0) return;
+  const int *ec = ecost + (size_t)S * cap;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const long long lo = base + (e > 0 ? ec[e - 1] : 0), hi = base + ec[e];
+    // runs w in [1, nruns) with lo <= floor(w C / nruns) < hi
+    long long w0 = (lo * nruns + C - 1) / C, w1 = (hi * nruns + C - 1) / C;
+    w0 = max(w0, 1ll);
+    w1 = min(w1, (long long)nruns);
+    for (long long w = w0; w < w1; ++w) {
+      pos[w] = ep[S] + (e & ~1);
+      posS[w] = S;
+    }
+  }
+}
+
+__global__ void k_runs_tiles(const int *__restrict__ count, int nruns, const int *__restrict__ ep,
+                             const int *__restrict__ pos, const int *__restrict__ posS, int *__restrict__ wstart,
+                             int4 *__restrict__ tiles, int *__restrict__ ntiles, int ntiles_cap) {
+  __shared__ int s_n[1024];
+  const int tid = threadIdx.x;
+  // run w: entries [pos[w], pos[w+1]), superclusters S0 .. S1
+  auto span = [&](int w, int &S0, int &S1) {
+    const int p0 = pos[w], p1 = pos[w + 1];
+    S0 = posS[w];
+    S1 = posS[w + 1] - (p1 == ep[posS[w + 1]] ? 1 : 0);
+    return p1 > p0 ? S1 - S0 + 1 : 0;
+  };
   int carry = 0;
   for (int base = 0; base < nruns; base += 1024) {
     const int w = base + tid;
-    int n = 0;
-    if (w < nruns) {
-      const int p0 = min(pos[w], E), p1 = min(max(pos[w + 1], p0), E);
-      if (p1 > p0) n = last_le<int>(ep, nsuper, p1 - 1) - last_le<int>(ep, nsuper, p0) + 1;
-    }
-    s_e[tid] = n;
+    int S0, S1;
+    const int n = w < nruns ? span(w, S0, S1) : 0;
+    s_n[tid] = n;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-      const int v = tid >= off ? s_e[tid - off] : 0;
+      const int v = tid >= off ? s_n[tid - off] : 0;
       __syncthreads();
-      s_e[tid] += v;
+      s_n[tid] += v;
       __syncthreads();
     }
-    if (w < nruns) wstart[w] = carry + s_e[tid] - n;
-    carry += s_e[1023];
+    if (w < nruns) {
+      int t = carry + s_n[tid] - n;
+      wstart[w] = min(t, ntiles_cap);
+      for (int S = S0; S < S0 + n && t < ntiles_cap; ++S, ++t) {
+        const int e0 = S == S0 ? pos[w] - ep[S] : 0;
+        const int e1 = S == S1 ? pos[w + 1] - ep[S] : count[S];
+        tiles[t] = make_int4(S, e0, e1, 0);
+      }
+    }
+    carry += s_n[1023];
     __syncthreads();
   }
   if (tid == 0) {
@@ -1040,18 +1082,6 @@ __global__ void k_make_tiles_balanced(int nsuper, const int *__restrict__ count,
     ntiles[0] = min(carry, ntiles_cap);
     ntiles[1] = 0;
     ntiles[2] = nruns;
-  }
-  __syncthreads();
-  for (int w = tid; w < nruns; w += 1024) {
-    const int p0 = min(pos[w], E), p1 = min(max(pos[w + 1], p0), E);
-    if (p1 <= p0) continue;
-    const int S0 = last_le<int>(ep, nsuper, p0), S1 = last_le<int>(ep, nsuper, p1 - 1);
-    int t = wstart[w];
-    for (int S = S0; S <= S1 && t < ntiles_cap; ++S, ++t) {
-      const int e0 = S == S0 ? p0 - ep[S] : 0;
-      const int e1 = S == S1 ? p1 - ep[S] : count[S];
-      tiles[t] = make_int4(S, e0, e1, 0);
-    }
   }
 }
 
