@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 FLOP_PER_PAIR = 69          # algorithmic FLOPs per in-cutoff pair (LJ C2 + PME real, counted once)
 # HBM bytes per k_pair launch (mixed form, skin 0.7 A): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
 # scales from a plain 256 MiB copy (profiles/k_pair_traffic_polymer26000.md)
-PAIR_TRAFFIC_BYTES = 57.7e6
+PAIR_TRAFFIC_BYTES = 43.98e6
 PAIR_TRAFFIC_SKIN = 0.7
 LAUNCH_KNOBS = {1: "tiles_per_cu", 2: "fft_lines", 3: "pair_blocks_per_cu", 4: "pair_schedule",
                 5: "pair_runs_per_wave"}
