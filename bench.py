@@ -38,7 +38,7 @@ FLOP_PER_PAIR = 69          # algorithmic FLOPs per in-cutoff pair (LJ C2 + PME 
 PAIR_TRAFFIC_BYTES = 43.98e6
 PAIR_TRAFFIC_SKIN = 0.7
 LAUNCH_KNOBS = {1: "tiles_per_cu", 2: "fft_lines", 3: "pair_blocks_per_cu", 4: "pair_schedule",
-                5: "pair_runs_per_wave"}
+                5: "pair_runs_per_wave", 6: "spread_on_second_stream"}
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
 TIMESTEP_FS = 1.0

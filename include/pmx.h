@@ -165,7 +165,10 @@ int pmx_sys_time_pair(pmx_sys *sys, int reps, float *ms_per_launch);
  *                        TILES_PER_CU asks for shorter tiles);
  * PMX_TUNE_PAIR_RUNS     balanced runs per resident wave (1..4, default 1:
  *                        fully static); each wave takes one run, the rest
- *                        are taken in order from one device-wide counter.
+ *                        are taken in order from one device-wide counter;
+ * PMX_TUNE_SPREAD_STREAM 0 (default): the PME charge spread runs on the main
+ *                        stream before the pair kernel; 1: it runs first on
+ *                        the second stream, beside the pair kernel.
  * The resident state is kept.  Blocks per CU and FFT lines do not change
  * any sum (beyond the order of the FP64 form's atomics); the tile length
  * regroups the FP32 partial sums of the mixed form's i-forces (one per tile,
@@ -175,7 +178,8 @@ enum {
   PMX_TUNE_FFT_LINES = 2,
   PMX_TUNE_PAIR_BLOCKS_PER_CU = 3,
   PMX_TUNE_PAIR_SCHEDULE = 4,
-  PMX_TUNE_PAIR_RUNS = 5
+  PMX_TUNE_PAIR_RUNS = 5,
+  PMX_TUNE_SPREAD_STREAM = 6
 };
 int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value);
 

@@ -75,6 +75,7 @@ struct Sys {
   int tune_pair_blocks = 0;    // pair-kernel blocks per CU (0: as many as fit)
   int tune_schedule = 0;       // 0: balanced runs per wave; 1: fixed-length tiles (counter / round robin)
   int tune_runs = 1;           // balanced runs per resident wave (1: static, more: the rest from a counter)
+  int tune_spread_stream2 = 0; // 1: the PME spread runs on the second stream, beside the pair kernel
   int pair_grid = 0;           // blocks of the pair-kernel launch
   long long nbins = 0;
   int nscanblocks = 0;
@@ -787,8 +788,7 @@ void launch_gather(Sys *s, hipStream_t st) {
 // runs at full rate, next to the pair kernel its many small waves would
 // not); the charge grid was zeroed by the k_zero launch of the pass.
 template <bool MIXED>
-void run_spread(Sys *s) {
-  hipStream_t st = s->ctx->stream;
+void run_spread(Sys *s, hipStream_t st) {
   switch (s->pme.order) {
     case 2: launch_spread<MIXED, 2>(s, st); break;
     case 4: launch_spread<MIXED, 4>(s, st); break;
@@ -960,19 +960,28 @@ int forces(Sys *s, unsigned flags, bool energies) {
     hipLaunchKernelGGL(k_zero, dim3(std::max(1u, nblk(3ll * npad, 256)), nr), dim3(256), 0, st, z);
   s->acc_clean = false;
   if (timing) hipEventRecord(s->ev[2], st);
-  if (s->pme_on && (s->d.pme_terms & PMX_PME_RECIP)) {
-    if (s->mixed)
-      run_spread<true>(s);
-    else
-      run_spread<false>(s);
-  }
   // fork: FFT, gather, exclusion correction and bonded terms on the second
   // stream beside the pair kernel; all add into the zeroed accumulators.
-  // The pair stage's start event is recorded after the fork, right before
-  // the pair kernel.
+  // The spread runs before the fork on the main stream, or first on the
+  // second stream (tuning knob).  The pair stage's start event is recorded
+  // after the fork, right before the pair kernel.
   hipStream_t st2 = s->ctx->stream2;
+  const bool spread = s->pme_on && (s->d.pme_terms & PMX_PME_RECIP);
+  const bool spread2 = spread && s->tune_spread_stream2;
+  if (spread && !spread2) {
+    if (s->mixed)
+      run_spread<true>(s, st);
+    else
+      run_spread<false>(s, st);
+  }
   HIPCHK(hipEventRecord(s->fork_ev, st));
   HIPCHK(hipStreamWaitEvent(st2, s->fork_ev, 0));
+  if (spread2) {
+    if (s->mixed)
+      run_spread<true>(s, st2);
+    else
+      run_spread<false>(s, st2);
+  }
   if (timing) hipEventRecord(s->ev[4], st);
   if (s->mixed)
     run_pair_phase<true>(s, stats, 0);
@@ -1343,6 +1352,10 @@ int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value) {
     case PMX_TUNE_PAIR_SCHEDULE:
       if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "pair schedule must be 0 or 1");
       s->tune_schedule = value;
+      break;
+    case PMX_TUNE_SPREAD_STREAM:
+      if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "spread stream must be 0 or 1");
+      s->tune_spread_stream2 = value;
       break;
     case PMX_TUNE_PAIR_RUNS:
       if (value < 1 || value > kMaxPairRunsPerCu / kMaxPairWavesPerCu)

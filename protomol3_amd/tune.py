@@ -11,8 +11,8 @@ short trajectory from the same state for every candidate:
   (balanced runs of equal estimated work, or fixed-length tiles from a
   counter), the balanced runs per wave (one static, the rest from a
   counter), the pair-kernel blocks per CU (fewer leave room for the PME
-  chain on the second stream) and the FFT lines per block (the PME grid
-  blocking).
+  chain on the second stream), the FFT lines per block (the PME grid
+  blocking) and the stream of the PME spread.
 
 FFT lines change no sum; the schedule and, under the balanced schedule, the
 runs and blocks per CU regroup the mixed form's FP32 partial sums of the i-forces
@@ -30,6 +30,7 @@ LAUNCH_CANDIDATES = (
     (pmx.TUNE_PAIR_RUNS, (1, 2)),
     (pmx.TUNE_PAIR_BLOCKS_PER_CU, (0, -1, -2)),
     (pmx.TUNE_FFT_LINES, (16, 8, 4)),
+    (pmx.TUNE_SPREAD_STREAM, (0, 1)),
 )
 
 
