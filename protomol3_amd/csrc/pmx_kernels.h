@@ -226,32 +226,41 @@ __global__ void k_scan_final(uint32_t *__restrict__ in, long long m, const uint3
   }
 }
 
-// Exclusive scan of m <= kScanSingleMax counts in one block: each thread
-// scans a contiguous chunk, chunk totals are scanned across the block.  The
-// counts are cleared for the next histogram.
+// Exclusive scan of m <= kScanSingleMax counts in one block: each wave owns
+// a contiguous segment and walks it in coalesced 64-count chunks (a wave
+// prefix per chunk, a running offset across chunks); segment totals are
+// combined across the block first.  The counts are cleared for the next
+// histogram.
 constexpr long long kScanSingleMax = 1 << 16;
 __global__ void k_scan_single(uint32_t *__restrict__ in, long long m, uint32_t *__restrict__ out,
                               uint32_t *__restrict__ out2) {
-  __shared__ uint32_t s[kScanBlock];
-  const long long per = (m + kScanBlock - 1) / kScanBlock;
-  const long long b = threadIdx.x * per, e = min(m, b + per);
+  constexpr int NW = kScanBlock / kWave;
+  __shared__ uint32_t s_tot[NW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long long seg = ((m + NW - 1) / NW + kWave - 1) / kWave * kWave;
+  const long long b = min(m, (long long)w * seg), e = min(m, b + seg);
   uint32_t t = 0;
-  for (long long i = b; i < e; ++i) t += in[i];
-  s[threadIdx.x] = t;
+  for (long long i = b + lane; i < e; i += kWave) t += in[i];
+  t = wave_sum(t);
+  if (lane == 0) s_tot[w] = t;
   __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    const uint32_t u = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += u;
-    __syncthreads();
-  }
-  uint32_t r = s[threadIdx.x] - t;
-  for (long long i = b; i < e; ++i) {
-    const uint32_t v = in[i];
-    out[i] = r;
-    out2[i] = r;
-    in[i] = 0u;
-    r += v;
+  uint32_t base = 0;
+  for (int k = 0; k < w; ++k) base += s_tot[k];
+  for (long long c = b; c < e; c += kWave) {
+    const long long i = c + lane;
+    const uint32_t v = i < e ? in[i] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (i < e) {
+      out[i] = base + x - v;
+      out2[i] = base + x - v;
+      in[i] = 0u;
+    }
+    base += __shfl(x, kWave - 1);
   }
 }
 
@@ -688,6 +697,7 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
   __shared__ int s_cnt[NW];
   __shared__ int s_cost[NW];
   __shared__ int s_sc[kListBlock];
+  __shared__ int s_ra[kWave], s_rpre[kWave], s_rtot;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int S = blockIdx.x;
   const int col = sc_col[S];
@@ -708,47 +718,82 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
   int xr[2][2], yr[2][2];
   const int cx = col / q.ncy, cy = col - cx * q.ncy;
   const int nxr = col_runs(cx, q.kx, q.ncx, xr), nyr = col_runs(cy, q.ky, q.ncy, yr);
+  // the candidate j-superclusters form index ranges, one per (x, y run) in
+  // ascending order; they are enumerated as one flat sequence (64 ranges at
+  // a time) so that each pass of 256 threads tests 256 candidates
+  const int nx0 = xr[0][1] - xr[0][0] + 1, nxt = nx0 + (nxr > 1 ? xr[1][1] - xr[1][0] + 1 : 0);
+  const int R = nxt * nyr;
   int total = 0;   // block-uniform
   int ctotal = 0;  // block-uniform work of the entries so far
-  for (int rx = 0; rx < nxr; ++rx)
-    for (int x = xr[rx][0]; x <= xr[rx][1]; ++x)
-      for (int ry = 0; ry < nyr; ++ry) {
-        const int a = max(S, colsc[x * q.ncy + yr[ry][0]]);
-        const int b = colsc[x * q.ncy + yr[ry][1] + 1];
-        for (int base = a; base < b; base += kListBlock) {
-          const int sc = base + threadIdx.x;
-          bool pass = false;
-          if (sc < b) {
-            const float4 jh = shalf[sc];
-            pass = jh.x >= 0.f && bbox_dist2(Sc, Sh, sctr[sc], jh, q.L) <= q.rlist2_sc;
-          }
-          const unsigned long long act = __ballot(pass);
-          if (lane == 0) s_cnt[wv] = __popcll(act);
-          __syncthreads();
-          int before = 0, npass = 0;
-#pragma unroll
-          for (int w = 0; w < NW; ++w) {
-            const int c = s_cnt[w];
-            before += w < wv ? c : 0;
-            npass += c;
-          }
-          if (pass) s_sc[before + __popcll(act & ((1ull << lane) - 1ull))] = sc;
-          __syncthreads();
-          for (int k0 = 0; k0 < npass; k0 += kScPerBatch) {
-            const int k = k0 + (int)(threadIdx.x / kClusterSize);
-            int cj = -1;
-            float4 jc = make_float4(0, 0, 0, 0), jh = make_float4(-1, -1, -1, 0);
-            if (k < npass) {
-              cj = s_sc[k] * kClusterSize + (int)(threadIdx.x & (kClusterSize - 1));
-              jc = ctr[cj];
-              jh = half[cj];
-            }
-            list_append(q, S, ic, ih, cj, jc, jh, entries, maskwork, nmaskwork, maxmaskwork, overflow, s_cnt, s_cost,
-                        ecost, total, ctotal);
-          }
-          __syncthreads();  // s_sc is rewritten by the next range
-        }
+  for (int g0 = 0; g0 < R; g0 += kWave) {
+    const int nrg = min(kWave, R - g0);
+    __syncthreads();  // every thread has read the previous group's ranges
+    if (threadIdx.x < kWave) {
+      const int r = g0 + lane;
+      int a = 0, len = 0;
+      if (r < R) {
+        const int xi = r / nyr, ry = r - xi * nyr;
+        const int x = xi < nx0 ? xr[0][0] + xi : xr[1][0] + (xi - nx0);
+        a = max(S, colsc[x * q.ncy + yr[ry][0]]);
+        len = max(0, colsc[x * q.ncy + yr[ry][1] + 1] - a);
       }
+      int incl = len;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+      }
+      s_ra[lane] = a;
+      s_rpre[lane] = incl - len;
+      if (lane == kWave - 1) s_rtot = incl;
+    }
+    __syncthreads();
+    const int T = s_rtot;
+    for (int base = 0; base < T; base += kListBlock) {
+      const int k = base + threadIdx.x;
+      bool pass = false;
+      int sc = 0;
+      if (k < T) {
+        // range of candidate k: the last range starting at or before k
+        int lo = 0, hi = nrg - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_rpre[mid] <= k)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        sc = s_ra[lo] + (k - s_rpre[lo]);
+        const float4 jh = shalf[sc];
+        pass = jh.x >= 0.f && bbox_dist2(Sc, Sh, sctr[sc], jh, q.L) <= q.rlist2_sc;
+      }
+      const unsigned long long act = __ballot(pass);
+      if (lane == 0) s_cnt[wv] = __popcll(act);
+      __syncthreads();
+      int before = 0, npass = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        const int c = s_cnt[w];
+        before += w < wv ? c : 0;
+        npass += c;
+      }
+      if (pass) s_sc[before + __popcll(act & ((1ull << lane) - 1ull))] = sc;
+      __syncthreads();
+      for (int k0 = 0; k0 < npass; k0 += kScPerBatch) {
+        const int kk = k0 + (int)(threadIdx.x / kClusterSize);
+        int cj = -1;
+        float4 jc = make_float4(0, 0, 0, 0), jh = make_float4(-1, -1, -1, 0);
+        if (kk < npass) {
+          cj = s_sc[kk] * kClusterSize + (int)(threadIdx.x & (kClusterSize - 1));
+          jc = ctr[cj];
+          jh = half[cj];
+        }
+        list_append(q, S, ic, ih, cj, jc, jh, entries, maskwork, nmaskwork, maxmaskwork, overflow, s_cnt, s_cost,
+                    ecost, total, ctotal);
+      }
+      __syncthreads();  // s_sc and s_cnt are rewritten by the next pass
+    }
+  }
   if (threadIdx.x == 0) {
     const int n = total < q.cap ? total : q.cap;
     count[S] = n;
