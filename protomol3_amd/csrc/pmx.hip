@@ -921,7 +921,11 @@ int rebuild(Sys *s, bool timing) {
                        (int)s->etab.size(), P<int4>(s->etab_d), P<int>(s->slot), P<int4>(s->entries),
                        P<int>(s->count), s->cap, P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork,
                        P<int>(s->overflow));
-  hipLaunchKernelGGL(k_build_masks, dim3(s->ctx->cus * 4), dim3(256), 0, st, P<int>(s->maskwork),
+  // a wave per mask slot, up to 16 blocks per CU (the slots in use are known
+  // on the device only; waves beyond them return at once)
+  hipLaunchKernelGGL(k_build_masks,
+                     dim3((unsigned)std::min<long long>(nblk((long long)s->maxmaskwork, 4), 16ll * s->ctx->cus)),
+                     dim3(256), 0, st, P<int>(s->maskwork),
                      P<int>(s->nmaskwork), s->maxmaskwork, P<int4>(s->entries), s->cap, P<int>(s->permp),
                      P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned short>(s->masks));
   make_tiles(s, st);
@@ -933,7 +937,10 @@ int rebuild(Sys *s, bool timing) {
 // Forces (and optionally energies) at the current positions with the
 // current list: pair + boundary pairs, PME, correction, bonded terms, then
 // the FP64 force array in the caller's atom order.
-int forces(Sys *s, unsigned flags, bool energies) {
+// next: the following MD step's kick (with next_dmax, its displacement
+// record): the pass ends with that step's kick and drift fused into the
+// force unsort.
+int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, unsigned *next_dmax = nullptr) {
   hipStream_t st = s->ctx->stream;
   const bool timing = flags & PMX_EVAL_TIMING;
   const bool stats = flags & PMX_EVAL_PAIRSTATS;
@@ -1007,7 +1014,19 @@ int forces(Sys *s, unsigned flags, bool energies) {
   else
     run_pair_phase<false>(s, stats, 1);
   HIPCHK(hipStreamWaitEvent(st, s->join_ev, 0));
-  if (s->mixed)
+  if (next && s->mixed)
+    hipLaunchKernelGGL((k_unsort_kick_drift<true, float>), dim3(nblk(npad, 256)), dim3(256), 0, st, npad, *next,
+                       s->md.h, P<int>(s->permp), P<unsigned long long>(s->facc), P<double>(s->fout),
+                       P<int>(s->tilectr), P<int>(s->nborder), P<double>(s->md.minv), P<double>(s->md.v),
+                       P<double>(s->xorig), P<double>(s->md.xbuild), P<float4>(s->xs), P<double4>(s->xs64),
+                       P<float4>(s->sorg), next_dmax);
+  else if (next)
+    hipLaunchKernelGGL((k_unsort_kick_drift<false, double>), dim3(nblk(npad, 256)), dim3(256), 0, st, npad, *next,
+                       s->md.h, P<int>(s->permp), P<double>(s->facc), P<double>(s->fout), P<int>(s->tilectr),
+                       P<int>(s->nborder), P<double>(s->md.minv), P<double>(s->md.v), P<double>(s->xorig),
+                       P<double>(s->md.xbuild), P<double4>(s->xs), P<double4>(s->xs64), P<float4>(s->sorg),
+                       next_dmax);
+  else if (s->mixed)
     hipLaunchKernelGGL(k_unsort_forces<true>, dim3(nblk(n, 256)), dim3(256), 0, st, n, P<int>(s->slot),
                        P<unsigned long long>(s->facc), P<double>(s->fout), P<int>(s->tilectr),
                        P<int>(s->nborder));
@@ -1550,15 +1569,20 @@ Kick make_kick(const Sys::Md &m, double kick, long long step, const unsigned lon
   return kk;
 }
 
+// kicked: the previous step's force pass already did this step's kick and
+// drift; next (with next_dslot): fuse the following step's into this one's.
 int md_step(Sys *s, double kick, long long step, bool rebuild_here, int dslot,
-            const unsigned long long *step_dev = nullptr) {
+            const unsigned long long *step_dev = nullptr, bool kicked = false, const Kick *next = nullptr,
+            int next_dslot = 0) {
   hipStream_t st = s->ctx->stream;
   Sys::Md &m = s->md;
   const bool timing = s->timing_active;
   const Kick kk = make_kick(m, kick, step, step_dev);
   // stage 0 of a timed MD step is integrate (+ sort when the list is rebuilt)
   timing_begin_call(s, timing);
-  if (s->mixed)
+  if (kicked)
+    ;
+  else if (s->mixed)
     hipLaunchKernelGGL(k_kick_drift<float>, dim3(nblk(s->npad, 256)), dim3(256), 0, st, s->npad, kk, m.h,
                        P<int>(s->permp), P<double>(s->fout), P<double>(m.minv), P<double>(m.v), P<double>(s->xorig),
                        P<double>(m.xbuild), P<float4>(s->xs), P<double4>(s->xs64),
@@ -1575,7 +1599,7 @@ int md_step(Sys *s, double kick, long long step, bool rebuild_here, int dslot,
     hipEventRecord(s->ev[1], st);
     hipEventRecord(s->ev[2], st);
   }
-  return forces(s, timing ? PMX_EVAL_TIMING : 0u, false);
+  return forces(s, timing ? PMX_EVAL_TIMING : 0u, false, next, next ? P<unsigned>(m.dmax) + next_dslot : nullptr);
 }
 
 // A chunk of K steps from the committed state: its state is backed up in
@@ -1601,9 +1625,15 @@ int chunk_body(Sys *s, const Chunk &c, bool rel) {
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kMaxChunk), 0, st, P<uint32_t>(m.dmax) + c.slot * kMaxChunk, 0u,
                      (long long)kMaxChunk);
   const unsigned long long *sd = rel ? P<unsigned long long>(m.step_dev) : nullptr;
+  // inside the chunk each force pass ends with the next step's kick and
+  // drift (one launch instead of two); timed steps keep them apart
+  const bool fuse = !s->timing_active;
   for (int k = 0; k < c.K; ++k) {
+    Kick nk{};
+    const bool nx = fuse && k + 1 < c.K;
+    if (nx) nk = make_kick(m, 1.0, rel ? k + 1 : c.step0 + k + 1, sd);
     int rc = md_step(s, (c.first && k == 0) ? 0.5 : 1.0, rel ? k : c.step0 + k, k == 0 || m.skin == 0.0,
-                     c.slot * kMaxChunk + k, sd);
+                     c.slot * kMaxChunk + k, sd, fuse && k > 0, nx ? &nk : nullptr, c.slot * kMaxChunk + k + 1);
     if (rc) return rc;
   }
   unsigned *h = m.host + c.slot * (kMaxChunk + 1);

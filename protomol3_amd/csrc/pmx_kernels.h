@@ -2708,6 +2708,59 @@ __global__ void k_kick_drift(int n, Kick kk, double h, const int *__restrict__ p
   if ((threadIdx.x & 63) == 0) atomicMax(dmax, __float_as_uint(d2f));
 }
 
+// The end of one MD step and the start of the next in one pass over the
+// sorted slots: the step's forces leave the accumulators (cleared, with the
+// pass counters) for the caller's atom order, and the next step's kick and
+// drift use them at once (k_unsort_forces followed by k_kick_drift).
+template <bool MIXED, class R>
+__global__ void k_unsort_kick_drift(int n, Kick kk, double h, const int *__restrict__ perm,
+                                    typename AccT<MIXED>::type *__restrict__ facc, double *__restrict__ fout,
+                                    int *__restrict__ tilectr, int *__restrict__ nborder,
+                                    const double *__restrict__ minv, double *__restrict__ v,
+                                    double *__restrict__ x, const double *__restrict__ xbuild,
+                                    typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64,
+                                    const float4 *__restrict__ sorg, unsigned int *__restrict__ dmax) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) {
+    *tilectr = 0;
+    *nborder = 0;
+  }
+  float d2f = 0.f;
+  const int a = s < n ? perm[s] : -1;  // n counts slots; padding slots have a < 0
+  if (a >= 0) {
+    double ff[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (MIXED)
+        ff[k] = (double)(long long)facc[3 * s + k] * (1.0 / kForceScale);
+      else
+        ff[k] = ((const double *)facc)[3 * s + k];
+      facc[3 * s + k] = 0;
+      fout[3 * a + k] = ff[k];
+    }
+    double dd[3], vv[3] = {v[3 * a], v[3 * a + 1], v[3 * a + 2]};
+    apply_kick(kk, a, ff, minv[a], vv);
+    double d2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      v[3 * a + k] = vv[k];
+      const double xk = x[3 * a + k] + vv[k] * h;
+      x[3 * a + k] = xk;
+      dd[k] = xk - xbuild[3 * a + k];
+      d2 += dd[k] * dd[k];
+    }
+    d2f = (float)d2;
+    const double4 b = xs64[s];
+    const float4 o = sorg[s / kSuperSize];
+    typename Real4<R>::type r = xs[s];
+    r.x = (R)((b.x + dd[0]) - (double)o.x);
+    r.y = (R)((b.y + dd[1]) - (double)o.y);
+    r.z = (R)((b.z + dd[2]) - (double)o.z);
+    xs[s] = r;
+  }
+  for (int m = 32; m >= 1; m >>= 1) d2f = fmaxf(d2f, __shfl_xor(d2f, m));
+  if ((threadIdx.x & 63) == 0) atomicMax(dmax, __float_as_uint(d2f));
+}
+
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
 __global__ void k_half_kick(int n, Kick kk, const double *__restrict__ fout, const double *__restrict__ minv,
