@@ -61,6 +61,10 @@ def ns_day(ms_per_step: float) -> float:
     return 86400.0 / (ms_per_step * 1e-3) * TIMESTEP_FS * 1e-6
 
 
+def _round(x, nd):
+    return None if x is None else round(x, nd)
+
+
 def run_form(ctx, s, spec, precision, steps, warmup, skin, drift_steps, tune_launch=False):
     from protomol3_amd import pmx, tune
     sy = pmx.System(ctx, s, spec, precision)
@@ -369,8 +373,8 @@ def main():
         "mixed_over_fp64": round(fp64["ms_per_step"] / mixed["ms_per_step"], 3),
         "rms_rel_force_err_mixed_vs_fp64": float(f"{err:.3e}"),
         "rel_energy_err_mixed_vs_fp64": rel_e,
-        "nve_drift_kcal_mol_per_ns": {"mixed": round(mixed["drift_kcal_mol_ns"], 3),
-                                      "fp64": round(fp64["drift_kcal_mol_ns"], 3),
+        "nve_drift_kcal_mol_per_ns": {"mixed": _round(mixed["drift_kcal_mol_ns"], 3),
+                                      "fp64": _round(fp64["drift_kcal_mol_ns"], 3),
                                       "steps": a.drift_steps},
         "launch_tuning": {form: None if r["launch"] is None else
                           {"chosen": {LAUNCH_KNOBS[k]: v for k, v in r["launch"][0].items()},
