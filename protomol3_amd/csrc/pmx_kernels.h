@@ -1251,10 +1251,12 @@ __device__ inline void pair_eval(const PairParams &pp, const SwR<R> &sl, const S
     // CoulombForce.h:29-40
     R e = qq * rinv;
     R f = e * rinv2;
-    R sv, sd;
-    switch_val<CSW, R>(pp.co.kind, sc, r2s, rinv, sv, sd);
-    f = f * sv - e * sd;
-    e = e * sv;
+    if constexpr (CSW != SW_CUTOFF) {  // a plain cutoff switches nothing (1 and 0)
+      R sv, sd;
+      switch_val<CSW, R>(pp.co.kind, sc, r2s, rinv, sv, sd);
+      f = f * sv - e * sd;
+      e = e * sv;
+    }
     eco = in_c ? e : (R)0;
     fr += in_c ? f : (R)0;
   } else if (COUL == COUL_PME) {
@@ -1264,10 +1266,12 @@ __device__ inline void pair_eval(const PairParams &pp, const SwR<R> &sl, const S
     const R ex = qq * fast_exp(-ar * ar);
     R e = poly5<R>(ar) * ex * rinv;
     R f = (e + a2pi * ex) * rinv2;
-    R sv, sd;
-    switch_val<CSW, R>(pp.co.kind, sc, r2s, rinv, sv, sd);
-    f = f * sv - e * sd;
-    e = e * sv;
+    if constexpr (CSW != SW_CUTOFF) {
+      R sv, sd;
+      switch_val<CSW, R>(pp.co.kind, sc, r2s, rinv, sv, sd);
+      f = f * sv - e * sd;
+      e = e * sv;
+    }
     eco = in_c ? e : (R)0;
     fr += in_c ? f : (R)0;
   }
@@ -1418,6 +1422,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   sc.load(pp.co);
   const R guard = (R)pp.guard;
   const R alpha = (R)pp.alpha, a2pi = (R)pp.a2pi, csf = (R)pp.csf;
+  const bool one_cut = sl.rc2 == sc.rc2;  // one guard band serves both cutoffs
   const int ntiles = ntiles_p[0], tlen = ntiles_p[1];
   // the boundary-pair sums are filled after this kernel (same stream)
   if (blockIdx.x == 0 && threadIdx.x < 4) out.border_acc[threadIdx.x] = 0ull;
@@ -1565,8 +1570,12 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
           pair_delta<R, R4>(xi, xj[g], E, shv[g] & kShiftPerPair, dx, dy, dz);
           const R r2 = dx * dx + dy * dy + dz * dz;
           bool border = false;
-          if (LJSW != SW_NONE) border |= fabs(r2 - sl.rc2) <= guard;
-          if (COUL != COUL_NONE) border |= fabs(r2 - sc.rc2) <= guard;
+          if (LJSW != SW_NONE && COUL != COUL_NONE && one_cut) {
+            border = fabs(r2 - sl.rc2) <= guard;
+          } else {
+            if (LJSW != SW_NONE) border |= fabs(r2 - sl.rc2) <= guard;
+            if (COUL != COUL_NONE) border |= fabs(r2 - sc.rc2) <= guard;
+          }
           border = border && cand;
           const bool ok = cand && !border;
           const bool in_lj = LJSW != SW_NONE && ok && r2 <= sl.rc2;
