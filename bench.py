@@ -295,8 +295,8 @@ def main():
     ap.add_argument("--replica-lanes", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
-              "the multi-GPU sharding design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {a.gpus}); "
+              "sharding the force path over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
     from protomol3_amd import pmx, tune
     ctx = pmx.Context(0)
