@@ -43,7 +43,7 @@ def _time_run(sy, x, v, steps: int, warmup: int = 20) -> float:
 
 
 def tune_skin(ctx, s, spec: pmx.ForceSpec, precision: int, x, v, timestep_fs: float = 1.0,
-              candidates=(0.5, 0.7, 0.9, 1.2), steps: int = 200) -> tuple[float, dict]:
+              candidates=(0.5, 0.7, 0.9, 1.2, 1.5), steps: int = 200) -> tuple[float, dict]:
     """Returns (best skin, {skin: ms per step})."""
     times = {}
     for skin in candidates:
