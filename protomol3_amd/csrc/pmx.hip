@@ -1618,12 +1618,13 @@ int chunk_body(Sys *s, const Chunk &c, bool rel) {
   Sys::Md &m = s->md;
   hipStream_t st = s->ctx->stream;
   const size_t n = s->n, off = c.slot * 3 * n;
-  HIPCHK(hipMemcpyAsync(P<double>(m.xbak) + off, s->xorig.p, 24 * n, hipMemcpyDeviceToDevice, st));
-  HIPCHK(hipMemcpyAsync(P<double>(m.vbak) + off, m.v.p, 24 * n, hipMemcpyDeviceToDevice, st));
-  HIPCHK(hipMemcpyAsync(P<double>(m.fbak) + off, s->fout.p, 24 * n, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kMaxChunk), 0, st, P<uint32_t>(m.dmax) + c.slot * kMaxChunk, 0u,
-                     (long long)kMaxChunk);
+  // backups of the committed state, cleared overflow flags and displacement
+  // record, in one launch
+  const long long m3 = 3ll * (long long)n;
+  hipLaunchKernelGGL(k_chunk_begin, dim3(nblk(std::max<long long>(m3, kMaxChunk), 256)), dim3(256), 0, st, m3,
+                     P<double>(s->xorig), P<double>(m.v), P<double>(s->fout), P<double>(m.xbak) + off,
+                     P<double>(m.vbak) + off, P<double>(m.fbak) + off, P<int>(s->overflow),
+                     P<unsigned>(m.dmax) + c.slot * kMaxChunk, kMaxChunk);
   const unsigned long long *sd = rel ? P<unsigned long long>(m.step_dev) : nullptr;
   // inside the chunk each force pass ends with the next step's kick and
   // drift (one launch instead of two); timed steps keep them apart

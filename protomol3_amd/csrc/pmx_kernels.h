@@ -2770,6 +2770,23 @@ __global__ void k_unsort_kick_drift(int n, Kick kk, double h, const int *__restr
   if ((threadIdx.x & 63) == 0) atomicMax(dmax, __float_as_uint(d2f));
 }
 
+// Start of an MD chunk in one launch: the committed state (x, v, f: m3 =
+// 3N doubles each) is backed up for a rollback, the overflow flags (2) and
+// the chunk's displacement record (nrec) are cleared.
+__global__ void k_chunk_begin(long long m3, const double *__restrict__ x, const double *__restrict__ v,
+                              const double *__restrict__ f, double *__restrict__ xbak, double *__restrict__ vbak,
+                              double *__restrict__ fbak, int *__restrict__ overflow, unsigned *__restrict__ dmax,
+                              int nrec) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m3) {
+    xbak[i] = x[i];
+    vbak[i] = v[i];
+    fbak[i] = f[i];
+  }
+  if (i < 2) overflow[i] = 0;
+  if (i < nrec) dmax[i] = 0u;
+}
+
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
 __global__ void k_half_kick(int n, Kick kk, const double *__restrict__ fout, const double *__restrict__ minv,
