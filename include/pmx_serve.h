@@ -45,7 +45,9 @@ int pmx_server_create(int device, int slots, unsigned long long hbm_budget, pmx_
 /* Queues a request; *ticket identifies it. */
 int pmx_server_submit(pmx_server *srv, const pmx_request *req, long long *ticket);
 /* Blocks until the request is done; returns its pmx_status.  Times in ms
- * since its submit: when it started on a slot and when it finished. */
+ * since its submit: when it started on a slot and when it finished.  One
+ * wait per ticket: a ticket already collected, or one another thread is
+ * waiting on, gives PMX_ERR_ARG. */
 int pmx_server_wait(pmx_server *srv, long long ticket, double *start_ms, double *done_ms);
 /* Requests queued + running. */
 int pmx_server_pending(pmx_server *srv, int *n);

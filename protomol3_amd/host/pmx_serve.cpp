@@ -36,6 +36,7 @@ struct Job {
   unsigned long long bytes = 0;
   Clock::time_point t_submit, t_start, t_done;
   bool done = false;
+  bool waited = false;  // a pmx_server_wait holds the ticket
   int status = PMX_OK;
 };
 
@@ -203,10 +204,12 @@ int pmx_server_wait(pmx_server *srv, long long ticket, double *start_ms, double 
   {
     std::unique_lock<std::mutex> lk(srv->mu);
     auto it = srv->jobs.find(ticket);
-    if (it == srv->jobs.end()) return PMX_ERR_ARG;
+    // unknown, already collected, or another thread is waiting on it
+    if (it == srv->jobs.end() || it->second->waited) return PMX_ERR_ARG;
     j = it->second;
+    j->waited = true;
     srv->cv_done.wait(lk, [&] { return j->done; });
-    srv->jobs.erase(it);
+    srv->jobs.erase(ticket);
   }
   auto ms = [&](Clock::time_point t) {
     return std::chrono::duration<double, std::milli>(t - j->t_submit).count();

@@ -182,12 +182,29 @@ class ForceSpec:
         self.cellsize, self.lj, self.coul, self.pme = cellsize, lj, coul, pme
 
 
+def _check_shape(a: np.ndarray, shape: tuple, what: str) -> None:
+    if a.shape != shape:
+        raise ValueError(f"{what}: shape {a.shape}, expected {shape}")
+
+
 def make_desc(s: MolSystem, spec: ForceSpec, precision: int = FP64):
-    """(pmx_sys_desc, arrays it points into) for a system and force spec."""
+    """(pmx_sys_desc, arrays it points into) for a system and force spec.
+    Raises ValueError when an array's shape does not match the atom count
+    (the library reads exactly n, ntypes^2 * 4 and 3 * nexcl values)."""
     keep = [np.ascontiguousarray(s.charge, np.float64), np.ascontiguousarray(s.atype, np.int32),
             np.ascontiguousarray(s.molecule, np.int32), np.ascontiguousarray(s.lj, np.float64),
             np.ascontiguousarray(s.exclusions, np.int32).reshape(-1, 3)]
     q, t, m, lj, ex = keep
+    n = s.n
+    _check_shape(q, (n,), "charge")
+    _check_shape(t, (n,), "atype")
+    _check_shape(m, (n,), "molecule")
+    if lj.ndim != 3 or lj.shape[0] != lj.shape[1] or lj.shape[2] != 4:
+        raise ValueError(f"lj: shape {lj.shape}, expected (ntypes, ntypes, 4)")
+    if n and (t.min() < 0 or t.max() >= lj.shape[0]):
+        raise ValueError("atype: LJ type outside the table")
+    if ex.size and (ex[:, :2].min() < 0 or ex[:, :2].max() >= n):
+        raise ValueError("exclusions: atom index outside the system")
     d = SysDesc()
     d.n, d.ntypes, d.nexcl = s.n, lj.shape[0], ex.shape[0]
     d.charge = _dp(q)
@@ -229,6 +246,13 @@ def make_md_desc(s: MolSystem, timestep_fs: float, skin: float = 1.0, integrator
             np.ascontiguousarray(s.angle_par, np.float64).reshape(-1, 2),
             np.ascontiguousarray(s.mass, np.float64)]
     b, bp, a, ap, m = keep
+    _check_shape(m, (s.n,), "mass")
+    _check_shape(bp, (b.shape[0], 2), "bond_par")
+    _check_shape(ap, (a.shape[0], 2), "angle_par")
+    if b.size and (b.min() < 0 or b.max() >= s.n):
+        raise ValueError("bonds: atom index outside the system")
+    if a.size and (a.min() < 0 or a.max() >= s.n):
+        raise ValueError("angles: atom index outside the system")
     d = MdDesc()
     d.nbonds, d.nangles = b.shape[0], a.shape[0]
     d.bonds = b.ctypes.data_as(C.POINTER(C.c_int))
@@ -278,6 +302,7 @@ class System:
     def evaluate(self, pos: np.ndarray, flags: int = 0):
         """Returns (forces (N,3), e_coulomb, e_lj) for positions (N,3)."""
         x = np.ascontiguousarray(pos, dtype=np.float64)
+        _check_shape(x, (self.n, 3), "positions")
         f = np.zeros((self.n, 3))
         e = np.zeros(2)
         self.ctx.check(self.ctx.lib.pmx_sys_evaluate(self.h, _dp(x), _dp(f), _dp(e), flags), "pmx_sys_evaluate")
@@ -285,6 +310,7 @@ class System:
 
     def set_positions(self, pos: np.ndarray):
         x = np.ascontiguousarray(pos, dtype=np.float64)
+        _check_shape(x, (self.n, 3), "positions")
         self.ctx.check(self.ctx.lib.pmx_sys_set_positions(self.h, _dp(x)), "pmx_sys_set_positions")
 
     def compute(self, flags: int = 0):
@@ -339,6 +365,8 @@ class System:
     def md_set_state(self, x: np.ndarray, v: np.ndarray):
         x = np.ascontiguousarray(x, np.float64)
         v = np.ascontiguousarray(v, np.float64)
+        _check_shape(x, (self.n, 3), "positions")
+        _check_shape(v, (self.n, 3), "velocities")
         self.ctx.check(self.ctx.lib.pmx_md_set_state(self.h, _dp(x), _dp(v)), "pmx_md_set_state")
 
     def md_run(self, nsteps: int):

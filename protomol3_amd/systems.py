@@ -40,24 +40,31 @@ M_O, M_H = 15.9994, 1.008
 # Bead polymer: bond 3.8 A, LJ eps 0.2 / sigma 4.0, charges +-0.5 alternating
 BEAD_BOND, BEAD_EPS, BEAD_SIGMA, BEAD_Q = 3.8, 0.2, 4.0, 0.5
 BEAD_KB, BEAD_MASS = 100.0, 100.0
+# 1-4 LJ of the bead when the chain carries scaled 1-4 pairs (CHARMM's
+# separate eps14 / Rmin14/2 columns of a PAR file)
+BEAD_EPS14, BEAD_SIGMA14 = 0.1, 3.5
 
 
 def rmin_half(sigma: float) -> float:
     return sigma * 2.0 ** (1.0 / 6.0) / 2.0
 
 
-def charmm_table(eps: list[float], rmin2: list[float]) -> np.ndarray:
+def charmm_table(eps: list[float], rmin2: list[float], eps14: list[float] | None = None,
+                 rmin2_14: list[float] | None = None) -> np.ndarray:
     """T x T x 4 table {A, B, A14, B14} with CHARMM combining
-    (protomol/topology/BuildTopology.cpp:663-673); 1-4 parameters equal the
-    normal ones (no separate 1-4 LJ in these models)."""
+    (protomol/topology/BuildTopology.cpp:663-673).  The 1-4 columns combine
+    eps14 / rmin2_14 the same way; without them they equal the normal ones."""
     t = len(eps)
+    eps14 = eps if eps14 is None else eps14
+    rmin2_14 = rmin2 if rmin2_14 is None else rmin2_14
     tab = np.zeros((t, t, 4))
     for i in range(t):
         for j in range(t):
             r = rmin2[i] + rmin2[j]
             e = math.sqrt(eps[i] * eps[j])
-            a, b = r ** 12 * e, 2.0 * r ** 6 * e
-            tab[i, j] = (a, b, a, b)
+            r14 = rmin2_14[i] + rmin2_14[j]
+            e14 = math.sqrt(eps14[i] * eps14[j])
+            tab[i, j] = (r ** 12 * e, 2.0 * r ** 6 * e, r14 ** 12 * e14, 2.0 * r14 ** 6 * e14)
     return tab
 
 
@@ -275,13 +282,20 @@ def _clear_of(wat: np.ndarray, beads: np.ndarray, box: float, overlap: float) ->
 
 
 def polymer_water(n_total: int = 26000, box: float = 64.0, n_beads: int = 500, seed: int = 1,
-                  overlap: float = 2.4, jitter: float = 0.3, oversample: float = 1.25) -> MolSystem:
+                  overlap: float = 2.4, jitter: float = 0.3, oversample: float = 1.25,
+                  scaled14: float | None = None) -> MolSystem:
     """Solvated bead polymer: a seeded random-walk chain of n_beads (bond 3.8 A,
     LJ eps 0.2 / sigma 4.0, charges +-0.5 alternating) plus 3-site waters on a
     jittered lattice; waters with any atom within `overlap` A of a bead are
     removed and the remaining lattice sites are drawn in seeded order until the
     system holds exactly n_total atoms.  The lattice holds `oversample`
-    times the waters needed."""
+    times the waters needed.  An odd n_beads leaves a net charge of +0.5 e.
+
+    Chain exclusions: bonded pairs (i, i+1) are FULL.  With scaled14 set, the
+    chain's exclusions follow ProtoMol's `exclude scaled1-4` rule
+    (BuildTopology.cpp:1110-1158): 1-2 and 1-3 pairs FULL, 1-4 pairs (i, i+3)
+    MODIFIED, coulombScalingFactor = scaled14, and the bead's 1-4 LJ columns
+    from eps 0.1 / sigma 3.5."""
     rng = np.random.default_rng(seed)
     n_water = (n_total - n_beads) // 3
     if n_beads + 3 * n_water != n_total:
@@ -298,15 +312,27 @@ def polymer_water(n_total: int = 26000, box: float = 64.0, n_beads: int = 500, s
     wpart = _water_part(o[sites], np.random.default_rng(seed + 7), o_type=0, h_type=1)
     wpart["pos"] = wat[sites].reshape(-1, 3)
     bonds = np.stack([np.arange(n_beads - 1), np.arange(1, n_beads)], 1)
+    excl = np.concatenate([bonds, np.full((n_beads - 1, 1), FULL)], 1)
+    if scaled14 is not None:
+        b = np.arange(n_beads)
+        e13 = np.stack([b[:-2], b[2:], np.full(n_beads - 2, FULL)], 1)
+        e14 = np.stack([b[:-3], b[3:], np.full(n_beads - 3, MODIFIED)], 1)
+        excl = np.concatenate([excl, e13, e14])
     ppart = dict(pos=beads,
                  charge=np.where(np.arange(n_beads) % 2 == 0, BEAD_Q, -BEAD_Q),
                  atype=np.full(n_beads, 2), mass=np.full(n_beads, BEAD_MASS),
                  molecule=np.zeros(n_beads, int),
                  bonds=bonds, bond_par=np.tile([BEAD_KB, BEAD_BOND], (n_beads - 1, 1)),
                  angles=np.zeros((0, 3), int), angle_par=np.zeros((0, 2)),
-                 exclusions=np.concatenate([bonds, np.full((n_beads - 1, 1), FULL)], 1))
-    lj = charmm_table([EPS_O, 0.0, BEAD_EPS], [rmin_half(SIGMA_O), 0.0, rmin_half(BEAD_SIGMA)])
-    s = _assemble(f"polymer_water{n_total}", [box] * 3, [ppart, wpart], lj)
+                 exclusions=excl)
+    eps, rmin2 = [EPS_O, 0.0, BEAD_EPS], [rmin_half(SIGMA_O), 0.0, rmin_half(BEAD_SIGMA)]
+    if scaled14 is None:
+        lj = charmm_table(eps, rmin2)
+    else:
+        lj = charmm_table(eps, rmin2, [EPS_O, 0.0, BEAD_EPS14],
+                          [rmin_half(SIGMA_O), 0.0, rmin_half(BEAD_SIGMA14)])
+    s = _assemble(f"polymer_water{n_total}", [box] * 3, [ppart, wpart], lj,
+                  csf=1.0 if scaled14 is None else float(scaled14))
     # keep every coordinate inside [0, box): the lattice and walk already are,
     # except walk beads that wandered; wrap them (positions are periodic)
     s.pos = s.pos - box * np.floor(s.pos / box)

@@ -66,6 +66,22 @@ CASES = {
                              ["lj_c2", "10", "12", "pme", "32", "32", "32", "4", "12", "1e-6", "-1", "111"], 12.0, 1),
     "polymer26000_pme_auto": ("polymer_water", dict(n_total=26000, box=64.0, n_beads=500, seed=1), 6.0,
                               ["lj_c2", "10", "12", "pme", "64", "64", "64", "4", "12", "1e-6", "-1", "111"], 12.0, 13),
+    # scaled 1-4 chain pairs (MODIFIED class, coulombScalingFactor 0.5, separate A14/B14)
+    "polymer3200_mod14_pme": ("polymer_water", dict(n_total=3200, box=32.0, n_beads=50, seed=4, scaled14=0.5),
+                              6.0, ["lj_c2", "10", "12", "pme", "32", "32", "32", "4", "12", "1e-6", "-1", "111"],
+                              12.0, 1),
+    "polymer3200_mod14_pme_corr": ("polymer_water",
+                                   dict(n_total=3200, box=32.0, n_beads=50, seed=4, scaled14=0.5), 6.0,
+                                   ["pme", "32", "32", "32", "4", "12", "1e-6", "-1", "001"], 12.0, 1),
+    # scaled 1-4 with cutoff Coulomb, LJ and Coulomb cutoffs apart (9 and 11.5 A)
+    "polymer3200_mod14_ljc2_coulc2": ("polymer_water",
+                                      dict(n_total=3200, box=32.0, n_beads=50, seed=4, scaled14=0.5), 5.0,
+                                      ["lj_c2", "7", "9", "coul_c2", "9", "11.5"], 11.5, 1),
+    # net charge +0.5 e (odd bead count): the charged-system energy term;
+    # LJ cutoff 10 A beside a PME real-space cutoff of 12 A
+    "polymer3201_charged_pme": ("polymer_water", dict(n_total=3201, box=32.0, n_beads=51, seed=4), 6.0,
+                                ["lj_c2", "8", "10", "pme", "32", "32", "32", "4", "12", "1e-6", "-1", "111"],
+                                12.0, 1),
 }
 
 

@@ -63,7 +63,8 @@ def test_force_line_grammar_errors_are_reported(line, msg):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["water423_ljc2_coulc1", "water1536_ljc2_pme", "water1536_pme_c1",
-                                  "polymer3200_pme_auto"])
+                                  "polymer3200_pme_auto", "polymer26000_pme_auto", "polymer3200_mod14_pme",
+                                  "polymer3200_mod14_ljc2_coulc2", "polymer3201_charged_pme"])
 def test_conf_force_lines_on_gpu_match_reference_fixture(name):
     fx = FX.load(name)
     s = FX.system(fx)

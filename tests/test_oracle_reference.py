@@ -102,3 +102,34 @@ def test_oracle_matches_live_reference_random(seed):
     assert abs(ec - ec_r) <= 1e-11 * max(1.0, abs(ec_r))
     assert abs(elj - elj_r) <= 1e-11 * max(1.0, abs(elj_r))
     assert np.abs(f - f_r).max() <= 1e-11 * max(1.0, np.abs(f_r).max())
+
+
+@pytest.mark.parametrize("name,change", [
+    ("polymer3200_mod14_pme", "csf"), ("polymer3200_mod14_pme", "lj14"), ("polymer3200_mod14_pme_corr", "csf"),
+    ("polymer3200_mod14_ljc2_coulc2", "csf"), ("polymer3201_charged_pme", "neutral"),
+])
+def test_fixtures_pin_modified_and_charged_branches(name, change):
+    """The scaled 1-4 and net-charge fixtures fail an oracle that drops the
+    branch they are there for: coulombScalingFactor taken as 1, the 1-4 LJ
+    columns replaced by the normal ones, or the charged-system energy term
+    left out."""
+    fx = FX.load(name)
+    s = FX.system(fx)
+    sp = FX.spec(fx)
+    if change == "csf":
+        s.csf = 1.0
+    elif change == "lj14":
+        s.lj = s.lj.copy()
+        s.lj[:, :, 2:] = s.lj[:, :, :2]
+    f, ec, elj = O.evaluate(s, oracle_spec(sp), nthreads=8)
+    if change == "neutral":
+        # the same evaluation without the charged-system energy term
+        charged = O.pme_terms(s, oracle_spec(sp))["charged"]
+        assert charged != 0.0
+        ec -= charged
+    idx = fx["force_index"]
+    fref = fx["forces"]
+    ok_f = np.abs(f[idx] - fref).max() <= 1e-10 * max(1.0, np.abs(fref).max())
+    ok_e = (abs(ec - float(fx["e_coulomb"])) <= 1e-10 * max(1.0, abs(float(fx["e_coulomb"])))
+            and abs(elj - float(fx["e_lj"])) <= 1e-10 * max(1.0, abs(float(fx["e_lj"]))))
+    assert not (ok_f and ok_e)
