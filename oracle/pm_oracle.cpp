@@ -109,19 +109,21 @@ struct Pbc {
 
 // ---------------------------------------------------------------------------
 // Exclusion lookup: PM/topology/ExclusionTable.cpp:103-120 (add keeps the
-// last class per pair for lookup and a de-duplicated table in insertion
-// order for the PME correction), ExclusionTable.h:116-131 (check).
+// last class per pair for lookup, and a table in insertion order for the PME
+// correction that holds each pair once, with the class it was first added
+// with: the set it is filtered by orders ExclusionPair on (a1, a2) only,
+// ExclusionTable.cpp:20-27), ExclusionTable.h:116-131 (check).
 struct Excl {
   std::map<std::pair<int, int>, int> cls;
   std::vector<std::array<int, 3>> table;
   void build(const pmo_system *s) {
-    std::set<std::tuple<int, int, int>> seen;
+    std::set<std::pair<int, int>> seen;
     for (int e = 0; e < s->nexcl; ++e) {
       int a = s->excl[3 * e], b = s->excl[3 * e + 1], c = s->excl[3 * e + 2];
       if (a > b) std::swap(a, b);
-      if (c != 0 && a != b && !seen.count({a, b, c})) {
+      if (c != 0 && a != b && !seen.count({a, b})) {
         table.push_back({a, b, c});
-        seen.insert({a, b, c});
+        seen.insert({a, b});
       }
       cls[{a, b}] = c;
     }

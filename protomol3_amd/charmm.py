@@ -1,12 +1,17 @@
 """Minimal CHARMM input readers (PSF, PDB, PAR) that build a MolSystem the way
-ProtoMol's topology builder does for systems of bonds and angles.
+ProtoMol's topology builder does for the nonbonded path.
 
-Follows protomol/src/protomol/io/PSFReader.cpp (atoms, bonds, angles),
-io/PDBReader.cpp (ATOM/HETATM coordinates), io/PARReader.cpp:416-450
-(NONBONDED: atom, ignored, epsilon, Rmin/2 [, ignored, epsilon14, Rmin/2_14]),
-topology/BuildTopology.cpp:92 (scaled charges), :663-673 (CHARMM combining),
-:900-1095 (molecules as connected components) and :1102-1158 (exclusions).
-Dihedrals and impropers are not part of the nonbonded path and are rejected.
+Follows protomol/src/protomol/io/PSFReader.cpp (atoms, bonds, angles,
+dihedrals, impropers), io/PDBReader.cpp (ATOM/HETATM coordinates),
+io/PARReader.cpp:416-450 (NONBONDED: atom, ignored, epsilon, Rmin/2
+[, ignored, epsilon14, Rmin/2_14]), topology/BuildTopology.cpp:92 (scaled
+charges), :663-673 (CHARMM combining), :900-1095 (molecules: connected
+components over bonds, angles, dihedrals and impropers, numbered by lowest
+atom) and :1102-1158 (exclusions by the .conf `exclude` keyword: 1-2 pairs
+from bonds, 1-3 from angles, 1-4 from dihedrals, FULL except 1-4 pairs under
+scaled1-4, which are MODIFIED).  Dihedral and improper energies are not part
+of this engine (the device MD step evaluates bonds and angles only); their
+atom lists are kept for exclusions and molecules.
 """
 from __future__ import annotations
 
@@ -20,7 +25,8 @@ from .systems import FULL, MODIFIED, SQRTCOULOMBCONSTANT, MolSystem
 def read_psf(path: str) -> dict:
     with open(path) as f:
         lines = f.read().splitlines()
-    out = {"atoms": [], "bonds": [], "angles": [], "ndihedrals": 0, "nimpropers": 0}
+    out = {"atoms": [], "bonds": [], "angles": [], "dihedrals": np.zeros((0, 4), np.int32),
+           "impropers": np.zeros((0, 4), np.int32)}
     i = 0
     while i < len(lines):
         ln = lines[i]
@@ -30,21 +36,18 @@ def read_psf(path: str) -> dict:
                 t = lines[i + 1 + k].split()
                 out["atoms"].append(dict(name=t[4], type=t[5], charge=float(t[6]), mass=float(t[7])))
             i += n
-        elif "!NBOND" in ln or "!NTHETA" in ln:
+        elif any(k in ln for k in ("!NBOND", "!NTHETA", "!NPHI", "!NIMPHI")):
             n = int(ln.split()[0])
-            per = 2 if "!NBOND" in ln else 3
+            key, per = next((k, p) for t, k, p in (("!NBOND", "bonds", 2), ("!NTHETA", "angles", 3),
+                                                   ("!NPHI", "dihedrals", 4), ("!NIMPHI", "impropers", 4))
+                             if t in ln)
             vals = []
             j = i + 1
             while len(vals) < n * per:
                 vals += [int(v) - 1 for v in lines[j].split()]
                 j += 1
-            arr = np.array(vals, dtype=np.int32).reshape(n, per)
-            out["bonds" if per == 2 else "angles"] = arr
+            out[key] = np.array(vals, dtype=np.int32).reshape(n, per)
             i = j - 1
-        elif "!NPHI" in ln:
-            out["ndihedrals"] = int(ln.split()[0])
-        elif "!NIMPHI" in ln:
-            out["nimpropers"] = int(ln.split()[0])
         i += 1
     return out
 
@@ -98,11 +101,16 @@ def read_par(path: str) -> dict:
     return par
 
 
+EXCLUDE = ("none", "1-2", "1-3", "1-4", "scaled1-4")
+
+
 def build_system(psf: str, pdb: str, par: str, box, origin=(0.0, 0.0, 0.0), exclude: str = "scaled1-4",
                  csf: float = 1.0, name: str = "charmm") -> MolSystem:
+    """The MolSystem ProtoMol builds from a PSF/PDB/PAR set for the .conf
+    keywords `exclude` and `coulombScalingFactor` (csf)."""
+    if exclude not in EXCLUDE:
+        raise ValueError(f"exclude must be one of {EXCLUDE}")
     p = read_psf(psf)
-    if p["ndihedrals"] or p["nimpropers"]:
-        raise ValueError("dihedrals/impropers are outside the nonbonded engine's topology builder")
     pos = read_pdb(pdb)
     prm = read_par(par)
     atoms = p["atoms"]
@@ -131,7 +139,10 @@ def build_system(psf: str, pdb: str, par: str, box, origin=(0.0, 0.0, 0.0), excl
     apar = np.array([(k, math.radians(t)) for k, t in
                      (prm["angles"][(atoms[a]["type"], atoms[b]["type"], atoms[c]["type"])]
                       for a, b, c in angles)]).reshape(-1, 2)
-    # molecules: connected components over bonds and angles
+    dihedrals = np.asarray(p["dihedrals"], dtype=np.int32).reshape(-1, 4)
+    impropers = np.asarray(p["impropers"], dtype=np.int32).reshape(-1, 4)
+    # molecules: connected components over bonds, angles, dihedrals and
+    # impropers (BuildTopology.cpp:900-1060), numbered by lowest atom
     parent = list(range(n))
 
     def find(x):
@@ -145,17 +156,28 @@ def build_system(psf: str, pdb: str, par: str, box, origin=(0.0, 0.0, 0.0), excl
     for a, b, c in angles:
         parent[find(a)] = find(b)
         parent[find(c)] = find(b)
+    for quad in (*dihedrals, *impropers):
+        for x in quad[1:]:
+            parent[find(x)] = find(quad[0])
     roots = {}
     mol = np.array([roots.setdefault(find(i), len(roots)) for i in range(n)], dtype=np.int32)
-    excl = []
+    # exclusion entries in the reference's insertion order: bonds, angles,
+    # dihedrals (BuildTopology.cpp:1118-1156).  A pair may come twice with
+    # different classes (a 1-3 pair that is also 1-4 around a five-membered
+    # ring); the engine keeps both rules of ExclusionTable::add for it: the
+    # last class for membership, the first for the PME correction table.
+    table = []
     if exclude != "none":
-        excl += [(min(a, b), max(a, b), FULL) for a, b in bonds]
+        table += [(min(a, b), max(a, b), FULL) for a, b in bonds]
         if exclude != "1-2":
-            excl += [(min(a, c), max(a, c), FULL) for a, _, c in angles]
-    ex = np.array(excl, dtype=np.int32).reshape(-1, 3)
+            table += [(min(a, c), max(a, c), FULL) for a, _, c in angles]
+            if exclude != "1-3":
+                cls = FULL if exclude == "1-4" else MODIFIED
+                table += [(min(a, d), max(a, d), cls) for a, _, _, d in dihedrals]
+    ex = np.array(table, dtype=np.int32).reshape(-1, 3)
     return MolSystem(
         name=name, pos=pos, charge=np.array([a["charge"] for a in atoms]) * SQRTCOULOMBCONSTANT,
         atype=np.array([tid[a["type"]] for a in atoms], dtype=np.int32), molecule=mol,
         mass=np.array([a["mass"] for a in atoms]), lj=lj, box=np.asarray(box, dtype=np.float64),
         origin=np.asarray(origin, dtype=np.float64), bonds=bonds, bond_par=bpar, angles=angles,
-        angle_par=apar, exclusions=ex, csf=csf)
+        angle_par=apar, exclusions=ex, csf=csf, dihedrals=dihedrals, impropers=impropers)

@@ -56,6 +56,7 @@ struct Sys {
   int n = 0, npad = 0, ncl = 0, nsuper = 0, ncols = 0;  // nsuper: upper bound (slots = 64 nsuper)
   // host copies
   std::vector<double> q, lj;
+  std::vector<float> lj32;         // FP32 copy of lj (source of ljt32)
   std::vector<int> type;
   std::vector<int4> etab;          // de-duplicated exclusion table a1 a2 cls
   std::vector<int> eptr;           // CSR of partners per atom (orig index)
@@ -84,7 +85,7 @@ struct Sys {
   // device buffers
   std::vector<DevBuf *> all;
   DevBuf xorig, xwrap, q_d, type_d, key, hist, start, cursor, scansum, perm, permp, slot, refcell, xs, xs64, ts,
-      perm2, sc_info, sc_col, colsc, sctr, shalf, sorg, ctr, half, entries, count, ecost, lcost, tep, tcp, tpos, tposS, wstart, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, facc, fout,
+      perm2, sc_info, sc_col, colsc, sctr, shalf, sorg, ctr, half, entries, count, ecost, lcost, tep, tcp, tpos, tposS, wstart, maskwork, nmaskwork, masks, overflow, tiles, ntiles, tilectr, ljtab, ljt32, facc, fout,
       pair_e, npairs, hash, eptr_d, elist_d, etab_d, corr_e, recip_e, eout, np_out, hash_out, grid64, gridfx, spec, spec2,
       greal, theta, twx, twy, twz, border, nborder, border_acc;
   int maxborder = 65536;
@@ -245,7 +246,7 @@ int validate(pmx_ctx *c, const pmx_sys_desc *d) {
   if (!d) return fail(c, PMX_ERR_ARG, "null descriptor");
   if (d->n <= 0 || d->ntypes <= 0 || d->nexcl < 0) return fail(c, PMX_ERR_ARG, "bad sizes");
   if (!d->charge || !d->type || !d->lj || (d->nexcl > 0 && !d->excl)) return fail(c, PMX_ERR_ARG, "null array");
-  if (d->ntypes > kMaxTypes) return fail(c, PMX_ERR_UNSUPPORTED, "more than 16 LJ types");
+  if (d->ntypes > kMaxTypesTotal) return fail(c, PMX_ERR_UNSUPPORTED, "more than 4096 LJ types");
   for (int k = 0; k < 3; ++k)
     if (!(d->box[k] > 0)) return fail(c, PMX_ERR_ARG, "box edge must be > 0");
   if (d->lj_switch < 0 || d->lj_switch > 3 || d->coul_kind < 0 || d->coul_kind > 2)
@@ -315,17 +316,19 @@ int setup(Sys *s) {
   s->type.assign(d.type, d.type + d.n);
   s->lj.assign(d.lj, d.lj + (size_t)d.ntypes * d.ntypes * 4);
 
-  // Exclusions (ExclusionTable.cpp:103-120): table de-duplicated on
-  // (a1,a2,class) in insertion order; lookup class = last added.
-  std::set<std::tuple<int, int, int>> seen;
+  // Exclusions (ExclusionTable.cpp:103-120): the correction table holds each
+  // pair once, in insertion order, with the class it was first added with
+  // (its filter set orders pairs on (a1, a2) only, ExclusionTable.cpp:20-27);
+  // the lookup class of a pair is the last one added.
+  std::set<std::pair<int, int>> seen;
   std::map<std::pair<int, int>, int> cls;
   s->etab.clear();
   for (int e = 0; e < d.nexcl; ++e) {
     int a = d.excl[3 * e], b = d.excl[3 * e + 1], c = d.excl[3 * e + 2];
     if (a > b) std::swap(a, b);
-    if (c != 0 && a != b && !seen.count({a, b, c})) {
+    if (c != 0 && a != b && !seen.count({a, b})) {
       s->etab.push_back(make_int4(a, b, c, 0));
-      seen.insert({a, b, c});
+      seen.insert({a, b});
     }
     if (a != b) cls[{a, b}] = c;
   }
@@ -505,7 +508,8 @@ int setup(Sys *s) {
       (rc = A(s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
       (rc = A(s->nmaskwork, 8)) || (rc = A(s->masks, 16 * 8 * (size_t)s->maxmaskwork)) ||
       (rc = A(s->overflow, 8)) || (rc = A(s->tiles, 16 * (size_t)s->ntiles_cap)) || (rc = A(s->ntiles, 16)) ||
-      (rc = A(s->tilectr, 8)) || (rc = A(s->ljtab, 8 * s->lj.size())) || (rc = A(s->facc, 24 * NP)) ||
+      (rc = A(s->tilectr, 8)) || (rc = A(s->ljtab, 8 * s->lj.size())) ||
+      (rc = A(s->ljt32, 4 * s->lj.size())) || (rc = A(s->facc, 24 * NP)) ||
       (rc = A(s->fout, 24 * N)) || (rc = A(s->pair_e, 16 * (size_t)s->ntiles_cap)) ||
       (rc = A(s->npairs, 8 * (size_t)s->ntiles_cap)) || (rc = A(s->hash, 8 * (size_t)s->ntiles_cap)) ||
       (rc = A(s->eptr_d, 4 * (N + 1))) || (rc = A(s->elist_d, 8 * s->elist.size())) ||
@@ -537,6 +541,9 @@ int setup(Sys *s) {
   HIPCHK(hipMemcpyAsync(s->q_d.p, s->q.data(), 8 * N, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(s->type_d.p, s->type.data(), 4 * N, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(s->ljtab.p, s->lj.data(), 8 * s->lj.size(), hipMemcpyHostToDevice, st));
+  // FP32 copy for the mixed pair kernel when the table does not fit its LDS
+  s->lj32.assign(s->lj.begin(), s->lj.end());
+  HIPCHK(hipMemcpyAsync(s->ljt32.p, s->lj32.data(), 4 * s->lj32.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(s->eptr_d.p, s->eptr.data(), 4 * (N + 1), hipMemcpyHostToDevice, st));
   if (!s->elist.empty())
     HIPCHK(hipMemcpyAsync(s->elist_d.p, s->elist.data(), 8 * s->elist.size(), hipMemcpyHostToDevice, st));
@@ -689,7 +696,9 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
     hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, stats ? 1 : 0,
                        P<int4>(s->tiles), P<int>(s->ntiles), P<int>(s->wstart), P<int>(s->tilectr), P<int4>(s->entries),
                        s->cap, P<unsigned short>(s->masks), P<R4>(s->xs),
-                       P<float4>(s->sorg), P<int>(s->ts), P<int>(s->permp), P<double>(s->ljtab), facc, out);
+                       P<float4>(s->sorg), P<int>(s->ts), P<int>(s->permp), P<double>(s->ljtab),
+                       MIXED ? reinterpret_cast<const R4 *>(s->ljt32.p) : reinterpret_cast<const R4 *>(s->ljtab.p),
+                       facc, out);
     return;
   }
   FixupArgs<MIXED> a{};

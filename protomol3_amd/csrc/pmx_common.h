@@ -15,7 +15,8 @@ constexpr int kWave = 64;
 constexpr int kPairBlock = 256;    // 4 waves
 constexpr double kForceScale = 4294967296.0;      // 2^32: int64 fixed-point forces
 constexpr double kGridScale = 1099511627776.0;    // 2^40: int64 fixed-point PME charge grid
-constexpr int kMaxTypes = 16;
+constexpr int kMaxTypes = 16;        // LJ types whose table the pair kernel keeps in LDS
+constexpr int kMaxTypesTotal = 4096; // LJ types accepted (tables beyond kMaxTypes are read from HBM)
 
 enum { SW_NONE = 0, SW_CUTOFF = 1, SW_C1 = 2, SW_C2 = 3 };
 enum { COUL_NONE = 0, COUL_CUTOFF = 1, COUL_PME = 2 };

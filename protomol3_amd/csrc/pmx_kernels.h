@@ -1389,7 +1389,9 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
     const unsigned short *__restrict__ masks,
     const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
     const float4 *__restrict__ sorg, const int *__restrict__ ts, const int *__restrict__ perm,
-    const double *__restrict__ ljtab, typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
+    const double *__restrict__ ljtab,
+    const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ ljt,
+    typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
   using R = typename std::conditional<MIXED, float, double>::type;
   using R4 = typename Real4<R>::type;
   using Acc = AccT<MIXED>;
@@ -1402,7 +1404,10 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   __shared__ JAtom<R> s_j[W][kGroup * kClusterSize];
 
   const int nt = pp.ntypes;
-  for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) {
+  // up to kMaxTypes LJ types the table sits in LDS; beyond that the typed
+  // copy ljt is read through the caches
+  const bool lj_lds = nt <= kMaxTypes;
+  for (int k = threadIdx.x; lj_lds && k < nt * nt; k += blockDim.x) {
     R4 v;
     v.x = (R)ljtab[4 * k];
     v.y = (R)ljtab[4 * k + 1];
@@ -1492,7 +1497,8 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
         const bool ic = COUL != COUL_NONE && e.r2 <= sc.rc2;
         R A = 0, B = 0;
         if (LJSW != SW_NONE) {
-          const R4 lj = s_lj[s_ty[wv][ia] * nt + jd.ty];
+          const int li = s_ty[wv][ia] * nt + jd.ty;
+          const R4 lj = lj_lds ? s_lj[li] : ljt[li];
           A = mod ? lj.z : lj.x;
           B = mod ? lj.w : lj.y;
         }

@@ -86,6 +86,10 @@ class MolSystem:
     exclusions: np.ndarray     # (NE, 3) int32 a1 < a2, class
     csf: float = 1.0
     vel: np.ndarray | None = None
+    # dihedral and improper atom lists (CHARMM inputs): they define 1-4
+    # exclusions and molecules; their energies are not evaluated here
+    dihedrals: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 4), np.int32))
+    impropers: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 4), np.int32))
 
     @property
     def n(self) -> int:

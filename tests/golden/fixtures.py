@@ -19,12 +19,28 @@ SW_NONE, SW_CUTOFF, SW_C1, SW_C2 = 0, 1, 2, 3
 
 
 def names() -> list[str]:
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz")))
+    """Force fixtures (the topology fixtures topo_*.npz are listed by topo_names)."""
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz"))
+                  if not os.path.basename(p).startswith("topo_"))
+
+
+def topo_names() -> list[str]:
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "topo_*.npz")))
 
 
 def load(name: str) -> dict:
     with np.load(os.path.join(HERE, f"{name}.npz"), allow_pickle=False) as z:
         return {k: z[k] for k in z.files}
+
+
+def charmm_file_system(stem: str, box: float, exclude: str, csf: float):
+    """A reference test molecule (charmm/<stem>.psf/.pdb, CHARMM27
+    parameters) in a cubic periodic box centred on the molecule."""
+    psf, pdb = os.path.join(HERE, "charmm", stem + ".psf"), os.path.join(HERE, "charmm", stem + ".pdb")
+    pos = charmm.read_pdb(pdb)
+    centre = 0.5 * (pos.min(0) + pos.max(0))
+    return charmm.build_system(psf, pdb, os.path.join(W423, "par_all27_prot_lipid.inp"), box=(box, box, box),
+                               origin=tuple(centre - 0.5 * box), exclude=exclude, csf=csf, name=stem)
 
 
 def system(fx: dict):
@@ -33,6 +49,8 @@ def system(fx: dict):
     if gen == "charmm_water423":
         s = charmm.build_system(os.path.join(W423, "water_423.psf"), os.path.join(W423, "water_423.pos.pdb"),
                                 os.path.join(W423, "par_all27_prot_lipid.inp"), box=(28.0, 28.0, 28.0))
+    elif gen == "charmm_file":
+        s = charmm_file_system(**args)
     else:
         s = getattr(systems, gen)(**args)
     if fx["pos"].shape[0]:

@@ -6,7 +6,13 @@ holds /root/reference:
 
     make -f oracle/ref/Makefile -j8 && python tests/golden/make_fixtures.py
 
-Each fixture holds the inputs (positions; the generator and seed that made
+The topology fixtures (topo_*.npz) hold what the reference's topology builder
+(compiled into oracle/_ref/pm_ref_topo) makes of the reference's own test
+molecules in tests/golden/charmm/ for each `exclude` keyword: LJ type, scaled
+charge and molecule of every atom, the LJ table, and the exclusion table with
+the class check() returns for each entry.
+
+Each force fixture holds the inputs (positions; the generator and seed that made
 the rest of the system) and the reference's outputs: forces, Coulomb and
 Lennard-Jones energies, the PME alpha the reference chose, the cell of every
 atom, and the count and set hash of the in-cutoff non-excluded pairs visited
@@ -39,7 +45,24 @@ def make_system(gen: str, args: dict):
     if gen == "charmm_water423":
         return charmm.build_system(os.path.join(W423, "water_423.psf"), os.path.join(W423, "water_423.pos.pdb"),
                                    os.path.join(W423, "par_all27_prot_lipid.inp"), box=(28.0, 28.0, 28.0))
+    if gen == "charmm_file":
+        return charmm_file_system(**args)
     return getattr(systems, gen)(**args)
+
+
+CHARMM = os.path.join(OUT, "charmm")
+PAR27 = os.path.join(W423, "par_all27_prot_lipid.inp")
+
+
+def charmm_file_system(stem: str, box: float, exclude: str, csf: float):
+    """A reference test molecule (tests/golden/charmm/<stem>.psf/.pdb with the
+    CHARMM27 parameters) in a cubic periodic box whose origin puts the
+    molecule's centre at the box centre."""
+    psf, pdb = os.path.join(CHARMM, stem + ".psf"), os.path.join(CHARMM, stem + ".pdb")
+    pos = charmm.read_pdb(pdb)
+    centre = 0.5 * (pos.min(0) + pos.max(0))
+    return charmm.build_system(psf, pdb, PAR27, box=(box, box, box), origin=tuple(centre - 0.5 * box),
+                               exclude=exclude, csf=csf, name=stem)
 
 
 # name: (generator, generator args, cellsize, pm_ref force spec, membership cutoff, force stride)
@@ -77,6 +100,15 @@ CASES = {
     "polymer3200_mod14_ljc2_coulc2": ("polymer_water",
                                       dict(n_total=3200, box=32.0, n_beads=50, seed=4, scaled14=0.5), 5.0,
                                       ["lj_c2", "7", "9", "coul_c2", "9", "11.5"], 11.5, 1),
+    # reference test molecules with dihedrals and impropers (CHARMM27):
+    # 1-4 pairs from the topology builder, rings where a pair is both 1-3 and
+    # 1-4, a protein with net charge
+    "wwdomain_scaled14_pme": ("charmm_file", dict(stem="ww_domain", box=48.0, exclude="scaled1-4", csf=1.0), 5.0,
+                              ["lj_c2", "8", "10", "pme", "48", "48", "48", "4", "10", "1e-6", "-1", "111"], 10.0, 1),
+    "droplet_alan_scaled14_coulc2": ("charmm_file",
+                                     dict(stem="large_water_droplet_alan", box=40.0, exclude="scaled1-4",
+                                          csf=0.8333), 5.0,
+                                     ["lj_c2", "8", "10", "coul_c2", "8", "10"], 10.0, 1),
     # net charge +0.5 e (odd bead count): the charged-system energy term;
     # LJ cutoff 10 A beside a PME real-space cutoff of 12 A
     "polymer3201_charged_pme": ("polymer_water", dict(n_total=3201, box=32.0, n_beads=51, seed=4), 6.0,
@@ -118,11 +150,47 @@ def run_case(name: str, gen: str, args: dict, cellsize: float, spec: list, rc: f
         pair_hash=np.uint64(phash))
 
 
+PM_REF_TOPO = os.path.join(ROOT, "oracle", "_ref", "pm_ref_topo")
+
+# topology fixtures: (molecule stem, exclude keyword) -> the reference
+# topology builder's output for that PSF with the CHARMM27 parameters
+TOPO_CASES = {
+    "ww_domain_scaled14": ("ww_domain", "scaled1-4"),
+    "ww_domain_one4": ("ww_domain", "1-4"),
+    "ww_domain_one3": ("ww_domain", "1-3"),
+    "droplet_alan_scaled14": ("large_water_droplet_alan", "scaled1-4"),
+    "droplet_alan_one2": ("large_water_droplet_alan", "1-2"),
+}
+
+
+def run_topo_case(stem: str, exclude: str) -> dict:
+    with tempfile.TemporaryDirectory() as td:
+        of = os.path.join(td, "topo")
+        subprocess.run([PM_REF_TOPO, os.path.join(CHARMM, stem + ".psf"), PAR27, exclude, of], check=True,
+                       capture_output=True)
+        with open(of) as f:
+            n, t, ne = (int(v) for v in f.readline().split())
+            atoms = np.array([f.readline().split() for _ in range(n)], dtype=object)
+            lj = np.array([[float(v) for v in f.readline().split()] for _ in range(t * t)]).reshape(t, t, 4)
+            table = np.array([[int(v) for v in f.readline().split()] for _ in range(ne)],
+                             dtype=np.int32).reshape(-1, 4)
+            counts = np.array([int(v) for v in f.readline().split()], dtype=np.int32)
+    return dict(stem=np.array(stem), exclude=np.array(exclude), type=atoms[:, 0].astype(np.int32),
+                scaled_charge=atoms[:, 1].astype(np.float64), molecule=atoms[:, 2].astype(np.int32), lj=lj,
+                table=table, counts=counts)
+
+
 def main():
     if not os.path.exists(PM_REF):
         sys.exit(f"{PM_REF} missing: build it with make -f oracle/ref/Makefile")
-    names = sys.argv[1:] or list(CASES)
+    names = sys.argv[1:] or list(CASES) + [f"topo_{k}" for k in TOPO_CASES]
     for name in names:
+        if name.startswith("topo_"):
+            d = run_topo_case(*TOPO_CASES[name[5:]])
+            np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **d)
+            print(f"{name}: N={len(d['type'])} types={d['lj'].shape[0]} exclusions={len(d['table'])} "
+                  f"bonded={d['counts'].tolist()}")
+            continue
         d = run_case(name, *CASES[name])
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **d)
         print(f"{name}: N={int(d['n'])} Ecoul={float(d['e_coulomb']):.10g} Elj={float(d['e_lj']):.10g} "
