@@ -62,6 +62,7 @@ struct Sys {
   std::vector<int> eptr;           // CSR of partners per atom (orig index)
   std::vector<int2> elist;
   double alpha = 0, self_e = 0, charged_e = 0;
+  double qpair = 1.0;              // charge factor of the pair pass (0: PME real term off)
   Box box{};
   BinParams bin{};
   PairParams pp{};
@@ -428,6 +429,9 @@ int setup(Sys *s) {
   }
 
   s->pme_on = d.coul_kind == COUL_PME;
+  // PME without -real: the pair pass still decides membership (pair
+  // statistics) but its Coulomb part sees zero charges
+  s->qpair = s->pme_on && !(d.pme_terms & PMX_PME_REAL) ? 0.0 : 1.0;
   s->alpha = 0;
   s->self_e = s->charged_e = 0;
   if (s->pme_on) {
@@ -913,13 +917,13 @@ int rebuild(Sys *s, bool timing) {
                        P<int2>(s->sc_info), P<int>(s->perm2), P<double>(s->xwrap), P<int>(s->permp), P<int>(s->slot),
                        P<double>(s->q_d), P<int>(s->type_d), P<float4>(s->xs), P<double4>(s->xs64), P<int>(s->ts),
                        P<double>(s->xorig), xbuild, P<float4>(s->sorg), P<float4>(s->ctr), P<float4>(s->half),
-                       P<float4>(s->sctr), P<float4>(s->shalf), P<int>(s->nmaskwork));
+                       P<float4>(s->sctr), P<float4>(s->shalf), P<int>(s->nmaskwork), s->qpair);
   else
     hipLaunchKernelGGL(k_split_fill<double>, dim3(nblk((long long)s->nsuper * 64, 256)), dim3(256), 0, st, s->nsuper,
                        P<int2>(s->sc_info), P<int>(s->perm2), P<double>(s->xwrap), P<int>(s->permp), P<int>(s->slot),
                        P<double>(s->q_d), P<int>(s->type_d), P<double4>(s->xs), P<double4>(s->xs64), P<int>(s->ts),
                        P<double>(s->xorig), xbuild, P<float4>(s->sorg), P<float4>(s->ctr), P<float4>(s->half),
-                       P<float4>(s->sctr), P<float4>(s->shalf), P<int>(s->nmaskwork));
+                       P<float4>(s->sctr), P<float4>(s->shalf), P<int>(s->nmaskwork), s->qpair);
   if (timing) hipEventRecord(s->ev[1], st);
   hipLaunchKernelGGL(k_build_list, dim3(s->nsuper), dim3(kListBlock), 0, st, s->lp, P<int>(s->sc_col),
                      P<int>(s->colsc), P<float4>(s->sctr), P<float4>(s->shalf), P<float4>(s->ctr),

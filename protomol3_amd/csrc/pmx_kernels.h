@@ -359,7 +359,8 @@ constexpr float kScSlack = 1e-3f;
 //    quarters by z, so slots 8c..8c+7 form compact clusters; permp (slot ->
 //    atom, -1 padding) and slot (atom -> slot) are written;
 //  - sorted SoA: coordinates relative to the supercluster origin (the
-//    position of the first slot's atom rounded to 1/16 A, in sorg) + charge,
+//    position of the first slot's atom rounded to 1/16 A, in sorg) + charge
+//    times qpair (0 when the pair pass carries no Coulomb term),
 //    FP64 copy in xs64, LJ type (padding: type -1); xbuild (MD only, else
 //    null) receives the positions the list is built from;
 //  - bounding boxes (FP32, list building only): per cluster (8-lane groups;
@@ -376,7 +377,8 @@ __global__ void __launch_bounds__(256) k_split_fill(int nsuper_max, const int2 *
                                                     const double *__restrict__ xorig, double *__restrict__ xbuild,
                                                     float4 *__restrict__ sorg, float4 *__restrict__ ctr,
                                                     float4 *__restrict__ half, float4 *__restrict__ sctr,
-                                                    float4 *__restrict__ shalf, int *__restrict__ nmaskwork) {
+                                                    float4 *__restrict__ shalf, int *__restrict__ nmaskwork,
+                                                    double qpair) {
   const int sc = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (blockIdx.x == 0 && threadIdx.x == 0) *nmaskwork = 0;
@@ -416,7 +418,7 @@ __global__ void __launch_bounds__(256) k_split_fill(int nsuper_max, const int2 *
     v.x = (R)(p[0] - o[0]);
     v.y = (R)(p[1] - o[1]);
     v.z = (R)(p[2] - o[2]);
-    v.w = (R)q[a];
+    v.w = (R)(q[a] * qpair);
     ts[s] = type[a];
     xs64[s] = make_double4(p[0], p[1], p[2], q[a]);
   } else {
