@@ -108,6 +108,21 @@ int pmx_sys_set_positions(pmx_sys *sys, const double *x_aos);
 int pmx_sys_compute(pmx_sys *sys, unsigned flags);
 int pmx_sys_get_forces(pmx_sys *sys, double *f_aos, double *energies);
 int pmx_sys_sync(pmx_sys *sys);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pmx_sys_compute with no allocation, ever.  The force
+ * pass's per-pass scratch (per-tile energy and pair-statistic partials, the
+ * boundary-pair records and their counters) lives in one static device
+ * array of 32 MiB per device, held until the process ends, instead of in the
+ * system's buffers; a pass that would need more (about 1,000,000 atoms at
+ * a 12 A list radius) is rejected with PMX_ERR_ARG.  A list that overflows
+ * its capacity is reported by pmx_sys_get_forces as PMX_ERR_OVERFLOW and is
+ * not regrown (pmx_sys_evaluate grows it and retries).  Also rejected:
+ * PMX_EVAL_TIMING, and a system whose tables pmx_sys_uncache dropped.
+ * Every call on one device shares that device's array: calls that overlap
+ * on one GPU (systems of different contexts, or threads) race.  Results are
+ * read with pmx_sys_get_forces as after pmx_sys_compute. */
+int pmx_sys_compute_noalloc(pmx_sys *sys, unsigned flags);
 
 /* Device bytes pmx_sys_create would allocate for desc, computed on the host
  * without allocating (kind 1 adds the device MD state of pmx_md_init, with

@@ -25,12 +25,28 @@ using namespace pmx;
 struct pmx_ctx {
   int device = 0;
   int cus = 256;  // compute units of the device
+  void *scratch = nullptr;  // device address of this device's copy of `scratch` (no-allocation force pass)
   hipStream_t stream = nullptr;   // main stream: sort, list, pair kernel, boundary pairs, finalize
   hipStream_t stream2 = nullptr;  // PME, exclusion correction and bonded terms, concurrent with the pair kernel
   std::string err;
 };
 
+// Scratch of pmx_sys_compute_noalloc: the per-pass partial results, counters
+// and records of a force pass (per-tile energies and pair statistics, the
+// boundary-pair records and their counters) live here instead of in the
+// system's own buffers.  Each device holds one copy for the life of the
+// process, shared by every call on that device.  In host code the name is a
+// stand-in, not a device address: host code uses only the address
+// hipGetSymbolAddress returns for the current device (pmx_ctx::scratch).
+constexpr size_t kScratchBytes = 32ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+// Device address of `scratch` per device, looked up once per device (with
+// that device current) by the first pmx_ctx_create on it.
+constexpr int kMaxDevices = 64;
+void *g_scratch_addr[kMaxDevices] = {};
 
 struct DevBuf {
   void *p = nullptr;
@@ -1141,8 +1157,15 @@ int pmx_ctx_create(int device, pmx_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return PMX_ERR_HIP;
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PMX_ERR_NODEV;
+  if (device >= kMaxDevices) return PMX_ERR_UNSUPPORTED;
+  if (!g_scratch_addr[device]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return PMX_ERR_HIP;
+    g_scratch_addr[device] = ptr;
+  }
   pmx_ctx *c = new pmx_ctx();
   c->device = device;
+  c->scratch = g_scratch_addr[device];
   c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
@@ -1235,6 +1258,44 @@ int pmx_sys_compute(pmx_sys *sys, unsigned flags) {
     if (rc) return rc;
   }
   return compute(s, flags);
+}
+
+int pmx_sys_compute_noalloc(pmx_sys *sys, unsigned flags) {
+  if (!sys) return PMX_ERR_ARG;
+  Sys *s = &sys->s;
+  if (flags & PMX_EVAL_TIMING)
+    return fail(s->ctx, PMX_ERR_ARG, "pmx_sys_compute_noalloc: PMX_EVAL_TIMING needs event creation");
+  if (!s->have_positions) return fail(s->ctx, PMX_ERR_ARG, "positions not set");
+  if (!s->cached)
+    return fail(s->ctx, PMX_ERR_ARG,
+                "pmx_sys_compute_noalloc: tables dropped by pmx_sys_uncache (pmx_sys_compute rebuilds them)");
+  if (!s->ctx->scratch) return fail(s->ctx, PMX_ERR_NODEV, "no scratch address for this device");
+  // layout: counters (tile counter, boundary count, 4 boundary sums), then
+  // boundary records, per-tile energies, pair counts and hashes
+  auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+  const size_t nb = (size_t)s->maxborder, nt = (size_t)s->ntiles_cap;
+  const size_t o_border = 256, o_pe = up(o_border + 8 * nb), o_np = up(o_pe + 16 * nt), o_h = up(o_np + 8 * nt),
+               end = o_h + 8 * nt;
+  if (end > kScratchBytes)
+    return fail(s->ctx, PMX_ERR_ARG,
+                "pmx_sys_compute_noalloc: the pass needs " + std::to_string(end) + " bytes of scratch, the maximum is " +
+                    std::to_string(kScratchBytes) + " (about 1,000,000 atoms at a 12 A list radius)");
+  hipSetDevice(s->ctx->device);
+  unsigned char *base = static_cast<unsigned char *>(s->ctx->scratch);
+  DevBuf *bufs[7] = {&s->tilectr, &s->nborder, &s->border_acc, &s->border, &s->pair_e, &s->npairs, &s->hash};
+  void *const addr[7] = {base, base + 8, base + 16, base + o_border, base + o_pe, base + o_np, base + o_h};
+  void *saved[7];
+  for (int k = 0; k < 7; ++k) {
+    saved[k] = bufs[k]->p;
+    bufs[k]->p = addr[k];
+  }
+  // the counters keep the last call's values (another system may have used
+  // them): clear them with a fill kernel, which graph capture also records
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->ctx->stream, reinterpret_cast<unsigned long long *>(base),
+                     0ull, 6ll);
+  const int rc = compute(s, flags);
+  for (int k = 0; k < 7; ++k) bufs[k]->p = saved[k];
+  return rc;
 }
 
 int pmx_sys_get_forces(pmx_sys *sys, double *f_aos, double *energies) {

@@ -66,7 +66,7 @@ EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_crea
             "pmx_sys_cells", "pmx_sys_stage_times", "pmx_sys_timing_begin", "pmx_sys_timing_end",
             "pmx_md_init", "pmx_md_set_state", "pmx_md_run", "pmx_md_get_state", "pmx_md_stats",
             "pmx_langevin_noise", "pmx_sys_device_bytes", "pmx_estimate_bytes", "pmx_md_run_many",
-            "pmx_sys_time_pair", "pmx_sys_set_tuning"]
+            "pmx_sys_time_pair", "pmx_sys_set_tuning", "pmx_sys_compute_noalloc"]
 
 _lib = None
 
@@ -91,6 +91,7 @@ def load() -> C.CDLL:
     L.pmx_sys_evaluate.argtypes = [vp, dp, dp, dp, C.c_uint]
     L.pmx_sys_set_positions.argtypes = [vp, dp]
     L.pmx_sys_compute.argtypes = [vp, C.c_uint]
+    L.pmx_sys_compute_noalloc.argtypes = [vp, C.c_uint]
     L.pmx_sys_get_forces.argtypes = [vp, dp, dp]
     L.pmx_sys_sync.argtypes = [vp]
     L.pmx_sys_alpha.argtypes = [vp, dp]
@@ -315,6 +316,11 @@ class System:
 
     def compute(self, flags: int = 0):
         self.ctx.check(self.ctx.lib.pmx_sys_compute(self.h, flags), "pmx_sys_compute")
+
+    def compute_noalloc(self, flags: int = 0):
+        """pmx_sys_compute without any device allocation (the pass's scratch
+        is the library's static per-device array)."""
+        self.ctx.check(self.ctx.lib.pmx_sys_compute_noalloc(self.h, flags), "pmx_sys_compute_noalloc")
 
     def sync(self):
         self.ctx.check(self.ctx.lib.pmx_sys_sync(self.h), "pmx_sys_sync")
