@@ -142,6 +142,49 @@ def large(ctx, steps: int, skin: float, log=print):
     return out
 
 
+def drift_study(ctx, s, spec, ps: float = 10.0, log=print):
+    """NVE total-energy drift of the 26,000-atom workload against the time
+    step: for each form and h in (1, 0.5, 0.25) fs, ps picoseconds of
+    leapfrog from the same start, total energy sampled every 0.1 ps, slope of
+    a least-squares line (kcal/mol/ns); and at 1 fs the same run with the
+    pair list rebuilt every step (skin 0) beside the tuned skin of 1.5 A.
+    Leapfrog's energy error is O(h^2), so a drift caused by the time step
+    falls about 4x per halving of h, and one caused by a force that is not the
+    gradient of the energy does not."""
+    from protomol3_amd import pmx
+    out = []
+    cases = [(pmx.FP64, h, 1.5) for h in (1.0, 0.5, 0.25)] + [(pmx.FP64, 1.0, 0.0)] + \
+            [(pmx.MIXED, h, 1.5) for h in (1.0, 0.5, 0.25)]
+    for prec, h, skin in cases:
+        sy = pmx.System(ctx, s, spec, prec)
+        try:
+            sy.md_init(s, h, skin=skin)
+            sy.md_set_state(s.pos, s.vel)
+            per = int(round(100.0 / h))  # steps per 0.1 ps
+            nsamp = int(round(ps * 10))
+            t0 = time.perf_counter()
+            sy.md_run(1)
+            _, _, e = sy.md_get_state()
+            etot, kin = [sum(e.values())], [e.get("kinetic", float("nan"))]
+            for _ in range(nsamp):
+                sy.md_run(per)
+                _, _, e = sy.md_get_state()
+                etot.append(sum(e.values()))
+                kin.append(e.get("kinetic", float("nan")))
+            wall = time.perf_counter() - t0
+        finally:
+            sy.close()
+        tns = np.arange(len(etot)) * 0.1e-3
+        slope = float(np.polyfit(tns, etot, 1)[0])
+        r = {"form": "mixed" if prec == pmx.MIXED else "fp64", "timestep_fs": h, "skin_A": skin, "ps": ps,
+             "drift_kcal_mol_per_ns": round(slope, 3), "etot_start": round(etot[0], 4),
+             "etot_end": round(etot[-1], 4), "etot_std": round(float(np.std(etot)), 4),
+             "kinetic_start": round(kin[0], 3), "kinetic_end": round(kin[-1], 3), "wall_s": round(wall, 2)}
+        log(f"drift: {r}")
+        out.append(r)
+    return {"workload": "polymer_water26000 NVE leapfrog from the relaxed 300 K start", "runs": out}
+
+
 def time_pair(ctx, s, spec, x, skin, launch=None, reps: int = 200) -> float:
     """Average device ms of one k_pair launch (mixed form) on the list an MD
     step with this skin builds at positions x: the kernel alone, launched
@@ -251,26 +294,47 @@ def replicas(nsys: int, steps: int, lanes: int, seed: int = 11, log=print):
                         "PME ~1 A grid, Langevin 300 K 1/ps, mixed, one pmx_md_run_many call"}
 
 
+# Reference build / oracle time per force evaluation of the 26,000-atom workload
+# on one core of the same host (profiles/cpu_calibration.md, oracle/calibrate.py)
+REF_OVER_ORACLE_1T = 1.015
+
+
 def cpu_baseline(s, budget_s: float = 20.0):
     """The oracle (FP64 C++ restatement of the reference path + leapfrog) on
-    the host's cores: whole MD steps of the same workload until ~budget_s."""
+    the host: whole MD steps of the same workload, first on one thread, then
+    on every core the box gives this job (at most 16), about budget_s in all.
+    The reference itself cannot run on the GPU host; its speed there is
+    estimated from the serial figure with the reference/oracle time ratio
+    measured on one core of another host."""
     from oracle import md
     from oracle import oracle as O
-    cores = min(os.cpu_count() or 1, 16)
+    nproc = os.cpu_count() or 1
+    cores = min(nproc, 16)
     ospec = O.make_spec(6.0, lj=(O.SW_C2, 10.0, 12.0),
                         pme=dict(rc=12.0, grid=(64, 64, 64), order=4, alpha=-1.0, accuracy=1e-6))
     x, v, f, _ = md.leapfrog_run(s, ospec, s.pos, s.vel, TIMESTEP_FS, 1, nthreads=cores)  # relaxed start
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        x, v, f, _ = md.leapfrog_run(s, ospec, x, v, TIMESTEP_FS, 1, f=f, nthreads=cores)
-        n += 1
-        if time.perf_counter() - t0 > budget_s or n >= 50:
-            break
-    dt = (time.perf_counter() - t0) / n
-    return dict(value=round(ns_day(dt * 1e3), 5), unit="ns/day", cores=cores, kind="port",
-                sample=f"{n} leapfrog MD steps of the 26,000-atom workload (oracle FP64 nonbonded on {cores} "
-                       f"threads + bonded + integrator), {dt:.3f} s per step")
+
+    def steps_for(nthreads, budget):
+        nonlocal x, v, f
+        n, t0 = 0, time.perf_counter()
+        while True:
+            x, v, f, _ = md.leapfrog_run(s, ospec, x, v, TIMESTEP_FS, 1, f=f, nthreads=nthreads)
+            n += 1
+            if time.perf_counter() - t0 > budget or n >= 50:
+                return n, (time.perf_counter() - t0) / n
+
+    n1, dt1 = steps_for(1, budget_s / 3.0)
+    nc, dtc = steps_for(cores, 2.0 * budget_s / 3.0)
+    return dict(value=round(ns_day(dtc * 1e3), 5), unit="ns/day", cores=cores, kind="port", nproc=nproc,
+                sample=f"{nc} leapfrog MD steps of the 26,000-atom workload (oracle FP64 nonbonded on {cores} "
+                       f"threads + bonded + integrator), {dtc:.3f} s per step; host has {nproc} logical cores",
+                serial={"value": round(ns_day(dt1 * 1e3), 5), "cores": 1, "s_per_step": round(dt1, 4),
+                        "steps": n1},
+                reference_serial_estimate={
+                    "value": round(ns_day(dt1 * REF_OVER_ORACLE_1T * 1e3), 5), "cores": 1,
+                    "ref_over_oracle_1thread": REF_OVER_ORACLE_1T,
+                    "note": "serial oracle time x the reference/oracle ratio measured on identical input "
+                            "(profiles/cpu_calibration.md); no MPI toolchain on the host"})
 
 
 def main():
@@ -291,6 +355,8 @@ def main():
                     help="only the 999,000-atom water box, both forms timed per stage (--steps steps, default 1000)")
     ap.add_argument("--pair-only", action="store_true",
                     help="only time the pair kernel alone (at --skin, default 0.7 A) and print it")
+    ap.add_argument("--drift-study", action="store_true",
+                    help="only the NVE drift against the time step (1, 0.5, 0.25 fs; skin 0 and 1.5 A)")
     ap.add_argument("--replica-steps", type=int, default=500)
     ap.add_argument("--replica-lanes", type=int, default=4)
     a = ap.parse_args()
@@ -309,6 +375,10 @@ def main():
     s, spec = workload(ctx)
     log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
     log(f"{s.n} atoms, steps={a.steps} warmup={a.warmup}")
+    if a.drift_study:
+        print(json.dumps(drift_study(ctx, s, spec, log=log)), flush=True)
+        ctx.close()
+        return
     if a.pair_only:
         skin = a.skin if a.skin >= 0 else 0.7
         ms = time_pair(ctx, s, spec, s.pos, skin)

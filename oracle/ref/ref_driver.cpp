@@ -52,6 +52,7 @@ using namespace ProtoMol::Report;
 #include "BSpline.h"
 #include "NonbondedPMEwaldSystemForce.h"
 
+#include <sys/time.h>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -218,6 +219,30 @@ int main(int argc, char **argv) {
     owned[n]->preProcess(&topo, &pos);
     owned[n]->evaluate(&topo, &pos, &forces, &energies);
     owned[n]->postProcess(&topo, &energies, &forces);
+  }
+  // With env PM_REF_TIME_REPS=k: k more evaluations of the same forces into
+  // scratch arrays (the first one above set up the PME tables), timed
+  // together; prints "eval_s <seconds per evaluation>".
+  if (const char *reps_s = std::getenv("PM_REF_TIME_REPS")) {
+    const int reps = std::atoi(reps_s);
+    Vector3DBlock fs;
+    fs.resize(N);
+    ScalarStructure es;
+    struct timeval tv0, tv1;
+    gettimeofday(&tv0, 0);
+    for (int r = 0; r < reps; ++r) {
+      fs.zero();
+      es.clear();
+      topo.uncacheCellList();
+      for (size_t n = 0; n < owned.size(); ++n) {
+        owned[n]->preProcess(&topo, &pos);
+        owned[n]->evaluate(&topo, &pos, &fs, &es);
+        owned[n]->postProcess(&topo, &es, &fs);
+      }
+    }
+    gettimeofday(&tv1, 0);
+    const double dt = (tv1.tv_sec - tv0.tv_sec) + 1e-6 * (tv1.tv_usec - tv0.tv_usec);
+    std::printf("eval_s %.6f\n", reps > 0 ? dt / reps : 0.0);
   }
 
   // Membership dump: every atom's cell and the cell-pair x atom-pair visit order

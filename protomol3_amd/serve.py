@@ -17,7 +17,7 @@ class Request(C.Structure):
     _fields_ = [("kind", C.c_int), ("sys", C.POINTER(pmx.SysDesc)), ("md", C.POINTER(pmx.MdDesc)),
                 ("x", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)), ("nsteps", C.c_int),
                 ("out_a", C.POINTER(C.c_double)), ("out_v", C.POINTER(C.c_double)),
-                ("out_energies", C.POINTER(C.c_double))]
+                ("out_energies", C.POINTER(C.c_double)), ("session", C.c_longlong)]
 
 
 _lib = None
@@ -35,6 +35,11 @@ def load():
     L.pmx_server_create.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)]
     L.pmx_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_longlong)]
     L.pmx_server_wait.argtypes = [vp, C.c_longlong, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.pmx_server_wait_times.argtypes = [vp, C.c_longlong, C.POINTER(C.c_double)]
+    L.pmx_server_open.argtypes = [vp, C.POINTER(pmx.SysDesc), C.POINTER(pmx.MdDesc), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    L.pmx_server_close.argtypes = [vp, C.c_longlong]
+    L.pmx_server_stats.argtypes = [vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_int)]
     L.pmx_server_pending.argtypes = [vp, C.POINTER(C.c_int)]
     L.pmx_server_destroy.argtypes = [vp]
     L.pmx_server_last_error.argtypes = [vp]
@@ -52,7 +57,51 @@ class Ticket:
         self.v = np.zeros((n, 3)) if kind == REQ_TRAJECTORY else None
         self.e = np.zeros(5 if kind == REQ_TRAJECTORY else 2)
         self.id = C.c_longlong()
+        # ms since the request's submit (start, done) and since the server's
+        # creation (submit_at, start_at, done_at)
         self.start_ms = self.done_ms = None
+        self.submit_at = self.start_at = self.done_at = None
+
+
+class Session:
+    """A system kept resident on the device between requests
+    (pmx_server_open / pmx_server_close)."""
+
+    def __init__(self, srv, sid, n, keep, md: bool):
+        self.srv, self.id, self.n, self.keep, self.md = srv, sid, n, keep, md
+
+    def forces(self, x: np.ndarray) -> Ticket:
+        """Forces and energies at positions x (a session opened without MD)."""
+        x = np.ascontiguousarray(x, np.float64)
+        pmx._check_shape(x, (self.n, 3), "positions")
+        t = Ticket(REQ_FORCES, self.n, [x])
+        r = Request(kind=REQ_FORCES, x=pmx._dp(x), out_a=pmx._dp(t.a), out_energies=pmx._dp(t.e), session=self.id)
+        t.keep.append(r)
+        self.srv._submit(r, t)
+        return t
+
+    def run(self, nsteps: int, x: np.ndarray | None = None, v: np.ndarray | None = None) -> Ticket:
+        """nsteps more MD steps from the resident state (or from x, v)."""
+        t = Ticket(REQ_TRAJECTORY, self.n, [])
+        r = Request(kind=REQ_TRAJECTORY, nsteps=nsteps, out_a=pmx._dp(t.a), out_v=pmx._dp(t.v),
+                    out_energies=pmx._dp(t.e), session=self.id)
+        if x is not None:
+            x = np.ascontiguousarray(x, np.float64)
+            v = np.ascontiguousarray(v, np.float64)
+            pmx._check_shape(x, (self.n, 3), "positions")
+            pmx._check_shape(v, (self.n, 3), "velocities")
+            t.keep += [x, v]
+            r.x, r.v = pmx._dp(x), pmx._dp(v)
+        t.keep.append(r)
+        self.srv._submit(r, t)
+        return t
+
+    def close(self):
+        if self.id:
+            rc = self.srv.lib.pmx_server_close(self.srv.h, self.id)
+            self.id = 0
+            if rc:
+                raise pmx.PmxError(rc, "pmx_server_close")
 
 
 class Server:
@@ -86,17 +135,44 @@ class Server:
         self._submit(r, t)
         return t
 
+    def open(self, s: systems.MolSystem, spec: pmx.ForceSpec, precision: int, md: dict | None = None) -> Session:
+        """Opens a session: the system stays resident on the device until
+        close().  md: None for a force-only session, else the arguments of
+        make_md_desc after the system (timestep_fs, skin, integrator,
+        temperature, gamma, seed); the state starts at s.pos, s.vel."""
+        d, keep = pmx.make_desc(s, spec, precision)
+        keep = [d, keep]
+        sid = C.c_longlong()
+        if md is None:
+            rc = self.lib.pmx_server_open(self.h, C.byref(d), None, None, None, C.byref(sid))
+        else:
+            m, mkeep = pmx.make_md_desc(s, **md)
+            x = np.ascontiguousarray(s.pos, np.float64)
+            v = np.ascontiguousarray(s.vel, np.float64)
+            keep += [m, mkeep, x, v]
+            rc = self.lib.pmx_server_open(self.h, C.byref(d), C.byref(m), pmx._dp(x), pmx._dp(v), C.byref(sid))
+        if rc:
+            raise pmx.PmxError(rc, "pmx_server_open")
+        return Session(self, sid.value, s.n, keep, md is not None)
+
+    def stats(self) -> tuple[int, int]:
+        """(largest footprint sum resident at once, most items running at once)."""
+        b, r = C.c_ulonglong(), C.c_int()
+        self.lib.pmx_server_stats(self.h, C.byref(b), C.byref(r))
+        return int(b.value), int(r.value)
+
     def _submit(self, r, t):
         rc = self.lib.pmx_server_submit(self.h, C.byref(r), C.byref(t.id))
         if rc:
             raise pmx.PmxError(rc, "pmx_server_submit")
 
     def wait(self, t: Ticket) -> Ticket:
-        s0, s1 = C.c_double(), C.c_double()
-        rc = self.lib.pmx_server_wait(self.h, t.id, C.byref(s0), C.byref(s1))
+        tm = (C.c_double * 3)()
+        rc = self.lib.pmx_server_wait_times(self.h, t.id, tm)
         if rc:
             raise pmx.PmxError(rc, "request failed: " + self.lib.pmx_server_last_error(self.h).decode())
-        t.start_ms, t.done_ms = s0.value, s1.value
+        t.submit_at, t.start_at, t.done_at = tm[0], tm[1], tm[2]
+        t.start_ms, t.done_ms = tm[1] - tm[0], tm[2] - tm[0]
         return t
 
     def close(self):

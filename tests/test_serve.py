@@ -79,3 +79,107 @@ def test_gpu_requests_alone_and_in_the_mix_agree():
         print(f"n={s.n} kind={kind} max diff {d:.2e} energy diff {de:.2e} start {t.start_ms:.1f} ms "
               f"done {t.done_ms:.1f} ms", flush=True)
         assert d <= 1e-12 and de <= 1e-9
+
+
+MD = dict(timestep_fs=1.0, skin=1.0, integrator=pmx.LANGEVIN, temperature=300.0, gamma=1.0, seed=5)
+
+
+@pytest.mark.gpu
+def test_gpu_session_keeps_the_system_resident_and_resumes():
+    """A session's trajectory requests continue from the resident state: two
+    requests of 25 and 35 steps give, to the last bit, what one system run
+    for 25 and then 35 steps gives; force requests on a force session match
+    one-shot requests at the same positions."""
+    s = systems.replica_box(5000, 31).with_velocities(300.0, seed=32)
+    spec = serve.request_spec(s)
+    ctx = pmx.Context(0)
+    try:
+        sy = pmx.System(ctx, s, spec, pmx.MIXED)
+        try:
+            sy.md_init(s, **MD)
+            sy.md_set_state(s.pos, s.vel)
+            sy.md_run(25)
+            x25, v25, _ = sy.md_get_state()
+            sy.md_run(35)
+            x60, v60, e60 = sy.md_get_state()
+        finally:
+            sy.close()
+    finally:
+        ctx.close()
+    srv = serve.Server(0, slots=2)
+    try:
+        traj = srv.open(s, spec, pmx.MIXED, MD)
+        frc = srv.open(s, spec, pmx.MIXED)
+        t1, t2 = traj.run(25), traj.run(35)
+        f1, f2 = frc.forces(x25), frc.forces(x60)
+        one = srv.submit_forces(s, spec, pmx.MIXED, x=x60)
+        traj.close()
+        frc.close()
+        for t in (t1, t2, f1, f2, one):
+            srv.wait(t)
+        peak, _ = srv.stats()
+    finally:
+        srv.close()
+    print(f"session: dx25={np.abs(t1.a - x25).max():.2e} dx60={np.abs(t2.a - x60).max():.2e} "
+          f"df={np.abs(f2.a - one.a).max():.2e} peak={peak}", flush=True)
+    assert np.array_equal(t1.a, x25) and np.array_equal(t1.v, v25)
+    assert np.array_equal(t2.a, x60) and np.array_equal(t2.v, v60)
+    assert np.allclose(t2.e, [e60[k] for k in ("coulomb", "lj", "bond", "angle", "kinetic")], rtol=1e-12, atol=0)
+    assert np.array_equal(f2.a, one.a) and np.array_equal(f2.e, one.e)
+    assert not np.array_equal(f1.a, f2.a)
+
+
+@pytest.mark.gpu
+def test_gpu_admission_keeps_resident_footprints_within_the_budget():
+    """Serving-size requests (1,000-100,000 atoms) under a 320 MB budget: the
+    footprints resident at any moment (running one-shot requests plus open
+    sessions) never exceed it, so some requests wait for others to finish;
+    every result equals the same request run alone."""
+    sizes = [100000, 90000, 60000, 40000, 20000, 10000, 5000, 3000, 2000, 1000, 70000, 30000]
+    reqs = []
+    for i, n in enumerate(sizes):
+        s = systems.replica_box(n, 700 + i).with_velocities(300.0, seed=800 + i)
+        reqs.append((s, serve.request_spec(s), i % 2))
+    budget = 320 << 20
+    srv = serve.Server(0, slots=4, hbm_budget=budget)
+    try:
+        ts = []
+        for s, spec, kind in reqs:
+            if kind == 0:
+                ts.append(srv.submit_forces(s, spec, pmx.MIXED))
+            else:
+                ts.append(srv.submit_trajectory(s, spec, pmx.MIXED, 20, skin=1.0, gamma=1.0, seed=9))
+        for t in ts:
+            srv.wait(t)
+        peak, peak_running = srv.stats()
+    finally:
+        srv.close()
+    foot = [pmx.estimate_bytes(s, spec, pmx.MIXED, trajectory=bool(kind)) for s, spec, kind in reqs]
+    print(f"admission: budget {budget} peak {peak} peak_running {peak_running} footprints {foot}", flush=True)
+    assert max(foot) <= budget
+    assert peak <= budget
+    # the resident sum at every start, from the requests' own time lines
+    waited = 0
+    for i, t in enumerate(ts):
+        resident = sum(f for f, u in zip(foot, ts) if u.start_at <= t.start_at < u.done_at)
+        assert resident <= budget, (i, resident)
+        waited += any(u.done_at <= t.start_at for u in ts if u is not t and u.submit_at <= t.submit_at)
+    assert waited > 0  # the budget made requests queue
+    ctx = pmx.Context(0)
+    try:
+        for (s, spec, kind), t in zip(reqs, ts):
+            sy = pmx.System(ctx, s, spec, pmx.MIXED)
+            try:
+                if kind == 0:
+                    f, ec, elj = sy.evaluate(s.pos)
+                    assert np.array_equal(t.a, f) and t.e[0] == ec and t.e[1] == elj
+                else:
+                    sy.md_init(s, 1.0, skin=1.0, integrator=pmx.LANGEVIN, temperature=300.0, gamma=1.0, seed=9)
+                    sy.md_set_state(s.pos, s.vel)
+                    sy.md_run(20)
+                    x, v, _ = sy.md_get_state()
+                    assert np.array_equal(t.a, x) and np.array_equal(t.v, v)
+            finally:
+                sy.close()
+    finally:
+        ctx.close()
