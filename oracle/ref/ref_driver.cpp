@@ -31,6 +31,12 @@
 // Output: line 1 "E_coulomb E_lj alpha", then N lines "fx fy fz" (%.17g).
 // With env PM_REF_PAIRS=<file>, also writes the cell of every atom and the
 // ordered (i, j) visit list of the first cutoff force (membership oracle).
+// With env PM_REF_MASSES=<file> (N numbers) the atoms get those masses
+// (scaledMass; 1 otherwise).  With env PM_REF_COMPARE=1 the first two force
+// specs are wrapped as the reference's compare pair (the `compare` force
+// keyword: SystemCompareForce of the first, then of the second compared with
+// the first) and evaluated twice; the reference prints its "Comparing"
+// lines, and its averaged summary when the pair is destroyed.
 
 #include <protomol/topology/Topology.h>
 #include <protomol/topology/PeriodicBoundaryConditions.h>
@@ -48,6 +54,7 @@
 #include <protomol/type/ScalarStructure.h>
 #include <protomol/type/Vector3DBlock.h>
 #include <protomol/base/Report.h>
+#include <protomol/force/CompareForce.h>
 using namespace ProtoMol::Report;
 #include "BSpline.h"
 #include "NonbondedPMEwaldSystemForce.h"
@@ -121,6 +128,10 @@ int main(int argc, char **argv) {
     topo.atoms[i].molecule = (int)ri(in);
     topo.atoms[i].scaledMass = 1.0;
     if (topo.atoms[i].molecule + 1 > nmol) nmol = topo.atoms[i].molecule + 1;
+  }
+  if (const char *mf = std::getenv("PM_REF_MASSES")) {
+    std::ifstream min(mf);
+    for (int i = 0; i < N; ++i) topo.atoms[i].scaledMass = rd(min);
   }
   topo.exclusions.resize(N);
   topo.exclusions.clear();
@@ -210,6 +221,23 @@ int main(int argc, char **argv) {
       return 2;
     }
     owned.push_back(f);
+  }
+
+  if (std::getenv("PM_REF_COMPARE") && owned.size() >= 2) {
+    SystemForce *a = dynamic_cast<SystemForce *>(owned[0]->makeCompareForce(owned[0], 0));
+    SystemForce *b =
+        dynamic_cast<SystemForce *>(owned[1]->makeCompareForce(owned[1], dynamic_cast<CompareForce *>(a)));
+    for (int rep = 0; rep < 2; ++rep) {
+      forces.zero();
+      energies.clear();
+      topo.uncacheCellList();
+      a->evaluate(&topo, &pos, &forces, &energies);
+      b->evaluate(&topo, &pos, &forces, &energies);
+    }
+    delete b;  // prints the averaged summary; deletes the wrapped forces
+    delete a;
+    std::fflush(stdout);
+    return 0;
   }
 
   // Same sequence as ForceGroup::evaluateSystemForces: cell lists are marked

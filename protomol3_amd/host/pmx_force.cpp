@@ -5,7 +5,13 @@
 // rows.
 //
 //   pmx_force <sysfile> <outfile> cellsize <cs> [--mixed] [--device d]
-//             --force "<ProtoMol force line>" [--force "<line>"]
+//             [--ranks R] --force "<ProtoMol force line>" [--force "<line>"]
+//
+// With --ranks R the force group is evaluated the way R MPI ranks evaluate it
+// under ProtoMol's static work distribution, one rank after the other in this
+// process: each rank calls parallelEvaluate into its own force and energy
+// arrays, and the ranks' arrays are summed (Parallel's reduction).  stderr
+// reports which ranks evaluated.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -66,7 +72,7 @@ int main(int argc, char **argv) {
   Vector3DBlock pos(n), forces(n);
   for (double &v : pos.c) v = std::strtod(tok(in).c_str(), nullptr);
 
-  int precision = PMX_FP64, device = 0;
+  int precision = PMX_FP64, device = 0, ranks = 0;
   std::vector<std::string> lines;
   for (int a = 3; a < argc; ++a) {
     std::string k = argv[a];
@@ -76,6 +82,8 @@ int main(int argc, char **argv) {
       precision = PMX_MIXED;
     else if (k == "--device" && a + 1 < argc)
       device = std::atoi(argv[++a]);
+    else if (k == "--ranks" && a + 1 < argc)
+      ranks = std::atoi(argv[++a]);
     else if (k == "--force" && a + 1 < argc)
       lines.push_back(argv[++a]);
     else {
@@ -92,7 +100,26 @@ int main(int argc, char **argv) {
   ScalarStructure energies;
   energies.clear();
   forces.zero();
-  f->evaluate(&topo, &pos, &forces, &energies);
+  if (ranks <= 0) {
+    f->preProcess(&topo, &pos);
+    f->evaluate(&topo, &pos, &forces, &energies);
+    f->postProcess(&topo, &energies, &forces);
+  } else {
+    for (int r = 0; r < ranks; ++r) {
+      Parallel::init(Parallel::STATIC, r, ranks);
+      Parallel::resetNext();
+      Vector3DBlock fr(n);
+      ScalarStructure er;
+      er.clear();
+      f->parallelEvaluate(&topo, &pos, &fr, &er);
+      double mag = 0;
+      for (double v : fr.c) mag += v * v;
+      std::fprintf(stderr, "rank %d of %d: blocks %u, %s\n", r, ranks, f->numberOfBlocks(&topo, &pos),
+                   mag > 0 ? "evaluated" : "nothing");
+      for (size_t k = 0; k < fr.c.size(); ++k) forces.c[k] += fr.c[k];
+      for (int k = 0; k < ScalarStructure::LAST; ++k) energies.v[k] += er.v[k];
+    }
+  }
   const double alpha = static_cast<GpuNonbondedSystemForce *>(f.get())->alpha();
   FILE *fo = std::fopen(argv[2], "w");
   if (!fo) return 2;

@@ -64,11 +64,54 @@ struct Topology {
   std::abort();
 }
 
+// Work-package distribution of ProtoMol's Parallel as a force sees it
+// (protomol/src/protomol/parallel/Parallel.cpp:911-1040): SEQUENTIAL, where
+// next() is always true, and STATIC, where package k of a force group goes to
+// rank k mod size.  (The reference's dynamic MASTER/SLAVE mode hands packages
+// out on request over MPI; a one-package force is evaluated by exactly one
+// rank there too.)
+class Parallel {
+ public:
+  enum Mode { SEQUENTIAL, STATIC };
+  static void init(Mode mode, int rank, int size) {
+    mode_ = mode;
+    rank_ = rank;
+    size_ = size;
+    next_ = 0;
+  }
+  static bool isParallel() { return mode_ != SEQUENTIAL && size_ > 1; }
+  // Parallel::distribute at the start of a force group's evaluation
+  static void resetNext() { next_ = 0; }
+  static bool next() {
+    if (mode_ == SEQUENTIAL) return true;
+    const bool mine = next_ == rank_;
+    next_ = (next_ + 1) % size_;
+    return mine;
+  }
+
+ private:
+  static inline Mode mode_ = SEQUENTIAL;
+  static inline int rank_ = 0, size_ = 1, next_ = 0;
+};
+
 class SystemForce {
  public:
   virtual ~SystemForce() = default;
   virtual void evaluate(const Topology *topo, const Vector3DBlock *positions, Vector3DBlock *forces,
                         ScalarStructure *energies) = 0;
+  // Parallel evaluation (SystemForce.h:107-112, NonbondedCutoffSystemForce.h:
+  // 57-77): a force split into numberOfBlocks packages evaluates the packages
+  // Parallel::next() gives this rank; the ranks' forces and energies are then
+  // summed.  The GPU force is one package, evaluated whole by one rank.
+  virtual unsigned numberOfBlocks(const Topology *, const Vector3DBlock *) { return 1; }
+  virtual void parallelEvaluate(const Topology *topo, const Vector3DBlock *positions, Vector3DBlock *forces,
+                                ScalarStructure *energies) {
+    if (Parallel::next()) evaluate(topo, positions, forces, energies);
+  }
+  // ForceGroup.cpp:65-69 calls these around evaluate; the nonbonded forces
+  // have nothing to do there
+  virtual void preProcess(const Topology *, const Vector3DBlock *) {}
+  virtual void postProcess(const Topology *, ScalarStructure *, Vector3DBlock *) {}
   virtual void uncache() {}
   virtual std::string getKeyword() const = 0;
   virtual std::string getIdNoAlias() const = 0;

@@ -19,9 +19,11 @@ SW_NONE, SW_CUTOFF, SW_C1, SW_C2 = 0, 1, 2, 3
 
 
 def names() -> list[str]:
-    """Force fixtures (the topology fixtures topo_*.npz are listed by topo_names)."""
+    """Force fixtures (the topology fixtures topo_*.npz are listed by
+    topo_names, the force-comparison reports compare_*.npz are read by their
+    own test)."""
     return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz"))
-                  if not os.path.basename(p).startswith("topo_"))
+                  if not os.path.basename(p).startswith(("topo_", "compare_")))
 
 
 def topo_names() -> list[str]:

@@ -180,11 +180,48 @@ def run_topo_case(stem: str, exclude: str) -> dict:
                 table=table, counts=counts)
 
 
+# compare fixtures: the reference's `compare` report (CompareForce.cpp:82-130)
+# of force spec B against force spec A on one system with O/H masses
+COMPARE_CASES = {
+    "compare_water1536_ljc2": ("water_box", dict(seed=3), 5.0, ["lj_c2", "8", "10"], ["lj_c2", "7", "10"]),
+    "compare_water1536_pme": ("water_box", dict(seed=3), 5.0,
+                              ["pme", "32", "32", "32", "4", "10", "1e-6", "0.32", "111"],
+                              ["pme", "24", "24", "24", "4", "10", "1e-6", "0.32", "111"]),
+}
+
+
+def run_compare_case(gen: str, args: dict, cellsize: float, spec_a: list, spec_b: list) -> dict:
+    s = make_system(gen, args)
+    mass = np.where(np.arange(s.n) % 3 == 0, 15.9994, 1.008)
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        sf, of, mf = (os.path.join(td, x) for x in ("sys", "out", "mass"))
+        sysfile.write_sysfile(sf, s)
+        np.savetxt(mf, mass, fmt="%.17g")
+        for tag, spec in (("a", spec_a), ("b", spec_b)):
+            subprocess.run([PM_REF, sf, of, "cellsize", repr(cellsize), *spec], capture_output=True, text=True,
+                           check=True)
+            ec, elj, frc = sysfile.read_forces(of)
+            out[f"forces_{tag}"] = frc
+            out[f"pe_{tag}"] = np.float64(ec + elj)
+        r = subprocess.run([PM_REF, sf, of, "cellsize", repr(cellsize), *spec_a, *spec_b], capture_output=True,
+                           text=True, check=True, env=dict(os.environ, PM_REF_COMPARE="1", PM_REF_MASSES=mf))
+    lines = [ln for ln in (r.stdout + r.stderr).splitlines() if ln.startswith("Comparing")]
+    out.update(generator=np.array(gen), generator_args=np.array(json.dumps(args)), mass=mass,
+               pos=s.pos, report_lines=np.array(lines))
+    return out
+
+
 def main():
     if not os.path.exists(PM_REF):
         sys.exit(f"{PM_REF} missing: build it with make -f oracle/ref/Makefile")
-    names = sys.argv[1:] or list(CASES) + [f"topo_{k}" for k in TOPO_CASES]
+    names = sys.argv[1:] or list(CASES) + [f"topo_{k}" for k in TOPO_CASES] + list(COMPARE_CASES)
     for name in names:
+        if name in COMPARE_CASES:
+            d = run_compare_case(*COMPARE_CASES[name])
+            np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **d)
+            print(f"{name}: " + " | ".join(str(x) for x in d["report_lines"]))
+            continue
         if name.startswith("topo_"):
             d = run_topo_case(*TOPO_CASES[name[5:]])
             np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **d)

@@ -36,11 +36,11 @@ def force_lines(sp):
     return out
 
 
-def run_exe(s, cellsize, lines, mixed=False):
+def run_exe(s, cellsize, lines, mixed=False, extra=()):
     with tempfile.TemporaryDirectory() as td:
         sf, of = os.path.join(td, "sys"), os.path.join(td, "out")
         sysfile.write_sysfile(sf, s)
-        args = [EXE, sf, of, "cellsize", repr(cellsize)] + (["--mixed"] if mixed else [])
+        args = [EXE, sf, of, "cellsize", repr(cellsize)] + (["--mixed"] if mixed else []) + list(extra)
         for ln in lines:
             args += ["--force", ln]
         r = subprocess.run(args, capture_output=True, text=True, timeout=120)
@@ -77,3 +77,23 @@ def test_conf_force_lines_on_gpu_match_reference_fixture(name):
     assert np.abs(f[idx] - fref).max() <= 1e-9 * max(1.0, np.abs(fref).max())
     assert abs(ec - float(fx["e_coulomb"])) <= 1e-9 * max(1.0, abs(float(fx["e_coulomb"])))
     assert abs(elj - float(fx["e_lj"])) <= 1e-9 * max(1.0, abs(float(fx["e_lj"])))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ranks", [1, 3])
+def test_parallel_evaluate_gives_one_rank_the_whole_force(ranks):
+    """Under ProtoMol's static work distribution the GPU force is one work
+    package: exactly one of the ranks evaluates it (the others add nothing),
+    and the ranks' summed forces and energies are the sequential ones
+    (NonbondedCutoffSystemForce.h:57-77, SystemForce.h:107-112)."""
+    fx = FX.load("polymer3200_pme_auto")
+    s = FX.system(fx)
+    sp = FX.spec(fx)
+    rc, err0, seq = run_exe(s, sp["cellsize"], force_lines(sp), mixed=True)
+    assert rc == 0, err0
+    rc, err, par = run_exe(s, sp["cellsize"], force_lines(sp), mixed=True, extra=["--ranks", str(ranks)])
+    assert rc == 0, err
+    print(err, flush=True)
+    assert err.count("evaluated") == 1 and err.count("nothing") == ranks - 1
+    assert err.count("blocks 1,") == ranks
+    assert np.array_equal(par[2], seq[2]) and par[0] == seq[0] and par[1] == seq[1]
