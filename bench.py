@@ -37,8 +37,8 @@ FLOP_PER_PAIR = 69          # algorithmic FLOPs per in-cutoff pair (LJ C2 + PME 
 # scales from a plain 256 MiB copy (profiles/k_pair_traffic_polymer26000.md)
 PAIR_TRAFFIC_BYTES = 43.98e6
 PAIR_TRAFFIC_SKIN = 0.7
-LAUNCH_KNOBS = {1: "tiles_per_cu", 2: "fft_lines", 3: "pair_blocks_per_cu", 4: "pair_schedule",
-                5: "pair_runs_per_wave", 6: "spread_on_second_stream"}
+LAUNCH_KNOBS = {2: "fft_lines", 3: "pair_blocks_per_cu", 4: "pair_schedule", 6: "spread_on_second_stream",
+                7: "list_prune"}
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
 TIMESTEP_FS = 1.0
@@ -140,6 +140,80 @@ def large(ctx, steps: int, skin: float, log=print):
     out["rms_rel_force_err_mixed_vs_fp64"] = float(
         f"{np.sqrt(np.mean(np.sum(d * d, 1))) / np.sqrt(np.mean(np.sum(f[pmx.FP64] ** 2, 1))):.3e}")
     return out
+
+
+def water1536_leg(ctx, steps: int = 1000, cpu_steps: int = 20, log=print):
+    """The 1,536-atom water box (512 flexible 3-site waters in a 24.8 A box,
+    LJ C2 8-10 A on O, PME 32^3 order 4 alpha 0.32 rc 10 A), leapfrog NVE
+    1 fs from 300 K: `steps` steps of each form; mixed vs FP64 force error
+    at the end configuration; the oracle's FP64 MD step on one thread for
+    cpu_steps steps beside it."""
+    from protomol3_amd import pmx, systems
+    s = systems.water_box().with_velocities(300.0, seed=4)
+    spec = pmx.ForceSpec(5.0, lj=(pmx.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 32, 32), order=4, alpha=0.32))
+    out = {"workload": "water1536: 24.8 A box, LJ C2 8-10 A, PME 32^3 order 4 rc 10 A alpha 0.32, leapfrog NVE 1 fs",
+           "atoms": s.n, "steps": steps}
+    xs = None
+    for prec, name in ((pmx.MIXED, "mixed"), (pmx.FP64, "fp64")):
+        r = run_form(ctx, s, spec, prec, steps, 50, 1.0, steps)
+        log(f"water1536 {name}: {r['ms_per_step']:.4f} ms/step stages {r['stages']}")
+        out[name] = {"ns_per_day": round(ns_day(r["ms_per_step"]), 3), "ms_per_step": round(r["ms_per_step"], 5),
+                     "nve_drift_kcal_mol_per_ns": _round(r["drift_kcal_mol_ns"], 3), "pairs_per_step": int(r["npairs"])}
+        xs = r["x"] if xs is None else xs
+    out["mixed_over_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    f = {}
+    for prec in (pmx.FP64, pmx.MIXED):
+        sy = pmx.System(ctx, s, spec, prec)
+        f[prec] = sy.evaluate(xs)
+        sy.close()
+    d = f[pmx.MIXED][0] - f[pmx.FP64][0]
+    out["rms_rel_force_err_mixed_vs_fp64"] = float(
+        f"{np.sqrt(np.mean(np.sum(d * d, 1))) / np.sqrt(np.mean(np.sum(f[pmx.FP64][0] ** 2, 1))):.3e}")
+    e64 = f[pmx.FP64][1] + f[pmx.FP64][2]
+    out["rel_energy_err_mixed_vs_fp64"] = float(f"{abs(f[pmx.MIXED][1] + f[pmx.MIXED][2] - e64) / abs(e64):.3e}")
+    if cpu_steps > 0:
+        from oracle import md
+        from oracle import oracle as O
+        ospec = O.make_spec(5.0, lj=(O.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 32, 32), order=4, alpha=0.32))
+        x, v, fo, _ = md.leapfrog_run(s, ospec, s.pos, s.vel, TIMESTEP_FS, 1, nthreads=1)
+        t0 = time.perf_counter()
+        md.leapfrog_run(s, ospec, x, v, TIMESTEP_FS, cpu_steps, f=fo, nthreads=1)
+        dt = (time.perf_counter() - t0) / cpu_steps
+        out["cpu_baseline"] = {"value": round(ns_day(dt * 1e3), 5), "unit": "ns/day", "cores": 1, "kind": "port",
+                               "sample": f"{cpu_steps} leapfrog MD steps of the oracle (FP64), {dt:.4f} s per step"}
+    return out
+
+
+def copies(s, spec, ncopies: int, steps: int, lanes: int, skin: float, log=print):
+    """ncopies independent copies of the 26,000-atom workload (same start
+    positions, velocities from per-copy seeds), mixed form, leapfrog NVE,
+    `steps` steps each, run by one pmx_md_run_many call over `lanes`
+    contexts; aggregate ns/day = copies x simulated time / wall time."""
+    from protomol3_amd import pmx
+    v0 = s.vel
+    vels = [s.with_velocities(300.0, seed=100 + i).vel.copy() for i in range(ncopies)]
+    s.vel = v0
+    ctxs = [pmx.Context(0) for _ in range(lanes)]
+    sys_ = []
+    try:
+        for i, vel in enumerate(vels):
+            sy = pmx.System(ctxs[i % lanes], s, spec, pmx.MIXED)
+            sys_.append(sy)
+            sy.md_init(s, TIMESTEP_FS, skin=skin)
+            sy.md_set_state(s.pos, vel)
+        pmx.run_many(sys_, 20)
+        t0 = time.perf_counter()
+        pmx.run_many(sys_, steps)
+        wall = time.perf_counter() - t0
+    finally:
+        for sy in sys_:
+            sy.close()
+        for c in ctxs:
+            c.close()
+    agg = ncopies * steps * TIMESTEP_FS * 1e-6 / wall * 86400.0
+    log(f"copies: {ncopies} x {s.n} atoms over {lanes} contexts: {agg:.1f} ns/day aggregate ({wall:.2f} s)")
+    return {"copies": ncopies, "lanes": lanes, "steps": steps, "skin_A": skin, "wall_s": round(wall, 3),
+            "aggregate_ns_per_day": round(agg, 3)}
 
 
 def drift_study(ctx, s, spec, ps: float = 10.0, log=print):
@@ -350,7 +424,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--serve-requests", type=int, default=64, help="requests in the serving stream (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--replicas", type=int, default=0, help="batched replicas to run after the main metric (0: skip)")
+    ap.add_argument("--replicas", type=int, default=256,
+                    help="batched replicas (log-uniform 500-20,000 atoms) run after the main metric (0: skip)")
+    ap.add_argument("--copies", type=int, default=8,
+                    help="copies of the 26,000-atom workload run side by side for an aggregate rate (0: skip)")
+    ap.add_argument("--no-water1536", action="store_true", help="skip the 1,536-atom water box leg")
     ap.add_argument("--large", action="store_true",
                     help="only the 999,000-atom water box, both forms timed per stage (--steps steps, default 1000)")
     ap.add_argument("--pair-only", action="store_true",
@@ -408,17 +486,24 @@ def main():
         forms[prec] = sy.evaluate(mixed["x"])
         sy.close()
     pair_launch_ms = time_pair(ctx, s, spec, mixed["x"], skin_m, mixed["launch"])
+    small_out = None if a.no_water1536 else water1536_leg(ctx, cpu_steps=0 if a.no_cpu else 20, log=log)
     ctx.close()
     f64, fm = forms[pmx.FP64][0], forms[pmx.MIXED][0]
     e64, em = forms[pmx.FP64], forms[pmx.MIXED]
     rel_e = {"coulomb": float(f"{abs(em[1] - e64[1]) / abs(e64[1]):.3e}"),
              "lj": float(f"{abs(em[2] - e64[2]) / abs(e64[2]):.3e}")}
     err = float(np.sqrt(np.mean(np.sum((fm - f64) ** 2, 1))) / np.sqrt(np.mean(np.sum(f64 ** 2, 1))))
+    from protomol3_amd import outputs
+    cmp = outputs.compare_forces(f64, e64[1] + e64[2], fm, em[1] + em[2], s.mass)
     pair_ms = pair_launch_ms
     achieved = FLOP_PER_PAIR * mixed["npairs"] / (pair_ms * 1e-3) / 1e12
     cpu = None if a.no_cpu else cpu_baseline(s, a.cpu_budget)
     serve_out = serving(a.serve_requests, a.serve_slots, log=log) if a.serve_requests > 0 else None
     rep_out = replicas(a.replicas, a.replica_steps, a.replica_lanes, log=log) if a.replicas > 0 else None
+    copies_out = copies(s, spec, a.copies, a.steps, a.replica_lanes, skin_m, log=log) if a.copies > 0 else None
+    if copies_out is not None:
+        copies_out["over_fp64_single"] = round(
+            copies_out["aggregate_ns_per_day"] / ns_day(fp64["ms_per_step"]), 3)
     out = {
         "metric": "ns/day (26k-atom PME) mixed vs FP64 GPU at <=1e-5 force err; p50/p99 latency",
         "value": round(ns_day(mixed["ms_per_step"]), 3),
@@ -443,6 +528,7 @@ def main():
         "mixed_over_fp64": round(fp64["ms_per_step"] / mixed["ms_per_step"], 3),
         "rms_rel_force_err_mixed_vs_fp64": float(f"{err:.3e}"),
         "rel_energy_err_mixed_vs_fp64": rel_e,
+        "compare_mixed_vs_fp64": {k: float(f"{v:.4e}") for k, v in cmp.items()},
         "nve_drift_kcal_mol_per_ns": {"mixed": _round(mixed["drift_kcal_mol_ns"], 3),
                                       "fp64": _round(fp64["drift_kcal_mol_ns"], 3),
                                       "steps": a.drift_steps},
@@ -468,8 +554,9 @@ def main():
         "cpu_baseline": cpu,
         "serving": serve_out,
     }
-    if rep_out is not None:
-        out["replicas"] = rep_out
+    out["water1536"] = small_out
+    out["replicas"] = rep_out
+    out["copies_polymer26000"] = copies_out
     print(json.dumps(out), flush=True)
 
 

@@ -166,35 +166,37 @@ int pmx_sys_timing_end(pmx_sys *sys, float *avg_ms, int *ncalls);
  * the forces of the last evaluation are kept. */
 int pmx_sys_time_pair(pmx_sys *sys, int reps, float *ms_per_launch);
 /* Launch shapes for the autotuner (no counterpart in the reference):
- * PMX_TUNE_TILES_PER_CU  (schedule 1) target of pair-kernel tiles per CU below which the
- *                        list is cut into shorter tiles (0..64, default 16;
- *                        0: always 16 entries per tile);
  * PMX_TUNE_FFT_LINES     most lines per FFT block (1..64, default 16; the
  *                        LDS limit of 24 KiB per block also applies);
  * PMX_TUNE_PAIR_BLOCKS_PER_CU  pair-kernel blocks per CU (0: as many as fit,
  *                        -k: k fewer than fit, at least 1; fewer leave room
  *                        for the PME chain beside it);
- * PMX_TUNE_PAIR_SCHEDULE 0 (default): the list is cut into one run of equal
- *                        estimated work per resident wave; 1: tiles of up to
- *                        16 entries taken from a counter (or round robin when
- *                        TILES_PER_CU asks for shorter tiles);
- * PMX_TUNE_PAIR_RUNS     balanced runs per resident wave (1..4, default 1:
- *                        fully static); each wave takes one run, the rest
- *                        are taken in order from one device-wide counter;
+ * PMX_TUNE_PAIR_SCHEDULE 0 (default): the pair list's runs (a number fixed by
+ *                        the list, a multiple of 4,096, of equal estimated
+ *                        work each) are dealt to the resident waves in
+ *                        contiguous shares; 1: each wave's first run is
+ *                        static, later ones come one at a time from a
+ *                        device-wide counter (better when fewer waves are
+ *                        resident than the default launch's 4,096);
  * PMX_TUNE_SPREAD_STREAM 0 (default): the PME charge spread runs on the main
  *                        stream before the pair kernel; 1: it runs first on
- *                        the second stream, beside the pair kernel.
- * The resident state is kept.  Blocks per CU and FFT lines do not change
- * any sum (beyond the order of the FP64 form's atomics); the tile length
- * regroups the FP32 partial sums of the mixed form's i-forces (one per tile,
- * about 1e-7 relative). */
+ *                        the second stream, beside the pair kernel;
+ * PMX_TUNE_LIST_PRUNE    1 (default): the first pair pass after each list
+ *                        build drops, for the later passes, the cluster pairs
+ *                        with no candidate pair inside the list radius;
+ *                        0: keep them.
+ * The resident state is kept.  No knob changes a result: the mixed form's
+ * forces and energies are the same to the last bit under every setting (the
+ * runs, and the FP32 partial sums within them, depend on the list only; a
+ * pruned cluster pair has no member), the FP64 form's up to the order of its
+ * atomic additions.  (Knobs 1 and 5, tiles per CU and runs per wave, are
+ * gone: the run count no longer depends on the launch.) */
 enum {
-  PMX_TUNE_TILES_PER_CU = 1,
   PMX_TUNE_FFT_LINES = 2,
   PMX_TUNE_PAIR_BLOCKS_PER_CU = 3,
   PMX_TUNE_PAIR_SCHEDULE = 4,
-  PMX_TUNE_PAIR_RUNS = 5,
-  PMX_TUNE_SPREAD_STREAM = 6
+  PMX_TUNE_SPREAD_STREAM = 6,
+  PMX_TUNE_LIST_PRUNE = 7
 };
 int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value);
 

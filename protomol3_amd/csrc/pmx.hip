@@ -86,14 +86,14 @@ struct Sys {
   PmeParams pme{};
   FftPlan plx{}, ply{}, plz{};
   bool pme_on = false;
-  int cap = 0, maxmaskwork = 0, ntiles_cap = 0, tile_target = 0;
+  int cap = 0, maxmaskwork = 0, ntiles_cap = 0;
   // launch shapes the autotuner may change (pmx_sys_set_tuning)
-  int tune_tiles_per_cu = 16;  // tile-count target per CU for short tiles (0: 16-entry tiles)
   int tune_fft_lines = 16;     // most lines per FFT block
   int tune_pair_blocks = 0;    // pair-kernel blocks per CU (0: as many as fit)
-  int tune_schedule = 0;       // 0: balanced runs per wave; 1: fixed-length tiles (counter / round robin)
-  int tune_runs = 1;           // balanced runs per resident wave (1: static, more: the rest from a counter)
+  int tune_schedule = 0;       // 0: runs round robin over the waves; 1: later runs from a counter
   int tune_spread_stream2 = 0; // 1: the PME spread runs on the second stream, beside the pair kernel
+  int tune_prune = 1;          // 1: the first pair pass after a list build prunes the list's cluster pairs
+  bool prune_pending = false;  // a list was built and its first pair pass has not run
   int pair_grid = 0;           // blocks of the pair-kernel launch
   long long nbins = 0;
   int nscanblocks = 0;
@@ -167,21 +167,68 @@ int fail(pmx_ctx *c, int code, const std::string &m) {
 std::recursive_mutex g_rt_mu;
 using RtLock = std::lock_guard<std::recursive_mutex>;
 
+// Device block cache.  Systems come and go (a request loop builds one per
+// request), so released device blocks are kept per device in size classes
+// (powers of 2 split in quarters above 4 KiB) and handed to the next request
+// of the same class instead of going back to hipFree / hipMalloc.  A block is
+// released only after the streams that used it are synchronized.  When
+// hipMalloc fails, the cached blocks of that device are freed and the
+// allocation is tried once more.  Guarded by g_rt_mu.
+size_t size_class(size_t bytes) {
+  if (bytes <= 4096) return bytes <= 256 ? 256 : (size_t)1 << (64 - __builtin_clzll(bytes - 1));
+  const int e = 63 - __builtin_clzll(bytes);  // 2^e <= bytes < 2^(e+1)
+  const size_t q = (size_t)1 << (e - 2);
+  return (bytes + q - 1) / q * q;
+}
+std::map<int, std::multimap<size_t, void *>> g_block_cache;
+size_t g_cached_bytes = 0;
+constexpr size_t kBlockCacheMax = 16ull << 30;  // cached bytes kept per process, at most
+
+void *block_get(int device, size_t cls) {
+  auto &m = g_block_cache[device];
+  auto it = m.find(cls);
+  if (it != m.end()) {
+    void *p = it->second;
+    m.erase(it);
+    g_cached_bytes -= cls;
+    return p;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, cls) == hipSuccess) return p;
+  for (auto &kv : m) hipFree(kv.second), g_cached_bytes -= kv.first;
+  m.clear();
+  p = nullptr;
+  return hipMalloc(&p, cls) == hipSuccess ? p : nullptr;
+}
+
+void block_put(int device, void *p, size_t cls) {
+  if (!p) return;
+  if (g_cached_bytes + cls > kBlockCacheMax) {
+    hipFree(p);
+    return;
+  }
+  g_block_cache[device].emplace(cls, p);
+  g_cached_bytes += cls;
+}
+
 int dalloc(Sys *s, DevBuf &b, size_t bytes) {
   if (bytes == 0) bytes = 8;
   if (s->dry) {
-    s->dry_bytes += bytes;
+    s->dry_bytes += size_class(bytes);
     return PMX_OK;
   }
   RtLock lk(g_rt_mu);
   if (b.p && b.bytes >= bytes) return PMX_OK;
-  if (b.p) hipFree(b.p);
-  b.p = nullptr;
-  if (hipMalloc(&b.p, bytes) != hipSuccess) {
-    b.p = nullptr;
-    return fail(s->ctx, PMX_ERR_OOM, "hipMalloc failed (" + std::to_string(bytes) + " bytes)");
+  if (b.p) {
+    // a growing buffer: its last users must be done before the block is reused
+    hipStreamSynchronize(s->ctx->stream);
+    hipStreamSynchronize(s->ctx->stream2);
+    block_put(s->ctx->device, b.p, b.bytes);
   }
-  b.bytes = bytes;
+  const size_t cls = size_class(bytes);
+  b.p = block_get(s->ctx->device, cls);
+  if (!b.p) return fail(s->ctx, PMX_ERR_OOM, "hipMalloc failed (" + std::to_string(cls) + " bytes)");
+  b.bytes = cls;
   if (!b.listed) s->all.push_back(&b);
   b.listed = true;
   return PMX_OK;
@@ -472,6 +519,7 @@ int setup(Sys *s) {
   }
   lp.skin = (float)s->md.skin;
   lp.rlist2 = (float)((rcmax + margin + s->md.skin) * (rcmax + margin + s->md.skin));
+  s->pp.prune_r2 = (double)lp.rlist2 * (1.0 + 1e-5);
   lp.rc = (float)rcmax;
   lp.ncl = s->ncl;
   lp.nsuper = s->nsuper;
@@ -495,13 +543,7 @@ int setup(Sys *s) {
     s->cap = (s->cap + kTileEntries - 1) / kTileEntries * kTileEntries;
   }
   lp.cap = s->cap;
-  {
-    const int cus = s->ctx ? s->ctx->cus : 256;
-    // default: a tile per resident pair-kernel wave; capacity for the largest
-    // target the tuning accepts (64 per CU)
-    s->tile_target = s->tune_tiles_per_cu * cus;
-    s->ntiles_cap = s->nsuper * (s->cap / kTileEntries) + 2 * 64 * cus + s->nsuper;
-  }
+  s->ntiles_cap = kMaxRuns + 2 * s->nsuper;  // a run touches one supercluster or more
   s->maxmaskwork = std::max(1024, s->nsuper * 128);
 
   int rc;
@@ -522,9 +564,9 @@ int setup(Sys *s) {
       (rc = A(s->count, 4 * (size_t)s->nsuper)) || (rc = A(s->lcost, 4 * (size_t)s->nsuper)) ||
       (rc = A(s->ecost, 4 * (size_t)s->nsuper * s->cap)) ||
       (rc = A(s->tep, 4 * ((size_t)s->nsuper + 1))) || (rc = A(s->tcp, 8 * ((size_t)s->nsuper + 1))) ||
-      (rc = A(s->tpos, 4 * ((size_t)kMaxPairRunsPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
-      (rc = A(s->tposS, 4 * ((size_t)kMaxPairRunsPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
-      (rc = A(s->wstart, 4 * ((size_t)kMaxPairRunsPerCu * (s->ctx ? s->ctx->cus : 256) + 1))) ||
+      (rc = A(s->tpos, 4 * ((size_t)kMaxRuns + 1))) ||
+      (rc = A(s->tposS, 4 * ((size_t)kMaxRuns + 1))) ||
+      (rc = A(s->wstart, 4 * ((size_t)kMaxRuns + 1))) ||
       (rc = A(s->maskwork, 4 * (size_t)s->maxmaskwork)) ||
       (rc = A(s->nmaskwork, 8)) || (rc = A(s->masks, 16 * 8 * (size_t)s->maxmaskwork)) ||
       (rc = A(s->overflow, 8)) || (rc = A(s->tiles, 16 * (size_t)s->ntiles_cap)) || (rc = A(s->ntiles, 16)) ||
@@ -545,7 +587,9 @@ int setup(Sys *s) {
     if (s->pme_on) {
       const size_t nx = d.pme_grid[0], ny = d.pme_grid[1], nz = d.pme_grid[2], nzh = nz / 2 + 1;
       const size_t G = nx * ny * nz, GH = nx * ny * nzh, cxb = s->mixed ? 8 : 16, rb = s->mixed ? 4 : 8;
-      s->dry_bytes += 8 * G + 2 * cxb * GH + rb * G + rb * GH + cxb * (nx + ny + nz) + 8 * ny * nzh + 8 * 5;
+      for (size_t b : {8 * G, cxb * GH, cxb * GH, rb * G, rb * GH, cxb * nx, cxb * ny, cxb * nz, 8 * ny * nzh, (size_t)8,
+                       (size_t)8, (size_t)8, (size_t)8, (size_t)8})
+        s->dry_bytes += size_class(b);
     }
     return PMX_OK;
   }
@@ -686,7 +730,7 @@ int setup(Sys *s) {
 // (phase 0, main stream), the boundary pairs (phase 1, main stream) and the
 // exclusion correction with the bonded terms (phase 2, second stream); the
 // last two are one kernel with different block ranges.
-template <bool MIXED, int C, int LJ, int CS>
+template <bool MIXED, int C, int LJ, int CS, bool BIGLJ = false>
 void launch_pair_phase(Sys *s, bool stats, int phase) {
   hipStream_t st = phase == 2 ? s->ctx->stream2 : s->ctx->stream;
   using R4 = typename Real4<typename std::conditional<MIXED, float, double>::type>::type;
@@ -696,7 +740,8 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
     // wave its first tile, or its whole run, statically)
     static int per_cu = 0;
     if (!per_cu) {
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair<MIXED, C, LJ, CS>, kPairBlock, 0) !=
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair<MIXED, C, LJ, CS, BIGLJ, false>, kPairBlock,
+                                                       0) !=
               hipSuccess ||
           per_cu < 1)
         per_cu = 1;
@@ -713,12 +758,23 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
     PairOut out{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash),
                 P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow),
                 P<unsigned long long>(s->border_acc)};
-    hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS>), dim3(grid), dim3(kPairBlock), 0, st, s->pp, stats ? 1 : 0,
-                       P<int4>(s->tiles), P<int>(s->ntiles), P<int>(s->wstart), P<int>(s->tilectr), P<int4>(s->entries),
-                       s->cap, P<unsigned short>(s->masks), P<R4>(s->xs),
-                       P<float4>(s->sorg), P<int>(s->ts), P<int>(s->permp), P<double>(s->ljtab),
-                       MIXED ? reinterpret_cast<const R4 *>(s->ljt32.p) : reinterpret_cast<const R4 *>(s->ljtab.p),
-                       facc, out);
+    // the list prune runs in the first pass after a build (not for tables
+    // beyond the LDS size, which are rare)
+    const bool prune = !BIGLJ && s->tune_prune && s->prune_pending;
+    s->prune_pending = false;
+    const R4 *ljt = MIXED ? reinterpret_cast<const R4 *>(s->ljt32.p) : reinterpret_cast<const R4 *>(s->ljtab.p);
+    if (prune)
+      hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS, false, true>), dim3(grid), dim3(kPairBlock), 0, st, s->pp,
+                         stats ? 1 : 0, s->tune_schedule, P<int4>(s->tiles), P<int>(s->ntiles), P<int>(s->wstart),
+                         P<int>(s->tilectr), P<int4>(s->entries), P<int4>(s->entries), s->cap,
+                         P<unsigned short>(s->masks), P<R4>(s->xs), P<float4>(s->sorg), P<int>(s->ts),
+                         P<int>(s->permp), P<double>(s->ljtab), ljt, facc, out);
+    else
+      hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS, BIGLJ, false>), dim3(grid), dim3(kPairBlock), 0, st, s->pp,
+                         stats ? 1 : 0, s->tune_schedule, P<int4>(s->tiles), P<int>(s->ntiles), P<int>(s->wstart),
+                         P<int>(s->tilectr), P<int4>(s->entries), P<int4>(s->entries), s->cap,
+                         P<unsigned short>(s->masks), P<R4>(s->xs), P<float4>(s->sorg), P<int>(s->ts),
+                         P<int>(s->permp), P<double>(s->ljtab), ljt, facc, out);
     return;
   }
   FixupArgs<MIXED> a{};
@@ -767,6 +823,16 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
 template <bool MIXED>
 void run_pair_phase(Sys *s, bool stats, int phase) {
   const int lj = s->d.lj_switch, cs = s->d.coul_switch;
+  if (s->d.ntypes > kMaxTypes && lj != SW_NONE) {
+    // LJ table too large for the kernel's LDS: read from the caches
+    if (s->pp.coul == COUL_PME)
+      launch_pair_phase<MIXED, COUL_PME, SW_ANY, SW_ANY, true>(s, stats, phase);
+    else if (s->pp.coul == COUL_CUTOFF)
+      launch_pair_phase<MIXED, COUL_CUTOFF, SW_ANY, SW_ANY, true>(s, stats, phase);
+    else
+      launch_pair_phase<MIXED, COUL_NONE, SW_ANY, SW_ANY, true>(s, stats, phase);
+    return;
+  }
   if (s->pp.coul == COUL_PME) {
     if (lj == SW_C2 && cs == SW_CUTOFF)
       launch_pair_phase<MIXED, COUL_PME, SW_C2, SW_CUTOFF>(s, stats, phase);
@@ -866,7 +932,7 @@ void reduce_energies(Sys *s, bool stats) {
   const bool recip = s->pme_on && (s->d.pme_terms & PMX_PME_RECIP);
   const bool corr = s->pme_on && (s->d.pme_terms & PMX_PME_CORR) && !s->etab.empty();
   hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(1024), 0, s->ctx->stream, s->ntiles_cap, P<int>(s->ntiles),
-                     P<double>(s->pair_e), recip ? s->recip_blocks : 0, P<double>(s->recip_e),
+                     P<double>(s->pair_e), s->mixed ? 1 : 0, recip ? s->recip_blocks : 0, P<double>(s->recip_e),
                      corr ? s->corr_blocks : 0, P<double>(s->corr_e), P<long long>(s->npairs),
                      P<unsigned long long>(s->hash), stats ? 1 : 0, P<unsigned long long>(s->border_acc),
                      P<unsigned long long>(s->border_acc) + 2, P<double>(s->eout), P<long long>(s->np_out),
@@ -890,19 +956,13 @@ void timing_begin_call(Sys *s, bool timing) {
 // ForceGroup.cpp:57; the MD path rebuilds when the skin requires it).
 // Work tiles of the current list for the current schedule and pair grid.
 void make_tiles(Sys *s, hipStream_t st) {
-  if (s->tune_schedule == 0) {
-    const int nruns = s->pair_grid * (kPairBlock / kWave) * s->tune_runs;
-    hipLaunchKernelGGL(k_runs_prefix, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int>(s->lcost),
-                       nruns, P<int>(s->tep), P<long long>(s->tcp), P<int>(s->tpos), P<int>(s->tposS));
-    hipLaunchKernelGGL(k_runs_cut, dim3(s->nsuper), dim3(256), 0, st, s->nsuper, P<int>(s->count), P<int>(s->ecost),
-                       s->cap, nruns, P<int>(s->tep), P<long long>(s->tcp), P<int>(s->tpos), P<int>(s->tposS));
-    hipLaunchKernelGGL(k_runs_tiles, dim3(1), dim3(1024), 0, st, P<int>(s->count), nruns, P<int>(s->tep),
-                       P<int>(s->tpos), P<int>(s->tposS), P<int>(s->wstart), P<int4>(s->tiles), P<int>(s->ntiles),
-                       s->ntiles_cap);
-  } else {
-    hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int4>(s->tiles),
-                       P<int>(s->ntiles), s->ntiles_cap, s->tile_target);
-  }
+  int *const nruns = P<int>(s->ntiles) + 2;
+  hipLaunchKernelGGL(k_runs_prefix, dim3(1), dim3(1024), 0, st, s->nsuper, P<int>(s->count), P<int>(s->lcost), nruns,
+                     P<int>(s->tep), P<long long>(s->tcp), P<int>(s->tpos), P<int>(s->tposS));
+  hipLaunchKernelGGL(k_runs_cut, dim3(s->nsuper), dim3(256), 0, st, s->nsuper, P<int>(s->count), P<int>(s->ecost),
+                     s->cap, nruns, P<int>(s->tep), P<long long>(s->tcp), P<int>(s->tpos), P<int>(s->tposS));
+  hipLaunchKernelGGL(k_runs_tiles, dim3(1), dim3(1024), 0, st, P<int>(s->count), P<int>(s->tep), P<int>(s->tpos),
+                     P<int>(s->tposS), P<int>(s->wstart), P<int4>(s->tiles), P<int>(s->ntiles), s->ntiles_cap);
 }
 
 int rebuild(Sys *s, bool timing) {
@@ -959,6 +1019,7 @@ int rebuild(Sys *s, bool timing) {
                      P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned short>(s->masks));
   make_tiles(s, st);
   s->list_valid = true;
+  s->prune_pending = true;
   if (timing) hipEventRecord(s->ev[2], st);
   return PMX_OK;
 }
@@ -1118,7 +1179,7 @@ int grow_and_retry(Sys *s) {
   s->cap = std::min(s->ncl + kTileEntries, 2 * s->cap);
   s->cap = (s->cap + kTileEntries - 1) / kTileEntries * kTileEntries;
   s->lp.cap = s->cap;
-  s->ntiles_cap = s->nsuper * (s->cap / kTileEntries) + 2 * 64 * s->ctx->cus + s->nsuper;
+  s->ntiles_cap = kMaxRuns + 2 * s->nsuper;
   s->maxmaskwork *= 2;
   s->maxborder *= 2;
   if (int rc0 = dalloc(s, s->border, 8 * (size_t)s->maxborder)) return rc0;
@@ -1218,8 +1279,7 @@ int pmx_sys_destroy(pmx_sys *sys) {
   hipStreamSynchronize(s->ctx->stream2);
   RtLock lk(g_rt_mu);
   drop_graphs(s);
-  for (DevBuf *b : s->all)
-    if (b->p) hipFree(b->p);
+  for (DevBuf *b : s->all) block_put(s->ctx->device, b->p, b->bytes);
   if (s->fork_ev) hipEventDestroy(s->fork_ev);
   if (s->join_ev) hipEventDestroy(s->join_ev);
   for (auto &e : s->md.chunk_ev)
@@ -1429,11 +1489,6 @@ int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value) {
   if (!sys) return PMX_ERR_ARG;
   Sys *s = &sys->s;
   switch (knob) {
-    case PMX_TUNE_TILES_PER_CU:
-      if (value < 0 || value > 64) return fail(s->ctx, PMX_ERR_ARG, "tiles per CU must be 0..64");
-      s->tune_tiles_per_cu = value;
-      s->tile_target = value * s->ctx->cus;
-      break;
     case PMX_TUNE_FFT_LINES:
       if (value < 1 || value > 64) return fail(s->ctx, PMX_ERR_ARG, "FFT lines per block must be 1..64");
       s->tune_fft_lines = value;
@@ -1450,10 +1505,9 @@ int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value) {
       if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "spread stream must be 0 or 1");
       s->tune_spread_stream2 = value;
       break;
-    case PMX_TUNE_PAIR_RUNS:
-      if (value < 1 || value > kMaxPairRunsPerCu / kMaxPairWavesPerCu)
-        return fail(s->ctx, PMX_ERR_ARG, "pair runs per wave must be 1..4");
-      s->tune_runs = value;
+    case PMX_TUNE_LIST_PRUNE:
+      if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "list prune must be 0 or 1");
+      s->tune_prune = value;
       break;
     default:
       return fail(s->ctx, PMX_ERR_ARG, "unknown tuning knob");

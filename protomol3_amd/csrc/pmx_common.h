@@ -8,9 +8,21 @@ namespace pmx {
 
 constexpr int kClusterSize = 8;    // atoms per cluster (one 8-lane group of a wave)
 constexpr int kSuperSize = 64;     // atoms per supercluster (8 clusters, one wave)
-constexpr int kTileEntries = 16;   // list entries per work tile
+constexpr int kTileEntries = 16;   // granularity of the per-supercluster list capacity
 constexpr int kMaxPairWavesPerCu = 32;  // resident pair-kernel waves per CU, at most
-constexpr int kMaxPairRunsPerCu = 4 * kMaxPairWavesPerCu;  // balanced runs: up to 4 per wave
+// Work runs of the pair kernel: the list is cut into runs of equal estimated
+// work whose number depends only on the list (never on the device or the
+// launch shape): kMinRuns k, k = ceil(total work / (kMinRuns kRunCost)),
+// 1 <= k <= kMaxRuns / kMinRuns.  A wave sums its i-forces in FP32 over a
+// run's part of one supercluster (a tile) and adds the sums to the force
+// array in fixed point, so forces and energies are the same whichever wave
+// takes which run.  kMinRuns is the pair kernel's resident wave count on
+// MI355X (256 CUs x 4 blocks of 4 waves): the static schedule gives every
+// wave k equal runs there (one for systems up to about 10^5 atoms: a run
+// boundary costs a flush of the i-forces, so runs are long).
+constexpr int kMinRuns = 4096;
+constexpr int kMaxRuns = 5 * kMinRuns;
+constexpr int kRunCost = 8192;     // work units (list cost model) per run, at least
 constexpr int kWave = 64;
 constexpr int kPairBlock = 256;    // 4 waves
 constexpr double kForceScale = 4294967296.0;      // 2^32: int64 fixed-point forces
@@ -46,6 +58,10 @@ struct PairParams {
   // box edges split as Lhi + Llo with Lhi a multiple of 1/16 A, so that image
   // shifts of supercluster origins (also multiples of 1/16) add exactly
   double Lhi[3], Llo[3];
+  // pair-list radius^2 (cutoff + margin + skin, with slack for the FP32 r^2):
+  // the pass after a list build drops cluster pairs without a candidate
+  // pair inside it, which no later pass before the next build can need
+  double prune_r2;
 };
 
 // Sorted coordinates are stored relative to the origin of their supercluster

@@ -590,6 +590,7 @@ constexpr int kListBlock = 256;
 // supercluster (a wave whose run crosses into it loads its i-atoms and adds
 // their forces once more).
 constexpr int kCpCost = 8, kMemCost = 9, kEntryCost = 14, kTileCost = 80;
+
 __device__ inline void list_append(const ListParams &q, int S, const float4 (&ic)[8], const float4 (&ih)[8], int cj,
                                    float4 jc, float4 jh, int4 *__restrict__ entries, int *__restrict__ maskwork,
                                    int *__restrict__ nmaskwork, int maxmaskwork, int *__restrict__ overflow,
@@ -936,59 +937,6 @@ __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ mas
 }
 
 // Compact tile list: tiles of up to kTileEntries entries per supercluster.
-__global__ void k_make_tiles(int nsuper, const int *__restrict__ count, int4 *__restrict__ tiles,
-                             int *__restrict__ ntiles, int ntiles_cap, int target) {
-  // single block of 1024 threads.  Tile length: the largest of 16, 8, 4 and
-  // 2 entries that gives at least `target` tiles (one per resident wave of
-  // the pair kernel, so small systems still spread over the device), as long
-  // as the smaller length's tiles fit in ntiles_cap.
-  __shared__ int s[1024];
-  __shared__ int s_tot[4][1024 / kWave];
-  int tot[4] = {0, 0, 0, 0};
-  for (int S = threadIdx.x; S < nsuper; S += blockDim.x) {
-    const int c = count[S];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) tot[k] += (c + (kTileEntries >> k) - 1) / (kTileEntries >> k);
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int v = wave_sum(tot[k]);
-    if ((threadIdx.x & 63) == 0) s_tot[k][threadIdx.x >> 6] = v;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    tot[k] = 0;
-    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) tot[k] += s_tot[k][w];
-  }
-  int lg = 0;
-  while (lg < 3 && tot[lg] < target && tot[lg + 1] <= ntiles_cap) ++lg;
-  const int T = kTileEntries >> lg;
-  int carry = 0;
-  for (int base = 0; base < nsuper; base += 1024) {
-    int S = base + threadIdx.x;
-    int t = S < nsuper ? (count[S] + T - 1) / T : 0;
-    s[threadIdx.x] = t;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      int v = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
-      __syncthreads();
-      s[threadIdx.x] += v;
-      __syncthreads();
-    }
-    int excl = carry + s[threadIdx.x] - t;
-    for (int k = 0; k < t; ++k) tiles[excl + k] = make_int4(S, k * T, min(k * T + T, count[S]), 0);
-    __syncthreads();
-    carry += s[1023];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    ntiles[0] = carry;
-    ntiles[1] = T;
-  }
-}
-
-// Largest index k in [0, n) with a[k] <= v (a ascending, a[0] <= v).
 template <class T>
 __device__ inline int last_le(const T *__restrict__ a, int n, T v) {
   int lo = 0, hi = n - 1;
@@ -1002,23 +950,24 @@ __device__ inline int last_le(const T *__restrict__ a, int n, T v) {
   return lo;
 }
 
-// Balanced schedule: the list entries of all superclusters, in supercluster
+// Run schedule: the list entries of all superclusters, in supercluster
 // order, are cut into nruns contiguous runs of equal estimated work (the
 // work estimates of k_build_list: per supercluster, and as an inclusive
-// prefix over its entries); nruns is a multiple of the pair kernel's
-// resident waves.  Run w starts at entry position pos[w] (ep[S] + offset,
-// offsets even) in supercluster posS[w], where the global work prefix first
-// exceeds b_w = floor(w C / nruns), C the total.  A run is split into one
-// tile per supercluster it touches (at most nruns + nsuper tiles); run r is
-// tiles [wstart[r], wstart[r + 1]).  Three passes, none searching:
+// prefix over its entries); nruns, a multiple of kMinRuns set by C, the
+// total, depends only on the list.  Run w starts at entry
+// position pos[w] (ep[S] + offset, offsets even) in supercluster posS[w],
+// where the global work prefix first exceeds b_w = floor(w C / nruns).  A run
+// is split into one tile per supercluster it touches (at most nruns + nsuper
+// tiles); run r is tiles [wstart[r], wstart[r + 1]).  Three passes, none
+// searching:
 //   k_runs_prefix (one block): prefixes ep (entries) and cp (work) over the
-//     superclusters, run boundaries preset to the end;
+//     superclusters, the run count, run boundaries preset to the end;
 //   k_runs_cut (a block per supercluster): each entry places the run
 //     boundaries b_w that fall in its work interval;
 //   k_runs_tiles (one block): tiles per run, their prefix, the tiles.
-__global__ void k_runs_prefix(int nsuper, const int *__restrict__ count, const int *__restrict__ cost, int nruns,
-                              int *__restrict__ ep, long long *__restrict__ cp, int *__restrict__ pos,
-                              int *__restrict__ posS) {
+__global__ void k_runs_prefix(int nsuper, const int *__restrict__ count, const int *__restrict__ cost,
+                              int *__restrict__ nruns_p, int *__restrict__ ep, long long *__restrict__ cp,
+                              int *__restrict__ pos, int *__restrict__ posS) {
   __shared__ int s_e[1024];
   __shared__ long long s_c[1024];
   __shared__ int s_last[1024 / kWave];
@@ -1054,9 +1003,13 @@ __global__ void k_runs_prefix(int nsuper, const int *__restrict__ count, const i
   __syncthreads();
   int lastS = -1;
   for (int k = 0; k < 1024 / kWave; ++k) lastS = max(lastS, s_last[k]);
+  const long long per = (long long)kMinRuns * kRunCost;
+  const long long k = (ccarry + per - 1) / per;
+  const int nruns = kMinRuns * (int)(k < 1 ? 1 : (k > kMaxRuns / kMinRuns ? kMaxRuns / kMinRuns : k));
   if (tid == 0) {
     ep[nsuper] = ecarry;
     cp[nsuper] = ccarry;
+    *nruns_p = nruns;
   }
   // every boundary starts at the end (k_runs_cut moves those that fall in
   // the list); run 0 starts at the beginning
@@ -1067,9 +1020,10 @@ __global__ void k_runs_prefix(int nsuper, const int *__restrict__ count, const i
 }
 
 __global__ void k_runs_cut(int nsuper, const int *__restrict__ count, const int *__restrict__ ecost, int cap,
-                           int nruns, const int *__restrict__ ep, const long long *__restrict__ cp,
-                           int *__restrict__ pos, int *__restrict__ posS) {
+                           const int *__restrict__ nruns_p, const int *__restrict__ ep,
+                           const long long *__restrict__ cp, int *__restrict__ pos, int *__restrict__ posS) {
   const int S = blockIdx.x;
+  const int nruns = *nruns_p;
   const int n = count[S];
   const long long C = cp[nsuper], base = cp[S];
   if (C <= 0) return;
@@ -1081,17 +1035,18 @@ __global__ void k_runs_cut(int nsuper, const int *__restrict__ count, const int 
     w0 = max(w0, 1ll);
     w1 = min(w1, (long long)nruns);
     for (long long w = w0; w < w1; ++w) {
-      pos[w] = ep[S] + (e & ~1);
+      pos[w] = ep[S] + (e & ~1);  // runs hold whole pair-kernel groups
       posS[w] = S;
     }
   }
 }
 
-__global__ void k_runs_tiles(const int *__restrict__ count, int nruns, const int *__restrict__ ep,
-                             const int *__restrict__ pos, const int *__restrict__ posS, int *__restrict__ wstart,
-                             int4 *__restrict__ tiles, int *__restrict__ ntiles, int ntiles_cap) {
+__global__ void k_runs_tiles(const int *__restrict__ count, const int *__restrict__ ep, const int *__restrict__ pos,
+                             const int *__restrict__ posS, int *__restrict__ wstart, int4 *__restrict__ tiles,
+                             int *__restrict__ ntiles, int ntiles_cap) {
   __shared__ int s_n[1024];
   const int tid = threadIdx.x;
+  const int nruns = ntiles[2];  // from k_runs_prefix
   // run w: entries [pos[w], pos[w+1]), superclusters S0 .. S1
   auto span = [&](int w, int &S0, int &S1) {
     const int p0 = pos[w], p1 = pos[w + 1];
@@ -1127,8 +1082,6 @@ __global__ void k_runs_tiles(const int *__restrict__ count, int nruns, const int
   if (tid == 0) {
     wstart[nruns] = min(carry, ntiles_cap);
     ntiles[0] = min(carry, ntiles_cap);
-    ntiles[1] = 0;
-    ntiles[2] = nruns;
   }
 }
 
@@ -1293,17 +1246,8 @@ __device__ inline double group8_sum(double v) {
   v += __shfl_xor(v, 4);
   return v;
 }
-// Sum over lanes differing in bits 3..5.
-template <class R>
-__device__ inline R col8_sum(R v) {
-  v += __shfl_xor(v, 8);
-  v += __shfl_xor(v, 16);
-  v += __shfl_xor(v, 32);
-  return v;
-}
-
 struct PairOut {
-  double *energy;            // [2 * ntiles_max]: per tile (coulomb, lj)
+  double *energy;            // [2 * ntiles_max]: per tile (coulomb, lj); int64 fixed point 2^-32 in the mixed form
   long long *npairs;         // [ntiles_max] when stats
   unsigned long long *hash;  // [ntiles_max]
   int2 *border;              // recorded boundary pairs (sorted slots i, j)
@@ -1333,6 +1277,40 @@ struct JAtom {
 
 constexpr int kGroup = 2;    // list entries whose pairs share one queue flush
 constexpr int kQueue = 128;  // queue ring per wave (holds < 64 + 64 pairs)
+static_assert(kGroup == 2, "runs are cut at even entries (k_runs_cut)");
+
+// Wave-private LDS area of the pair kernel: the force slots of a group and
+// the queue; at a segment end, when both are empty, it holds the transpose of
+// one component of the i-force partial sums (8 values per lane).  (Kept
+// small: an FFT block of the PME chain must fit on a CU beside the pair
+// kernel's blocks.)
+template <class R>
+struct PairArea {
+  static constexpr int kSlotBytes = (int)sizeof(R) * kGroup * 8 * kWave;
+  static constexpr int kQueueBytes = (int)sizeof(PairQ<R>) * kQueue;
+  static constexpr int kTransBytes = (int)sizeof(R) * 8 * kWave;
+  static constexpr int kBytes =
+      kSlotBytes + kQueueBytes > kTransBytes ? kSlotBytes + kQueueBytes : kTransBytes;
+};
+
+// Fixed-point (mixed form) or FP64 sums of a wave's own i-atom forces and of
+// its pair energies.
+template <bool MIXED>
+struct WaveAcc;
+template <>
+struct WaveAcc<true> {
+  using T = long long;
+  __device__ static T of(float v) { return __float2ll_rn(v * 4294967296.0f); }
+  __device__ static void add(unsigned long long *p, T v) { atomicAdd(p, (unsigned long long)v); }
+  __device__ static double energy_bits(T v) { return __longlong_as_double(v); }
+};
+template <>
+struct WaveAcc<false> {
+  using T = double;
+  __device__ static T of(double v) { return v; }
+  __device__ static void add(double *p, T v) { unsafeAtomicAdd(p, v); }
+  __device__ static double energy_bits(T v) { return v; }
+};
 
 // Box edges in the kernel's type, split as hi + lo (see PairParams::Lhi).
 template <class R>
@@ -1380,14 +1358,26 @@ __device__ inline void pair_delta(const R4 &xi, const R4 &xj, const EdgesR<R> &E
 //   B. the last partial batch is evaluated; each pair's scalar force lands in
 //      its slot of an LDS array (zero for non-members);
 //   C. the candidate lanes read their slot and accumulate F = f d into the
-//      i-forces (registers, whole tile) and j-forces (summed over the 8
-//      i-lanes with DPP, one atomic add per j-atom component and entry).
+//      i-forces (registers) and j-forces (summed over the 8 i-lanes with DPP,
+//      one fixed-point atomic add per j-atom component and entry).
 // Only phase B does the expensive math (rsqrt, exp, erfc polynomial,
 // switches), on members only, which are about a quarter of the candidates.
-template <bool MIXED, int COUL, int LJSW, int CSW>
-__global__ void __launch_bounds__(kPairBlock) k_pair(
-    PairParams pp, int stats, const int4 *__restrict__ tiles, const int *__restrict__ ntiles_p,
-    const int *__restrict__ wstart, int *__restrict__ tile_counter, const int4 *__restrict__ entries, int cap,
+// Deterministic sums: a tile (a run's part of one supercluster's list) is the
+// same whichever wave takes it, and at its end the i-force partial sums are
+// transposed through LDS, lane 8 c + ti sums its i-atom's eight partials in
+// a fixed order and adds the sum to the force array in fixed point; the
+// tile's FP32 pair-energy sums go to fixed point too.  So the mixed form's forces and energies do not depend
+// on the schedule, the grid or the device.
+//
+// PRUNE (the first pass after a list build): cluster pairs without a
+// candidate pair inside the list radius are dropped from the entries' masks
+// (through entries_w, which aliases entries; each entry is read and written
+// by the one wave that takes it).
+template <bool MIXED, int COUL, int LJSW, int CSW, bool BIGLJ, bool PRUNE>
+__global__ void __launch_bounds__(kPairBlock, 4) k_pair(
+    PairParams pp, int stats, int dynamic, const int4 *__restrict__ tiles, const int *__restrict__ ntiles_p,
+    const int *__restrict__ wstart, int *__restrict__ tile_counter, const int4 *__restrict__ entries, int4 *entries_w,
+    int cap,
     const unsigned short *__restrict__ masks,
     const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
     const float4 *__restrict__ sorg, const int *__restrict__ ts, const int *__restrict__ perm,
@@ -1397,19 +1387,19 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   using R = typename std::conditional<MIXED, float, double>::type;
   using R4 = typename Real4<R>::type;
   using Acc = AccT<MIXED>;
+  using WA = WaveAcc<MIXED>;
+  using AT = typename WA::T;
   constexpr int W = kPairBlock / kWave;
   __shared__ R4 s_i[W][kSuperSize];
   __shared__ int s_ty[W][kSuperSize];
   __shared__ R4 s_lj[kMaxTypes * kMaxTypes];  // {A, B, A14, B14}
-  __shared__ PairQ<R> s_q[W][kQueue];
-  __shared__ __align__(16) R s_fr[W][kGroup * 8 * kWave];
+  __shared__ __align__(16) unsigned char s_area[W][PairArea<R>::kBytes];
   __shared__ JAtom<R> s_j[W][kGroup * kClusterSize];
 
   const int nt = pp.ntypes;
-  // up to kMaxTypes LJ types the table sits in LDS; beyond that the typed
-  // copy ljt is read through the caches
-  const bool lj_lds = nt <= kMaxTypes;
-  for (int k = threadIdx.x; lj_lds && k < nt * nt; k += blockDim.x) {
+  // up to kMaxTypes LJ types the table sits in LDS; beyond that (BIGLJ) the
+  // typed copy ljt is read through the caches
+  for (int k = threadIdx.x; !BIGLJ && k < nt * nt; k += blockDim.x) {
     R4 v;
     v.x = (R)ljtab[4 * k];
     v.y = (R)ljtab[4 * k + 1];
@@ -1430,30 +1420,32 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
   const R guard = (R)pp.guard;
   const R alpha = (R)pp.alpha, a2pi = (R)pp.a2pi, csf = (R)pp.csf;
   const bool one_cut = sl.rc2 == sc.rc2;  // one guard band serves both cutoffs
-  const int ntiles = ntiles_p[0], tlen = ntiles_p[1];
+  const R prune_r2 = (R)pp.prune_r2;
   // the boundary-pair sums are filled after this kernel (same stream)
   if (blockIdx.x == 0 && threadIdx.x < 4) out.border_acc[threadIdx.x] = 0ull;
   R4 *const si = s_i[wv];
-  PairQ<R> *const q = s_q[wv];
-  R *const fr_slot = s_fr[wv];
+  R *const fr_slot = reinterpret_cast<R *>(s_area[wv]);
+  PairQ<R> *const q = reinterpret_cast<PairQ<R> *>(s_area[wv] + PairArea<R>::kSlotBytes);
+  R *const tr = reinterpret_cast<R *>(s_area[wv]);
 
-  // Schedules (the grid holds the waves that fit on the device at once):
-  //  tlen == 0: balanced runs, run r is tiles [wstart[r], wstart[r+1]);
-  //    wave w takes run w, then runs from the counter;
-  //  tlen == kTileEntries: the first tile of a wave is its global wave
-  //    index, later tiles come from the shared counter;
-  //  shorter tiles (a small system split finely): round robin, since a
-  //    counter hit once per short tile would serialize the waves.
+  // Runs (the grid holds the waves that fit on the device at once): wave w
+  // takes the contiguous runs [w nruns / nwaves, (w + 1) nruns / nwaves)
+  // (dynamic == 0: consecutive tiles often share a supercluster, whose
+  // i-atoms then stay loaded), or
+  // run w and then runs in order from a device-wide counter (dynamic == 1).
+  // Run r is tiles [wstart[r], wstart[r + 1]).  Which wave takes a run
+  // changes no result.
   const int nwaves = gridDim.x * W;
   const int gw = blockIdx.x * W + wv;
-  const int nruns = ntiles_p[2];  // balanced: runs (wave w starts with run w)
-  int t = tlen == 0 ? wstart[gw] : gw;
-  int tstop = tlen == 0 ? wstart[gw + 1] : ntiles;
+  const int nruns = ntiles_p[2];
+  const int r0 = dynamic ? gw : (int)((long long)gw * nruns / nwaves);
+  const int r1 = dynamic ? min(gw + 1, nruns) : (int)((long long)(gw + 1) * nruns / nwaves);
+  int t = r0 < r1 ? wstart[r0] : 0, tstop = r0 < r1 ? wstart[r1] : 0;
+  int curS = -1;      // supercluster whose i-atoms are loaded
+  bool ireal = false;  // slot `lane` of curS holds an atom
   while (true) {
     if (t >= tstop) {
-      // balanced runs beyond one per wave are taken from the counter in
-      // order, so that waves that started late or ran slow take fewer
-      if (tlen != 0 || nruns <= nwaves) break;
+      if (!dynamic) break;
       int rn = 0;
       if (lane == 0) rn = nwaves + atomicAdd(tile_counter, 1);
       const int r = __builtin_amdgcn_readfirstlane(__shfl(rn, 0));
@@ -1466,17 +1458,19 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
     const int S = __builtin_amdgcn_readfirstlane(tile.x);
     const int e0 = __builtin_amdgcn_readfirstlane(tile.y);
     const float4 oS = sorg[S];
-    si[lane] = xs[S * kSuperSize + lane];
-    {
+    if (S != curS) {
+      curS = S;
+      si[lane] = xs[S * kSuperSize + lane];
       const int v = ts[S * kSuperSize + lane];
+      ireal = v >= 0;
       s_ty[wv][lane] = v < 0 ? 0 : v;
+      __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_wave_barrier();
 
     R fi[8][3];
 #pragma unroll
     for (int c = 0; c < 8; ++c) fi[c][0] = fi[c][1] = fi[c][2] = (R)0;
-    double elj = 0.0, eco = 0.0;
+    R elT = 0, ecT = 0;  // this lane's pair energies over the tile
     int npair = 0;
     unsigned long long hsum = 0;
     int qhead = 0, qtail = 0;  // wave-uniform queue positions
@@ -1500,7 +1494,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
         R A = 0, B = 0;
         if (LJSW != SW_NONE) {
           const int li = s_ty[wv][ia] * nt + jd.ty;
-          const R4 lj = lj_lds ? s_lj[li] : ljt[li];
+          const R4 lj = BIGLJ ? ljt[li] : s_lj[li];
           A = mod ? lj.z : lj.x;
           B = mod ? lj.w : lj.y;
         }
@@ -1511,6 +1505,26 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
         fr_slot[slot] = f;
       }
       qhead += cnt;
+    };
+
+    // Tile end (queue empty, slots free): the tile's FP32 i-force partial
+    // sums go through LDS one component at a time so that lane 8 c + ti sums
+    // the eight partials of its i-atom (from lanes (ti, tj'), tj' = 0..7 in
+    // order) and adds the sum, in fixed point, to the atom's force.
+    auto flush = [&]() {
+      const int sidx = S * kSuperSize + lane;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int c = 0; c < 8; ++c) tr[lane * 8 + c] = fi[c][k];
+        __builtin_amdgcn_wave_barrier();
+        R sk = 0;
+#pragma unroll
+        for (int tq = 0; tq < 8; ++tq) sk += tr[(8 * tq + ti) * 8 + tj];
+        if (ireal) WA::add(&facc[3 * sidx + k], WA::of(sk));
+      }
+      __builtin_amdgcn_wave_barrier();
     };
 
     const int eend = __builtin_amdgcn_readfirstlane(tile.z);
@@ -1561,6 +1575,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
       // ---- phase A: membership, queue, full batches
       R qel = 0, qec = 0;
       unsigned used = 0;  // bit g * 8 + c: the cluster pair has a member
+      unsigned kept = 0;  // bit g * 8 + c: (prune pass) a candidate pair within the list radius
 #pragma unroll
       for (int g = 0; g < kGroup; ++g) {
         const int sj = cjv[g] * kClusterSize + tj;
@@ -1590,6 +1605,7 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
           const bool in_c = COUL != COUL_NONE && ok && r2 <= sc.rc2;
           const bool in = in_lj || in_c;
           const unsigned long long bm = __builtin_amdgcn_ballot_w64(in);
+          if (PRUNE && __builtin_amdgcn_ballot_w64(cand && r2 <= prune_r2)) kept |= 1u << (g * 8 + c);
           if (border) {
             const int b = atomicAdd(out.nborder, 1);
             if (b < out.maxborder)
@@ -1617,10 +1633,21 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
           }
         }
       }
+      // prune pass: cluster pairs without a candidate pair inside the list
+      // radius cannot have a member before the next list build (every atom
+      // stays within half the skin of its build position); later passes skip
+      // them.  They have no member now either, so no result changes.
+      if (PRUNE) {
+#pragma unroll
+        for (int g = 0; g < kGroup; ++g) {
+          const int km = (kept >> (g * 8)) & 0xff;
+          if (g < ng && lane == 0 && km != emv[g]) entries_w[(size_t)S * cap + eg + g].y = km;
+        }
+      }
       // ---- phase B: the rest of the queue
       if (qtail > qhead) eval_batch(qtail - qhead, qel, qec);
-      elj += (double)qel;
-      eco += (double)qec;
+      elT += qel;
+      ecT += qec;
       __builtin_amdgcn_wave_barrier();
 
       // ---- phase C: forces from the slots
@@ -1657,51 +1684,24 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(
       }
       __builtin_amdgcn_wave_barrier();
     }
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      fi[c][0] = col8_sum(fi[c][0]);
-      fi[c][1] = col8_sum(fi[c][1]);
-      fi[c][2] = col8_sum(fi[c][2]);
-    }
-    R my0 = 0, my1 = 0, my2 = 0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-      if (tj == c) {
-        my0 = fi[c][0];
-        my1 = fi[c][1];
-        my2 = fi[c][2];
-      }
-    const int sidx = S * kSuperSize + tj * 8 + ti;
-    if (ts[sidx] >= 0) {
-      Acc::add(&facc[3 * sidx], my0);
-      Acc::add(&facc[3 * sidx + 1], my1);
-      Acc::add(&facc[3 * sidx + 2], my2);
-    }
-    elj = wave_sum(elj);
-    eco = wave_sum(eco);
+    flush();
+    const AT eljA = wave_sum(WA::of(elT));
+    const AT ecoA = wave_sum(WA::of(ecT));
     long long np = 0;
     if (stats) {
       np = wave_sum((long long)npair);
       hsum = wave_sum(hsum);
     }
     if (lane == 0) {
-      out.energy[2 * t] = eco;
-      out.energy[2 * t + 1] = elj;
+      out.energy[2 * t] = WA::energy_bits(ecoA);
+      out.energy[2 * t + 1] = WA::energy_bits(eljA);
       if (stats) {
         out.npairs[t] = np;
         out.hash[t] = hsum;
       }
     }
     __builtin_amdgcn_wave_barrier();
-    if (tlen == 0) {
-      ++t;
-    } else if (tlen < kTileEntries) {
-      t += nwaves;
-    } else {
-      int tn = 0;
-      if (lane == 0) tn = nwaves + atomicAdd(tile_counter, 1);
-      t = __builtin_amdgcn_readfirstlane(__shfl(tn, 0));
-    }
+    ++t;
   }
 }
 
@@ -2431,8 +2431,11 @@ __global__ void k_unsort_forces(int n, const int *__restrict__ slot, typename Ac
 }
 
 // Deterministic energy/pair-stat reduction in one block, in index order.
+// Pair energies per tile are int64 fixed point (2^-32) when pair_fixed (the
+// mixed form: their sum is exact, whatever tiles the schedule made), FP64
+// otherwise.
 __global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p, const double *__restrict__ pair_e,
-                                int nrecip, const double *__restrict__ recip_e, int ncorr,
+                                int pair_fixed, int nrecip, const double *__restrict__ recip_e, int ncorr,
                                 const double *__restrict__ corr_e, const long long *__restrict__ npairs,
                                 const unsigned long long *__restrict__ hash, int stats,
                                 const unsigned long long *__restrict__ border_e,
@@ -2441,13 +2444,19 @@ __global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p
   __shared__ double sd[4][1024 / kWave];
   __shared__ long long sn[1024 / kWave];
   __shared__ unsigned long long sh[1024 / kWave];
+  __shared__ long long sf[2][1024 / kWave];
   const int nt = min(*ntiles_p, ntiles_cap);
   double c = 0, l = 0, r = 0, k = 0;
-  long long np = 0;
+  long long np = 0, cf = 0, lf = 0;
   unsigned long long h = 0;
   for (int i = threadIdx.x; i < nt; i += blockDim.x) {
-    c += pair_e[2 * i];
-    l += pair_e[2 * i + 1];
+    if (pair_fixed) {
+      cf += __double_as_longlong(pair_e[2 * i]);
+      lf += __double_as_longlong(pair_e[2 * i + 1]);
+    } else {
+      c += pair_e[2 * i];
+      l += pair_e[2 * i + 1];
+    }
     if (stats) {
       np += npairs[i];
       h += hash[i];
@@ -2457,6 +2466,8 @@ __global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p
   for (int i = threadIdx.x; i < ncorr; i += blockDim.x) k += corr_e[i];
   c = wave_sum(c);
   l = wave_sum(l);
+  cf = wave_sum(cf);
+  lf = wave_sum(lf);
   r = wave_sum(r);
   k = wave_sum(k);
   np = wave_sum(np);
@@ -2469,13 +2480,17 @@ __global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p
     sd[3][w] = k;
     sn[w] = np;
     sh[w] = h;
+    sf[0][w] = cf;
+    sf[1][w] = lf;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double C = 0, Lj = 0, Rr = 0, K = 0;
-    long long N = 0;
+    long long N = 0, CF = 0, LF = 0;
     unsigned long long H = 0;
     for (int i = 0; i < (int)(blockDim.x / kWave); ++i) {
+      CF += sf[0][i];
+      LF += sf[1][i];
       C += sd[0][i];
       Lj += sd[1][i];
       Rr += sd[2][i];
@@ -2483,8 +2498,8 @@ __global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p
       N += sn[i];
       H += sh[i];
     }
-    out[0] = C + (double)(long long)border_e[0] * (1.0 / kForceScale);
-    out[1] = Lj + (double)(long long)border_e[1] * (1.0 / kForceScale);
+    out[0] = C + (double)(CF + (long long)border_e[0]) * (1.0 / kForceScale);
+    out[1] = Lj + (double)(LF + (long long)border_e[1]) * (1.0 / kForceScale);
     out[2] = 0.5 * Rr;
     out[3] = K;
     *out_np = N + (long long)border_stat[0];

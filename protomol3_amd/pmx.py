@@ -56,8 +56,8 @@ class MdDesc(C.Structure):
 
 
 LEAPFROG, LANGEVIN = 0, 1
-TUNE_TILES_PER_CU, TUNE_FFT_LINES, TUNE_PAIR_BLOCKS_PER_CU, TUNE_PAIR_SCHEDULE, TUNE_PAIR_RUNS = 1, 2, 3, 4, 5
-TUNE_SPREAD_STREAM = 6
+TUNE_FFT_LINES, TUNE_PAIR_BLOCKS_PER_CU, TUNE_PAIR_SCHEDULE = 2, 3, 4
+TUNE_SPREAD_STREAM, TUNE_LIST_PRUNE = 6, 7
 
 
 EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_create", "pmx_sys_destroy",
@@ -385,9 +385,9 @@ class System:
         return float(ms.value)
 
     def set_tuning(self, knob: int, value: int):
-        """Sets a launch shape (TUNE_PAIR_SCHEDULE, TUNE_PAIR_RUNS,
-        TUNE_TILES_PER_CU, TUNE_FFT_LINES, TUNE_PAIR_BLOCKS_PER_CU or
-        TUNE_SPREAD_STREAM); the resident state is kept."""
+        """Sets a launch shape (TUNE_PAIR_SCHEDULE, TUNE_FFT_LINES,
+        TUNE_PAIR_BLOCKS_PER_CU, TUNE_SPREAD_STREAM or TUNE_LIST_PRUNE);
+        the resident state is kept."""
         self.ctx.check(self.ctx.lib.pmx_sys_set_tuning(self.h, knob, value), "pmx_sys_set_tuning")
 
     def md_get_state(self):

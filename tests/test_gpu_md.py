@@ -6,7 +6,7 @@ import pytest
 from oracle import md
 from oracle import oracle as O
 from protomol3_amd import pmx, systems
-from test_golden_trajectory import SPEC, load_golden, water423
+from test_golden_trajectory import SPEC, check_outputs, load_golden, water423
 
 pytestmark = pytest.mark.gpu
 
@@ -18,12 +18,17 @@ def ctx():
     c.close()
 
 
-def run_gpu_md(ctx, s, spec, v0, dt, nsteps, precision, skin, chunks=1):
+def run_gpu_md(ctx, s, spec, v0, dt, nsteps, precision, skin, chunks=1, initial=None):
+    """nsteps of leapfrog on the device; initial (a list) receives the total
+    forces at the start configuration."""
     sy = pmx.System(ctx, s, pmx.ForceSpec(spec["cellsize"], lj=spec.get("lj"), coul=spec.get("coul"),
                                           pme=spec.get("pme")), precision)
     try:
         sy.md_init(s, dt, skin=skin)
         sy.md_set_state(s.pos, v0)
+        if initial is not None:
+            sy.md_get_state()
+            initial.append(sy.get_forces()[0])
         for _ in range(chunks):
             sy.md_run(nsteps // chunks)
         x, v, e = sy.md_get_state()
@@ -35,12 +40,17 @@ def run_gpu_md(ctx, s, spec, v0, dt, nsteps, precision, skin, chunks=1):
 
 
 @pytest.mark.parametrize("skin", [0.0, 1.0])
-def test_gpu_md_reproduces_reference_golden_trajectory(ctx, skin):
+def test_gpu_md_reproduces_reference_golden_trajectory(ctx, skin, tmp_path):
     """water_CHARMM_PERIODIC_L: 200 steps of 0.5 fs on the GPU (FP64 form),
-    compared with the reference's expected outputs at its harness epsilon."""
+    compared with the reference's expected outputs at its harness epsilon;
+    the force trajectory (XYZForceFile) and energies line written from the
+    GPU's results match the reference's expected files line by line."""
     s, v0 = water423()
     pos, vel, f0, f200, en = load_golden()
-    x, v, f, e, stats = run_gpu_md(ctx, s, SPEC, v0, 0.5, 200, pmx.FP64, skin)
+    f_init = []
+    x, v, f, e, stats = run_gpu_md(ctx, s, SPEC, v0, 0.5, 200, pmx.FP64, skin, initial=f_init)
+    assert np.abs(f_init[0] - f0).max() < 1e-7
+    check_outputs(s, f_init[0], f, e, tmp_path)
     print(f"skin={skin} stats={stats} dpos={np.abs(x - pos).max():.2e} dvel={np.abs(v - vel).max():.2e} "
           f"dforce={np.abs(f - f200).max():.2e} energies={e}", flush=True)
     assert stats["steps"] == 200
@@ -150,13 +160,12 @@ def test_gpu_md_graph_replay_matches_kernel_launches(ctx, precision):
 @pytest.mark.parametrize("precision", [pmx.MIXED, pmx.FP64])
 def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
     """The autotuner's launch shapes change how the work is cut, not what is
-    computed.  Under the counter schedule, pair blocks per CU and FFT lines
-    per block leave every sum as it was: a run whose shapes change between
-    its parts follows the unchanged run (the mixed form to the last bit:
-    fixed-point accumulation).  The schedule, the tile length and (balanced
-    schedule) the blocks per CU regroup the FP32 partial sums of the
-    i-forces (one per tile), so forces at one configuration agree to FP32
-    rounding in the mixed form."""
+    computed.  A run whose shapes change between its parts follows the
+    unchanged run (the mixed form to the last bit: fixed-point accumulation).
+    Every schedule cuts the pair list at the same 8-entry segments, whose
+    FP32 partial sums go to fixed point before they are combined, so the
+    mixed form's forces and energies at one configuration are bit-identical
+    under every schedule, run count, tile length and block count."""
     s = systems.water_box(seed=32).with_velocities(300.0, seed=9)
     spec = pmx.ForceSpec(5.0, lj=(pmx.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 30, 24), order=4, alpha=0.32))
     shapes = [{}, {pmx.TUNE_FFT_LINES: 1, pmx.TUNE_SPREAD_STREAM: 1},
@@ -191,11 +200,10 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
     sy = pmx.System(ctx, s, spec, precision)
     try:
         f0, ec0, elj0 = sy.evaluate(x)
-        ftol = 1e-6 if precision == pmx.MIXED else 1e-12
-        cases = [((pmx.TUNE_PAIR_BLOCKS_PER_CU, 1),), ((pmx.TUNE_PAIR_RUNS, 1),), ((pmx.TUNE_PAIR_RUNS, 4),),
-                 ((pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),),
-                 ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 0), (pmx.TUNE_PAIR_SCHEDULE, 1))]
-        cases += [((pmx.TUNE_TILES_PER_CU, t),) for t in (0, 8, 64)]
+        ftol = 0.0 if precision == pmx.MIXED else 1e-12
+        cases = [((pmx.TUNE_PAIR_BLOCKS_PER_CU, 1),), ((pmx.TUNE_PAIR_SCHEDULE, 1),),
+                 ((pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),), ((pmx.TUNE_LIST_PRUNE, 0),),
+                 ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 0), (pmx.TUNE_PAIR_SCHEDULE, 0), (pmx.TUNE_LIST_PRUNE, 1))]
         for case in cases:
             for k, v in case:
                 sy.set_tuning(k, v)
@@ -203,10 +211,11 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
             rel = np.sqrt(np.mean(np.sum((f - f0) ** 2, 1)) / np.mean(np.sum(f0 ** 2, 1)))
             print(f"prec={precision} {case} rms_rel {rel:.3e}", flush=True)
             assert rel <= ftol
+            if precision == pmx.MIXED:
+                assert np.array_equal(f, f0) and ec == ec0 and elj == elj0
             assert abs(ec - ec0) <= 1e-9 * abs(ec0) and abs(elj - elj0) <= 1e-9 * max(1.0, abs(elj0))
-        for knob, bad in ((pmx.TUNE_FFT_LINES, 0), (pmx.TUNE_TILES_PER_CU, 65), (pmx.TUNE_PAIR_BLOCKS_PER_CU, -8),
-                          (pmx.TUNE_PAIR_SCHEDULE, 2), (pmx.TUNE_PAIR_RUNS, 0), (pmx.TUNE_PAIR_RUNS, 5),
-                          (pmx.TUNE_SPREAD_STREAM, 2), (99, 1)):
+        for knob, bad in ((pmx.TUNE_FFT_LINES, 0), (pmx.TUNE_PAIR_BLOCKS_PER_CU, -8), (pmx.TUNE_PAIR_SCHEDULE, 2),
+                          (pmx.TUNE_SPREAD_STREAM, 2), (pmx.TUNE_LIST_PRUNE, 2), (1, 16), (5, 1), (99, 1)):
             with pytest.raises(pmx.PmxError):
                 sy.set_tuning(knob, bad)
     finally:

@@ -146,3 +146,30 @@ def test_time_pair_measures_and_keeps_results(ctx):
     print(f"pair launch {ms:.4f} ms", flush=True)
     assert ms > 0
     assert np.array_equal(f1, f2) and ec1 == ec2 and elj1 == elj2
+
+
+def test_mixed_forces_do_not_depend_on_the_schedule(ctx):
+    """The 26,000-atom workload's mixed-form forces and energies are
+    bit-identical under every pair schedule, block count, list prune and PME
+    launch shape: the list's runs depend on the list only, each tile's FP32
+    i-force sums and every pair's j-force go to int64 fixed point, and
+    fixed-point sums do not depend on their order."""
+    s = systems.polymer_water(n_total=26000, box=64.0, n_beads=500, seed=1)
+    spec = pmx.ForceSpec(6.0, lj=(pmx.SW_C2, 10.0, 12.0),
+                         pme=dict(rc=12.0, grid=(64, 64, 64), order=4, alpha=-1.0, accuracy=1e-6))
+    sy = pmx.System(ctx, s, spec, pmx.MIXED)
+    try:
+        f0, ec0, elj0 = sy.evaluate(s.pos)
+        cases = [((pmx.TUNE_PAIR_SCHEDULE, 1),), ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 1),),
+                 ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 2),), ((pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),),
+                 ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 0), (pmx.TUNE_PAIR_SCHEDULE, 0)), ((pmx.TUNE_LIST_PRUNE, 0),),
+                 ((pmx.TUNE_LIST_PRUNE, 1), (pmx.TUNE_SPREAD_STREAM, 1), (pmx.TUNE_FFT_LINES, 4))]
+        for case in cases:
+            for k, v in case:
+                sy.set_tuning(k, v)
+            f, ec, elj = sy.evaluate(s.pos)
+            print(f"schedule {case}: max |df| {np.abs(f - f0).max():.3e} dEc {ec - ec0:.3e} dElj {elj - elj0:.3e}",
+                  flush=True)
+            assert np.array_equal(f, f0) and ec == ec0 and elj == elj0
+    finally:
+        sy.close()
