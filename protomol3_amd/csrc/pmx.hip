@@ -740,8 +740,8 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
     // wave its first tile, or its whole run, statically)
     static int per_cu = 0;
     if (!per_cu) {
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair<MIXED, C, LJ, CS, BIGLJ, false>, kPairBlock,
-                                                       0) !=
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair<MIXED, C, LJ, CS, BIGLJ, kPairPlain>,
+                                                       kPairBlock, 0) !=
               hipSuccess ||
           per_cu < 1)
         per_cu = 1;
@@ -758,23 +758,27 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
     PairOut out{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash),
                 P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow),
                 P<unsigned long long>(s->border_acc)};
-    // the list prune runs in the first pass after a build (not for tables
-    // beyond the LDS size, which are rare)
-    const bool prune = !BIGLJ && s->tune_prune && s->prune_pending;
+    // the list prune runs in the first pass after a build, at the build
+    // positions (a pruned cluster pair cannot gain a member before the next
+    // build only when judged there); not in a pass that counts pairs, nor
+    // for tables beyond the LDS size, which are rare
+    const bool prune = !BIGLJ && !stats && s->tune_prune && s->prune_pending;
     s->prune_pending = false;
     const R4 *ljt = MIXED ? reinterpret_cast<const R4 *>(s->ljt32.p) : reinterpret_cast<const R4 *>(s->ljtab.p);
-    if (prune)
-      hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS, false, true>), dim3(grid), dim3(kPairBlock), 0, st, s->pp,
-                         stats ? 1 : 0, s->tune_schedule, P<int4>(s->tiles), P<int>(s->ntiles), P<int>(s->wstart),
-                         P<int>(s->tilectr), P<int4>(s->entries), P<int4>(s->entries), s->cap,
-                         P<unsigned short>(s->masks), P<R4>(s->xs), P<float4>(s->sorg), P<int>(s->ts),
-                         P<int>(s->permp), P<double>(s->ljtab), ljt, facc, out);
+#define PMX_PAIR_ARGS                                                                                              \
+  s->pp, s->tune_schedule, P<int4>(s->tiles), P<int>(s->ntiles), P<int>(s->wstart), P<int>(s->tilectr),            \
+      P<int4>(s->entries), P<int4>(s->entries), s->cap, P<unsigned short>(s->masks), P<R4>(s->xs),                 \
+      P<float4>(s->sorg), P<int>(s->ts), P<int>(s->permp), P<double>(s->ljtab), ljt, facc, out
+    if (stats)
+      hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS, BIGLJ, kPairStats>), dim3(grid), dim3(kPairBlock), 0, st,
+                         PMX_PAIR_ARGS);
+    else if (prune)
+      hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS, false, kPairPrune>), dim3(grid), dim3(kPairBlock), 0, st,
+                         PMX_PAIR_ARGS);
     else
-      hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS, BIGLJ, false>), dim3(grid), dim3(kPairBlock), 0, st, s->pp,
-                         stats ? 1 : 0, s->tune_schedule, P<int4>(s->tiles), P<int>(s->ntiles), P<int>(s->wstart),
-                         P<int>(s->tilectr), P<int4>(s->entries), P<int4>(s->entries), s->cap,
-                         P<unsigned short>(s->masks), P<R4>(s->xs), P<float4>(s->sorg), P<int>(s->ts),
-                         P<int>(s->permp), P<double>(s->ljtab), ljt, facc, out);
+      hipLaunchKernelGGL((k_pair<MIXED, C, LJ, CS, BIGLJ, kPairPlain>), dim3(grid), dim3(kPairBlock), 0, st,
+                         PMX_PAIR_ARGS);
+#undef PMX_PAIR_ARGS
     return;
   }
   FixupArgs<MIXED> a{};

@@ -1259,14 +1259,16 @@ struct PairOut {
 
 // Queue entry of the compacted pair kernel: one in-cutoff pair awaiting its
 // force.  info packs its force slot (bits 0..9: entry of the group, i-cluster
-// and lane, kGroup * 8 * 64 slots) and MODIFIED (bit 10); the evaluating lane
-// recovers the atoms' charges and types from the slot.
+// and lane, kGroup * 8 * 64 slots), MODIFIED (bit 10), the i-atom of the
+// supercluster (bits 11..16) and the group's j-atom (bits 17..20); the
+// evaluating lane reads the atoms' charges and types from LDS.
 template <class R>
 struct PairQ {
   R r2;
   int info;
 };
 constexpr int kSlotBits = 10;
+constexpr int kPairPlain = 0, kPairPrune = 1, kPairStats = 2;  // k_pair MODE
 
 // Charge and LJ type (clamped to 0 for padding) of a j-atom of the group.
 template <class R>
@@ -1369,13 +1371,14 @@ __device__ inline void pair_delta(const R4 &xi, const R4 &xj, const EdgesR<R> &E
 // tile's FP32 pair-energy sums go to fixed point too.  So the mixed form's forces and energies do not depend
 // on the schedule, the grid or the device.
 //
-// PRUNE (the first pass after a list build): cluster pairs without a
-// candidate pair inside the list radius are dropped from the entries' masks
-// (through entries_w, which aliases entries; each entry is read and written
-// by the one wave that takes it).
-template <bool MIXED, int COUL, int LJSW, int CSW, bool BIGLJ, bool PRUNE>
+// MODE kPairPrune (the first pass after a list build): cluster pairs without
+// a candidate pair inside the list radius are dropped from the entries'
+// masks (through entries_w, which aliases entries; each entry is read and
+// written by the one wave that takes it).  MODE kPairStats: the pass also
+// counts the in-cutoff pairs and sums their hashes.
+template <bool MIXED, int COUL, int LJSW, int CSW, bool BIGLJ, int MODE>
 __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
-    PairParams pp, int stats, int dynamic, const int4 *__restrict__ tiles, const int *__restrict__ ntiles_p,
+    PairParams pp, int dynamic, const int4 *__restrict__ tiles, const int *__restrict__ ntiles_p,
     const int *__restrict__ wstart, int *__restrict__ tile_counter, const int4 *__restrict__ entries, int4 *entries_w,
     int cap,
     const unsigned short *__restrict__ masks,
@@ -1390,6 +1393,7 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
   using WA = WaveAcc<MIXED>;
   using AT = typename WA::T;
   constexpr int W = kPairBlock / kWave;
+  constexpr bool PRUNE = MODE == kPairPrune, STATS = MODE == kPairStats;
   __shared__ R4 s_i[W][kSuperSize];
   __shared__ int s_ty[W][kSuperSize];
   __shared__ R4 s_lj[kMaxTypes * kMaxTypes];  // {A, B, A14, B14}
@@ -1412,6 +1416,10 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int ti = lane & 7, tj = lane >> 3;
+  // queue info of this lane's pairs: force slot (bits 0..9, with the group
+  // entry and i-cluster added), MODIFIED (bit 10), i-atom c * 8 + ti (bits
+  // 11..16, c added), j-atom g * 8 + tj (bits 17..20, g added)
+  const int info_lane = lane | (ti << 11) | (tj << 17);
   EdgesR<R> E;
   E.load(pp);
   SwR<R> sl, sc;
@@ -1420,6 +1428,8 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
   const R guard = (R)pp.guard;
   const R alpha = (R)pp.alpha, a2pi = (R)pp.a2pi, csf = (R)pp.csf;
   const bool one_cut = sl.rc2 == sc.rc2;  // one guard band serves both cutoffs
+  // membership: r^2 within the larger cutoff of the kinds present
+  const R rmax2 = LJSW == SW_NONE ? sc.rc2 : (COUL == COUL_NONE ? sl.rc2 : (sl.rc2 > sc.rc2 ? sl.rc2 : sc.rc2));
   const R prune_r2 = (R)pp.prune_r2;
   // the boundary-pair sums are filled after this kernel (same stream)
   if (blockIdx.x == 0 && threadIdx.x < 4) out.border_acc[threadIdx.x] = 0ull;
@@ -1463,7 +1473,7 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
       si[lane] = xs[S * kSuperSize + lane];
       const int v = ts[S * kSuperSize + lane];
       ireal = v >= 0;
-      s_ty[wv][lane] = v < 0 ? 0 : v;
+      s_ty[wv][lane] = (v < 0 ? 0 : v) * nt;  // row of the LJ table
       __builtin_amdgcn_wave_barrier();
     }
 
@@ -1484,8 +1494,8 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
         const PairQ<R> e = q[(qhead + lane) & (kQueue - 1)];
         const int slot = e.info & ((1 << kSlotBits) - 1);
         const bool mod = (e.info >> kSlotBits) & 1;
-        const int ia = ((slot >> 6) & 7) * 8 + (slot & 7);  // c * 8 + ti
-        const int ja = ((slot >> 9) << 3) + ((slot >> 3) & 7);  // g * 8 + tj
+        const int ia = (e.info >> 11) & 63;  // c * 8 + ti
+        const int ja = (e.info >> 17) & 15;  // g * 8 + tj
         const JAtom<R> jd = s_j[wv][ja];
         R qq = si[ia].w * jd.q;
         if (mod) qq *= csf;
@@ -1493,7 +1503,7 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
         const bool ic = COUL != COUL_NONE && e.r2 <= sc.rc2;
         R A = 0, B = 0;
         if (LJSW != SW_NONE) {
-          const int li = s_ty[wv][ia] * nt + jd.ty;
+          const int li = s_ty[wv][ia] + jd.ty;
           const R4 lj = BIGLJ ? ljt[li] : s_lj[li];
           A = mod ? lj.z : lj.x;
           B = mod ? lj.w : lj.y;
@@ -1592,28 +1602,28 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
           R dx, dy, dz;
           pair_delta<R, R4>(xi, xj[g], E, shv[g] & kShiftPerPair, dx, dy, dz);
           const R r2 = dx * dx + dy * dy + dz * dz;
-          bool border = false;
-          if (LJSW != SW_NONE && COUL != COUL_NONE && one_cut) {
-            border = fabs(r2 - sl.rc2) <= guard;
+          // a pair within the guard band of a cutoff is left to the FP64
+          // boundary pass; a member is in range of a cutoff and outside
+          // every band
+          bool border = false, in;
+          if ((LJSW == SW_NONE || COUL == COUL_NONE || one_cut)) {
+            const R d = r2 - rmax2;
+            border = cand && fabs(d) <= guard;
+            in = cand && d < -guard;
           } else {
-            if (LJSW != SW_NONE) border |= fabs(r2 - sl.rc2) <= guard;
-            if (COUL != COUL_NONE) border |= fabs(r2 - sc.rc2) <= guard;
+            border = (fabs(r2 - sl.rc2) <= guard || fabs(r2 - sc.rc2) <= guard) && cand;
+            in = cand && !border && r2 <= rmax2;
           }
-          border = border && cand;
-          const bool ok = cand && !border;
-          const bool in_lj = LJSW != SW_NONE && ok && r2 <= sl.rc2;
-          const bool in_c = COUL != COUL_NONE && ok && r2 <= sc.rc2;
-          const bool in = in_lj || in_c;
           const unsigned long long bm = __builtin_amdgcn_ballot_w64(in);
           if (PRUNE && __builtin_amdgcn_ballot_w64(cand && r2 <= prune_r2)) kept |= 1u << (g * 8 + c);
-          if (border) {
+          if (__builtin_amdgcn_ballot_w64(border) && border) {
             const int b = atomicAdd(out.nborder, 1);
             if (b < out.maxborder)
               out.border[b] = make_int2(S * kSuperSize + c * 8 + ti, sj);
             else
               atomicExch(out.overflow, 1);
           }
-          if (stats && in) {
+          if (STATS && in) {
             const int ai = perm[S * kSuperSize + c * 8 + ti], aj = perm[sj];
             ++npair;
             hsum += pair_hash(ai < aj ? ai : aj, ai < aj ? aj : ai);
@@ -1625,7 +1635,7 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
                   __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
               PairQ<R> e;
               e.r2 = r2;
-              e.info = ((g * 8 + c) * kWave + lane) | ((int)modified << kSlotBits);
+              e.info = info_lane | (((g * 8 + c) << 6) | (c << 14) | (g << 20)) | ((int)modified << kSlotBits);
               q[(qtail + (int)below) & (kQueue - 1)] = e;
             }
             qtail += __popcll(bm);
@@ -1688,14 +1698,14 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
     const AT eljA = wave_sum(WA::of(elT));
     const AT ecoA = wave_sum(WA::of(ecT));
     long long np = 0;
-    if (stats) {
+    if (STATS) {
       np = wave_sum((long long)npair);
       hsum = wave_sum(hsum);
     }
     if (lane == 0) {
       out.energy[2 * t] = WA::energy_bits(ecoA);
       out.energy[2 * t + 1] = WA::energy_bits(eljA);
-      if (stats) {
+      if (STATS) {
         out.npairs[t] = np;
         out.hash[t] = hsum;
       }
