@@ -33,10 +33,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 69          # algorithmic FLOPs per in-cutoff pair (LJ C2 + PME real, counted once)
-# HBM bytes per k_pair launch (mixed form, skin 0.7 A): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
+# HBM bytes per k_pair launch (mixed form, skin 1.5 A): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
 # scales from a plain 256 MiB copy (profiles/k_pair_traffic_polymer26000.md)
-PAIR_TRAFFIC_BYTES = 43.98e6
-PAIR_TRAFFIC_SKIN = 0.7
+PAIR_TRAFFIC_BYTES = 44.35e6
+PAIR_TRAFFIC_SKIN = 1.5
 LAUNCH_KNOBS = {2: "fft_lines", 3: "pair_blocks_per_cu", 4: "pair_schedule", 6: "spread_on_second_stream",
                 7: "list_prune"}
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
