@@ -1428,6 +1428,7 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
   const R guard = (R)pp.guard;
   const R alpha = (R)pp.alpha, a2pi = (R)pp.a2pi, csf = (R)pp.csf;
   const bool one_cut = sl.rc2 == sc.rc2;  // one guard band serves both cutoffs
+  const bool same_cut = LJSW == SW_NONE || COUL == COUL_NONE || one_cut;
   // membership: r^2 within the larger cutoff of the kinds present
   const R rmax2 = LJSW == SW_NONE ? sc.rc2 : (COUL == COUL_NONE ? sl.rc2 : (sl.rc2 > sc.rc2 ? sl.rc2 : sc.rc2));
   const R prune_r2 = (R)pp.prune_r2;
@@ -1488,7 +1489,10 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
     // Evaluates queue pairs [qhead, qhead + cnt) with lane l on pair qhead + l.
     // A queued pair passed the membership test: it is in range of at least
     // one of the two cutoffs, and not within the guard band of either.
-    auto eval_batch = [&](int cnt, R &qel, R &qec) {
+    // `inside` is std::true_type when every queued pair is inside the cutoff
+    // of each kind present (one cutoff for both kinds), so no per-pair test
+    // or select is needed.
+    auto eval_batch = [&](auto inside, int cnt, R &qel, R &qec) {
       __builtin_amdgcn_wave_barrier();
       if (lane < cnt) {
         const PairQ<R> e = q[(qhead + lane) & (kQueue - 1)];
@@ -1499,8 +1503,8 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
         const JAtom<R> jd = s_j[wv][ja];
         R qq = si[ia].w * jd.q;
         if (mod) qq *= csf;
-        const bool il = LJSW != SW_NONE && e.r2 <= sl.rc2;
-        const bool ic = COUL != COUL_NONE && e.r2 <= sc.rc2;
+        const bool il = LJSW != SW_NONE && (decltype(inside)::value || e.r2 <= sl.rc2);
+        const bool ic = COUL != COUL_NONE && (decltype(inside)::value || e.r2 <= sc.rc2);
         R A = 0, B = 0;
         if (LJSW != SW_NONE) {
           const int li = s_ty[wv][ia] + jd.ty;
@@ -1606,7 +1610,7 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
           // boundary pass; a member is in range of a cutoff and outside
           // every band
           bool border = false, in;
-          if ((LJSW == SW_NONE || COUL == COUL_NONE || one_cut)) {
+          if (same_cut) {
             const R d = r2 - rmax2;
             border = cand && fabs(d) <= guard;
             in = cand && d < -guard;
@@ -1639,7 +1643,12 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
               q[(qtail + (int)below) & (kQueue - 1)] = e;
             }
             qtail += __popcll(bm);
-            while (qtail - qhead >= kWave) eval_batch(kWave, qel, qec);
+            while (qtail - qhead >= kWave) {
+              if (same_cut)
+                eval_batch(std::true_type{}, kWave, qel, qec);
+              else
+                eval_batch(std::false_type{}, kWave, qel, qec);
+            }
           }
         }
       }
@@ -1655,7 +1664,12 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
         }
       }
       // ---- phase B: the rest of the queue
-      if (qtail > qhead) eval_batch(qtail - qhead, qel, qec);
+      if (qtail > qhead) {
+        if (same_cut)
+          eval_batch(std::true_type{}, qtail - qhead, qel, qec);
+        else
+          eval_batch(std::false_type{}, qtail - qhead, qel, qec);
+      }
       elT += qel;
       ecT += qec;
       __builtin_amdgcn_wave_barrier();
