@@ -277,56 +277,65 @@ def time_pair(ctx, s, spec, x, skin, launch=None, reps: int = 200) -> float:
         sy.close()
 
 
-def serving(nreq: int, slots: int, seed: int = 7, log=print):
+def serving(nreq: int, slots: int, seed: int = 7, pool: int = 256, log=print):
     """Serving stream: requests of log-uniform size in [1,000, 100,000] atoms
-    (water + bead-polymer solute at 0.0334 atoms/A^3, per-request seeds),
+    (water + bead-polymer solute at 0.0334 atoms/A^3, per-system seeds),
     half single force evaluations and half 100-step Langevin trajectories
-    (300 K, 1/ps), mixed precision, LJ C2 7-9 A + PME ~1 A grid.  First all
-    requests at once (capacity), then Poisson arrivals at 75% of that
+    (300 K, 1/ps, per-request noise seeds), mixed precision, LJ C2 7-9 A +
+    PME ~1 A grid.  Up to `pool` distinct systems are built; request i uses
+    system i mod pool.  First the first min(nreq, pool) requests at once
+    (capacity), then all nreq requests as Poisson arrivals at 75% of that
     capacity; latency = submit to result, per request."""
     from protomol3_amd import pmx, serve, systems
     rng = np.random.default_rng(seed)
-    sizes = np.exp(rng.uniform(np.log(1000), np.log(100000), nreq)).astype(int)
+    npool = min(nreq, pool)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(100000), npool)).astype(int)
     kinds = rng.integers(0, 2, nreq)
-    reqs = []
-    for i, (n, kd) in enumerate(zip(sizes, kinds)):
+    built = []
+    for i, n in enumerate(sizes):
         s = systems.replica_box(int(n), 1000 + i).with_velocities(300.0, seed=5000 + i)
-        reqs.append((s, serve.request_spec(s), int(kd)))
-    natoms = int(sum(s.n for s, _, _ in reqs))
-    log(f"serving: {nreq} requests, {natoms} atoms in total, {slots} slots")
+        built.append((s, serve.request_spec(s)))
+    reqs = [(built[i % npool][0], built[i % npool][1], int(kinds[i]), i) for i in range(nreq)]
+    natoms = int(sum(s.n for s, _, _, _ in reqs))
+    log(f"serving: {nreq} requests over {npool} systems, {natoms} atoms in total, {slots} slots")
 
-    def submit(srv, s, spec, kd):
+    def submit(srv, s, spec, kd, i):
         if kd == 0:
             return srv.submit_forces(s, spec, pmx.MIXED)
-        return srv.submit_trajectory(s, spec, pmx.MIXED, 100, skin=1.0, gamma=1.0, seed=17)
+        return srv.submit_trajectory(s, spec, pmx.MIXED, 100, skin=1.0, gamma=1.0, seed=17 + i)
 
     srv = serve.Server(0, slots=slots)
     try:
         t0 = time.perf_counter()
-        ts = [submit(srv, *r) for r in reqs]
+        ts = [submit(srv, *r) for r in reqs[:npool]]
         for t in ts:
             srv.wait(t)
         makespan = time.perf_counter() - t0
-        rate = 0.75 * nreq / makespan
+        capacity = npool / makespan
+        rate = 0.75 * capacity
         gaps = rng.exponential(1.0 / rate, nreq)
         t0 = time.perf_counter()
         ts, due = [], t0
-        for r, g in zip(reqs, gaps):
+        for k, (r, g) in enumerate(zip(reqs, gaps)):
             due += g
             now = time.perf_counter()
             if due > now:
                 time.sleep(due - now)
             ts.append(submit(srv, *r))
+            if k and k % 1000 == 0:
+                log(f"serving: {k} requests submitted")
         lat = np.array([srv.wait(t).done_ms for t in ts])
         wall = time.perf_counter() - t0
+        peak_bytes, peak_running = srv.stats()
     finally:
         srv.close()
-    traj_steps = 100 * sum(1 for _, _, kd in reqs if kd == 1)
-    return {"requests": nreq, "slots": slots, "atoms_total": natoms,
-            "capacity_req_per_s": round(nreq / makespan, 3), "arrival_rate_req_per_s": round(rate, 3),
+    traj_steps = 100 * sum(1 for _, _, kd, _ in reqs if kd == 1)
+    return {"requests": nreq, "distinct_systems": npool, "slots": slots, "atoms_total": natoms,
+            "capacity_req_per_s": round(capacity, 3), "arrival_rate_req_per_s": round(rate, 3),
             "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
             "aggregate_ns_per_day": round(traj_steps * 1e-6 / wall * 86400.0, 3),
+            "wall_s": round(wall, 2), "peak_device_bytes": int(peak_bytes), "peak_running": int(peak_running),
             "workload": "log-uniform 1k-100k atoms, 50% force evaluations / 50% 100-step Langevin 1 fs, mixed, "
                         "Poisson arrivals at 75% of measured capacity"}
 
@@ -424,6 +433,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--serve-requests", type=int, default=64, help="requests in the serving stream (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving stream (--serve-requests requests) and print it")
     ap.add_argument("--replicas", type=int, default=256,
                     help="batched replicas (log-uniform 500-20,000 atoms) run after the main metric (0: skip)")
     ap.add_argument("--copies", type=int, default=8,
@@ -443,6 +454,10 @@ def main():
               "sharding the force path over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
     from protomol3_amd import pmx, tune
+    if a.serve_only:
+        log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
+        print(json.dumps({"serving": serving(a.serve_requests, a.serve_slots, log=log)}), flush=True)
+        return
     ctx = pmx.Context(0)
     if a.large:
         log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
