@@ -184,7 +184,12 @@ int pmx_sys_time_pair(pmx_sys *sys, int reps, float *ms_per_launch);
  * PMX_TUNE_LIST_PRUNE    1 (default): the first pair pass after each list
  *                        build drops, for the later passes, the cluster pairs
  *                        with no candidate pair inside the list radius;
- *                        0: keep them.
+ *                        0: keep them;
+ * PMX_TUNE_FIXUP_STREAM  0 (default): the exclusion correction and the bonded
+ *                        terms run on the second stream after the PME chain;
+ *                        1: on the main stream, in one launch with the
+ *                        boundary pairs after the pair kernel (better where
+ *                        the PME chain is the longer branch).
  * The resident state is kept.  No knob changes a result: the mixed form's
  * forces and energies are the same to the last bit under every setting (the
  * runs, and the FP32 partial sums within them, depend on the list only; a
@@ -196,7 +201,8 @@ enum {
   PMX_TUNE_PAIR_BLOCKS_PER_CU = 3,
   PMX_TUNE_PAIR_SCHEDULE = 4,
   PMX_TUNE_SPREAD_STREAM = 6,
-  PMX_TUNE_LIST_PRUNE = 7
+  PMX_TUNE_LIST_PRUNE = 7,
+  PMX_TUNE_FIXUP_STREAM = 8
 };
 int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value);
 

@@ -92,6 +92,8 @@ struct Sys {
   int tune_pair_blocks = 0;    // pair-kernel blocks per CU (0: as many as fit)
   int tune_schedule = 0;       // 0: runs round robin over the waves; 1: later runs from a counter
   int tune_spread_stream2 = 0; // 1: the PME spread runs on the second stream, beside the pair kernel
+  int tune_fixup_main = 0;     // 1: exclusion correction and bonded terms on the main stream after the pair kernel
+  int border_blocks = 4;       // blocks of the boundary-pair pass
   int tune_prune = 1;          // 1: the first pair pass after a list build prunes the list's cluster pairs
   bool prune_pending = false;  // a list was built and its first pair pass has not run
   int pair_grid = 0;           // blocks of the pair-kernel launch
@@ -580,6 +582,8 @@ int setup(Sys *s) {
       (rc = A(s->nborder, 8)) || (rc = A(s->border_acc, 32)))
     return rc;
   s->corr_blocks = std::max(1, (int)nblk((long long)s->etab.size(), 256));
+  // boundary pairs grow with the pair count: one block per 16 superclusters
+  s->border_blocks = std::min(128, std::max(4, (int)(s->nsuper / 16)));
   if ((rc = A(s->corr_e, 8 * (size_t)s->corr_blocks))) return rc;
   if (s->dry) {
     // PME buffers as allocated below (grid, half spectrum, real potential,
@@ -727,12 +731,14 @@ int setup(Sys *s) {
 }
 
 // Launches of the kernels templated on the force kinds: the pair kernel
-// (phase 0, main stream), the boundary pairs (phase 1, main stream) and the
-// exclusion correction with the bonded terms (phase 2, second stream); the
-// last two are one kernel with different block ranges.
+// (phase 0, main stream), the boundary pairs (phase 1, main stream), the
+// exclusion correction with the bonded terms (phase 2, second stream), or all
+// three fixups in one launch on the main stream (phase 3); the fixups are one
+// kernel with different block ranges.
 template <bool MIXED, int C, int LJ, int CS, bool BIGLJ = false>
 void launch_pair_phase(Sys *s, bool stats, int phase) {
   hipStream_t st = phase == 2 ? s->ctx->stream2 : s->ctx->stream;
+  const bool border_here = phase == 1 || phase == 3, rest_here = phase == 2 || phase == 3;
   using R4 = typename Real4<typename std::conditional<MIXED, float, double>::type>::type;
   auto *facc = P<typename AccT<MIXED>::type>(s->facc);
   if (phase < 0) {
@@ -782,7 +788,7 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
     return;
   }
   FixupArgs<MIXED> a{};
-  a.border_blocks = phase == 1 ? 4 : 0;
+  a.border_blocks = border_here ? s->border_blocks : 0;
   a.pp = s->pp;
   a.stats = stats ? 1 : 0;
   a.border = P<int2>(s->border);
@@ -799,7 +805,7 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
   a.border_e = P<unsigned long long>(s->border_acc);
   a.border_stat = P<unsigned long long>(s->border_acc) + 2;
   const bool corr = s->pme_on && (s->d.pme_terms & PMX_PME_CORR) && !s->etab.empty();
-  a.corr_blocks = corr && phase == 2 ? s->corr_blocks : 0;
+  a.corr_blocks = corr && rest_here ? s->corr_blocks : 0;
   a.ne = (int)s->etab.size();
   a.etab = P<int4>(s->etab_d);
   a.q = P<double>(s->q_d);
@@ -808,7 +814,7 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
   a.csf = s->d.csf;
   a.corr_e = P<double>(s->corr_e);
   int bonded_blocks = 0;
-  if (phase == 2 && s->md.on && s->md.nb + s->md.na > 0) {
+  if (rest_here && s->md.on && s->md.nb + s->md.na > 0) {
     bonded_blocks = (int)nblk(s->md.nb + s->md.na, 256);
     a.nbond = s->md.nb;
     a.nangle = s->md.na;
@@ -1096,17 +1102,23 @@ int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, un
     hipEventRecord(s->ev[5], st2);
     hipEventRecord(s->ev[6], st2);
   }
-  if (s->mixed)
-    run_pair_phase<true>(s, stats, 2);
-  else
-    run_pair_phase<false>(s, stats, 2);
+  // the exclusion correction and bonded terms on the second stream (after
+  // the PME chain), or with the boundary pairs on the main stream after the
+  // pair kernel (tuning knob)
+  const int fix_phase = s->tune_fixup_main ? 3 : 1;
+  if (!s->tune_fixup_main) {
+    if (s->mixed)
+      run_pair_phase<true>(s, stats, 2);
+    else
+      run_pair_phase<false>(s, stats, 2);
+  }
   if (timing) hipEventRecord(s->ev[7], st2);
   HIPCHK(hipEventRecord(s->join_ev, st2));
   // boundary pairs after the pair kernel, then join
   if (s->mixed)
-    run_pair_phase<true>(s, stats, 1);
+    run_pair_phase<true>(s, stats, fix_phase);
   else
-    run_pair_phase<false>(s, stats, 1);
+    run_pair_phase<false>(s, stats, fix_phase);
   HIPCHK(hipStreamWaitEvent(st, s->join_ev, 0));
   if (next && s->mixed)
     hipLaunchKernelGGL((k_unsort_kick_drift<true, float>), dim3(nblk(npad, 256)), dim3(256), 0, st, npad, *next,
@@ -1512,6 +1524,10 @@ int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value) {
     case PMX_TUNE_LIST_PRUNE:
       if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "list prune must be 0 or 1");
       s->tune_prune = value;
+      break;
+    case PMX_TUNE_FIXUP_STREAM:
+      if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "fixup stream must be 0 or 1");
+      s->tune_fixup_main = value;
       break;
     default:
       return fail(s->ctx, PMX_ERR_ARG, "unknown tuning knob");

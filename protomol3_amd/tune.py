@@ -28,6 +28,7 @@ from . import pmx
 LAUNCH_CANDIDATES = (
     (pmx.TUNE_PAIR_SCHEDULE, (0, 1)),
     (pmx.TUNE_LIST_PRUNE, (1, 0)),
+    (pmx.TUNE_FIXUP_STREAM, (0, 1)),
     (pmx.TUNE_PAIR_BLOCKS_PER_CU, (0, -1, -2)),
     (pmx.TUNE_FFT_LINES, (16, 8, 4)),
     (pmx.TUNE_SPREAD_STREAM, (0, 1)),

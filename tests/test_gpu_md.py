@@ -202,7 +202,7 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
         f0, ec0, elj0 = sy.evaluate(x)
         ftol = 0.0 if precision == pmx.MIXED else 1e-12
         cases = [((pmx.TUNE_PAIR_BLOCKS_PER_CU, 1),), ((pmx.TUNE_PAIR_SCHEDULE, 1),),
-                 ((pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),), ((pmx.TUNE_LIST_PRUNE, 0),),
+                 ((pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),), ((pmx.TUNE_LIST_PRUNE, 0),), ((pmx.TUNE_FIXUP_STREAM, 1),),
                  ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 0), (pmx.TUNE_PAIR_SCHEDULE, 0), (pmx.TUNE_LIST_PRUNE, 1))]
         for case in cases:
             for k, v in case:
@@ -215,7 +215,7 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
                 assert np.array_equal(f, f0) and ec == ec0 and elj == elj0
             assert abs(ec - ec0) <= 1e-9 * abs(ec0) and abs(elj - elj0) <= 1e-9 * max(1.0, abs(elj0))
         for knob, bad in ((pmx.TUNE_FFT_LINES, 0), (pmx.TUNE_PAIR_BLOCKS_PER_CU, -8), (pmx.TUNE_PAIR_SCHEDULE, 2),
-                          (pmx.TUNE_SPREAD_STREAM, 2), (pmx.TUNE_LIST_PRUNE, 2), (1, 16), (5, 1), (99, 1)):
+                          (pmx.TUNE_SPREAD_STREAM, 2), (pmx.TUNE_LIST_PRUNE, 2), (pmx.TUNE_FIXUP_STREAM, 2), (1, 16), (5, 1), (99, 1)):
             with pytest.raises(pmx.PmxError):
                 sy.set_tuning(knob, bad)
     finally:

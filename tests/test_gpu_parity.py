@@ -150,8 +150,8 @@ def test_time_pair_measures_and_keeps_results(ctx):
 
 def test_mixed_forces_do_not_depend_on_the_schedule(ctx):
     """The 26,000-atom workload's mixed-form forces and energies are
-    bit-identical under every pair schedule, block count, list prune and PME
-    launch shape: the list's runs depend on the list only, each tile's FP32
+    bit-identical under every pair schedule, block count, list prune, PME
+    launch shape and fixup stream: the list's runs depend on the list only, each tile's FP32
     i-force sums and every pair's j-force go to int64 fixed point, and
     fixed-point sums do not depend on their order."""
     s = systems.polymer_water(n_total=26000, box=64.0, n_beads=500, seed=1)
@@ -163,7 +163,8 @@ def test_mixed_forces_do_not_depend_on_the_schedule(ctx):
         cases = [((pmx.TUNE_PAIR_SCHEDULE, 1),), ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 1),),
                  ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 2),), ((pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),),
                  ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 0), (pmx.TUNE_PAIR_SCHEDULE, 0)), ((pmx.TUNE_LIST_PRUNE, 0),),
-                 ((pmx.TUNE_LIST_PRUNE, 1), (pmx.TUNE_SPREAD_STREAM, 1), (pmx.TUNE_FFT_LINES, 4))]
+                 ((pmx.TUNE_LIST_PRUNE, 1), (pmx.TUNE_SPREAD_STREAM, 1), (pmx.TUNE_FFT_LINES, 4)),
+                 ((pmx.TUNE_FIXUP_STREAM, 1),)]
         for case in cases:
             for k, v in case:
                 sy.set_tuning(k, v)
