@@ -603,8 +603,10 @@ int setup(Sys *s) {
   hipLaunchKernelGGL(k_fill_u32, dim3(nblk(s->nbins, 256)), dim3(256), 0, st, P<uint32_t>(s->hist), 0u, s->nbins);
   {
     RtLock lk(g_rt_mu);
-    if (!s->fork_ev) HIPCHK(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
-    if (!s->join_ev) HIPCHK(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
+    // the fork and join order the two streams of one device only (the host
+    // never waits on them), so they need no system-scope fence
+    if (!s->fork_ev) HIPCHK(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming | hipEventDisableSystemFence));
+    if (!s->join_ev) HIPCHK(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming | hipEventDisableSystemFence));
   }
   HIPCHK(hipMemcpyAsync(s->q_d.p, s->q.data(), 8 * N, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(s->type_d.p, s->type.data(), 4 * N, hipMemcpyHostToDevice, st));
