@@ -161,11 +161,11 @@ def test_gpu_md_graph_replay_matches_kernel_launches(ctx, precision):
 def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
     """The autotuner's launch shapes change how the work is cut, not what is
     computed.  A run whose shapes change between its parts follows the
-    unchanged run (the mixed form to the last bit: fixed-point accumulation).
-    Every schedule cuts the pair list at the same 8-entry segments, whose
-    FP32 partial sums go to fixed point before they are combined, so the
-    mixed form's forces and energies at one configuration are bit-identical
-    under every schedule, run count, tile length and block count."""
+    unchanged run (the mixed form to the last bit: the list's runs depend on
+    the list only and every FP32 partial sum is over a tile before it goes to
+    fixed point), so the mixed form's forces and energies at one
+    configuration are bit-identical under every schedule, block count, list
+    prune and fixup stream."""
     s = systems.water_box(seed=32).with_velocities(300.0, seed=9)
     spec = pmx.ForceSpec(5.0, lj=(pmx.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 30, 24), order=4, alpha=0.32))
     shapes = [{}, {pmx.TUNE_FFT_LINES: 1, pmx.TUNE_SPREAD_STREAM: 1},
@@ -215,7 +215,8 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
                 assert np.array_equal(f, f0) and ec == ec0 and elj == elj0
             assert abs(ec - ec0) <= 1e-9 * abs(ec0) and abs(elj - elj0) <= 1e-9 * max(1.0, abs(elj0))
         for knob, bad in ((pmx.TUNE_FFT_LINES, 0), (pmx.TUNE_PAIR_BLOCKS_PER_CU, -8), (pmx.TUNE_PAIR_SCHEDULE, 2),
-                          (pmx.TUNE_SPREAD_STREAM, 2), (pmx.TUNE_LIST_PRUNE, 2), (pmx.TUNE_FIXUP_STREAM, 2), (1, 16), (5, 1), (99, 1)):
+                          (pmx.TUNE_SPREAD_STREAM, 2), (pmx.TUNE_LIST_PRUNE, 2), (pmx.TUNE_FIXUP_STREAM, 2),
+                          (1, 16), (5, 1), (99, 1)):
             with pytest.raises(pmx.PmxError):
                 sy.set_tuning(knob, bad)
     finally:
