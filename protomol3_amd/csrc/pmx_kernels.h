@@ -580,7 +580,7 @@ __device__ inline int col_runs(int c, int k, int n, int r[2][2]) {
 // prefix count keeps the entries sorted by cj.  Diagonal entries and entries
 // with a partial cluster get a mask work slot (explicit pair masks).
 constexpr int kListBlock = 256;
-// Work estimate of a list entry for the balanced pair schedule, in units of
+// Work estimate of a list entry for cutting the list into runs, in units of
 // about 35 ns of one pair-kernel wave (fitted to per-wave times): kCpCost per
 // cluster pair (its 64 candidates: membership and force passes), kMemCost
 // times the estimated member fraction of the cluster pair (evaluation; the

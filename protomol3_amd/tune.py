@@ -8,16 +8,15 @@ short trajectory from the same state for every candidate:
   pairs.
 * tune_launch: the launch shapes of one system (pmx.TUNE_* knobs), one knob
   at a time with the others at their defaults: the pair-kernel schedule
-  (balanced runs of equal estimated work, or fixed-length tiles from a
-  counter), the balanced runs per wave (one static, the rest from a
-  counter), the pair-kernel blocks per CU (fewer leave room for the PME
-  chain on the second stream), the FFT lines per block (the PME grid
-  blocking) and the stream of the PME spread.
+  (the list's runs in contiguous shares per wave, or the first run static
+  and the rest from a counter), the list prune after each build, the stream
+  of the exclusion correction and bonded terms, the pair-kernel blocks per
+  CU (fewer leave room for the PME chain on the second stream), the FFT
+  lines per block (the PME grid blocking) and the stream of the PME spread.
 
-FFT lines change no sum; the schedule and, under the balanced schedule, the
-runs and blocks per CU regroup the mixed form's FP32 partial sums of the i-forces
-(one per tile), a change at FP32 rounding (about 1e-7 relative); the FP64
-form's atomics may add in another order.
+No knob changes a mixed-form result: the list's runs depend on the list
+only, every FP32 partial sum is over a tile, and the accumulation is in
+fixed point.  The FP64 form's atomics may add in another order.
 """
 from __future__ import annotations
 
