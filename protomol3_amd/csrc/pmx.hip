@@ -884,6 +884,12 @@ void launch_spread(Sys *s, hipStream_t st) {
 template <bool MIXED, int ORDER>
 void launch_gather(Sys *s, hipStream_t st) {
   using T = typename std::conditional<MIXED, float, double>::type;
+  if (ORDER >= 4) {
+    hipLaunchKernelGGL((k_gather_patch<MIXED, ORDER>), dim3(nblk(s->npad, kSpreadSlots)), dim3(256), 0, st, s->npad,
+                       s->pme, P<double>(s->xorig), P<int>(s->permp), P<double>(s->q_d), P<T>(s->greal),
+                       P<typename AccT<MIXED>::type>(s->facc));
+    return;
+  }
   const int waves = (int)nblk(s->npad, 64 / pme_tpa(ORDER));
   hipLaunchKernelGGL((k_gather<MIXED, ORDER>), dim3(nblk((long long)waves * 64, 256)), dim3(256), 0, st, s->npad,
                      waves,

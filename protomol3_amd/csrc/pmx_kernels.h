@@ -2378,6 +2378,147 @@ __global__ void __launch_bounds__(256) k_gather(int n, int waves, PmeParams p, c
   }
 }
 
+// Force interpolation through an LDS patch of the real potential: one block
+// takes kSpreadSlots consecutive sorted slots (a compact group of atoms, as
+// k_spread_patch); wave 0 finds their grid bases and the box of grid points
+// their stencils cover; the block loads that box once, then 8 lanes per atom
+// sum its P^2 stencil rows from LDS (row r on lane r mod 8), reduce across
+// the 8 lanes and add the force.  A group whose box exceeds the patch reads
+// the potential from the grid directly.  The grid is read about once per
+// block instead of once per atom's stencil row.
+template <bool MIXED, int P>
+__global__ void __launch_bounds__(256) k_gather_patch(int n, PmeParams p, const double *__restrict__ xorig,
+                                                      const int *__restrict__ perm, const double *__restrict__ q,
+                                                      const typename std::conditional<MIXED, float, double>::type
+                                                          *__restrict__ greal,
+                                                      typename AccT<MIXED>::type *__restrict__ facc) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  constexpr int PMAX = spread_patch_max<P>();
+  constexpr int LPA = 8;  // lanes per atom
+  static_assert(kSpreadSlots * LPA == 256, "one block of 256 threads per group");
+  __shared__ R s_patch[PMAX];
+  __shared__ int s_rb[kSpreadSlots][3];  // base relative to the patch origin (direct: the plain base)
+  __shared__ R s_w[kSpreadSlots][3];
+  __shared__ int s_org[3], s_ext[3], s_direct, s_any;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int s0 = blockIdx.x * kSpreadSlots;
+  const int K[3] = {p.nx, p.ny, p.nz};
+  if (wv == 0) {
+    const int t = lane;
+    const int sl = s0 + t;
+    const int a = (t < kSpreadSlots && sl < n) ? perm[sl] : -1;
+    int b[3] = {0, 0, 0};
+    double w[3] = {0, 0, 0};
+    if (a >= 0) {
+      grid_coord(xorig[3 * a], p.hxr, p.nx, p.offx, b[0], w[0]);
+      grid_coord(xorig[3 * a + 1], p.hyr, p.ny, p.offy, b[1], w[1]);
+      grid_coord(xorig[3 * a + 2], p.hzr, p.nz, p.offz, b[2], w[2]);
+    }
+    const unsigned long long ok = __ballot(a >= 0);
+    if (ok) {
+      const int first = __builtin_ctzll(ok);
+      int lo[3], hi[3], db[3], ref[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        ref[k] = __shfl(b[k], first);
+        int d = b[k] - ref[k];  // on the periodic grid, nearest to the reference
+        d -= K[k] * (int)floorf(((float)d + 0.5f * (float)K[k]) / (float)K[k]);
+        db[k] = d;
+        lo[k] = a >= 0 ? d : 0x3fffffff;
+        hi[k] = a >= 0 ? d : -0x3fffffff;
+      }
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = min(lo[k], __shfl_xor(lo[k], m));
+          hi[k] = max(hi[k], __shfl_xor(hi[k], m));
+        }
+      const int ex = hi[0] - lo[0] + P, ey = hi[1] - lo[1] + P, ez = hi[2] - lo[2] + P;
+      const bool direct = ex > K[0] || ey > K[1] || ez > K[2] || (long long)ex * ey * ez > PMAX;
+      if (t < kSpreadSlots) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          s_rb[t][k] = a < 0 ? -1 : (direct ? b[k] : db[k] - lo[k]);
+          s_w[t][k] = (R)w[k];
+        }
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s_org[k] = ref[k] + lo[k];
+        s_ext[0] = ex;
+        s_ext[1] = ey;
+        s_ext[2] = ez;
+        s_direct = direct ? 1 : 0;
+        s_any = 1;
+      }
+    } else if (lane == 0) {
+      s_any = 0;
+    }
+  }
+  __syncthreads();
+  if (!s_any) return;  // block-uniform: no atoms in this group
+  const bool direct = s_direct;
+  const int ex = s_ext[0], ey = s_ext[1], ez = s_ext[2];
+  if (!direct) {
+    const int vol = ex * ey * ez;
+    const int o0 = s_org[0], o1 = s_org[1], o2 = s_org[2];
+    for (int i = tid; i < vol; i += blockDim.x) {
+      const int x = i / (ey * ez), rem = i - x * (ey * ez), y = rem / ez, z = rem - y * ez;
+      s_patch[i] = greal[((size_t)wrap_grid(o0 + x, p.nx) * p.ny + wrap_grid(o1 + y, p.ny)) * p.nz +
+                         wrap_grid(o2 + z, p.nz)];
+    }
+  }
+  __syncthreads();
+  const int t = tid / LPA, r0 = tid % LPA;  // atom t of the group, lane r0 of its 8
+  R fx = 0, fy = 0, fz = 0;
+  const int rb0 = s_rb[t][0], rb1 = s_rb[t][1], rb2 = s_rb[t][2];
+  if (rb0 >= 0) {
+    R tx[P], ty[P], tz[P], dx[P], dy[P], dz[P];
+    bspline_w<P, R>(s_w[t][0], tx, dx);
+    bspline_w<P, R>(s_w[t][1], ty, dy);
+    bspline_w<P, R>(s_w[t][2], tz, dz);
+    for (int rr = r0; rr < P * P; rr += LPA) {
+      const int i = rr / P, j = rr - (rr / P) * P;
+      const R txi = pick<P>(tx, i), tyj = pick<P>(ty, j);
+      const R xij = pick<P>(dx, i) * tyj, yij = txi * pick<P>(dy, j), zij = txi * tyj;
+      if (!direct) {
+        const R *row = s_patch + ((rb0 + i) * ey + (rb1 + j)) * ez + rb2;
+#pragma unroll
+        for (int kk = 0; kk < P; kk++) {
+          const R term = row[kk];
+          fx -= term * xij * tz[kk];
+          fy -= term * yij * tz[kk];
+          fz -= term * zij * dz[kk];
+        }
+      } else {
+        const int i1 = wrap_grid(i + rb0, p.nx), j1 = wrap_grid(j + rb1, p.ny);
+        const size_t row = ((size_t)i1 * p.ny + j1) * p.nz;
+#pragma unroll
+        for (int kk = 0; kk < P; kk++) {
+          const R term = greal[row + wrap_grid(kk + rb2, p.nz)];
+          fx -= term * xij * tz[kk];
+          fy -= term * yij * tz[kk];
+          fz -= term * zij * dz[kk];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 1; m < LPA; m <<= 1) {
+    fx += __shfl_xor(fx, m);
+    fy += __shfl_xor(fy, m);
+    fz += __shfl_xor(fz, m);
+  }
+  const int sl = s0 + t;
+  if (r0 == 0 && rb0 >= 0) {
+    const R qa = (R)q[perm[sl]];
+    AccT<MIXED>::add(&facc[3 * sl], fx * (R)p.hxr * qa);
+    AccT<MIXED>::add(&facc[3 * sl + 1], fy * (R)p.hyr * qa);
+    AccT<MIXED>::add(&facc[3 * sl + 2], fz * (R)p.hzr * qa);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 5. Exclusion correction (NonbondedPMEwaldSystemForce.h:874-911), FP64, exact erf.
 // One block of 256 entries (block blk of the correction range).
