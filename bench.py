@@ -38,7 +38,7 @@ FLOP_PER_PAIR = 69          # algorithmic FLOPs per in-cutoff pair (LJ C2 + PME 
 PAIR_TRAFFIC_BYTES = 44.35e6
 PAIR_TRAFFIC_SKIN = 1.5
 LAUNCH_KNOBS = {2: "fft_lines", 3: "pair_blocks_per_cu", 4: "pair_schedule", 6: "spread_on_second_stream",
-                7: "list_prune", 8: "fixups_on_main_stream"}
+                7: "list_prune", 8: "fixups_on_main_stream", 9: "spread_fused_with_kick_drift"}
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
 TIMESTEP_FS = 1.0

@@ -189,7 +189,10 @@ int pmx_sys_time_pair(pmx_sys *sys, int reps, float *ms_per_launch);
  *                        terms run on the second stream after the PME chain;
  *                        1: on the main stream, in one launch with the
  *                        boundary pairs after the pair kernel (better where
- *                        the PME chain is the longer branch).
+ *                        the PME chain is the longer branch);
+ * PMX_TUNE_FUSED_SPREAD  1 (default): inside an MD chunk the PME charge spread
+ *                        of the next force pass runs in the launch that kicks
+ *                        and drifts the atoms; 0: in a launch of its own.
  * The resident state is kept.  No knob changes a result: the mixed form's
  * forces and energies are the same to the last bit under every setting (the
  * runs, and the FP32 partial sums within them, depend on the list only; a
@@ -202,7 +205,8 @@ enum {
   PMX_TUNE_PAIR_SCHEDULE = 4,
   PMX_TUNE_SPREAD_STREAM = 6,
   PMX_TUNE_LIST_PRUNE = 7,
-  PMX_TUNE_FIXUP_STREAM = 8
+  PMX_TUNE_FIXUP_STREAM = 8,
+  PMX_TUNE_FUSED_SPREAD = 9
 };
 int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value);
 

@@ -93,6 +93,7 @@ struct Sys {
   int tune_schedule = 0;       // 0: runs round robin over the waves; 1: later runs from a counter
   int tune_spread_stream2 = 0; // 1: the PME spread runs on the second stream, beside the pair kernel
   int tune_fixup_main = 0;     // 1: exclusion correction and bonded terms on the main stream after the pair kernel
+  int tune_fused_spread = 1;   // 1: inside an MD chunk the next pass's spread joins the kick-drift
   int border_blocks = 4;       // blocks of the boundary-pair pass
   int tune_prune = 1;          // 1: the first pair pass after a list build prunes the list's cluster pairs
   bool prune_pending = false;  // a list was built and its first pair pass has not run
@@ -121,6 +122,7 @@ struct Sys {
   bool cached = false;
   bool list_valid = false;             // entries, counts and work of a built list are in place
   bool acc_clean = false;              // force accumulators, PME grid and pass counters are zero
+  bool spread_ready = false;           // the grid holds the next pass's spread (fused into the last kick-drift)
   bool dry = false;                    // size only: allocations are counted, nothing runs
   unsigned long long dry_bytes = 0;
   double eout_h[4] = {0, 0, 0, 0};
@@ -897,6 +899,27 @@ void launch_gather(Sys *s, hipStream_t st) {
                      P<typename AccT<MIXED>::type>(s->facc));
 }
 
+// The fused unsort + kick + drift + next spread (orders 4 to 8).
+template <bool MIXED>
+void launch_unsort_kick_drift_spread(Sys *s, const Kick &kk, unsigned *dmax) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using R4 = typename Real4<R>::type;
+  hipStream_t st = s->ctx->stream;
+#define PMX_UKDS(ORD)                                                                                              \
+  hipLaunchKernelGGL((k_unsort_kick_drift_spread<MIXED, R, ORD>), dim3(nblk(s->npad, kSpreadSlots)), dim3(256), 0, \
+                     st, s->npad, kk, s->md.h, P<int>(s->permp), P<typename AccT<MIXED>::type>(s->facc),           \
+                     P<double>(s->fout), P<int>(s->tilectr), P<int>(s->nborder), P<double>(s->md.minv),             \
+                     P<double>(s->md.v), P<double>(s->xorig), P<double>(s->md.xbuild), P<R4>(s->xs),                \
+                     P<double4>(s->xs64), P<float4>(s->sorg), dmax, s->pme, P<double>(s->q_d),                      \
+                     P<double>(s->grid64), P<unsigned long long>(s->gridfx))
+  switch (s->pme.order) {
+    case 4: PMX_UKDS(4); break;
+    case 6: PMX_UKDS(6); break;
+    default: PMX_UKDS(8); break;
+  }
+#undef PMX_UKDS
+}
+
 // Charge spreading on the main stream (before the pair kernel: alone it
 // runs at full rate, next to the pair kernel its many small waves would
 // not); the charge grid was zeroed by the k_zero launch of the pass.
@@ -1048,7 +1071,8 @@ int rebuild(Sys *s, bool timing) {
 // next: the following MD step's kick (with next_dmax, its displacement
 // record): the pass ends with that step's kick and drift fused into the
 // force unsort.
-int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, unsigned *next_dmax = nullptr) {
+int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, unsigned *next_dmax = nullptr,
+           bool spread_done = false) {
   hipStream_t st = s->ctx->stream;
   const bool timing = flags & PMX_EVAL_TIMING;
   const bool stats = flags & PMX_EVAL_PAIRSTATS;
@@ -1070,7 +1094,11 @@ int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, un
   }
   // A completed pass leaves the accumulators, the PME grid and the counters
   // cleared (k_unsort_forces, k_fft_z_r2c, k_pair); only the first pass after
-  // (re)allocation or after an interrupted pass clears them here.
+  // (re)allocation or after an interrupted pass clears them here.  A spread
+  // fused into the previous kick-drift is this pass's only when the caller
+  // says so; otherwise it belongs to other positions and is cleared.
+  if (s->spread_ready && !spread_done) s->acc_clean = false;
+  s->spread_ready = false;
   if (!s->acc_clean)
     hipLaunchKernelGGL(k_zero, dim3(std::max(1u, nblk(3ll * npad, 256)), nr), dim3(256), 0, st, z);
   s->acc_clean = false;
@@ -1083,7 +1111,7 @@ int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, un
   hipStream_t st2 = s->ctx->stream2;
   const bool spread = s->pme_on && (s->d.pme_terms & PMX_PME_RECIP);
   const bool spread2 = spread && s->tune_spread_stream2;
-  if (spread && !spread2) {
+  if (spread && !spread2 && !spread_done) {
     if (s->mixed)
       run_spread<true>(s, st);
     else
@@ -1128,7 +1156,15 @@ int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, un
   else
     run_pair_phase<false>(s, stats, fix_phase);
   HIPCHK(hipStreamWaitEvent(st, s->join_ev, 0));
-  if (next && s->mixed)
+  // inside an MD chunk the next pass's spread joins the kick-drift
+  const bool fuse_spread = next && spread && !spread2 && s->pme.order >= 4 && s->tune_fused_spread;
+  if (fuse_spread) {
+    if (s->mixed)
+      launch_unsort_kick_drift_spread<true>(s, *next, next_dmax);
+    else
+      launch_unsort_kick_drift_spread<false>(s, *next, next_dmax);
+    s->spread_ready = true;
+  } else if (next && s->mixed)
     hipLaunchKernelGGL((k_unsort_kick_drift<true, float>), dim3(nblk(npad, 256)), dim3(256), 0, st, npad, *next,
                        s->md.h, P<int>(s->permp), P<unsigned long long>(s->facc), P<double>(s->fout),
                        P<int>(s->tilectr), P<int>(s->nborder), P<double>(s->md.minv), P<double>(s->md.v),
@@ -1537,6 +1573,10 @@ int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value) {
       if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "fixup stream must be 0 or 1");
       s->tune_fixup_main = value;
       break;
+    case PMX_TUNE_FUSED_SPREAD:
+      if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "fused spread must be 0 or 1");
+      s->tune_fused_spread = value;
+      break;
     default:
       return fail(s->ctx, PMX_ERR_ARG, "unknown tuning knob");
   }
@@ -1755,7 +1795,10 @@ int md_step(Sys *s, double kick, long long step, bool rebuild_here, int dslot,
     hipEventRecord(s->ev[1], st);
     hipEventRecord(s->ev[2], st);
   }
-  return forces(s, timing ? PMX_EVAL_TIMING : 0u, false, next, next ? P<unsigned>(m.dmax) + next_dslot : nullptr);
+  // the previous pass's kick-drift spread these positions when it was fused
+  const bool spread_done = kicked && s->spread_ready;
+  return forces(s, timing ? PMX_EVAL_TIMING : 0u, false, next, next ? P<unsigned>(m.dmax) + next_dslot : nullptr,
+                spread_done);
 }
 
 // A chunk of K steps from the committed state: its state is backed up in

@@ -1883,11 +1883,13 @@ __host__ __device__ constexpr int spread_patch_max() {
   return P <= 4 ? 2048 : 4096;
 }
 
+// The spread of one group (the block's kSpreadSlots slots from s0): lane t
+// of wave 0 brings atom a (-1: none) of slot s0 + t at position xa; the
+// other threads' a and xa are not read.
 template <bool MIXED, int P>
-__global__ void __launch_bounds__(256) k_spread_patch(int n, PmeParams p, const double *__restrict__ xorig,
-                                                      const int *__restrict__ perm, const double *__restrict__ q,
-                                                      double *__restrict__ grid64,
-                                                      unsigned long long *__restrict__ gridfx) {
+__device__ void spread_patch_group(int s0, const PmeParams &p, int a, const double xa[3],
+                                   const double *__restrict__ q, double *__restrict__ grid64,
+                                   unsigned long long *__restrict__ gridfx) {
   using R = typename std::conditional<MIXED, float, double>::type;
   using G = typename std::conditional<MIXED, unsigned long long, double>::type;
   constexpr int P3 = P * P * P;
@@ -1898,18 +1900,15 @@ __global__ void __launch_bounds__(256) k_spread_patch(int n, PmeParams p, const 
   __shared__ R s_q[kSpreadSlots];
   __shared__ int s_org[3], s_ext[3], s_direct, s_any;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int s0 = blockIdx.x * kSpreadSlots;
   const int K[3] = {p.nx, p.ny, p.nz};
   if (wv == 0) {
     const int t = lane;
-    const int s = s0 + t;
-    const int a = (t < kSpreadSlots && s < n) ? perm[s] : -1;
     int b[3] = {0, 0, 0};
     double w[3] = {0, 0, 0};
     if (a >= 0) {
-      grid_coord(xorig[3 * a], p.hxr, p.nx, p.offx, b[0], w[0]);
-      grid_coord(xorig[3 * a + 1], p.hyr, p.ny, p.offy, b[1], w[1]);
-      grid_coord(xorig[3 * a + 2], p.hzr, p.nz, p.offz, b[2], w[2]);
+      grid_coord(xa[0], p.hxr, p.nx, p.offx, b[0], w[0]);
+      grid_coord(xa[1], p.hyr, p.ny, p.offy, b[1], w[1]);
+      grid_coord(xa[2], p.hzr, p.nz, p.offz, b[2], w[2]);
     }
     const unsigned long long ok = __ballot(a >= 0);
     if (ok) {
@@ -2009,6 +2008,26 @@ __global__ void __launch_bounds__(256) k_spread_patch(int n, PmeParams p, const 
     else
       unsafeAtomicAdd(&grid64[g], (double)v);
   }
+}
+
+template <bool MIXED, int P>
+__global__ void __launch_bounds__(256) k_spread_patch(int n, PmeParams p, const double *__restrict__ xorig,
+                                                      const int *__restrict__ perm, const double *__restrict__ q,
+                                                      double *__restrict__ grid64,
+                                                      unsigned long long *__restrict__ gridfx) {
+  const int s0 = blockIdx.x * kSpreadSlots;
+  const int t = threadIdx.x, s = s0 + t;
+  int a = -1;
+  double xa[3] = {0, 0, 0};
+  if (t < kSpreadSlots && s < n) {
+    a = perm[s];
+    if (a >= 0) {
+      xa[0] = xorig[3 * a];
+      xa[1] = xorig[3 * a + 1];
+      xa[2] = xorig[3 * a + 2];
+    }
+  }
+  spread_patch_group<MIXED, P>(s0, p, a, xa, q, grid64, gridfx);
 }
 
 // Charge spreading (Grid::anterpolateCharge, Grid.h:107-154).  The P^3
@@ -2956,6 +2975,68 @@ __global__ void k_unsort_kick_drift(int n, Kick kk, double h, const int *__restr
   }
   for (int m = 32; m >= 1; m >>= 1) d2f = fmaxf(d2f, __shfl_xor(d2f, m));
   if ((threadIdx.x & 63) == 0) atomicMax(dmax, __float_as_uint(d2f));
+}
+
+// k_unsort_kick_drift and the charge spread of the next force pass in one
+// launch: block b takes the kSpreadSlots slots from 32 b; lanes 0..31 of wave
+// 0 unsort their slot's force, kick and drift its atom as k_unsort_kick_drift
+// does, and bring the new position to the group's spread (the grid was left
+// cleared by the previous pass).  Used inside an MD chunk, where the next
+// pass spreads these positions.
+template <bool MIXED, class R, int P>
+__global__ void __launch_bounds__(256) k_unsort_kick_drift_spread(
+    int n, Kick kk, double h, const int *__restrict__ perm, typename AccT<MIXED>::type *__restrict__ facc,
+    double *__restrict__ fout, int *__restrict__ tilectr, int *__restrict__ nborder, const double *__restrict__ minv,
+    double *__restrict__ v, double *__restrict__ x, const double *__restrict__ xbuild,
+    typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64, const float4 *__restrict__ sorg,
+    unsigned int *__restrict__ dmax, PmeParams pme, const double *__restrict__ q, double *__restrict__ grid64,
+    unsigned long long *__restrict__ gridfx) {
+  const int s0 = blockIdx.x * kSpreadSlots;
+  const int t = threadIdx.x, s = s0 + t;
+  if (blockIdx.x == 0 && t == 0) {
+    *tilectr = 0;
+    *nborder = 0;
+  }
+  int a = -1;
+  double xa[3] = {0, 0, 0};
+  if (t < kWave) {
+    float d2f = 0.f;
+    if (t < kSpreadSlots && s < n) a = perm[s];
+    if (a >= 0) {
+      double ff[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (MIXED)
+          ff[k] = (double)(long long)facc[3 * s + k] * (1.0 / kForceScale);
+        else
+          ff[k] = ((const double *)facc)[3 * s + k];
+        facc[3 * s + k] = 0;
+        fout[3 * a + k] = ff[k];
+      }
+      double dd[3], vv[3] = {v[3 * a], v[3 * a + 1], v[3 * a + 2]};
+      apply_kick(kk, a, ff, minv[a], vv);
+      double d2 = 0;
+      for (int k = 0; k < 3; ++k) {
+        v[3 * a + k] = vv[k];
+        const double xk = x[3 * a + k] + vv[k] * h;
+        x[3 * a + k] = xk;
+        xa[k] = xk;
+        dd[k] = xk - xbuild[3 * a + k];
+        d2 += dd[k] * dd[k];
+      }
+      d2f = (float)d2;
+      const double4 b = xs64[s];
+      const float4 o = sorg[s / kSuperSize];
+      typename Real4<R>::type r = xs[s];
+      r.x = (R)((b.x + dd[0]) - (double)o.x);
+      r.y = (R)((b.y + dd[1]) - (double)o.y);
+      r.z = (R)((b.z + dd[2]) - (double)o.z);
+      xs[s] = r;
+    }
+    for (int m = 32; m >= 1; m >>= 1) d2f = fmaxf(d2f, __shfl_xor(d2f, m));
+    if (t == 0) atomicMax(dmax, __float_as_uint(d2f));
+  }
+  spread_patch_group<MIXED, P>(s0, pme, a, xa, q, grid64, gridfx);
 }
 
 // Start of an MD chunk in one launch: the committed state (x, v, f: m3 =

@@ -57,7 +57,7 @@ class MdDesc(C.Structure):
 
 LEAPFROG, LANGEVIN = 0, 1
 TUNE_FFT_LINES, TUNE_PAIR_BLOCKS_PER_CU, TUNE_PAIR_SCHEDULE = 2, 3, 4
-TUNE_SPREAD_STREAM, TUNE_LIST_PRUNE, TUNE_FIXUP_STREAM = 6, 7, 8
+TUNE_SPREAD_STREAM, TUNE_LIST_PRUNE, TUNE_FIXUP_STREAM, TUNE_FUSED_SPREAD = 6, 7, 8, 9
 
 
 EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_create", "pmx_sys_destroy",
@@ -386,8 +386,8 @@ class System:
 
     def set_tuning(self, knob: int, value: int):
         """Sets a launch shape (TUNE_PAIR_SCHEDULE, TUNE_FFT_LINES,
-        TUNE_PAIR_BLOCKS_PER_CU, TUNE_SPREAD_STREAM, TUNE_LIST_PRUNE or
-        TUNE_FIXUP_STREAM);
+        TUNE_PAIR_BLOCKS_PER_CU, TUNE_SPREAD_STREAM, TUNE_LIST_PRUNE,
+        TUNE_FIXUP_STREAM or TUNE_FUSED_SPREAD);
         the resident state is kept."""
         self.ctx.check(self.ctx.lib.pmx_sys_set_tuning(self.h, knob, value), "pmx_sys_set_tuning")
 

@@ -10,7 +10,8 @@ short trajectory from the same state for every candidate:
   at a time with the others at their defaults: the pair-kernel schedule
   (the list's runs in contiguous shares per wave, or the first run static
   and the rest from a counter), the list prune after each build, the stream
-  of the exclusion correction and bonded terms, the pair-kernel blocks per
+  of the exclusion correction and bonded terms, the charge spread fused into
+  the kick-drift launch or apart, the pair-kernel blocks per
   CU (fewer leave room for the PME chain on the second stream), the FFT
   lines per block (the PME grid blocking) and the stream of the PME spread.
 
@@ -28,6 +29,7 @@ LAUNCH_CANDIDATES = (
     (pmx.TUNE_PAIR_SCHEDULE, (0, 1)),
     (pmx.TUNE_LIST_PRUNE, (1, 0)),
     (pmx.TUNE_FIXUP_STREAM, (0, 1)),
+    (pmx.TUNE_FUSED_SPREAD, (1, 0)),
     (pmx.TUNE_PAIR_BLOCKS_PER_CU, (0, -1, -2)),
     (pmx.TUNE_FFT_LINES, (16, 8, 4)),
     (pmx.TUNE_SPREAD_STREAM, (0, 1)),

@@ -169,8 +169,9 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
     s = systems.water_box(seed=32).with_velocities(300.0, seed=9)
     spec = pmx.ForceSpec(5.0, lj=(pmx.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 30, 24), order=4, alpha=0.32))
     shapes = [{}, {pmx.TUNE_FFT_LINES: 1, pmx.TUNE_SPREAD_STREAM: 1},
-              {pmx.TUNE_PAIR_BLOCKS_PER_CU: 1, pmx.TUNE_FFT_LINES: 64},
-              {pmx.TUNE_PAIR_BLOCKS_PER_CU: 3, pmx.TUNE_FFT_LINES: 5, pmx.TUNE_SPREAD_STREAM: 0}]
+              {pmx.TUNE_PAIR_BLOCKS_PER_CU: 1, pmx.TUNE_FFT_LINES: 64, pmx.TUNE_FUSED_SPREAD: 0},
+              {pmx.TUNE_PAIR_BLOCKS_PER_CU: 3, pmx.TUNE_FFT_LINES: 5, pmx.TUNE_SPREAD_STREAM: 0,
+               pmx.TUNE_FUSED_SPREAD: 1}]
     out = {}
     for tuned in (False, True):
         sy = pmx.System(ctx, s, spec, precision)
@@ -216,7 +217,7 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
             assert abs(ec - ec0) <= 1e-9 * abs(ec0) and abs(elj - elj0) <= 1e-9 * max(1.0, abs(elj0))
         for knob, bad in ((pmx.TUNE_FFT_LINES, 0), (pmx.TUNE_PAIR_BLOCKS_PER_CU, -8), (pmx.TUNE_PAIR_SCHEDULE, 2),
                           (pmx.TUNE_SPREAD_STREAM, 2), (pmx.TUNE_LIST_PRUNE, 2), (pmx.TUNE_FIXUP_STREAM, 2),
-                          (1, 16), (5, 1), (99, 1)):
+                          (pmx.TUNE_FUSED_SPREAD, 2), (1, 16), (5, 1), (99, 1)):
             with pytest.raises(pmx.PmxError):
                 sy.set_tuning(knob, bad)
     finally:
