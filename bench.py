@@ -38,7 +38,7 @@ FLOP_PER_PAIR = 69          # algorithmic FLOPs per in-cutoff pair (LJ C2 + PME 
 PAIR_TRAFFIC_BYTES = 44.35e6
 PAIR_TRAFFIC_SKIN = 1.5
 LAUNCH_KNOBS = {2: "fft_lines", 3: "pair_blocks_per_cu", 4: "pair_schedule", 6: "spread_on_second_stream",
-                7: "list_prune", 8: "fixups_on_main_stream", 9: "spread_fused_with_kick_drift"}
+                7: "list_prune", 8: "fixups_on_main_stream", 9: "spread_fused_with_kick_drift", 10: "fft_planes"}
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
 TIMESTEP_FS = 1.0
@@ -340,7 +340,7 @@ def serving(nreq: int, slots: int, seed: int = 7, pool: int = 256, log=print):
                         "Poisson arrivals at 75% of measured capacity"}
 
 
-def replicas(nsys: int, steps: int, lanes: int, seed: int = 11, log=print):
+def replicas(nsys: int, steps: int, lanes: int, seed: int = 11, skin: float = 1.0, graphs: bool = False, log=print):
     """Batched replicas: nsys independent systems with atom counts drawn
     log-uniform in [500, 20,000] (water + 5-10% bead-polymer solute at 0.0334
     atoms/A^3, per-system seeds), LJ C2 7-9 A + PME ~1 A grid, mixed
@@ -359,18 +359,24 @@ def replicas(nsys: int, steps: int, lanes: int, seed: int = 11, log=print):
         for i, b in enumerate(boxes):
             sy = pmx.System(ctxs[i % lanes], b, serve.request_spec(b), pmx.MIXED)
             sys_.append(sy)
-            sy.md_init(b, TIMESTEP_FS, skin=1.0, integrator=pmx.LANGEVIN, temperature=300.0, gamma=1.0, seed=i)
+            sy.md_init(b, TIMESTEP_FS, skin=skin, integrator=pmx.LANGEVIN, temperature=300.0, gamma=1.0, seed=i,
+                       graphs=graphs)
             sy.md_set_state(b.pos, b.vel)
         pmx.run_many(sys_, 5)  # warm-up (first list builds, kernel loads)
+        st0 = [sy.md_stats() for sy in sys_]
         t0 = time.perf_counter()
         pmx.run_many(sys_, steps)
         wall = time.perf_counter() - t0
+        st1 = [sy.md_stats() for sy in sys_]
+        rebuilds = sum(b["rebuilds"] - a["rebuilds"] for a, b in zip(st0, st1))
+        rollbacks = sum(b["rollbacks"] - a["rollbacks"] for a, b in zip(st0, st1))
     finally:
         for sy in sys_:
             sy.close()
         for c in ctxs:
             c.close()
     return {"systems": nsys, "atoms_total": natoms, "steps": steps, "lanes": lanes, "wall_s": round(wall, 3),
+            "skin_A": skin, "rebuilds": rebuilds, "rollbacks": rollbacks,
             "aggregate_ns_per_day": round(nsys * steps * TIMESTEP_FS * 1e-6 / wall * 86400.0, 3),
             "atom_steps_per_s": float(f"{natoms * steps / wall:.4e}"),
             "workload": "log-uniform 500-20,000 atoms, water + bead polymer at 0.0334 atoms/A^3, rc 9 A, "

@@ -185,14 +185,21 @@ int pmx_sys_time_pair(pmx_sys *sys, int reps, float *ms_per_launch);
  *                        build drops, for the later passes, the cluster pairs
  *                        with no candidate pair inside the list radius;
  *                        0: keep them;
- * PMX_TUNE_FIXUP_STREAM  0 (default): the exclusion correction and the bonded
- *                        terms run on the second stream after the PME chain;
- *                        1: on the main stream, in one launch with the
+ * PMX_TUNE_FIXUP_STREAM  0 (default from 100,000 atoms): the exclusion
+ *                        correction and the bonded terms run on the second
+ *                        stream after the PME chain; 1 (default below 100,000
+ *                        atoms): on the main stream, in one launch with the
  *                        boundary pairs after the pair kernel (better where
  *                        the PME chain is the longer branch);
  * PMX_TUNE_FUSED_SPREAD  1 (default): inside an MD chunk the PME charge spread
  *                        of the next force pass runs in the launch that kicks
  *                        and drifts the atoms; 0: in a launch of its own.
+ * PMX_TUNE_FFT_PLANES    1 (default): on PME grids whose x-y planes have at
+ *                        most 1,024 points, the y transform, the x transform
+ *                        with the convolution and the inverse y transform run
+ *                        as one launch with a block per plane; 0: three
+ *                        launches over lines (also the only form on larger
+ *                        planes).
  * The resident state is kept.  No knob changes a result: the mixed form's
  * forces and energies are the same to the last bit under every setting (the
  * runs, and the FP32 partial sums within them, depend on the list only; a
@@ -206,7 +213,8 @@ enum {
   PMX_TUNE_SPREAD_STREAM = 6,
   PMX_TUNE_LIST_PRUNE = 7,
   PMX_TUNE_FIXUP_STREAM = 8,
-  PMX_TUNE_FUSED_SPREAD = 9
+  PMX_TUNE_FUSED_SPREAD = 9,
+  PMX_TUNE_FFT_PLANES = 10
 };
 int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value);
 

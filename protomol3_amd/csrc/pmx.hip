@@ -93,7 +93,10 @@ struct Sys {
   int tune_schedule = 0;       // 0: runs round robin over the waves; 1: later runs from a counter
   int tune_spread_stream2 = 0; // 1: the PME spread runs on the second stream, beside the pair kernel
   int tune_fixup_main = 0;     // 1: exclusion correction and bonded terms on the main stream after the pair kernel
+                               // (default 1 below kFixupMainBelow atoms)
   int tune_fused_spread = 1;   // 1: inside an MD chunk the next pass's spread joins the kick-drift
+  int tune_fft_planes = 1;     // 1: y, x and convolution passes in one launch per plane on small planes
+  bool fft_plane = false;      // the plane launch is in use
   int border_blocks = 4;       // blocks of the boundary-pair pass
   int tune_prune = 1;          // 1: the first pair pass after a list build prunes the list's cluster pairs
   bool prune_pending = false;  // a list was built and its first pair pass has not run
@@ -150,6 +153,7 @@ struct Sys {
   } md;
 };
 constexpr int kMaxChunk = 128;
+constexpr int kFixupMainBelow = 100000;  // atoms; default of PMX_TUNE_FIXUP_STREAM
 constexpr double kInvTimeFactor = 0.02045482828087295;  // protomol/base/PMConstants.cpp:54
 constexpr double kBoltzmann = 0.001987191;              // protomol/base/PMConstants.cpp:70
 
@@ -367,7 +371,8 @@ int fft_blocking(Sys *s) {
   s->zl = std::max(1, std::min(lines, budget / pm.nz));
   s->tky = std::max(1, std::min(lines, budget / pm.ny));
   s->tkx = std::max(1, std::min(lines, budget / pm.nx));
-  s->recip_blocks = pm.nzh * ((pm.ny + s->tkx - 1) / s->tkx);
+  s->fft_plane = s->tune_fft_planes && pm.nx * pm.ny <= kFftPlanePoints;
+  s->recip_blocks = s->fft_plane ? pm.nzh : pm.nzh * ((pm.ny + s->tkx - 1) / s->tkx);
   return dalloc(s, s->recip_e, 8 * (size_t)s->recip_blocks);
 }
 
@@ -947,12 +952,19 @@ int run_pme(Sys *s, bool timing) {
     const int yb = pm.nzh * ((pm.nx + s->tky - 1) / s->tky);
     hipLaunchKernelGGL((k_fft_z_r2c<T, MIXED>), dim3(zb), dim3(256), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
                        P<double>(s->grid64), P<unsigned long long>(s->gridfx), P<Cx<T>>(s->twz), P<Cx<T>>(s->spec));
-    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(256), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, false,
-                       P<Cx<T>>(s->twy), P<Cx<T>>(s->spec), P<Cx<T>>(s->spec2));
-    hipLaunchKernelGGL((k_fft_x_conv<T>), dim3(s->recip_blocks), dim3(256), 2 * s->tkx * pm.nx * cx, st, pm, s->plx,
-                       s->tkx, P<Cx<T>>(s->twx), P<T>(s->theta), P<Cx<T>>(s->spec2), P<double>(s->recip_e));
-    hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(256), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, true,
-                       P<Cx<T>>(s->twy), P<Cx<T>>(s->spec2), P<Cx<T>>(s->spec));
+    if (s->fft_plane) {
+      hipLaunchKernelGGL((k_fft_plane_conv<T>), dim3(pm.nzh), dim3(256), 2 * (size_t)pm.nx * pm.ny * cx, st, pm,
+                         s->ply, s->plx, P<Cx<T>>(s->twy), P<Cx<T>>(s->twx), P<T>(s->theta), P<Cx<T>>(s->spec),
+                         P<double>(s->recip_e));
+    } else {
+      hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(256), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, false,
+                         P<Cx<T>>(s->twy), P<Cx<T>>(s->spec), P<Cx<T>>(s->spec2));
+      hipLaunchKernelGGL((k_fft_x_conv<T>), dim3(s->recip_blocks), dim3(256), 2 * s->tkx * pm.nx * cx, st, pm,
+                         s->plx, s->tkx, P<Cx<T>>(s->twx), P<T>(s->theta), P<Cx<T>>(s->spec2),
+                         P<double>(s->recip_e));
+      hipLaunchKernelGGL((k_fft_y<T>), dim3(yb), dim3(256), 2 * s->tky * pm.ny * cx, st, pm, s->ply, s->tky, true,
+                         P<Cx<T>>(s->twy), P<Cx<T>>(s->spec2), P<Cx<T>>(s->spec));
+    }
     hipLaunchKernelGGL((k_fft_z_c2r<T>), dim3(zb), dim3(256), 2 * s->zl * pm.nz * cx, st, pm, s->plz, s->zl,
                        P<Cx<T>>(s->twz), P<Cx<T>>(s->spec), P<T>(s->greal));
     if (timing) hipEventRecord(s->ev[5], st);
@@ -1322,6 +1334,9 @@ int pmx_sys_create(pmx_ctx *ctx, const pmx_sys_desc *desc, pmx_sys **out) {
   pmx_sys *ps = new pmx_sys();
   ps->s.ctx = ctx;
   ps->s.d = *desc;
+  // below this size the PME chain (a fixed number of short dependent
+  // launches) is the longer branch of a force pass
+  ps->s.tune_fixup_main = desc->n < kFixupMainBelow ? 1 : 0;
   rc = setup(&ps->s);
   if (rc) {
     pmx_sys_destroy(ps);
@@ -1576,6 +1591,10 @@ int pmx_sys_set_tuning(pmx_sys *sys, int knob, int value) {
     case PMX_TUNE_FUSED_SPREAD:
       if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "fused spread must be 0 or 1");
       s->tune_fused_spread = value;
+      break;
+    case PMX_TUNE_FFT_PLANES:
+      if (value < 0 || value > 1) return fail(s->ctx, PMX_ERR_ARG, "FFT planes must be 0 or 1");
+      s->tune_fft_planes = value;
       break;
     default:
       return fail(s->ctx, PMX_ERR_ARG, "unknown tuning knob");
@@ -1907,6 +1926,20 @@ int restore_chunk(Sys *s, int slot) {
   return PMX_OK;
 }
 
+// Length of the next chunk from the largest squared displacement d2 that
+// the last chunk reached at step k after its list build.  Displacement is
+// taken to grow linearly with time (the fastest atoms move ballistically
+// over a few steps; diffusive motion grows slower, so the estimate errs
+// short): the list stays valid up to step L = k sqrt(lim2 / d2).  The chunk
+// ends at 80 % of L, since a chunk that outlives its list is rolled back and
+// the chunk queued behind it discarded, and grows at most 2x per chunk.
+int next_chunk(int k, float d2, float lim2, int cur) {
+  const int grow = std::min(kMaxChunk, 2 * std::max(1, cur));
+  if (!(d2 > 0.f)) return grow;
+  const double L = k * std::sqrt((double)lim2 / (double)d2);
+  return std::max(1, std::min(grow, (int)std::floor(0.8 * L) + 1));
+}
+
 // Progress of one pmx_md_run on one system (see md_advance).
 struct MdRun {
   int nsteps = 0;
@@ -1994,7 +2027,7 @@ int md_advance(Sys *s, MdRun &r, bool block, bool *progressed) {
       r.done += c.K;
       m.steps += c.K;
       m.rebuilds += m.skin > 0 ? 1 : c.K;
-      if (m.skin > 0 && c.K == m.chunk && worst < 0.25f * lim2) m.chunk = std::min(kMaxChunk, 2 * m.chunk);
+      if (m.skin > 0 && c.K > 1) m.chunk = next_chunk(c.K - 1, worst, lim2, m.chunk);
     } else {
       // roll back to the chunk's start and replay the steps that used a
       // valid list (deterministic: the replay reproduces them exactly)
@@ -2009,7 +2042,7 @@ int md_advance(Sys *s, MdRun &r, bool block, bool *progressed) {
       m.steps += bad;
       m.rebuilds += 1;
       r.planned = r.done;
-      m.chunk = std::max(1, bad);
+      m.chunk = next_chunk(bad, worst, lim2, m.chunk);
     }
   }
   if (r.done == r.nsteps && r.inflight.empty()) {

@@ -2313,6 +2313,82 @@ __global__ void k_fft_x_conv(PmeParams p, FftPlan pl, int TJ, const Cx<T> *__res
   }
 }
 
+// Largest plane (nx ny points) that k_fft_plane_conv takes: two LDS copies
+// of 1,024 points take 16 KiB (mixed) or 32 KiB (FP64).  On larger planes
+// the three line launches, with many more blocks, are faster (measured on
+// 45 x 48 and 60 x 60 planes).
+constexpr int kFftPlanePoints = 1024;
+
+// The y pass, the x pass with the convolution and the inverse y pass of one
+// plane k of the half spectrum in one block (the work of k_fft_y, k_fft_x_conv
+// and k_fft_y in one launch, for small planes): SA plane k
+// is read as nx lines along y, transformed, transposed in LDS to ny lines
+// along x, transformed, multiplied by the influence function (energy summed
+// as in k_fft_x_conv), transformed back along x and, transposed, along y, and
+// written back to SA.  The conj-out of the inverse x transform and the
+// conj-in of the inverse y transform cancel, so only the last conj remains.
+template <class T>
+__global__ void __launch_bounds__(256) k_fft_plane_conv(PmeParams p, FftPlan ply, FftPlan plx,
+                                                        const Cx<T> *__restrict__ twy, const Cx<T> *__restrict__ twx,
+                                                        const T *__restrict__ theta, Cx<T> *__restrict__ sa,
+                                                        double *__restrict__ epart) {
+  extern __shared__ unsigned char smem_raw[];
+  const int nx = p.nx, ny = p.ny, nxy = nx * ny;
+  Cx<T> *const a = reinterpret_cast<Cx<T> *>(smem_raw);
+  Cx<T> *const b = a + nxy;
+  __shared__ double s_e[256 / kWave];
+  const int k = blockIdx.x;
+  for (int t = threadIdx.x; t < nxy; t += blockDim.x) {
+    const int i = t / ny, j = t - i * ny;
+    a[t] = sa[((size_t)i * p.nzh + k) * ny + j];
+  }
+  __syncthreads();
+  // forward y: lines i, layout [i][j]
+  Cx<T> *r = lds_fft<T>(a, b, nx, ply, twy);
+  Cx<T> *o = (r == a) ? b : a;
+  for (int t = threadIdx.x; t < nxy; t += blockDim.x) {
+    const int j = t / nx, i = t - j * nx;
+    o[t] = r[i * ny + j];
+  }
+  __syncthreads();
+  // forward x: lines j, layout [j][i] (the SB order of theta)
+  r = lds_fft<T>(o, r, ny, plx, twx);
+  const double wgt = (k > 0 && ((k != p.nz / 2) || (p.nz & 1))) ? 2.0 : 1.0;
+  const T *th_k = theta + (size_t)k * nxy;
+  double e = 0.0;
+  for (int t = threadIdx.x; t < nxy; t += blockDim.x) {
+    const T th = th_k[t];
+    const Cx<T> v = r[t];
+    e += (double)th * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+    r[t] = Cx<T>{v.x * th, -(v.y * th)};  // conj-in of the inverse x transform
+  }
+  __syncthreads();
+  o = (r == a) ? b : a;
+  r = lds_fft<T>(r, o, ny, plx, twx);
+  o = (r == a) ? b : a;
+  for (int t = threadIdx.x; t < nxy; t += blockDim.x) {
+    const int i = t / ny, j = t - i * ny;
+    o[t] = r[j * nx + i];
+  }
+  __syncthreads();
+  // inverse y (forward transform of the conjugate), conj-out into SA
+  r = lds_fft<T>(o, r, nx, ply, twy);
+  for (int t = threadIdx.x; t < nxy; t += blockDim.x) {
+    const int i = t / ny, j = t - i * ny;
+    Cx<T> v = r[t];
+    v.y = -v.y;
+    sa[((size_t)i * p.nzh + k) * ny + j] = v;
+  }
+  e = wave_sum(wgt * e);
+  if ((threadIdx.x & 63) == 0) s_e[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) tot += s_e[w];
+    epart[k] = tot;
+  }
+}
+
 // z pass, half complex -> real for the TY lines (i, j0 .. j0 + TY - 1) of SA
 // (Hermitian completion along z, inverse sign); out: the real potential.
 template <class T>

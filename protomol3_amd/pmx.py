@@ -58,6 +58,7 @@ class MdDesc(C.Structure):
 LEAPFROG, LANGEVIN = 0, 1
 TUNE_FFT_LINES, TUNE_PAIR_BLOCKS_PER_CU, TUNE_PAIR_SCHEDULE = 2, 3, 4
 TUNE_SPREAD_STREAM, TUNE_LIST_PRUNE, TUNE_FIXUP_STREAM, TUNE_FUSED_SPREAD = 6, 7, 8, 9
+TUNE_FFT_PLANES = 10
 
 
 EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_create", "pmx_sys_destroy",

@@ -13,7 +13,8 @@ short trajectory from the same state for every candidate:
   of the exclusion correction and bonded terms, the charge spread fused into
   the kick-drift launch or apart, the pair-kernel blocks per
   CU (fewer leave room for the PME chain on the second stream), the FFT
-  lines per block (the PME grid blocking) and the stream of the PME spread.
+  lines per block and whole-plane FFT launches (the PME grid blocking) and
+  the stream of the PME spread.
 
 No knob changes a mixed-form result: the list's runs depend on the list
 only, every FP32 partial sum is over a tile, and the accumulation is in
@@ -28,10 +29,11 @@ from . import pmx
 LAUNCH_CANDIDATES = (
     (pmx.TUNE_PAIR_SCHEDULE, (0, 1)),
     (pmx.TUNE_LIST_PRUNE, (1, 0)),
-    (pmx.TUNE_FIXUP_STREAM, (0, 1)),
+    (pmx.TUNE_FIXUP_STREAM, (1, 0)),
     (pmx.TUNE_FUSED_SPREAD, (1, 0)),
     (pmx.TUNE_PAIR_BLOCKS_PER_CU, (0, -1, -2)),
     (pmx.TUNE_FFT_LINES, (16, 8, 4)),
+    (pmx.TUNE_FFT_PLANES, (1, 0)),
     (pmx.TUNE_SPREAD_STREAM, (0, 1)),
 )
 
@@ -65,7 +67,8 @@ def tune_launch(sy, x, v, steps: int = 400, candidates=LAUNCH_CANDIDATES, rounds
     """Tunes the launch shapes of a System whose MD state is initialised
     (md_init done).  Leaves the best shapes set on it and returns
     ({knob: value}, {knob: {value: ms per step}}).  The first candidate of
-    each knob is its default; each time is the best of `rounds` runs, and a
+    each knob is its default (below 100,000 atoms for the fixup stream);
+    each time is the best of `rounds` runs, and a
     knob leaves its default only for a candidate faster by more than
     `margin` (run-to-run noise of a short timed run is about that)."""
     best = {k: vals[0] for k, vals in candidates}

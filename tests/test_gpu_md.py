@@ -165,13 +165,14 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
     the list only and every FP32 partial sum is over a tile before it goes to
     fixed point), so the mixed form's forces and energies at one
     configuration are bit-identical under every schedule, block count, list
-    prune and fixup stream."""
+    prune, fixup stream and FFT blocking (lines or whole planes)."""
     s = systems.water_box(seed=32).with_velocities(300.0, seed=9)
     spec = pmx.ForceSpec(5.0, lj=(pmx.SW_C2, 8.0, 10.0), pme=dict(rc=10.0, grid=(32, 30, 24), order=4, alpha=0.32))
     shapes = [{}, {pmx.TUNE_FFT_LINES: 1, pmx.TUNE_SPREAD_STREAM: 1},
-              {pmx.TUNE_PAIR_BLOCKS_PER_CU: 1, pmx.TUNE_FFT_LINES: 64, pmx.TUNE_FUSED_SPREAD: 0},
+              {pmx.TUNE_PAIR_BLOCKS_PER_CU: 1, pmx.TUNE_FFT_LINES: 64, pmx.TUNE_FUSED_SPREAD: 0,
+               pmx.TUNE_FFT_PLANES: 0},
               {pmx.TUNE_PAIR_BLOCKS_PER_CU: 3, pmx.TUNE_FFT_LINES: 5, pmx.TUNE_SPREAD_STREAM: 0,
-               pmx.TUNE_FUSED_SPREAD: 1}]
+               pmx.TUNE_FUSED_SPREAD: 1, pmx.TUNE_FFT_PLANES: 1}]
     out = {}
     for tuned in (False, True):
         sy = pmx.System(ctx, s, spec, precision)
@@ -204,6 +205,7 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
         ftol = 0.0 if precision == pmx.MIXED else 1e-12
         cases = [((pmx.TUNE_PAIR_BLOCKS_PER_CU, 1),), ((pmx.TUNE_PAIR_SCHEDULE, 1),),
                  ((pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),), ((pmx.TUNE_LIST_PRUNE, 0),), ((pmx.TUNE_FIXUP_STREAM, 1),),
+                 ((pmx.TUNE_FFT_PLANES, 0),), ((pmx.TUNE_FFT_PLANES, 1), (pmx.TUNE_FFT_LINES, 3)),
                  ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 0), (pmx.TUNE_PAIR_SCHEDULE, 0), (pmx.TUNE_LIST_PRUNE, 1))]
         for case in cases:
             for k, v in case:
@@ -217,7 +219,7 @@ def test_gpu_md_launch_shapes_keep_the_trajectory(ctx, precision):
             assert abs(ec - ec0) <= 1e-9 * abs(ec0) and abs(elj - elj0) <= 1e-9 * max(1.0, abs(elj0))
         for knob, bad in ((pmx.TUNE_FFT_LINES, 0), (pmx.TUNE_PAIR_BLOCKS_PER_CU, -8), (pmx.TUNE_PAIR_SCHEDULE, 2),
                           (pmx.TUNE_SPREAD_STREAM, 2), (pmx.TUNE_LIST_PRUNE, 2), (pmx.TUNE_FIXUP_STREAM, 2),
-                          (pmx.TUNE_FUSED_SPREAD, 2), (1, 16), (5, 1), (99, 1)):
+                          (pmx.TUNE_FUSED_SPREAD, 2), (pmx.TUNE_FFT_PLANES, 2), (1, 16), (5, 1), (99, 1)):
             with pytest.raises(pmx.PmxError):
                 sy.set_tuning(knob, bad)
     finally:

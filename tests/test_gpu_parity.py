@@ -164,7 +164,7 @@ def test_mixed_forces_do_not_depend_on_the_schedule(ctx):
                  ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 2),), ((pmx.TUNE_PAIR_BLOCKS_PER_CU, -1),),
                  ((pmx.TUNE_PAIR_BLOCKS_PER_CU, 0), (pmx.TUNE_PAIR_SCHEDULE, 0)), ((pmx.TUNE_LIST_PRUNE, 0),),
                  ((pmx.TUNE_LIST_PRUNE, 1), (pmx.TUNE_SPREAD_STREAM, 1), (pmx.TUNE_FFT_LINES, 4)),
-                 ((pmx.TUNE_FIXUP_STREAM, 1),)]
+                 ((pmx.TUNE_FIXUP_STREAM, 1),), ((pmx.TUNE_FFT_PLANES, 0),)]
         for case in cases:
             for k, v in case:
                 sy.set_tuning(k, v)
