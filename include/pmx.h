@@ -276,6 +276,22 @@ int pmx_md_run(pmx_sys *sys, int nsteps);
  * many independent replicas of uneven size (Hybrid Monte Carlo proposals,
  * replica ensembles).  Returns the first error of any system. */
 int pmx_md_run_many(pmx_sys *const *systems, int nsys, int nsteps);
+/* Batched replicas in batched launches: pmx_md_run(systems[i], nsteps) for
+ * every i, with every kernel of a step launched once for all the systems (a
+ * table in device memory maps each block to its system and that system's
+ * arguments).  The systems share one context, the precision, the kinds of
+ * force and a PME stencil order of 4, 6 or 8, and each has at most 65,536
+ * cell bins.  The list of a system is rebuilt on the device at the first step
+ * at which an atom has moved more than half the skin since the last build
+ * (no chunks rolled back); the host checks the lists' capacities every 64
+ * steps and runs such a stretch again after growing a list that overflowed.
+ * A system's trajectory does not depend on the batch it runs in (the mixed
+ * form to the last bit); it may differ from pmx_md_run's in the mixed form's
+ * last bits where pmx_md_run rebuilds the list at other steps (its chunk
+ * starts).  No counterpart in the reference, which runs one system per
+ * process (ProtoMolApp.cpp); it serves many independent small replicas, whose
+ * steps alone are bound by kernel launch latency. */
+int pmx_md_run_batch(pmx_sys *const *systems, int nsys, int nsteps);
 /* Current positions, velocities and energies (energies[PMX_MD_E_COUNT]:
  * potential terms at x, kinetic energy of v; right after pmx_md_set_state
  * the forces are computed first).  Any pointer may be NULL. */

@@ -7,7 +7,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib", "libpmx.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("pmx.hip", "pmx_kernels.h", "pmx_common.h")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("pmx.hip", "pmx_kernels.h", "pmx_batch.h", "pmx_common.h")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -19,6 +19,7 @@
 
 #include "../../include/pmx.h"
 #include "pmx_kernels.h"
+#include "pmx_batch.h"
 
 using namespace pmx;
 
@@ -744,6 +745,9 @@ int setup(Sys *s) {
 // exclusion correction with the bonded terms (phase 2, second stream), or all
 // three fixups in one launch on the main stream (phase 3); the fixups are one
 // kernel with different block ranges.
+template <bool MIXED>
+FixupArgs<MIXED> fixup_args(Sys *s, bool stats, bool border_here, bool rest_here, int *nblocks);
+
 template <bool MIXED, int C, int LJ, int CS, bool BIGLJ = false>
 void launch_pair_phase(Sys *s, bool stats, int phase) {
   hipStream_t st = phase == 2 ? s->ctx->stream2 : s->ctx->stream;
@@ -796,6 +800,18 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
 #undef PMX_PAIR_ARGS
     return;
   }
+  int nfix = 0;
+  const FixupArgs<MIXED> a = fixup_args<MIXED>(s, stats, border_here, rest_here, &nfix);
+  if (nfix > 0)
+    hipLaunchKernelGGL((k_fixups<MIXED, C, LJ, CS>), dim3(nfix), dim3(256), 0, st, a);
+}
+
+// Arguments of k_fixups for one force pass: the boundary pairs
+// (border_here), the exclusion correction and the bonded terms (rest_here);
+// *nblocks gets the launch's block count.
+template <bool MIXED>
+FixupArgs<MIXED> fixup_args(Sys *s, bool stats, bool border_here, bool rest_here, int *nblocks) {
+  using R4 = typename Real4<typename std::conditional<MIXED, float, double>::type>::type;
   FixupArgs<MIXED> a{};
   a.border_blocks = border_here ? s->border_blocks : 0;
   a.pp = s->pp;
@@ -833,10 +849,9 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
     a.apar = P<double2>(s->md.apar);
     a.bond_e = P<double>(s->md.bond_e);
   }
-  a.facc = facc;
-  if (a.border_blocks + a.corr_blocks + bonded_blocks > 0)
-    hipLaunchKernelGGL((k_fixups<MIXED, C, LJ, CS>), dim3(a.border_blocks + a.corr_blocks + bonded_blocks), dim3(256),
-                       0, st, a);
+  a.facc = P<typename AccT<MIXED>::type>(s->facc);
+  *nblocks = a.border_blocks + a.corr_blocks + bonded_blocks;
+  return a;
 }
 
 template <bool MIXED>
@@ -1083,12 +1098,9 @@ int rebuild(Sys *s, bool timing) {
 // next: the following MD step's kick (with next_dmax, its displacement
 // record): the pass ends with that step's kick and drift fused into the
 // force unsort.
-int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, unsigned *next_dmax = nullptr,
-           bool spread_done = false) {
-  hipStream_t st = s->ctx->stream;
-  const bool timing = flags & PMX_EVAL_TIMING;
-  const bool stats = flags & PMX_EVAL_PAIRSTATS;
-  const int n = s->n, npad = s->npad;
+// Clears the force accumulators, the pass counters and the PME charge grid.
+void launch_zero(Sys *s) {
+  const int npad = s->npad;
   ZeroRanges z{};
   int nr = 0;
   z.p[nr] = P<unsigned long long>(s->facc);
@@ -1104,6 +1116,15 @@ int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, un
     z.p[nr] = s->mixed ? P<unsigned long long>(s->gridfx) : P<unsigned long long>(s->grid64);
     z.n[nr++] = G;
   }
+  hipLaunchKernelGGL(k_zero, dim3(std::max(1u, nblk(3ll * npad, 256)), nr), dim3(256), 0, s->ctx->stream, z);
+}
+
+int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, unsigned *next_dmax = nullptr,
+           bool spread_done = false) {
+  hipStream_t st = s->ctx->stream;
+  const bool timing = flags & PMX_EVAL_TIMING;
+  const bool stats = flags & PMX_EVAL_PAIRSTATS;
+  const int n = s->n, npad = s->npad;
   // A completed pass leaves the accumulators, the PME grid and the counters
   // cleared (k_unsort_forces, k_fft_z_r2c, k_pair); only the first pass after
   // (re)allocation or after an interrupted pass clears them here.  A spread
@@ -1111,8 +1132,7 @@ int forces(Sys *s, unsigned flags, bool energies, const Kick *next = nullptr, un
   // says so; otherwise it belongs to other positions and is cleared.
   if (s->spread_ready && !spread_done) s->acc_clean = false;
   s->spread_ready = false;
-  if (!s->acc_clean)
-    hipLaunchKernelGGL(k_zero, dim3(std::max(1u, nblk(3ll * npad, 256)), nr), dim3(256), 0, st, z);
+  if (!s->acc_clean) launch_zero(s);
   s->acc_clean = false;
   if (timing) hipEventRecord(s->ev[2], st);
   // fork: FFT, gather, exclusion correction and bonded terms on the second
@@ -2061,6 +2081,449 @@ int md_end(Sys *s) {
   return md_check_overflow(s);
 }
 
+// ---- batched MD: every kernel of a step launched once for many systems ----
+// (C++ linkage for the templates inside the C ABI's extern "C" block)
+extern "C++" {
+
+#define BHIPCHK(expr)                                                                                   \
+  do {                                                                                                  \
+    hipError_t _e = (expr);                                                                             \
+    if (_e != hipSuccess) return fail(b.ctx, PMX_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+// A device block of the batch (from the block cache, given back at the end).
+struct BatchBuf {
+  int device = 0;
+  void *p = nullptr;
+  size_t cls = 0;
+  BatchBuf() = default;
+  BatchBuf(const BatchBuf &) = delete;
+  BatchBuf &operator=(const BatchBuf &) = delete;
+  ~BatchBuf() { release(); }
+  void release() {
+    if (!p) return;
+    RtLock lk(g_rt_mu);
+    block_put(device, p, cls);
+    p = nullptr;
+  }
+  bool get(int dev, size_t bytes) {
+    release();
+    device = dev;
+    cls = size_class(std::max<size_t>(bytes, 8));
+    RtLock lk(g_rt_mu);
+    p = block_get(dev, cls);
+    return p != nullptr;
+  }
+};
+
+// One batched launch: the argument table, each system's first block and the
+// system of each block.
+struct BatchLaunch {
+  BatchBuf tab, first, b2s;
+  unsigned blocks = 0;
+  size_t lds = 0;
+};
+
+struct Batch {
+  pmx_ctx *ctx = nullptr;
+  std::vector<Sys *> sys;
+  int nsys = 0;
+  BatchLaunch bin, scan, scatter, binrank, columns, split, list, excl, masks, rprefix, rcut, rtiles;
+  BatchLaunch kick, spread, pair, zr2c, yfwd, xconv, yinv, plane, zc2r, gather, fixups, ukds, unsort, begin, half;
+  BatchBuf bsys, need, base, flags, rebuilds;
+  BatchSys *bsys_h = nullptr;
+  unsigned long long *base_h = nullptr;
+  int *flags_h = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  ~Batch() {
+    RtLock lk(g_rt_mu);
+    if (bsys_h) hipHostFree(bsys_h);
+    if (base_h) hipHostFree(base_h);
+    if (flags_h) hipHostFree(flags_h);
+    if (fork) hipEventDestroy(fork);
+    if (join) hipEventDestroy(join);
+  }
+};
+
+// Uploads the table of one batched launch (args[s], nblocks[s] blocks of
+// system s; dynamic LDS `lds`).
+template <class Op>
+int batch_table(Batch &b, BatchLaunch &L, const std::vector<typename Op::Args> &args,
+                const std::vector<int> &nblocks, size_t lds = 0) {
+  std::vector<int> first(b.nsys + 1, 0);
+  for (int i = 0; i < b.nsys; ++i) first[i + 1] = first[i] + nblocks[i];
+  std::vector<int> b2s(std::max(1, first[b.nsys]));
+  for (int i = 0; i < b.nsys; ++i)
+    for (int k = first[i]; k < first[i + 1]; ++k) b2s[k] = i;
+  L.blocks = (unsigned)first[b.nsys];
+  L.lds = lds;
+  const int dev = b.ctx->device;
+  if (!L.tab.get(dev, sizeof(typename Op::Args) * args.size()) || !L.first.get(dev, 4 * first.size()) ||
+      !L.b2s.get(dev, 4 * b2s.size()))
+    return fail(b.ctx, PMX_ERR_OOM, "batch tables");
+  RtLock lk(g_rt_mu);
+  BHIPCHK(hipMemcpy(L.tab.p, args.data(), sizeof(typename Op::Args) * args.size(), hipMemcpyHostToDevice));
+  BHIPCHK(hipMemcpy(L.first.p, first.data(), 4 * first.size(), hipMemcpyHostToDevice));
+  BHIPCHK(hipMemcpy(L.b2s.p, b2s.data(), 4 * b2s.size(), hipMemcpyHostToDevice));
+  return PMX_OK;
+}
+
+template <class Op>
+void batch_go(const BatchLaunch &L, hipStream_t st, const BatchStep &bs) {
+  if (!L.blocks) return;
+  hipLaunchKernelGGL(k_batch<Op>, dim3(L.blocks), dim3(Op::kBlock), L.lds, st,
+                     reinterpret_cast<const typename Op::Args *>(L.tab.p), reinterpret_cast<const int *>(L.b2s.p),
+                     reinterpret_cast<const int *>(L.first.p), bs);
+}
+
+// The batch's kernel case: the pair kernel's cases of run_pair_phase for PME
+// systems (0: LJ C2 + Coulomb cutoff switch, 1: no LJ, 2: any switches, 3:
+// LJ table beyond the kernel's LDS).
+int batch_case(const Sys *s) {
+  const int lj = s->d.lj_switch, cs = s->d.coul_switch;
+  if (s->d.ntypes > kMaxTypes && lj != SW_NONE) return 3;
+  if (lj == SW_C2 && cs == SW_CUTOFF) return 0;
+  return lj == SW_NONE ? 1 : 2;
+}
+
+template <bool MIXED, int C, int LJ, int CS, bool BIGLJ, int ORDER>
+using PairOp = OpPair<MIXED, C, LJ, CS, BIGLJ>;
+
+// Fills every table of the batch from its systems (again after a system's
+// buffers grow).
+template <bool MIXED, int C, int LJ, int CS, bool BIGLJ, int ORDER>
+int batch_tables(Batch &b) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using R4 = typename Real4<R>::type;
+  using T = R;
+  using Acc = typename AccT<MIXED>::type;
+  const int ns = b.nsys;
+  const size_t cx = sizeof(Cx<T>);
+  std::vector<int> nb(ns);
+  int rc;
+  {
+    std::vector<OpBin::Args> args_bin(ns);
+    std::vector<OpScanSingle::Args> args_scan(ns);
+    std::vector<OpScatter::Args> args_scatter(ns);
+    std::vector<OpBinrank::Args> args_binrank(ns);
+    std::vector<OpColumns::Args> args_columns(ns);
+    std::vector<typename OpSplitFill<R>::Args> args_split(ns);
+    std::vector<OpBuildList::Args> args_list(ns);
+    std::vector<OpMarkExclusions::Args> args_excl(ns);
+    std::vector<OpBuildMasks::Args> args_masks(ns);
+    std::vector<OpRunsPrefix::Args> args_rprefix(ns);
+    std::vector<OpRunsCut::Args> args_rcut(ns);
+    std::vector<OpRunsTiles::Args> args_rtiles(ns);
+    for (int i = 0; i < ns; ++i) {
+      Sys *s = b.sys[i];
+      args_bin[i] = {s->n, P<double>(s->xorig), s->bin, P<uint32_t>(s->key), P<uint32_t>(s->hist),
+                     P<double>(s->xwrap), P<int>(s->refcell)};
+      args_scan[i] = {P<uint32_t>(s->hist), P<uint32_t>(s->start), P<uint32_t>(s->cursor), s->nbins};
+      args_scatter[i] = {s->n, P<uint32_t>(s->key), P<uint32_t>(s->cursor), P<int>(s->perm)};
+      args_binrank[i] = {s->n, s->nbins, P<uint32_t>(s->key), P<uint32_t>(s->start), P<int>(s->perm),
+                         P<int>(s->perm2)};
+      args_columns[i] = {s->ncols, s->bin.nzb, s->n, s->nsuper, P<uint32_t>(s->start), P<int2>(s->sc_info),
+                         P<int>(s->sc_col), P<int>(s->colsc)};
+      auto &sp = args_split[i];
+      sp.nsuper = s->nsuper;
+      sp.sc_info = P<int2>(s->sc_info);
+      sp.order = P<int>(s->perm2);
+      sp.xwrap = P<double>(s->xwrap);
+      sp.q = P<double>(s->q_d);
+      sp.permp = P<int>(s->permp);
+      sp.slot = P<int>(s->slot);
+      sp.type = P<int>(s->type_d);
+      sp.xs = P<R4>(s->xs);
+      sp.xs64 = P<double4>(s->xs64);
+      sp.ts = P<int>(s->ts);
+      sp.xorig = P<double>(s->xorig);
+      sp.xbuild = P<double>(s->md.xbuild);
+      sp.sorg = P<float4>(s->sorg);
+      sp.ctr = P<float4>(s->ctr);
+      sp.half = P<float4>(s->half);
+      sp.sctr = P<float4>(s->sctr);
+      sp.shalf = P<float4>(s->shalf);
+      sp.nmaskwork = P<int>(s->nmaskwork);
+      sp.qpair = s->qpair;
+      args_list[i] = {s->lp, P<int>(s->sc_col), P<int>(s->colsc), P<float4>(s->sctr), P<float4>(s->shalf),
+                      P<float4>(s->ctr), P<float4>(s->half), P<int4>(s->entries), P<int>(s->count),
+                      P<int>(s->ecost), P<int>(s->lcost), P<int>(s->maskwork), P<int>(s->nmaskwork),
+                      s->maxmaskwork, P<int>(s->overflow)};
+      args_excl[i] = {(int)s->etab.size(), s->cap, s->maxmaskwork, P<int4>(s->etab_d), P<int>(s->slot),
+                      P<int>(s->count), P<int4>(s->entries), P<int>(s->maskwork), P<int>(s->nmaskwork),
+                      P<int>(s->overflow)};
+      args_masks[i] = {P<int>(s->maskwork), P<int>(s->nmaskwork), s->maxmaskwork, s->cap, P<int4>(s->entries),
+                       P<int>(s->permp), P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned short>(s->masks)};
+      int *const nruns = P<int>(s->ntiles) + 2;
+      args_rprefix[i] = {s->nsuper, P<int>(s->count), P<int>(s->lcost), nruns, P<int>(s->tep), P<long long>(s->tcp),
+                         P<int>(s->tpos), P<int>(s->tposS)};
+      args_rcut[i] = {s->nsuper, s->cap, P<int>(s->count), P<int>(s->ecost), nruns, P<int>(s->tep),
+                      P<long long>(s->tcp), P<int>(s->tpos), P<int>(s->tposS)};
+      args_rtiles[i] = {P<int>(s->count), P<int>(s->tep), P<int>(s->tpos), P<int>(s->tposS), P<int>(s->wstart),
+                        P<int4>(s->tiles), P<int>(s->ntiles), s->ntiles_cap};
+    }
+    for (int i = 0; i < ns; ++i) nb[i] = (int)nblk(b.sys[i]->n, 256);
+    if ((rc = batch_table<OpBin>(b, b.bin, args_bin, nb, 0))) return rc;
+    if ((rc = batch_table<OpScatter>(b, b.scatter, args_scatter, nb, 0))) return rc;
+    if ((rc = batch_table<OpBinrank>(b, b.binrank, args_binrank, nb, 0))) return rc;
+    for (int i = 0; i < ns; ++i) nb[i] = 1;
+    if ((rc = batch_table<OpScanSingle>(b, b.scan, args_scan, nb, 0))) return rc;
+    if ((rc = batch_table<OpColumns>(b, b.columns, args_columns, nb, 0))) return rc;
+    if ((rc = batch_table<OpRunsPrefix>(b, b.rprefix, args_rprefix, nb, 0))) return rc;
+    if ((rc = batch_table<OpRunsTiles>(b, b.rtiles, args_rtiles, nb, 0))) return rc;
+    for (int i = 0; i < ns; ++i) nb[i] = (int)nblk((long long)b.sys[i]->nsuper * 64, 256);
+    if ((rc = batch_table<OpSplitFill<R>>(b, b.split, args_split, nb, 0))) return rc;
+    for (int i = 0; i < ns; ++i) nb[i] = b.sys[i]->nsuper;
+    if ((rc = batch_table<OpBuildList>(b, b.list, args_list, nb, 0))) return rc;
+    if ((rc = batch_table<OpRunsCut>(b, b.rcut, args_rcut, nb, 0))) return rc;
+    for (int i = 0; i < ns; ++i) nb[i] = b.sys[i]->etab.empty() ? 0 : (int)nblk((long long)b.sys[i]->etab.size(), 256);
+    if ((rc = batch_table<OpMarkExclusions>(b, b.excl, args_excl, nb, 0))) return rc;
+    // a wave per mask slot in use (known on the device only), up to 64 blocks
+    for (int i = 0; i < ns; ++i) nb[i] = (int)std::min<long long>(nblk((long long)b.sys[i]->maxmaskwork, 4), 64);
+    if ((rc = batch_table<OpBuildMasks>(b, b.masks, args_masks, nb, 0))) return rc;
+  }
+  {
+    std::vector<typename OpKickDrift<R>::Args> args_kick(ns);
+    std::vector<typename OpSpreadPatch<MIXED, ORDER>::Args> args_spread(ns);
+    std::vector<typename PairOp<MIXED, C, LJ, CS, BIGLJ, ORDER>::Args> args_pair(ns);
+    std::vector<typename OpFftZr2c<T, MIXED>::Args> args_zr2c(ns);
+    std::vector<typename OpFftY<T>::Args> args_yfwd(ns), args_yinv(ns);
+    std::vector<typename OpFftXConv<T>::Args> args_xconv(ns);
+    std::vector<typename OpFftPlane<T>::Args> args_plane(ns);
+    std::vector<typename OpFftZc2r<T>::Args> args_zc2r(ns);
+    std::vector<typename OpGatherPatch<MIXED, ORDER>::Args> args_gather(ns);
+    std::vector<FixupArgs<MIXED>> args_fixups(ns);
+    std::vector<typename OpUnsortKickDriftSpread<MIXED, R, ORDER>::Args> args_ukds(ns);
+    std::vector<typename OpUnsortForces<MIXED>::Args> args_unsort(ns);
+    std::vector<OpChunkBegin::Args> args_begin(ns);
+    std::vector<OpHalfKick::Args> args_half(ns);
+    std::vector<int> nfix(ns);
+    long long natoms = 0;
+    size_t lds_z = 0, lds_y = 0, lds_x = 0, lds_p = 0;
+    for (int i = 0; i < ns; ++i) {
+      Sys *s = b.sys[i];
+      Sys::Md &m = s->md;
+      natoms += s->n;
+      const Kick kk = make_kick(m, 1.0, 0, P<unsigned long long>(m.step_dev));
+      Acc *facc = P<Acc>(s->facc);
+      args_kick[i] = {s->npad, kk, m.h, P<int>(s->permp), P<double>(s->fout), P<double>(m.minv), P<double>(m.xbuild),
+                      P<double>(m.v), P<double>(s->xorig), P<R4>(s->xs), P<double4>(s->xs64), P<float4>(s->sorg),
+                      P<unsigned>(m.dmax)};
+      args_spread[i] = {s->npad, s->pme, P<double>(s->xorig), P<double>(s->q_d), P<int>(s->permp),
+                        P<double>(s->grid64), P<unsigned long long>(s->gridfx)};
+      const R4 *ljt = MIXED ? reinterpret_cast<const R4 *>(s->ljt32.p) : reinterpret_cast<const R4 *>(s->ljtab.p);
+      args_pair[i] = {s->pp, P<int4>(s->tiles), P<int>(s->ntiles), P<int>(s->wstart), P<int>(s->tilectr),
+                      P<int4>(s->entries), s->cap, P<unsigned short>(s->masks), P<R4>(s->xs), P<float4>(s->sorg),
+                      P<int>(s->ts), P<int>(s->permp), P<double>(s->ljtab), ljt, facc,
+                      PairOut{P<double>(s->pair_e), P<long long>(s->npairs), P<unsigned long long>(s->hash),
+                              P<int2>(s->border), P<int>(s->nborder), s->maxborder, P<int>(s->overflow),
+                              P<unsigned long long>(s->border_acc)}};
+      const PmeParams &pm = s->pme;
+      args_zr2c[i] = {pm, s->plz, s->zl, P<double>(s->grid64), P<unsigned long long>(s->gridfx), P<Cx<T>>(s->twz),
+                      P<Cx<T>>(s->spec)};
+      args_yfwd[i] = {pm, s->ply, s->tky, false, P<Cx<T>>(s->twy), P<Cx<T>>(s->spec), P<Cx<T>>(s->spec2)};
+      args_xconv[i] = {pm, s->plx, s->tkx, P<Cx<T>>(s->twx), P<T>(s->theta), P<Cx<T>>(s->spec2),
+                       P<double>(s->recip_e)};
+      args_yinv[i] = {pm, s->ply, s->tky, true, P<Cx<T>>(s->twy), P<Cx<T>>(s->spec2), P<Cx<T>>(s->spec)};
+      args_plane[i] = {pm, s->ply, s->plx, P<Cx<T>>(s->twy), P<Cx<T>>(s->twx), P<T>(s->theta), P<Cx<T>>(s->spec),
+                       P<double>(s->recip_e)};
+      args_zc2r[i] = {pm, s->plz, s->zl, P<Cx<T>>(s->twz), P<Cx<T>>(s->spec), P<T>(s->greal)};
+      args_gather[i] = {s->npad, pm, P<double>(s->xorig), P<double>(s->q_d), P<int>(s->permp), P<T>(s->greal), facc};
+      args_fixups[i] = fixup_args<MIXED>(s, false, true, true, &nfix[i]);
+      args_ukds[i] = {s->npad, kk, m.h, P<int>(s->permp), facc, P<double>(s->fout), P<int>(s->tilectr),
+                      P<int>(s->nborder), P<double>(m.minv), P<double>(m.xbuild), P<double>(s->q_d), P<double>(m.v),
+                      P<double>(s->xorig), P<R4>(s->xs), P<double4>(s->xs64), P<float4>(s->sorg),
+                      P<unsigned>(m.dmax), pm, P<double>(s->grid64), P<unsigned long long>(s->gridfx)};
+      args_unsort[i] = {s->n, P<int>(s->slot), facc, P<double>(s->fout), P<int>(s->tilectr), P<int>(s->nborder)};
+      args_begin[i] = {3ll * s->n, P<double>(s->xorig), P<double>(m.v), P<double>(s->fout), P<double>(m.xbak),
+                       P<double>(m.vbak), P<double>(m.fbak), P<int>(s->overflow), P<unsigned>(m.dmax), kMaxChunk};
+      args_half[i] = {s->n, kk, m.h, P<double>(s->fout), P<double>(m.minv), P<double>(m.v)};
+      lds_z = std::max(lds_z, 2 * (size_t)s->zl * pm.nz * cx);
+      lds_y = std::max(lds_y, 2 * (size_t)s->tky * pm.ny * cx);
+      lds_x = std::max(lds_x, 2 * (size_t)s->tkx * pm.nx * cx);
+      lds_p = std::max(lds_p, 2 * (size_t)pm.nx * pm.ny * cx);
+    }
+    for (int i = 0; i < ns; ++i) nb[i] = (int)nblk(b.sys[i]->npad, 256);
+    if ((rc = batch_table<OpKickDrift<R>>(b, b.kick, args_kick, nb, 0))) return rc;
+    for (int i = 0; i < ns; ++i) nb[i] = (int)nblk(b.sys[i]->npad, kSpreadSlots);
+    if ((rc = batch_table<OpSpreadPatch<MIXED, ORDER>>(b, b.spread, args_spread, nb, 0))) return rc;
+    if ((rc = batch_table<OpGatherPatch<MIXED, ORDER>>(b, b.gather, args_gather, nb, 0))) return rc;
+    if ((rc = batch_table<OpUnsortKickDriftSpread<MIXED, R, ORDER>>(b, b.ukds, args_ukds, nb, 0))) return rc;
+    // pair-kernel blocks in proportion to the atoms (a run's share of a
+    // system's waves changes no result), about the device's resident blocks
+    // in all
+    static int per_cu = 0;
+    if (!per_cu) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<PairOp<MIXED, C, LJ, CS, BIGLJ, ORDER>>,
+                                                       kPairBlock, 0) != hipSuccess ||
+          per_cu < 1)
+        per_cu = 1;
+      per_cu = std::min(per_cu, kMaxPairWavesPerCu / (kPairBlock / kWave));
+    }
+    const double pair_blocks = (double)b.ctx->cus * per_cu;
+    for (int i = 0; i < ns; ++i)
+      nb[i] = std::max(1, (int)std::lround(pair_blocks * b.sys[i]->n / (double)natoms));
+    if ((rc = batch_table<PairOp<MIXED, C, LJ, CS, BIGLJ, ORDER>>(b, b.pair, args_pair, nb, 0))) return rc;
+    for (int i = 0; i < ns; ++i) {
+      const Sys *s = b.sys[i];
+      nb[i] = s->pme.nx * ((s->pme.ny + s->zl - 1) / s->zl);
+    }
+    if ((rc = batch_table<OpFftZr2c<T, MIXED>>(b, b.zr2c, args_zr2c, nb, lds_z))) return rc;
+    if ((rc = batch_table<OpFftZc2r<T>>(b, b.zc2r, args_zc2r, nb, lds_z))) return rc;
+    // systems with small planes take the one-launch plane pass, the others
+    // the three line passes
+    for (int i = 0; i < ns; ++i) {
+      const Sys *s = b.sys[i];
+      nb[i] = s->fft_plane ? 0 : s->pme.nzh * ((s->pme.nx + s->tky - 1) / s->tky);
+    }
+    if ((rc = batch_table<OpFftY<T>>(b, b.yfwd, args_yfwd, nb, lds_y))) return rc;
+    if ((rc = batch_table<OpFftY<T>>(b, b.yinv, args_yinv, nb, lds_y))) return rc;
+    for (int i = 0; i < ns; ++i) nb[i] = b.sys[i]->fft_plane ? 0 : b.sys[i]->recip_blocks;
+    if ((rc = batch_table<OpFftXConv<T>>(b, b.xconv, args_xconv, nb, lds_x))) return rc;
+    for (int i = 0; i < ns; ++i) nb[i] = b.sys[i]->fft_plane ? b.sys[i]->pme.nzh : 0;
+    if ((rc = batch_table<OpFftPlane<T>>(b, b.plane, args_plane, nb, lds_p))) return rc;
+    if ((rc = batch_table<OpFixups<MIXED, C, LJ, CS>>(b, b.fixups, args_fixups, nfix, 0))) return rc;
+    for (int i = 0; i < ns; ++i) nb[i] = (int)nblk(b.sys[i]->n, 256);
+    if ((rc = batch_table<OpUnsortForces<MIXED>>(b, b.unsort, args_unsort, nb, 0))) return rc;
+    if ((rc = batch_table<OpHalfKick>(b, b.half, args_half, nb, 0))) return rc;
+    for (int i = 0; i < ns; ++i) nb[i] = (int)nblk(std::max<long long>(3ll * b.sys[i]->n, kMaxChunk), 256);
+    if ((rc = batch_table<OpChunkBegin>(b, b.begin, args_begin, nb, 0))) return rc;
+  }
+  // per-system bookkeeping
+  for (int i = 0; i < ns; ++i) {
+    Sys *s = b.sys[i];
+    b.bsys_h[i] = BatchSys{P<unsigned>(s->md.dmax), s->md.skin > 0 ? (float)(0.25 * s->md.skin * s->md.skin) : -1.f, 0,
+                           P<unsigned long long>(s->md.step_dev), P<int>(s->overflow),
+                           reinterpret_cast<long long *>(b.rebuilds.p) + i};
+  }
+  return PMX_OK;
+}
+
+// Queues one chunk of K steps of every system of the batch (the first step
+// of each system's run when `opening`), then the gathering of the overflow
+// flags into b.flags_h.  Step 0 kicks and drifts and spreads on its own;
+// every later step's kick, drift and spread run in the previous step's last
+// launch.
+template <bool MIXED, int C, int LJ, int CS, bool BIGLJ, int ORDER>
+int batch_chunk(Batch &b, int K, bool opening) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using T = R;
+  hipStream_t st = b.ctx->stream, st2 = b.ctx->stream2;
+  const int ns = b.nsys;
+  BHIPCHK(hipMemcpyAsync(b.bsys.p, b.bsys_h, sizeof(BatchSys) * ns, hipMemcpyHostToDevice, st));
+  BHIPCHK(hipMemcpyAsync(b.base.p, b.base_h, 8 * (size_t)ns, hipMemcpyHostToDevice, st));
+  const BatchSys *bs = reinterpret_cast<const BatchSys *>(b.bsys.p);
+  unsigned char *need = reinterpret_cast<unsigned char *>(b.need.p);
+  hipLaunchKernelGGL(k_batch_steps, dim3(nblk(ns, 256)), dim3(256), 0, st, ns, bs,
+                     reinterpret_cast<const unsigned long long *>(b.base.p));
+  const BatchStep b0{0, opening ? 1 : 0, need};
+  batch_go<OpChunkBegin>(b.begin, st, b0);
+  for (int k = 0; k < K; ++k) {
+    const BatchStep bk{k, opening ? 1 : 0, need};
+    if (k == 0) batch_go<OpKickDrift<R>>(b.kick, st, bk);
+    hipLaunchKernelGGL(k_batch_need, dim3(nblk(ns, 256)), dim3(256), 0, st, ns, bs, k, need);
+    batch_go<OpBin>(b.bin, st, bk);
+    batch_go<OpScanSingle>(b.scan, st, bk);
+    batch_go<OpScatter>(b.scatter, st, bk);
+    batch_go<OpBinrank>(b.binrank, st, bk);
+    batch_go<OpColumns>(b.columns, st, bk);
+    batch_go<OpSplitFill<R>>(b.split, st, bk);
+    batch_go<OpBuildList>(b.list, st, bk);
+    batch_go<OpMarkExclusions>(b.excl, st, bk);
+    batch_go<OpBuildMasks>(b.masks, st, bk);
+    batch_go<OpRunsPrefix>(b.rprefix, st, bk);
+    batch_go<OpRunsCut>(b.rcut, st, bk);
+    batch_go<OpRunsTiles>(b.rtiles, st, bk);
+    if (k == 0) batch_go<OpSpreadPatch<MIXED, ORDER>>(b.spread, st, bk);
+    // fork: the PME chain beside the pair kernel
+    BHIPCHK(hipEventRecord(b.fork, st));
+    BHIPCHK(hipStreamWaitEvent(st2, b.fork, 0));
+    batch_go<PairOp<MIXED, C, LJ, CS, BIGLJ, ORDER>>(b.pair, st, bk);
+    batch_go<OpFftZr2c<T, MIXED>>(b.zr2c, st2, bk);
+    batch_go<OpFftPlane<T>>(b.plane, st2, bk);
+    batch_go<OpFftY<T>>(b.yfwd, st2, bk);
+    batch_go<OpFftXConv<T>>(b.xconv, st2, bk);
+    batch_go<OpFftY<T>>(b.yinv, st2, bk);
+    batch_go<OpFftZc2r<T>>(b.zc2r, st2, bk);
+    batch_go<OpGatherPatch<MIXED, ORDER>>(b.gather, st2, bk);
+    BHIPCHK(hipEventRecord(b.join, st2));
+    // boundary pairs, exclusion correction and bonded terms after the pair
+    // kernel, then the join
+    batch_go<OpFixups<MIXED, C, LJ, CS>>(b.fixups, st, bk);
+    BHIPCHK(hipStreamWaitEvent(st, b.join, 0));
+    if (k + 1 < K)
+      batch_go<OpUnsortKickDriftSpread<MIXED, R, ORDER>>(b.ukds, st, bk);
+    else
+      batch_go<OpUnsortForces<MIXED>>(b.unsort, st, bk);
+  }
+  hipLaunchKernelGGL(k_batch_flags, dim3(nblk(ns, 256)), dim3(256), 0, st, ns, bs,
+                     reinterpret_cast<int *>(b.flags.p));
+  BHIPCHK(hipMemcpyAsync(b.flags_h, b.flags.p, 4 * (size_t)ns, hipMemcpyDeviceToHost, st));
+  BHIPCHK(hipGetLastError());
+  return PMX_OK;
+}
+
+constexpr int kBatchChunk = 64;  // steps between the host's checks of a batch
+
+// nsteps MD steps of every system of the batch.  Chunks of up to
+// kBatchChunk steps; after each the host reads the overflow flags, and a
+// chunk in which a list outgrew its buffers is run again from its start
+// (backups of every system) after those systems' buffers grow.
+template <bool MIXED, int C, int LJ, int CS, bool BIGLJ, int ORDER>
+int batch_run(Batch &b, int nsteps) {
+  hipStream_t st = b.ctx->stream;
+  int rc = batch_tables<MIXED, C, LJ, CS, BIGLJ, ORDER>(b);
+  if (rc) return rc;
+  BHIPCHK(hipMemsetAsync(b.rebuilds.p, 0, 8 * (size_t)b.nsys, st));
+  int done = 0, retries = 0, last = 0;
+  while (done < nsteps) {
+    const int K = std::min(kBatchChunk, nsteps - done);
+    last = K;
+    for (int i = 0; i < b.nsys; ++i) b.base_h[i] = (unsigned long long)(b.sys[i]->md.steps + done);
+    if ((rc = batch_chunk<MIXED, C, LJ, CS, BIGLJ, ORDER>(b, K, done == 0))) return rc;
+    BHIPCHK(hipStreamSynchronize(st));
+    bool grown = false;
+    for (int i = 0; i < b.nsys; ++i) {
+      if (!b.flags_h[i]) continue;
+      if (!grown && ++retries > 6) return fail(b.ctx, PMX_ERR_OVERFLOW, "pair list capacity exceeded during MD");
+      if (!grown) {
+        // every system back to the chunk's start
+        for (Sys *s : b.sys)
+          if ((rc = restore_chunk(s, 0))) return rc;
+      }
+      grown = true;
+      if ((rc = grow_and_retry(b.sys[i]))) return rc;
+    }
+    if (grown) {
+      if ((rc = batch_tables<MIXED, C, LJ, CS, BIGLJ, ORDER>(b))) return rc;
+      for (int i = 0; i < b.nsys; ++i) b.bsys_h[i].force = 1;
+      BHIPCHK(hipStreamSynchronize(st));
+      continue;
+    }
+    for (int i = 0; i < b.nsys; ++i) b.bsys_h[i].force = 0;
+    done += K;
+  }
+  // closing half kick of every system's last step
+  const BatchStep bk{last, 0, nullptr};
+  batch_go<OpHalfKick>(b.half, st, bk);
+  std::vector<long long> nreb(b.nsys);
+  BHIPCHK(hipMemcpyAsync(nreb.data(), b.rebuilds.p, 8 * (size_t)b.nsys, hipMemcpyDeviceToHost, st));
+  BHIPCHK(hipStreamSynchronize(st));
+  for (int i = 0; i < b.nsys; ++i) {
+    Sys *s = b.sys[i];
+    s->md.steps += nsteps;
+    s->md.rebuilds += nreb[i];
+    s->list_valid = true;
+    s->prune_pending = false;
+    s->spread_ready = false;
+    s->acc_clean = true;
+  }
+  return PMX_OK;
+}
+
+}  // extern "C++"
+
+
 }  // namespace
 
 int pmx_md_run(pmx_sys *sys, int nsteps) {
@@ -2146,6 +2609,92 @@ int pmx_md_run_many(pmx_sys *const *systems, int nsys, int nsteps) {
   for (int rc : status)
     if (rc) return rc;
   return PMX_OK;
+}
+
+extern "C++" {
+namespace {
+
+template <bool MIXED, int CASE>
+int batch_run_case(Batch &b, int order, int nsteps) {
+  constexpr int LJ = CASE == 0 ? SW_C2 : (CASE == 1 ? SW_NONE : SW_ANY);
+  constexpr int CS = CASE == 0 ? SW_CUTOFF : SW_ANY;
+  constexpr bool BIG = CASE == 3;
+  switch (order) {
+    case 4: return batch_run<MIXED, COUL_PME, LJ, CS, BIG, 4>(b, nsteps);
+    case 6: return batch_run<MIXED, COUL_PME, LJ, CS, BIG, 6>(b, nsteps);
+    default: return batch_run<MIXED, COUL_PME, LJ, CS, BIG, 8>(b, nsteps);
+  }
+}
+
+template <bool MIXED>
+int batch_run_form(Batch &b, int cs, int order, int nsteps) {
+  switch (cs) {
+    case 0: return batch_run_case<MIXED, 0>(b, order, nsteps);
+    case 1: return batch_run_case<MIXED, 1>(b, order, nsteps);
+    case 2: return batch_run_case<MIXED, 2>(b, order, nsteps);
+    default: return batch_run_case<MIXED, 3>(b, order, nsteps);
+  }
+}
+
+}  // namespace
+}  // extern "C++"
+
+int pmx_md_run_batch(pmx_sys *const *systems, int nsys, int nsteps) {
+  if (!systems || nsys <= 0 || nsteps < 0) return PMX_ERR_ARG;
+  for (int i = 0; i < nsys; ++i)
+    if (!systems[i]) return PMX_ERR_ARG;
+  Sys *s0 = &systems[0]->s;
+  pmx_ctx *ctx = s0->ctx;
+  const int cs = batch_case(s0), order = s0->pme.order;
+  std::set<const pmx_sys *> seen;
+  for (int i = 0; i < nsys; ++i) {
+    Sys *s = &systems[i]->s;
+    if (!seen.insert(systems[i]).second) return fail(ctx, PMX_ERR_ARG, "a system is listed twice");
+    if (s->ctx != ctx) return fail(ctx, PMX_ERR_ARG, "the systems of a batch share one context");
+    if (!s->md.on || !s->have_positions)
+      return fail(ctx, PMX_ERR_ARG, "pmx_md_init and pmx_md_set_state first (every system of the batch)");
+    if (s->mixed != s0->mixed || batch_case(s) != cs)
+      return fail(ctx, PMX_ERR_ARG, "the systems of a batch share the precision and the kinds of force");
+    if (!s->pme_on || !(s->d.pme_terms & PMX_PME_RECIP) || s->pme.order < 4 || s->pme.order != order)
+      return fail(ctx, PMX_ERR_UNSUPPORTED, "batched MD takes PME systems of one stencil order (4, 6 or 8)");
+    if (s->nbins > kScanSingleMax)
+      return fail(ctx, PMX_ERR_UNSUPPORTED, "batched MD takes systems of at most 65,536 cell bins");
+  }
+  if (nsteps == 0) return PMX_OK;
+  hipSetDevice(ctx->device);
+  Batch b;
+  b.ctx = ctx;
+  b.nsys = nsys;
+  for (int i = 0; i < nsys; ++i) {
+    Sys *s = &systems[i]->s;
+    b.sys.push_back(s);
+    if (int rc = md_ensure_forces(s)) return rc;
+    if (!s->acc_clean || s->spread_ready) {
+      s->spread_ready = false;
+      launch_zero(s);
+      s->acc_clean = true;
+    }
+  }
+  const int dev = ctx->device;
+  if (!b.bsys.get(dev, sizeof(BatchSys) * nsys) || !b.need.get(dev, nsys) || !b.base.get(dev, 8 * (size_t)nsys) ||
+      !b.flags.get(dev, 4 * (size_t)nsys) || !b.rebuilds.get(dev, 8 * (size_t)nsys))
+    return fail(ctx, PMX_ERR_OOM, "batch buffers");
+  {
+    RtLock lk(g_rt_mu);
+    if (hipHostMalloc((void **)&b.bsys_h, sizeof(BatchSys) * nsys, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&b.base_h, 8 * (size_t)nsys, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&b.flags_h, 4 * (size_t)nsys, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&b.fork, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&b.join, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess)
+      return fail(ctx, PMX_ERR_HIP, "batch host buffers and events");
+  }
+  // the systems' own streams are idle from here on; the batch runs on the
+  // context's two streams
+  for (Sys *s : b.sys) {
+    if (hipStreamSynchronize(s->ctx->stream) != hipSuccess || hipStreamSynchronize(s->ctx->stream2) != hipSuccess)
+      return fail(ctx, PMX_ERR_HIP, "stream synchronize");
+  }
+  return s0->mixed ? batch_run_form<true>(b, cs, order, nsteps) : batch_run_form<false>(b, cs, order, nsteps);
 }
 
 int pmx_md_get_state(pmx_sys *sys, double *x, double *v, double *energies) {

@@ -105,9 +105,10 @@ struct BinParams {
   double delta[3];    // origin - min
 };
 
-__global__ void k_bin(int n, const double *__restrict__ xorig, BinParams p, uint32_t *__restrict__ key,
-                      double *__restrict__ xwrap, int *__restrict__ refcell, uint32_t *__restrict__ hist) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bin_body(const int bid_, const int nbk_, int n, const double *__restrict__ xorig,
+                                         BinParams p, uint32_t *__restrict__ key, double *__restrict__ xwrap,
+                                         int *__restrict__ refcell, uint32_t *__restrict__ hist) {
+  int a = bid_ * blockDim.x + threadIdx.x;
   if (a >= n) return;
   double w[3];
 #pragma unroll
@@ -145,6 +146,11 @@ __global__ void k_bin(int n, const double *__restrict__ xorig, BinParams p, uint
     c[k] = ci;
   }
   refcell[a] = (c[0] * p.rc_n[1] + c[1]) * p.rc_n[2] + c[2];
+}
+
+__global__ void k_bin(int n, const double *__restrict__ xorig, BinParams p, uint32_t *__restrict__ key,
+                      double *__restrict__ xwrap, int *__restrict__ refcell, uint32_t *__restrict__ hist) {
+  bin_body(blockIdx.x, gridDim.x, n, xorig, p, key, xwrap, refcell, hist);
 }
 
 // Counting sort by key (one radix digit spanning the key range): histogram,
@@ -232,8 +238,8 @@ __global__ void k_scan_final(uint32_t *__restrict__ in, long long m, const uint3
 // combined across the block first.  The counts are cleared for the next
 // histogram.
 constexpr long long kScanSingleMax = 1 << 16;
-__global__ void k_scan_single(uint32_t *__restrict__ in, long long m, uint32_t *__restrict__ out,
-                              uint32_t *__restrict__ out2) {
+__device__ __forceinline__ void scan_single_body(const int bid_, const int nbk_, uint32_t *__restrict__ in, long long m,
+                                                 uint32_t *__restrict__ out, uint32_t *__restrict__ out2) {
   constexpr int NW = kScanBlock / kWave;
   __shared__ uint32_t s_tot[NW];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -264,18 +270,29 @@ __global__ void k_scan_single(uint32_t *__restrict__ in, long long m, uint32_t *
   }
 }
 
+__global__ void k_scan_single(uint32_t *__restrict__ in, long long m, uint32_t *__restrict__ out,
+                              uint32_t *__restrict__ out2) {
+  scan_single_body(blockIdx.x, gridDim.x, in, m, out, out2);
+}
+
+__device__ __forceinline__ void scatter_body(const int bid_, const int nbk_, int n, const uint32_t *__restrict__ key,
+                                             uint32_t *__restrict__ cursor, int *__restrict__ perm) {
+  int a = bid_ * blockDim.x + threadIdx.x;
+  if (a < n) perm[atomicAdd(&cursor[key[a]], 1u)] = a;
+}
+
 __global__ void k_scatter(int n, const uint32_t *__restrict__ key, uint32_t *__restrict__ cursor,
                           int *__restrict__ perm) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < n) perm[atomicAdd(&cursor[key[a]], 1u)] = a;
+  scatter_body(blockIdx.x, gridDim.x, n, key, cursor, perm);
 }
 
 // Orders each bin by atom index: position p of the scatter output moves to
 // its bin's start plus the number of smaller atom indices in the bin (bins
 // hold a handful of atoms; the result does not depend on the scatter order).
-__global__ void k_binrank(int n, long long nbins, const uint32_t *__restrict__ key,
-                          const uint32_t *__restrict__ start, const int *__restrict__ in, int *__restrict__ out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void binrank_body(const int bid_, const int nbk_, int n, long long nbins,
+                                             const uint32_t *__restrict__ key, const uint32_t *__restrict__ start,
+                                             const int *__restrict__ in, int *__restrict__ out) {
+  const int p = bid_ * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int a = in[p];
   const uint32_t b = key[a];
@@ -285,12 +302,19 @@ __global__ void k_binrank(int n, long long nbins, const uint32_t *__restrict__ k
   out[s0 + rank] = a;
 }
 
+__global__ void k_binrank(int n, long long nbins, const uint32_t *__restrict__ key,
+                          const uint32_t *__restrict__ start, const int *__restrict__ in, int *__restrict__ out) {
+  binrank_body(blockIdx.x, gridDim.x, n, nbins, key, start, in, out);
+}
+
 // Superclusters of each column: sc_info[sc] = (first sorted position, atom
 // count) and sc_col[sc] = its column for sc < total, (0, 0) and -1 beyond;
 // colsc[col] = first supercluster of column col, colsc[ncols] = total.  One
 // block; columns in chunks of the block size with a running offset.
-__global__ void k_columns(int ncols, int nzb, int n, const uint32_t *__restrict__ start, int nsuper_max,
-                          int2 *__restrict__ sc_info, int *__restrict__ sc_col, int *__restrict__ colsc) {
+__device__ __forceinline__ void columns_body(const int bid_, const int nbk_, int ncols, int nzb, int n,
+                                             const uint32_t *__restrict__ start, int nsuper_max,
+                                             int2 *__restrict__ sc_info, int *__restrict__ sc_col,
+                                             int *__restrict__ colsc) {
   __shared__ int s[1024];
   int carry = 0;
   for (int base = 0; base < ncols; base += blockDim.x) {
@@ -325,6 +349,11 @@ __global__ void k_columns(int ncols, int nzb, int n, const uint32_t *__restrict_
     sc_info[sc] = make_int2(0, 0);
     sc_col[sc] = -1;
   }
+}
+
+__global__ void k_columns(int ncols, int nzb, int n, const uint32_t *__restrict__ start, int nsuper_max,
+                          int2 *__restrict__ sc_info, int *__restrict__ sc_col, int *__restrict__ colsc) {
+  columns_body(blockIdx.x, gridDim.x, ncols, nzb, n, start, nsuper_max, sc_info, sc_col, colsc);
 }
 
 // Ascending bitonic sort of (key, idx) within aligned segments of SEG lanes.
@@ -368,20 +397,19 @@ constexpr float kScSlack = 1e-3f;
 //    supercluster (the whole wave, widened by kScSlack).
 // Thread 0 also clears the mask work counter of the list build that follows.
 template <class R>
-__global__ void __launch_bounds__(256) k_split_fill(int nsuper_max, const int2 *__restrict__ sc_info,
-                                                    const int *__restrict__ order, const double *__restrict__ xwrap,
-                                                    int *__restrict__ permp, int *__restrict__ slot,
-                                                    const double *__restrict__ q, const int *__restrict__ type,
-                                                    typename Real4<R>::type *__restrict__ xs,
-                                                    double4 *__restrict__ xs64, int *__restrict__ ts,
-                                                    const double *__restrict__ xorig, double *__restrict__ xbuild,
-                                                    float4 *__restrict__ sorg, float4 *__restrict__ ctr,
-                                                    float4 *__restrict__ half, float4 *__restrict__ sctr,
-                                                    float4 *__restrict__ shalf, int *__restrict__ nmaskwork,
-                                                    double qpair) {
-  const int sc = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+__device__ __forceinline__ void split_fill_body(const int bid_, const int nbk_, int nsuper_max,
+                                                const int2 *__restrict__ sc_info, const int *__restrict__ order,
+                                                const double *__restrict__ xwrap, int *__restrict__ permp,
+                                                int *__restrict__ slot, const double *__restrict__ q,
+                                                const int *__restrict__ type, typename Real4<R>::type *__restrict__ xs,
+                                                double4 *__restrict__ xs64, int *__restrict__ ts,
+                                                const double *__restrict__ xorig, double *__restrict__ xbuild,
+                                                float4 *__restrict__ sorg, float4 *__restrict__ ctr,
+                                                float4 *__restrict__ half, float4 *__restrict__ sctr,
+                                                float4 *__restrict__ shalf, int *__restrict__ nmaskwork, double qpair) {
+  const int sc = (bid_ * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *nmaskwork = 0;
+  if (bid_ == 0 && threadIdx.x == 0) *nmaskwork = 0;
   if (sc >= nsuper_max) return;  // whole waves
   const int2 info = sc_info[sc];
   int a = lane < info.y ? order[info.x + lane] : -1;
@@ -477,6 +505,22 @@ __global__ void __launch_bounds__(256) k_split_fill(int nsuper_max, const int2 *
                               (float)(0.5 * (hi[2] - lo[2])) + kScSlack, 0.f);
     }
   }
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_split_fill(int nsuper_max, const int2 *__restrict__ sc_info,
+                                                    const int *__restrict__ order, const double *__restrict__ xwrap,
+                                                    int *__restrict__ permp, int *__restrict__ slot,
+                                                    const double *__restrict__ q, const int *__restrict__ type,
+                                                    typename Real4<R>::type *__restrict__ xs,
+                                                    double4 *__restrict__ xs64, int *__restrict__ ts,
+                                                    const double *__restrict__ xorig, double *__restrict__ xbuild,
+                                                    float4 *__restrict__ sorg, float4 *__restrict__ ctr,
+                                                    float4 *__restrict__ half, float4 *__restrict__ sctr,
+                                                    float4 *__restrict__ shalf, int *__restrict__ nmaskwork,
+                                                    double qpair) {
+  split_fill_body<R>(blockIdx.x, gridDim.x, nsuper_max, sc_info, order, xwrap, permp, slot, q, type, xs, xs64, ts,
+                     xorig, xbuild, sorg, ctr, half, sctr, shalf, nmaskwork, qpair);
 }
 
 
@@ -685,16 +729,14 @@ __device__ inline void list_append(const ListParams &q, int S, const float4 (&ic
 // eight i-clusters of S, and entries are appended in ascending cj.  Only
 // cj >= 8S is listed, so each cluster pair appears once.  The work per S is
 // bounded by the neighbourhood, independent of the system size.
-__global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const int *__restrict__ sc_col,
-                                                           const int *__restrict__ colsc,
-                                                           const float4 *__restrict__ sctr,
-                                                           const float4 *__restrict__ shalf,
-                                                           const float4 *__restrict__ ctr,
-                                                           const float4 *__restrict__ half,
-                                                           int4 *__restrict__ entries, int *__restrict__ count,
-                                                           int *__restrict__ ecost, int *__restrict__ cost_out,
-                                                           int *__restrict__ maskwork, int *__restrict__ nmaskwork,
-                                                           int maxmaskwork, int *__restrict__ overflow) {
+__device__ __forceinline__ void build_list_body(const int bid_, const int nbk_, ListParams q,
+                                                const int *__restrict__ sc_col, const int *__restrict__ colsc,
+                                                const float4 *__restrict__ sctr, const float4 *__restrict__ shalf,
+                                                const float4 *__restrict__ ctr, const float4 *__restrict__ half,
+                                                int4 *__restrict__ entries, int *__restrict__ count,
+                                                int *__restrict__ ecost, int *__restrict__ cost_out,
+                                                int *__restrict__ maskwork, int *__restrict__ nmaskwork,
+                                                int maxmaskwork, int *__restrict__ overflow) {
   constexpr int NW = kListBlock / kWave;
   constexpr int kScPerBatch = kListBlock / kClusterSize;
   __shared__ int s_cnt[NW];
@@ -702,7 +744,7 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
   __shared__ int s_sc[kListBlock];
   __shared__ int s_ra[kWave], s_rpre[kWave], s_rtot;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int S = blockIdx.x;
+  const int S = bid_;
   const int col = sc_col[S];
   if (col < 0) {  // past the last occupied supercluster: no atoms
     if (threadIdx.x == 0) {
@@ -804,15 +846,30 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const i
   }
 }
 
+__global__ void __launch_bounds__(kListBlock) k_build_list(ListParams q, const int *__restrict__ sc_col,
+                                                           const int *__restrict__ colsc,
+                                                           const float4 *__restrict__ sctr,
+                                                           const float4 *__restrict__ shalf,
+                                                           const float4 *__restrict__ ctr,
+                                                           const float4 *__restrict__ half,
+                                                           int4 *__restrict__ entries, int *__restrict__ count,
+                                                           int *__restrict__ ecost, int *__restrict__ cost_out,
+                                                           int *__restrict__ maskwork, int *__restrict__ nmaskwork,
+                                                           int maxmaskwork, int *__restrict__ overflow) {
+  build_list_body(blockIdx.x, gridDim.x, q, sc_col, colsc, sctr, shalf, ctr, half, entries, count, ecost, cost_out,
+                  maskwork, nmaskwork, maxmaskwork, overflow);
+}
+
 // Flags the list entry that holds each excluded pair (FULL or MODIFIED) so
 // that k_build_masks gives it explicit pair masks.  Entries of supercluster S
 // are sorted by j-cluster, so the entry is found by binary search; pairs whose
 // clusters are not in the list are beyond the list radius and never evaluated.
-__global__ void k_mark_exclusions(int ne, const int4 *__restrict__ etab, const int *__restrict__ slot,
-                                  int4 *__restrict__ entries, const int *__restrict__ count, int cap,
-                                  int *__restrict__ maskwork, int *__restrict__ nmaskwork, int maxmaskwork,
-                                  int *__restrict__ overflow) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void mark_exclusions_body(const int bid_, const int nbk_, int ne,
+                                                     const int4 *__restrict__ etab, const int *__restrict__ slot,
+                                                     int4 *__restrict__ entries, const int *__restrict__ count, int cap,
+                                                     int *__restrict__ maskwork, int *__restrict__ nmaskwork,
+                                                     int maxmaskwork, int *__restrict__ overflow) {
+  const int e = bid_ * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   int hit = -1;  // entry index (S * cap + k) this thread claimed
   if (e < ne) {
@@ -854,6 +911,14 @@ __global__ void k_mark_exclusions(int ne, const int4 *__restrict__ etab, const i
   }
 }
 
+__global__ void k_mark_exclusions(int ne, const int4 *__restrict__ etab, const int *__restrict__ slot,
+                                  int4 *__restrict__ entries, const int *__restrict__ count, int cap,
+                                  int *__restrict__ maskwork, int *__restrict__ nmaskwork, int maxmaskwork,
+                                  int *__restrict__ overflow) {
+  mark_exclusions_body(blockIdx.x, gridDim.x, ne, etab, slot, entries, count, cap, maskwork, nmaskwork, maxmaskwork,
+                       overflow);
+}
+
 // Binary search of b in the sorted exclusion partners of a (original indices).
 __device__ inline int excl_class(const int *__restrict__ eptr, const int2 *__restrict__ elist, int a, int b) {
   int lo = eptr[a], hi = eptr[a + 1] - 1;
@@ -875,16 +940,16 @@ __device__ inline int excl_class(const int *__restrict__ eptr, const int2 *__res
 // atoms (perm >= 0), i precedes j within a shared cluster, and the pair is
 // not FULL-excluded; bit 8 + c marks MODIFIED.  Each lane of the pair kernel
 // loads its own word once per entry.  The entry's mask field becomes w.
-__global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ maskwork, const int *__restrict__ nmaskwork,
-                                                     int maxmaskwork, int4 *__restrict__ entries, int cap,
-                                                     const int *__restrict__ perm, const int *__restrict__ eptr,
-                                                     const int2 *__restrict__ elist,
-                                                     unsigned short *__restrict__ masks) {
+__device__ __forceinline__ void build_masks_body(const int bid_, const int nbk_, const int *__restrict__ maskwork,
+                                                 const int *__restrict__ nmaskwork, int maxmaskwork,
+                                                 int4 *__restrict__ entries, int cap, const int *__restrict__ perm,
+                                                 const int *__restrict__ eptr, const int2 *__restrict__ elist,
+                                                 unsigned short *__restrict__ masks) {
   const int lane = threadIdx.x & 63;
   int nw = *nmaskwork;
   if (nw > maxmaskwork) nw = maxmaskwork;
-  const int nwaves = gridDim.x * (blockDim.x / kWave);
-  for (int w = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6); w < nw; w += nwaves) {
+  const int nwaves = nbk_ * (blockDim.x / kWave);
+  for (int w = bid_ * (blockDim.x / kWave) + (threadIdx.x >> 6); w < nw; w += nwaves) {
     const int eidx = maskwork[w];
     const int S = eidx / cap;
     const int4 ent = entries[eidx];
@@ -936,6 +1001,14 @@ __global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ mas
   }
 }
 
+__global__ void __launch_bounds__(256) k_build_masks(const int *__restrict__ maskwork, const int *__restrict__ nmaskwork,
+                                                     int maxmaskwork, int4 *__restrict__ entries, int cap,
+                                                     const int *__restrict__ perm, const int *__restrict__ eptr,
+                                                     const int2 *__restrict__ elist,
+                                                     unsigned short *__restrict__ masks) {
+  build_masks_body(blockIdx.x, gridDim.x, maskwork, nmaskwork, maxmaskwork, entries, cap, perm, eptr, elist, masks);
+}
+
 // Compact tile list: tiles of up to kTileEntries entries per supercluster.
 template <class T>
 __device__ inline int last_le(const T *__restrict__ a, int n, T v) {
@@ -965,9 +1038,11 @@ __device__ inline int last_le(const T *__restrict__ a, int n, T v) {
 //   k_runs_cut (a block per supercluster): each entry places the run
 //     boundaries b_w that fall in its work interval;
 //   k_runs_tiles (one block): tiles per run, their prefix, the tiles.
-__global__ void k_runs_prefix(int nsuper, const int *__restrict__ count, const int *__restrict__ cost,
-                              int *__restrict__ nruns_p, int *__restrict__ ep, long long *__restrict__ cp,
-                              int *__restrict__ pos, int *__restrict__ posS) {
+__device__ __forceinline__ void runs_prefix_body(const int bid_, const int nbk_, int nsuper,
+                                                 const int *__restrict__ count, const int *__restrict__ cost,
+                                                 int *__restrict__ nruns_p, int *__restrict__ ep,
+                                                 long long *__restrict__ cp, int *__restrict__ pos,
+                                                 int *__restrict__ posS) {
   __shared__ int s_e[1024];
   __shared__ long long s_c[1024];
   __shared__ int s_last[1024 / kWave];
@@ -1019,10 +1094,17 @@ __global__ void k_runs_prefix(int nsuper, const int *__restrict__ count, const i
   }
 }
 
-__global__ void k_runs_cut(int nsuper, const int *__restrict__ count, const int *__restrict__ ecost, int cap,
-                           const int *__restrict__ nruns_p, const int *__restrict__ ep,
-                           const long long *__restrict__ cp, int *__restrict__ pos, int *__restrict__ posS) {
-  const int S = blockIdx.x;
+__global__ void k_runs_prefix(int nsuper, const int *__restrict__ count, const int *__restrict__ cost,
+                              int *__restrict__ nruns_p, int *__restrict__ ep, long long *__restrict__ cp,
+                              int *__restrict__ pos, int *__restrict__ posS) {
+  runs_prefix_body(blockIdx.x, gridDim.x, nsuper, count, cost, nruns_p, ep, cp, pos, posS);
+}
+
+__device__ __forceinline__ void runs_cut_body(const int bid_, const int nbk_, int nsuper, const int *__restrict__ count,
+                                              const int *__restrict__ ecost, int cap, const int *__restrict__ nruns_p,
+                                              const int *__restrict__ ep, const long long *__restrict__ cp,
+                                              int *__restrict__ pos, int *__restrict__ posS) {
+  const int S = bid_;
   const int nruns = *nruns_p;
   const int n = count[S];
   const long long C = cp[nsuper], base = cp[S];
@@ -1041,9 +1123,16 @@ __global__ void k_runs_cut(int nsuper, const int *__restrict__ count, const int 
   }
 }
 
-__global__ void k_runs_tiles(const int *__restrict__ count, const int *__restrict__ ep, const int *__restrict__ pos,
-                             const int *__restrict__ posS, int *__restrict__ wstart, int4 *__restrict__ tiles,
-                             int *__restrict__ ntiles, int ntiles_cap) {
+__global__ void k_runs_cut(int nsuper, const int *__restrict__ count, const int *__restrict__ ecost, int cap,
+                           const int *__restrict__ nruns_p, const int *__restrict__ ep,
+                           const long long *__restrict__ cp, int *__restrict__ pos, int *__restrict__ posS) {
+  runs_cut_body(blockIdx.x, gridDim.x, nsuper, count, ecost, cap, nruns_p, ep, cp, pos, posS);
+}
+
+__device__ __forceinline__ void runs_tiles_body(const int bid_, const int nbk_, const int *__restrict__ count,
+                                                const int *__restrict__ ep, const int *__restrict__ pos,
+                                                const int *__restrict__ posS, int *__restrict__ wstart,
+                                                int4 *__restrict__ tiles, int *__restrict__ ntiles, int ntiles_cap) {
   __shared__ int s_n[1024];
   const int tid = threadIdx.x;
   const int nruns = ntiles[2];  // from k_runs_prefix
@@ -1083,6 +1172,12 @@ __global__ void k_runs_tiles(const int *__restrict__ count, const int *__restric
     wstart[nruns] = min(carry, ntiles_cap);
     ntiles[0] = min(carry, ntiles_cap);
   }
+}
+
+__global__ void k_runs_tiles(const int *__restrict__ count, const int *__restrict__ ep, const int *__restrict__ pos,
+                             const int *__restrict__ posS, int *__restrict__ wstart, int4 *__restrict__ tiles,
+                             int *__restrict__ ntiles, int ntiles_cap) {
+  runs_tiles_body(blockIdx.x, gridDim.x, count, ep, pos, posS, wstart, tiles, ntiles, ntiles_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1377,10 +1472,9 @@ __device__ inline void pair_delta(const R4 &xi, const R4 &xj, const EdgesR<R> &E
 // written by the one wave that takes it).  MODE kPairStats: the pass also
 // counts the in-cutoff pairs and sums their hashes.
 template <bool MIXED, int COUL, int LJSW, int CSW, bool BIGLJ, int MODE>
-__global__ void __launch_bounds__(kPairBlock, 4) k_pair(
-    PairParams pp, int dynamic, const int4 *__restrict__ tiles, const int *__restrict__ ntiles_p,
-    const int *__restrict__ wstart, int *__restrict__ tile_counter, const int4 *__restrict__ entries, int4 *entries_w,
-    int cap,
+__device__ __forceinline__ void pair_body(const int bid_, const int nbk_, PairParams pp, int dynamic,
+    const int4 *__restrict__ tiles, const int *__restrict__ ntiles_p, const int *__restrict__ wstart,
+    int *__restrict__ tile_counter, const int4 *__restrict__ entries, int4 *entries_w, int cap,
     const unsigned short *__restrict__ masks,
     const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
     const float4 *__restrict__ sorg, const int *__restrict__ ts, const int *__restrict__ perm,
@@ -1433,7 +1527,7 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
   const R rmax2 = LJSW == SW_NONE ? sc.rc2 : (COUL == COUL_NONE ? sl.rc2 : (sl.rc2 > sc.rc2 ? sl.rc2 : sc.rc2));
   const R prune_r2 = (R)pp.prune_r2;
   // the boundary-pair sums are filled after this kernel (same stream)
-  if (blockIdx.x == 0 && threadIdx.x < 4) out.border_acc[threadIdx.x] = 0ull;
+  if (bid_ == 0 && threadIdx.x < 4) out.border_acc[threadIdx.x] = 0ull;
   R4 *const si = s_i[wv];
   R *const fr_slot = reinterpret_cast<R *>(s_area[wv]);
   PairQ<R> *const q = reinterpret_cast<PairQ<R> *>(s_area[wv] + PairArea<R>::kSlotBytes);
@@ -1446,8 +1540,8 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
   // run w and then runs in order from a device-wide counter (dynamic == 1).
   // Run r is tiles [wstart[r], wstart[r + 1]).  Which wave takes a run
   // changes no result.
-  const int nwaves = gridDim.x * W;
-  const int gw = blockIdx.x * W + wv;
+  const int nwaves = nbk_ * W;
+  const int gw = bid_ * W + wv;
   const int nruns = ntiles_p[2];
   const int r0 = dynamic ? gw : (int)((long long)gw * nruns / nwaves);
   const int r1 = dynamic ? min(gw + 1, nruns) : (int)((long long)(gw + 1) * nruns / nwaves);
@@ -1727,6 +1821,22 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
     __builtin_amdgcn_wave_barrier();
     ++t;
   }
+}
+
+template <bool MIXED, int COUL, int LJSW, int CSW, bool BIGLJ, int MODE>
+__global__ void __launch_bounds__(kPairBlock, 4) k_pair(
+    PairParams pp, int dynamic, const int4 *__restrict__ tiles, const int *__restrict__ ntiles_p,
+    const int *__restrict__ wstart, int *__restrict__ tile_counter, const int4 *__restrict__ entries, int4 *entries_w,
+    int cap,
+    const unsigned short *__restrict__ masks,
+    const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ xs,
+    const float4 *__restrict__ sorg, const int *__restrict__ ts, const int *__restrict__ perm,
+    const double *__restrict__ ljtab,
+    const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ ljt,
+    typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
+  pair_body<MIXED, COUL, LJSW, CSW, BIGLJ, MODE>(blockIdx.x, gridDim.x, pp, dynamic, tiles, ntiles_p, wstart,
+                                                 tile_counter, entries, entries_w, cap, masks, xs, sorg, ts, perm,
+                                                 ljtab, ljt, facc, out);
 }
 
 // Boundary pairs: exact FP64 membership on the original coordinates
@@ -2011,11 +2121,11 @@ __device__ void spread_patch_group(int s0, const PmeParams &p, int a, const doub
 }
 
 template <bool MIXED, int P>
-__global__ void __launch_bounds__(256) k_spread_patch(int n, PmeParams p, const double *__restrict__ xorig,
-                                                      const int *__restrict__ perm, const double *__restrict__ q,
-                                                      double *__restrict__ grid64,
-                                                      unsigned long long *__restrict__ gridfx) {
-  const int s0 = blockIdx.x * kSpreadSlots;
+__device__ __forceinline__ void spread_patch_body(const int bid_, const int nbk_, int n, PmeParams p,
+                                                  const double *__restrict__ xorig, const int *__restrict__ perm,
+                                                  const double *__restrict__ q, double *__restrict__ grid64,
+                                                  unsigned long long *__restrict__ gridfx) {
+  const int s0 = bid_ * kSpreadSlots;
   const int t = threadIdx.x, s = s0 + t;
   int a = -1;
   double xa[3] = {0, 0, 0};
@@ -2028,6 +2138,14 @@ __global__ void __launch_bounds__(256) k_spread_patch(int n, PmeParams p, const 
     }
   }
   spread_patch_group<MIXED, P>(s0, p, a, xa, q, grid64, gridfx);
+}
+
+template <bool MIXED, int P>
+__global__ void __launch_bounds__(256) k_spread_patch(int n, PmeParams p, const double *__restrict__ xorig,
+                                                      const int *__restrict__ perm, const double *__restrict__ q,
+                                                      double *__restrict__ grid64,
+                                                      unsigned long long *__restrict__ gridfx) {
+  spread_patch_body<MIXED, P>(blockIdx.x, gridDim.x, n, p, xorig, perm, q, grid64, gridfx);
 }
 
 // Charge spreading (Grid::anterpolateCharge, Grid.h:107-154).  The P^3
@@ -2200,13 +2318,14 @@ __device__ Cx<T> *lds_fft(Cx<T> *a, Cx<T> *b, int nb, const FftPlan &pl, const C
 // z pass, real -> half complex, for the TY lines (i, j0 .. j0 + TY - 1):
 // the grid points are read once and cleared for the next spread; out: SA.
 template <class T, bool FIXED>
-__global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int TY, double *__restrict__ g64,
-                            unsigned long long *__restrict__ gfx, const Cx<T> *__restrict__ tw, Cx<T> *__restrict__ sa) {
+__device__ __forceinline__ void fft_z_r2c_body(const int bid_, const int nbk_, PmeParams p, FftPlan pl, int TY,
+                                               double *__restrict__ g64, unsigned long long *__restrict__ gfx,
+                                               const Cx<T> *__restrict__ tw, Cx<T> *__restrict__ sa) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
   Cx<T> *b = a + TY * p.nz;
   const int jt = (p.ny + TY - 1) / TY;
-  const int i = blockIdx.x / jt, j0 = (blockIdx.x - i * jt) * TY;
+  const int i = bid_ / jt, j0 = (bid_ - i * jt) * TY;
   const int nb = min(TY, p.ny - j0);
   const size_t base = ((size_t)i * p.ny + j0) * p.nz;
   for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) {
@@ -2228,16 +2347,23 @@ __global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int TY, double *__restrict_
   }
 }
 
+template <class T, bool FIXED>
+__global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int TY, double *__restrict__ g64,
+                            unsigned long long *__restrict__ gfx, const Cx<T> *__restrict__ tw, Cx<T> *__restrict__ sa) {
+  fft_z_r2c_body<T, FIXED>(blockIdx.x, gridDim.x, p, pl, TY, g64, gfx, tw, sa);
+}
+
 // y pass for the TI lines (i0 .. i0 + TI - 1, k): forward SA -> SB; inverse
 // (conj-in / conj-out around the forward transform) SB -> SA.
 template <class T>
-__global__ void k_fft_y(PmeParams p, FftPlan pl, int TI, bool inverse, const Cx<T> *__restrict__ tw,
-                        const Cx<T> *__restrict__ src, Cx<T> *__restrict__ dst) {
+__device__ __forceinline__ void fft_y_body(const int bid_, const int nbk_, PmeParams p, FftPlan pl, int TI,
+                                           bool inverse, const Cx<T> *__restrict__ tw, const Cx<T> *__restrict__ src,
+                                           Cx<T> *__restrict__ dst) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
   Cx<T> *b = a + TI * p.ny;
   const int it = (p.nx + TI - 1) / TI;
-  const int k = blockIdx.x / it, i0 = (blockIdx.x - k * it) * TI;
+  const int k = bid_ / it, i0 = (bid_ - k * it) * TI;
   const int nb = min(TI, p.nx - i0);
   if (!inverse) {
     for (int t = threadIdx.x; t < nb * p.ny; t += blockDim.x) {
@@ -2269,18 +2395,25 @@ __global__ void k_fft_y(PmeParams p, FftPlan pl, int TI, bool inverse, const Cx<
   }
 }
 
+template <class T>
+__global__ void k_fft_y(PmeParams p, FftPlan pl, int TI, bool inverse, const Cx<T> *__restrict__ tw,
+                        const Cx<T> *__restrict__ src, Cx<T> *__restrict__ dst) {
+  fft_y_body<T>(blockIdx.x, gridDim.x, p, pl, TI, inverse, tw, src, dst);
+}
+
 // x pass for the TJ lines (k, j0 .. j0 + TJ - 1) of SB, in place: forward
 // FFT, influence function multiply with the energy (Grid.h:237-341; theta in
 // the SB layout), inverse FFT.
 template <class T>
-__global__ void k_fft_x_conv(PmeParams p, FftPlan pl, int TJ, const Cx<T> *__restrict__ tw,
-                             const T *__restrict__ theta, Cx<T> *__restrict__ sb, double *__restrict__ epart) {
+__device__ __forceinline__ void fft_x_conv_body(const int bid_, const int nbk_, PmeParams p, FftPlan pl, int TJ,
+                                                const Cx<T> *__restrict__ tw, const T *__restrict__ theta,
+                                                Cx<T> *__restrict__ sb, double *__restrict__ epart) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
   Cx<T> *b = a + TJ * p.nx;
   __shared__ double s_e[1024 / kWave];
   const int jt = (p.ny + TJ - 1) / TJ;
-  const int k = blockIdx.x / jt, j0 = (blockIdx.x - k * jt) * TJ;
+  const int k = bid_ / jt, j0 = (bid_ - k * jt) * TJ;
   const int nb = min(TJ, p.ny - j0);
   const size_t base = ((size_t)k * p.ny + j0) * p.nx;
   for (int t = threadIdx.x; t < nb * p.nx; t += blockDim.x) a[t] = sb[base + t];
@@ -2309,8 +2442,14 @@ __global__ void k_fft_x_conv(PmeParams p, FftPlan pl, int TJ, const Cx<T> *__res
   if (threadIdx.x == 0) {
     double tot = 0;
     for (int w = 0; w < (int)(blockDim.x / kWave); ++w) tot += s_e[w];
-    epart[blockIdx.x] = tot;
+    epart[bid_] = tot;
   }
+}
+
+template <class T>
+__global__ void k_fft_x_conv(PmeParams p, FftPlan pl, int TJ, const Cx<T> *__restrict__ tw,
+                             const T *__restrict__ theta, Cx<T> *__restrict__ sb, double *__restrict__ epart) {
+  fft_x_conv_body<T>(blockIdx.x, gridDim.x, p, pl, TJ, tw, theta, sb, epart);
 }
 
 // Largest plane (nx ny points) that k_fft_plane_conv takes: two LDS copies
@@ -2328,16 +2467,16 @@ constexpr int kFftPlanePoints = 1024;
 // written back to SA.  The conj-out of the inverse x transform and the
 // conj-in of the inverse y transform cancel, so only the last conj remains.
 template <class T>
-__global__ void __launch_bounds__(256) k_fft_plane_conv(PmeParams p, FftPlan ply, FftPlan plx,
-                                                        const Cx<T> *__restrict__ twy, const Cx<T> *__restrict__ twx,
-                                                        const T *__restrict__ theta, Cx<T> *__restrict__ sa,
-                                                        double *__restrict__ epart) {
+__device__ __forceinline__ void fft_plane_conv_body(const int bid_, const int nbk_, PmeParams p, FftPlan ply,
+                                                    FftPlan plx, const Cx<T> *__restrict__ twy,
+                                                    const Cx<T> *__restrict__ twx, const T *__restrict__ theta,
+                                                    Cx<T> *__restrict__ sa, double *__restrict__ epart) {
   extern __shared__ unsigned char smem_raw[];
   const int nx = p.nx, ny = p.ny, nxy = nx * ny;
   Cx<T> *const a = reinterpret_cast<Cx<T> *>(smem_raw);
   Cx<T> *const b = a + nxy;
   __shared__ double s_e[256 / kWave];
-  const int k = blockIdx.x;
+  const int k = bid_;
   for (int t = threadIdx.x; t < nxy; t += blockDim.x) {
     const int i = t / ny, j = t - i * ny;
     a[t] = sa[((size_t)i * p.nzh + k) * ny + j];
@@ -2389,16 +2528,25 @@ __global__ void __launch_bounds__(256) k_fft_plane_conv(PmeParams p, FftPlan ply
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(256) k_fft_plane_conv(PmeParams p, FftPlan ply, FftPlan plx,
+                                                        const Cx<T> *__restrict__ twy, const Cx<T> *__restrict__ twx,
+                                                        const T *__restrict__ theta, Cx<T> *__restrict__ sa,
+                                                        double *__restrict__ epart) {
+  fft_plane_conv_body<T>(blockIdx.x, gridDim.x, p, ply, plx, twy, twx, theta, sa, epart);
+}
+
 // z pass, half complex -> real for the TY lines (i, j0 .. j0 + TY - 1) of SA
 // (Hermitian completion along z, inverse sign); out: the real potential.
 template <class T>
-__global__ void k_fft_z_c2r(PmeParams p, FftPlan pl, int TY, const Cx<T> *__restrict__ tw,
-                            const Cx<T> *__restrict__ sa, T *__restrict__ greal) {
+__device__ __forceinline__ void fft_z_c2r_body(const int bid_, const int nbk_, PmeParams p, FftPlan pl, int TY,
+                                               const Cx<T> *__restrict__ tw, const Cx<T> *__restrict__ sa,
+                                               T *__restrict__ greal) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
   Cx<T> *b = a + TY * p.nz;
   const int jt = (p.ny + TY - 1) / TY;
-  const int i = blockIdx.x / jt, j0 = (blockIdx.x - i * jt) * TY;
+  const int i = bid_ / jt, j0 = (bid_ - i * jt) * TY;
   const int nb = min(TY, p.ny - j0);
   for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) {
     const int k = t / nb, jj = t - k * nb;
@@ -2415,6 +2563,12 @@ __global__ void k_fft_z_c2r(PmeParams p, FftPlan pl, int TY, const Cx<T> *__rest
   const Cx<T> *r = lds_fft<T>(a, b, nb, pl, tw);
   const size_t base = ((size_t)i * p.ny + j0) * p.nz;
   for (int t = threadIdx.x; t < nb * p.nz; t += blockDim.x) greal[base + t] = r[t].x;
+}
+
+template <class T>
+__global__ void k_fft_z_c2r(PmeParams p, FftPlan pl, int TY, const Cx<T> *__restrict__ tw,
+                            const Cx<T> *__restrict__ sa, T *__restrict__ greal) {
+  fft_z_c2r_body<T>(blockIdx.x, gridDim.x, p, pl, TY, tw, sa, greal);
 }
 
 // Force interpolation (Grid.h:159-189) into the sorted accumulators.  Lane
@@ -2482,11 +2636,10 @@ __global__ void __launch_bounds__(256) k_gather(int n, int waves, PmeParams p, c
 // the potential from the grid directly.  The grid is read about once per
 // block instead of once per atom's stencil row.
 template <bool MIXED, int P>
-__global__ void __launch_bounds__(256) k_gather_patch(int n, PmeParams p, const double *__restrict__ xorig,
-                                                      const int *__restrict__ perm, const double *__restrict__ q,
-                                                      const typename std::conditional<MIXED, float, double>::type
-                                                          *__restrict__ greal,
-                                                      typename AccT<MIXED>::type *__restrict__ facc) {
+__device__ __forceinline__ void gather_patch_body(const int bid_, const int nbk_, int n, PmeParams p,
+    const double *__restrict__ xorig, const int *__restrict__ perm, const double *__restrict__ q,
+    const typename std::conditional<MIXED, float, double>::type *__restrict__ greal,
+    typename AccT<MIXED>::type *__restrict__ facc) {
   using R = typename std::conditional<MIXED, float, double>::type;
   constexpr int PMAX = spread_patch_max<P>();
   constexpr int LPA = 8;  // lanes per atom
@@ -2496,7 +2649,7 @@ __global__ void __launch_bounds__(256) k_gather_patch(int n, PmeParams p, const 
   __shared__ R s_w[kSpreadSlots][3];
   __shared__ int s_org[3], s_ext[3], s_direct, s_any;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int s0 = blockIdx.x * kSpreadSlots;
+  const int s0 = bid_ * kSpreadSlots;
   const int K[3] = {p.nx, p.ny, p.nz};
   if (wv == 0) {
     const int t = lane;
@@ -2614,6 +2767,15 @@ __global__ void __launch_bounds__(256) k_gather_patch(int n, PmeParams p, const 
   }
 }
 
+template <bool MIXED, int P>
+__global__ void __launch_bounds__(256) k_gather_patch(int n, PmeParams p, const double *__restrict__ xorig,
+                                                      const int *__restrict__ perm, const double *__restrict__ q,
+                                                      const typename std::conditional<MIXED, float, double>::type
+                                                          *__restrict__ greal,
+                                                      typename AccT<MIXED>::type *__restrict__ facc) {
+  gather_patch_body<MIXED, P>(blockIdx.x, gridDim.x, n, p, xorig, perm, q, greal, facc);
+}
+
 // ---------------------------------------------------------------------------
 // 5. Exclusion correction (NonbondedPMEwaldSystemForce.h:874-911), FP64, exact erf.
 // One block of 256 entries (block blk of the correction range).
@@ -2672,9 +2834,11 @@ __device__ void correction_block(int blk, int ne, const int4 *__restrict__ etab,
 // starts from zero without a separate clearing launch (padding slots are
 // never accumulated into).
 template <bool MIXED>
-__global__ void k_unsort_forces(int n, const int *__restrict__ slot, typename AccT<MIXED>::type *__restrict__ facc,
-                                double *__restrict__ fout, int *__restrict__ tilectr, int *__restrict__ nborder) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void unsort_forces_body(const int bid_, const int nbk_, int n, const int *__restrict__ slot,
+                                                   typename AccT<MIXED>::type *__restrict__ facc,
+                                                   double *__restrict__ fout, int *__restrict__ tilectr,
+                                                   int *__restrict__ nborder) {
+  int a = bid_ * blockDim.x + threadIdx.x;
   if (a == 0) {
     *tilectr = 0;
     *nborder = 0;
@@ -2688,6 +2852,12 @@ __global__ void k_unsort_forces(int n, const int *__restrict__ slot, typename Ac
       fout[3 * a + k] = ((const double *)facc)[3 * s + k];
     facc[3 * s + k] = 0;
   }
+}
+
+template <bool MIXED>
+__global__ void k_unsort_forces(int n, const int *__restrict__ slot, typename AccT<MIXED>::type *__restrict__ facc,
+                                double *__restrict__ fout, int *__restrict__ tilectr, int *__restrict__ nborder) {
+  unsort_forces_body<MIXED>(blockIdx.x, gridDim.x, n, slot, facc, fout, tilectr, nborder);
 }
 
 // Deterministic energy/pair-stat reduction in one block, in index order.
@@ -2910,8 +3080,8 @@ struct FixupArgs {
 };
 
 template <bool MIXED, int COUL, int LJSW, int CSW>
-__global__ void __launch_bounds__(256) k_fixups(FixupArgs<MIXED> a) {
-  int b = blockIdx.x;
+__device__ __forceinline__ void fixups_body(const int bid_, const int nbk_, FixupArgs<MIXED> a) {
+  int b = bid_;
   if (b < a.border_blocks) {
     border_pairs<MIXED, COUL, LJSW, CSW>(b, a.border_blocks, a.pp, a.stats, a.border, a.nborder, a.maxborder, a.xs,
                                          a.sorg, a.ts, a.perm, a.xorig, a.ljtab, a.eptr, a.elist, a.facc, a.border_e,
@@ -2926,6 +3096,11 @@ __global__ void __launch_bounds__(256) k_fixups(FixupArgs<MIXED> a) {
   b -= a.corr_blocks;
   bonded_block<MIXED>(b, a.nbond, a.nangle, a.bonds, a.bpar, a.angles, a.apar, a.xorig, a.slot, a.pp.box, a.facc,
                       a.bond_e);
+}
+
+template <bool MIXED, int COUL, int LJSW, int CSW>
+__global__ void __launch_bounds__(256) k_fixups(FixupArgs<MIXED> a) {
+  fixups_body<MIXED, COUL, LJSW, CSW>(blockIdx.x, gridDim.x, a);
 }
 
 // Velocity kicks of one leapfrog step.  Leapfrog (LeapfrogIntegrator.cpp):
@@ -2966,12 +3141,14 @@ __device__ inline void apply_kick(const Kick &kk, int a, const double f[3], doub
 // build (xs64), since list entries carry a fixed image shift, and the largest
 // squared displacement since that build is folded into dmax (float bits).
 template <class R>
-__global__ void k_kick_drift(int n, Kick kk, double h, const int *__restrict__ perm,
-                             const double *__restrict__ fout, const double *__restrict__ minv, double *__restrict__ v,
-                             double *__restrict__ x, const double *__restrict__ xbuild,
-                             typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64,
-                             const float4 *__restrict__ sorg, unsigned int *__restrict__ dmax) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void kick_drift_body(const int bid_, const int nbk_, int n, Kick kk, double h,
+                                                const int *__restrict__ perm, const double *__restrict__ fout,
+                                                const double *__restrict__ minv, double *__restrict__ v,
+                                                double *__restrict__ x, const double *__restrict__ xbuild,
+                                                typename Real4<R>::type *__restrict__ xs,
+                                                const double4 *__restrict__ xs64, const float4 *__restrict__ sorg,
+                                                unsigned int *__restrict__ dmax) {
+  const int s = bid_ * blockDim.x + threadIdx.x;
   float d2f = 0.f;
   const int a = s < n ? perm[s] : -1;  // n counts slots; padding slots have a < 0
   if (a >= 0) {
@@ -2998,6 +3175,15 @@ __global__ void k_kick_drift(int n, Kick kk, double h, const int *__restrict__ p
   // wave max, one atomic per wave
   for (int m = 32; m >= 1; m >>= 1) d2f = fmaxf(d2f, __shfl_xor(d2f, m));
   if ((threadIdx.x & 63) == 0) atomicMax(dmax, __float_as_uint(d2f));
+}
+
+template <class R>
+__global__ void k_kick_drift(int n, Kick kk, double h, const int *__restrict__ perm,
+                             const double *__restrict__ fout, const double *__restrict__ minv, double *__restrict__ v,
+                             double *__restrict__ x, const double *__restrict__ xbuild,
+                             typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64,
+                             const float4 *__restrict__ sorg, unsigned int *__restrict__ dmax) {
+  kick_drift_body<R>(blockIdx.x, gridDim.x, n, kk, h, perm, fout, minv, v, x, xbuild, xs, xs64, sorg, dmax);
 }
 
 // The end of one MD step and the start of the next in one pass over the
@@ -3060,16 +3246,22 @@ __global__ void k_unsort_kick_drift(int n, Kick kk, double h, const int *__restr
 // cleared by the previous pass).  Used inside an MD chunk, where the next
 // pass spreads these positions.
 template <bool MIXED, class R, int P>
-__global__ void __launch_bounds__(256) k_unsort_kick_drift_spread(
-    int n, Kick kk, double h, const int *__restrict__ perm, typename AccT<MIXED>::type *__restrict__ facc,
-    double *__restrict__ fout, int *__restrict__ tilectr, int *__restrict__ nborder, const double *__restrict__ minv,
-    double *__restrict__ v, double *__restrict__ x, const double *__restrict__ xbuild,
-    typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64, const float4 *__restrict__ sorg,
-    unsigned int *__restrict__ dmax, PmeParams pme, const double *__restrict__ q, double *__restrict__ grid64,
-    unsigned long long *__restrict__ gridfx) {
-  const int s0 = blockIdx.x * kSpreadSlots;
+__device__ __forceinline__ void unsort_kick_drift_spread_body(const int bid_, const int nbk_, int n, Kick kk, double h,
+                                                              const int *__restrict__ perm,
+                                                              typename AccT<MIXED>::type *__restrict__ facc,
+                                                              double *__restrict__ fout, int *__restrict__ tilectr,
+                                                              int *__restrict__ nborder,
+                                                              const double *__restrict__ minv, double *__restrict__ v,
+                                                              double *__restrict__ x, const double *__restrict__ xbuild,
+                                                              typename Real4<R>::type *__restrict__ xs,
+                                                              const double4 *__restrict__ xs64,
+                                                              const float4 *__restrict__ sorg,
+                                                              unsigned int *__restrict__ dmax, PmeParams pme,
+                                                              const double *__restrict__ q, double *__restrict__ grid64,
+                                                              unsigned long long *__restrict__ gridfx) {
+  const int s0 = bid_ * kSpreadSlots;
   const int t = threadIdx.x, s = s0 + t;
-  if (blockIdx.x == 0 && t == 0) {
+  if (bid_ == 0 && t == 0) {
     *tilectr = 0;
     *nborder = 0;
   }
@@ -3115,14 +3307,27 @@ __global__ void __launch_bounds__(256) k_unsort_kick_drift_spread(
   spread_patch_group<MIXED, P>(s0, pme, a, xa, q, grid64, gridfx);
 }
 
+template <bool MIXED, class R, int P>
+__global__ void __launch_bounds__(256) k_unsort_kick_drift_spread(
+    int n, Kick kk, double h, const int *__restrict__ perm, typename AccT<MIXED>::type *__restrict__ facc,
+    double *__restrict__ fout, int *__restrict__ tilectr, int *__restrict__ nborder, const double *__restrict__ minv,
+    double *__restrict__ v, double *__restrict__ x, const double *__restrict__ xbuild,
+    typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64, const float4 *__restrict__ sorg,
+    unsigned int *__restrict__ dmax, PmeParams pme, const double *__restrict__ q, double *__restrict__ grid64,
+    unsigned long long *__restrict__ gridfx) {
+  unsort_kick_drift_spread_body<MIXED, R, P>(blockIdx.x, gridDim.x, n, kk, h, perm, facc, fout, tilectr, nborder, minv,
+                                             v, x, xbuild, xs, xs64, sorg, dmax, pme, q, grid64, gridfx);
+}
+
 // Start of an MD chunk in one launch: the committed state (x, v, f: m3 =
 // 3N doubles each) is backed up for a rollback, the overflow flags (2) and
 // the chunk's displacement record (nrec) are cleared.
-__global__ void k_chunk_begin(long long m3, const double *__restrict__ x, const double *__restrict__ v,
-                              const double *__restrict__ f, double *__restrict__ xbak, double *__restrict__ vbak,
-                              double *__restrict__ fbak, int *__restrict__ overflow, unsigned *__restrict__ dmax,
-                              int nrec) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void chunk_begin_body(const int bid_, const int nbk_, long long m3,
+                                                 const double *__restrict__ x, const double *__restrict__ v,
+                                                 const double *__restrict__ f, double *__restrict__ xbak,
+                                                 double *__restrict__ vbak, double *__restrict__ fbak,
+                                                 int *__restrict__ overflow, unsigned *__restrict__ dmax, int nrec) {
+  const long long i = (long long)bid_ * blockDim.x + threadIdx.x;
   if (i < m3) {
     xbak[i] = x[i];
     vbak[i] = v[i];
@@ -3132,16 +3337,29 @@ __global__ void k_chunk_begin(long long m3, const double *__restrict__ x, const 
   if (i < nrec) dmax[i] = 0u;
 }
 
+__global__ void k_chunk_begin(long long m3, const double *__restrict__ x, const double *__restrict__ v,
+                              const double *__restrict__ f, double *__restrict__ xbak, double *__restrict__ vbak,
+                              double *__restrict__ fbak, int *__restrict__ overflow, unsigned *__restrict__ dmax,
+                              int nrec) {
+  chunk_begin_body(blockIdx.x, gridDim.x, m3, x, v, f, xbak, vbak, fbak, overflow, dmax, nrec);
+}
+
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
-__global__ void k_half_kick(int n, Kick kk, const double *__restrict__ fout, const double *__restrict__ minv,
-                            double *__restrict__ v) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void half_kick_body(const int bid_, const int nbk_, int n, Kick kk,
+                                               const double *__restrict__ fout, const double *__restrict__ minv,
+                                               double *__restrict__ v) {
+  const int a = bid_ * blockDim.x + threadIdx.x;
   if (a >= n) return;
   double vv[3] = {v[3 * a], v[3 * a + 1], v[3 * a + 2]};
   const double ff[3] = {fout[3 * a], fout[3 * a + 1], fout[3 * a + 2]};
   apply_kick(kk, a, ff, minv[a], vv);
   for (int k = 0; k < 3; ++k) v[3 * a + k] = vv[k];
+}
+
+__global__ void k_half_kick(int n, Kick kk, const double *__restrict__ fout, const double *__restrict__ minv,
+                            double *__restrict__ v) {
+  half_kick_body(blockIdx.x, gridDim.x, n, kk, fout, minv, v);
 }
 
 // Kinetic energy 0.5 sum m v^2 into per-block partials.

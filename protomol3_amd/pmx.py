@@ -67,7 +67,7 @@ EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_crea
             "pmx_sys_cells", "pmx_sys_stage_times", "pmx_sys_timing_begin", "pmx_sys_timing_end",
             "pmx_md_init", "pmx_md_set_state", "pmx_md_run", "pmx_md_get_state", "pmx_md_stats",
             "pmx_langevin_noise", "pmx_sys_device_bytes", "pmx_estimate_bytes", "pmx_md_run_many",
-            "pmx_sys_time_pair", "pmx_sys_set_tuning", "pmx_sys_compute_noalloc"]
+            "pmx_sys_time_pair", "pmx_sys_set_tuning", "pmx_sys_compute_noalloc", "pmx_md_run_batch"]
 
 _lib = None
 
@@ -107,6 +107,7 @@ def load() -> C.CDLL:
     L.pmx_md_set_state.argtypes = [vp, dp, dp]
     L.pmx_md_run.argtypes = [vp, C.c_int]
     L.pmx_md_run_many.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
+    L.pmx_md_run_batch.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
     L.pmx_sys_time_pair.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
     L.pmx_sys_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.pmx_md_get_state.argtypes = [vp, dp, dp, dp]
@@ -283,6 +284,18 @@ def run_many(systems_: list, nsteps: int) -> None:
     if rc:
         msgs = {sy.ctx.lib.pmx_last_error(sy.ctx.h).decode() for sy in systems_}
         raise PmxError(rc, "pmx_md_run_many: " + "; ".join(m for m in msgs if m))
+
+
+def run_batch(systems_: list, nsteps: int) -> None:
+    """pmx_md_run_batch: nsteps MD steps on every System (each after
+    md_init and md_set_state; one Context, one precision, the same kinds of
+    force, PME of one stencil order), every kernel of a step launched once
+    for all of them."""
+    if not systems_:
+        return
+    ctx = systems_[0].ctx
+    arr = (C.c_void_p * len(systems_))(*[sy.h for sy in systems_])
+    ctx.check(ctx.lib.pmx_md_run_batch(arr, len(systems_), nsteps), "pmx_md_run_batch")
 
 
 def estimate_bytes(s: MolSystem, spec: ForceSpec, precision: int = FP64, trajectory: bool = False) -> int:

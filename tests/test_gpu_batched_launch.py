@@ -1,0 +1,127 @@
+"""Batched launches: pmx_md_run_batch runs every kernel of an MD step once for
+many systems of uneven size.
+
+* A system's trajectory does not depend on the batch it runs in: each system
+  run in a batch of one gives, in the mixed form, the same positions,
+  velocities and energies to the last bit as in the batch of all.
+* The trajectory is pmx_md_run's: in the FP64 form to 1e-9 (the two differ only
+  in the steps at which the list is rebuilt, which FP64 forces do not see,
+  and in the order of double atomics); in the mixed form to 1e-6 A after 60
+  steps (its FP32 tile sums follow the list).
+* The energies the batch leaves are those of a fresh evaluation at the final
+  positions (the device-side list rebuilds kept every list valid).
+* Batches that mix precisions or force kinds, repeat a system or hold a
+  non-PME system are refused."""
+import numpy as np
+import pytest
+
+from protomol3_amd import pmx, serve, systems
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [600, 1500, 2500, 4000, 900, 3100, 1200, 7000]
+STEPS = 60
+
+
+def _systems():
+    out = []
+    for i, n in enumerate(SIZES):
+        s = systems.replica_box(n, 300 + i).with_velocities(300.0, seed=400 + i)
+        out.append((s, serve.request_spec(s)))
+    return out
+
+
+def _make(ctx, s, spec, precision, seed, skin=1.0):
+    sy = pmx.System(ctx, s, spec, precision)
+    sy.md_init(s, 1.0, skin=skin, integrator=pmx.LANGEVIN, temperature=300.0, gamma=1.0, seed=seed)
+    sy.md_set_state(s.pos, s.vel)
+    return sy
+
+
+def _run(ctx, reqs, precision, how, idx=None):
+    idx = range(len(reqs)) if idx is None else idx
+    sys_ = [_make(ctx, *reqs[i], precision, 50 + i) for i in idx]
+    try:
+        if how == "batch":
+            pmx.run_batch(sys_, STEPS)
+        elif how == "alone_batch":
+            for sy in sys_:
+                pmx.run_batch([sy], STEPS)
+        else:
+            for sy in sys_:
+                sy.md_run(STEPS)
+        return [sy.md_get_state() for sy in sys_], [sy.md_stats() for sy in sys_]
+    finally:
+        for sy in sys_:
+            sy.close()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pmx.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("precision", [pmx.MIXED, pmx.FP64])
+def test_batch_matches_batch_of_one_and_md_run(ctx, precision):
+    reqs = _systems()
+    batch, bstats = _run(ctx, reqs, precision, "batch")
+    ones, _ = _run(ctx, reqs, precision, "alone_batch")
+    chunked, cstats = _run(ctx, reqs, precision, "md_run")
+    tol_one = 0.0 if precision == pmx.MIXED else 1e-9
+    tol_run = 1e-6 if precision == pmx.MIXED else 1e-9
+    for i, ((xb, vb, eb), (xo, vo, eo), (xr, vr, er)) in enumerate(zip(batch, ones, chunked)):
+        dxo, dxr = np.abs(xb - xo).max(), np.abs(xb - xr).max()
+        dvr = np.abs(vb - vr).max()
+        print(f"system {i} n={SIZES[i]} prec={precision} vs one {dxo:.3e} vs md_run {dxr:.3e} dv {dvr:.3e} "
+              f"stats {bstats[i]} md_run {cstats[i]}\n  energies batch {eb}\n  md_run {er}", flush=True)
+        assert bstats[i]["steps"] == STEPS and bstats[i]["rebuilds"] > 0
+        assert dxo <= tol_one * max(1.0, np.abs(xb).max())
+        assert np.abs(vb - vo).max() <= tol_one * max(1.0, np.abs(vb).max())
+        if precision == pmx.MIXED:
+            for k in eb:
+                assert eb[k] == eo[k], k
+        assert dxr <= tol_run * max(1.0, np.abs(xr).max())
+        assert dvr <= 10 * tol_run * max(1.0, np.abs(vr).max())
+        # terms of opposite sign can leave a small total (the Coulomb energy
+        # of these boxes): the bound is relative to the sum of the magnitudes
+        scale = sum(abs(e) for e in er.values())
+        for k in eb:
+            assert abs(eb[k] - er[k]) <= 1e-6 * scale, k
+
+
+def test_batch_energies_are_those_of_the_final_positions(ctx):
+    reqs = _systems()[:4]
+    sys_ = [_make(ctx, *reqs[i], pmx.FP64, 70 + i, skin=0.6) for i in range(len(reqs))]
+    try:
+        pmx.run_batch(sys_, 45)
+        states = [sy.md_get_state() for sy in sys_]
+    finally:
+        for sy in sys_:
+            sy.close()
+    for (s, spec), (x, v, e) in zip(reqs, states):
+        sy = pmx.System(ctx, s, spec, pmx.FP64)
+        try:
+            _, ec, elj = sy.evaluate(x)
+        finally:
+            sy.close()
+        print(f"n={s.n} batch Ec {e['coulomb']:.10g} fresh {ec:.10g} Elj {e['lj']:.10g} fresh {elj:.10g}", flush=True)
+        assert abs(e["coulomb"] - ec) <= 1e-9 * abs(ec)
+        assert abs(e["lj"] - elj) <= 1e-9 * max(1.0, abs(elj))
+
+
+def test_batch_refuses_mixed_batches(ctx):
+    reqs = _systems()[:2]
+    a = _make(ctx, *reqs[0], pmx.MIXED, 1)
+    b = _make(ctx, *reqs[1], pmx.FP64, 2)
+    s, _ = reqs[0]
+    c = _make(ctx, s, pmx.ForceSpec(5.0, lj=(pmx.SW_C2, 7.0, 9.0), coul=(pmx.SW_C2, 7.0, 9.0)), pmx.MIXED, 3)
+    try:
+        for bad in ([a, b], [a, a], [a, c]):
+            with pytest.raises(pmx.PmxError):
+                pmx.run_batch(bad, 5)
+        pmx.run_batch([a], 0)
+    finally:
+        for sy in (a, b, c):
+            sy.close()
