@@ -186,34 +186,47 @@ def water1536_leg(ctx, steps: int = 1000, cpu_steps: int = 20, log=print):
 
 def copies(s, spec, ncopies: int, steps: int, lanes: int, skin: float, log=print):
     """ncopies independent copies of the 26,000-atom workload (same start
-    positions, velocities from per-copy seeds), mixed form, leapfrog NVE,
-    `steps` steps each, run by one pmx_md_run_many call over `lanes`
-    contexts; aggregate ns/day = copies x simulated time / wall time."""
+    positions, velocities from per-copy seeds), leapfrog NVE, `steps` steps
+    each: in the mixed form by one pmx_md_run_batch call (every kernel of a
+    step launched once for all copies) and by one pmx_md_run_many call over
+    `lanes` contexts (each copy's kernels on its context's streams), and in
+    the FP64 form batched; aggregate ns/day = copies x simulated time / wall
+    time."""
     from protomol3_amd import pmx
     v0 = s.vel
     vels = [s.with_velocities(300.0, seed=100 + i).vel.copy() for i in range(ncopies)]
     s.vel = v0
-    ctxs = [pmx.Context(0) for _ in range(lanes)]
-    sys_ = []
-    try:
-        for i, vel in enumerate(vels):
-            sy = pmx.System(ctxs[i % lanes], s, spec, pmx.MIXED)
-            sys_.append(sy)
-            sy.md_init(s, TIMESTEP_FS, skin=skin)
-            sy.md_set_state(s.pos, vel)
-        pmx.run_many(sys_, 20)
-        t0 = time.perf_counter()
-        pmx.run_many(sys_, steps)
-        wall = time.perf_counter() - t0
-    finally:
-        for sy in sys_:
-            sy.close()
-        for c in ctxs:
-            c.close()
-    agg = ncopies * steps * TIMESTEP_FS * 1e-6 / wall * 86400.0
-    log(f"copies: {ncopies} x {s.n} atoms over {lanes} contexts: {agg:.1f} ns/day aggregate ({wall:.2f} s)")
-    return {"copies": ncopies, "lanes": lanes, "steps": steps, "skin_A": skin, "wall_s": round(wall, 3),
-            "aggregate_ns_per_day": round(agg, 3)}
+
+    def leg(precision, how):
+        ctxs = [pmx.Context(0) for _ in range(lanes if how == "many" else 1)]
+        sys_ = []
+        try:
+            for i, vel in enumerate(vels):
+                sy = pmx.System(ctxs[i % len(ctxs)], s, spec, precision)
+                sys_.append(sy)
+                sy.md_init(s, TIMESTEP_FS, skin=skin)
+                sy.md_set_state(s.pos, vel)
+            run = pmx.run_batch if how == "batch" else pmx.run_many
+            run(sys_, 20)
+            t0 = time.perf_counter()
+            run(sys_, steps)
+            wall = time.perf_counter() - t0
+        finally:
+            for sy in sys_:
+                sy.close()
+            for c in ctxs:
+                c.close()
+        return wall, ncopies * steps * TIMESTEP_FS * 1e-6 / wall * 86400.0
+
+    wall, agg = leg(pmx.MIXED, "batch")
+    _, agg_many = leg(pmx.MIXED, "many")
+    _, agg64 = leg(pmx.FP64, "batch")
+    log(f"copies: {ncopies} x {s.n} atoms batched: mixed {agg:.1f} ns/day aggregate ({wall:.2f} s), FP64 "
+        f"{agg64:.1f}; over {lanes} contexts {agg_many:.1f}")
+    return {"copies": ncopies, "steps": steps, "skin_A": skin, "wall_s": round(wall, 3),
+            "aggregate_ns_per_day": round(agg, 3), "fp64_aggregate_ns_per_day": round(agg64, 3),
+            "mixed_over_fp64_batched": round(agg / agg64, 3),
+            "per_context_streams": {"lanes": lanes, "aggregate_ns_per_day": round(agg_many, 3)}}
 
 
 def drift_study(ctx, s, spec, ps: float = 10.0, log=print):
@@ -340,12 +353,15 @@ def serving(nreq: int, slots: int, seed: int = 7, pool: int = 256, log=print):
                         "Poisson arrivals at 75% of measured capacity"}
 
 
-def replicas(nsys: int, steps: int, lanes: int, seed: int = 11, skin: float = 1.0, graphs: bool = False, log=print):
+def replicas(nsys: int, steps: int, lanes: int, seed: int = 11, skin: float = 1.0, graphs: bool = False,
+             how: str = "batch", precision: int = 1, log=print):
     """Batched replicas: nsys independent systems with atom counts drawn
     log-uniform in [500, 20,000] (water + 5-10% bead-polymer solute at 0.0334
     atoms/A^3, per-system seeds), LJ C2 7-9 A + PME ~1 A grid, mixed
     precision, `steps` Langevin steps (300 K, 1/ps) each, all run by one
-    pmx_md_run_many call over `lanes` contexts."""
+    pmx_md_run_batch call (how="batch": every kernel of a step launched once
+    for all systems) or one pmx_md_run_many call over `lanes` contexts
+    (how="many"); precision 1 is the mixed form, 0 FP64."""
     from protomol3_amd import pmx, serve, systems
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(500), np.log(20000), nsys)).astype(int)
@@ -353,19 +369,20 @@ def replicas(nsys: int, steps: int, lanes: int, seed: int = 11, skin: float = 1.
     boxes = [systems.replica_box(int(n), 2000 + i).with_velocities(300.0, seed=3000 + i) for i, n in enumerate(sizes)]
     natoms = int(sum(b.n for b in boxes))
     log(f"replicas: {nsys} systems, {natoms} atoms in total, built in {time.perf_counter() - t0:.1f} s")
-    ctxs = [pmx.Context(0) for _ in range(lanes)]
+    ctxs = [pmx.Context(0) for _ in range(lanes if how == "many" else 1)]
+    run = pmx.run_batch if how == "batch" else pmx.run_many
     sys_ = []
     try:
         for i, b in enumerate(boxes):
-            sy = pmx.System(ctxs[i % lanes], b, serve.request_spec(b), pmx.MIXED)
+            sy = pmx.System(ctxs[i % len(ctxs)], b, serve.request_spec(b), precision)
             sys_.append(sy)
             sy.md_init(b, TIMESTEP_FS, skin=skin, integrator=pmx.LANGEVIN, temperature=300.0, gamma=1.0, seed=i,
                        graphs=graphs)
             sy.md_set_state(b.pos, b.vel)
-        pmx.run_many(sys_, 5)  # warm-up (first list builds, kernel loads)
+        run(sys_, 5)  # warm-up (first list builds, kernel loads)
         st0 = [sy.md_stats() for sy in sys_]
         t0 = time.perf_counter()
-        pmx.run_many(sys_, steps)
+        run(sys_, steps)
         wall = time.perf_counter() - t0
         st1 = [sy.md_stats() for sy in sys_]
         rebuilds = sum(b["rebuilds"] - a["rebuilds"] for a, b in zip(st0, st1))
@@ -375,12 +392,13 @@ def replicas(nsys: int, steps: int, lanes: int, seed: int = 11, skin: float = 1.
             sy.close()
         for c in ctxs:
             c.close()
-    return {"systems": nsys, "atoms_total": natoms, "steps": steps, "lanes": lanes, "wall_s": round(wall, 3),
-            "skin_A": skin, "rebuilds": rebuilds, "rollbacks": rollbacks,
+    return {"systems": nsys, "atoms_total": natoms, "steps": steps, "launches": how, "lanes": len(ctxs),
+            "wall_s": round(wall, 3), "skin_A": skin, "rebuilds": rebuilds, "rollbacks": rollbacks,
             "aggregate_ns_per_day": round(nsys * steps * TIMESTEP_FS * 1e-6 / wall * 86400.0, 3),
             "atom_steps_per_s": float(f"{natoms * steps / wall:.4e}"),
             "workload": "log-uniform 500-20,000 atoms, water + bead polymer at 0.0334 atoms/A^3, rc 9 A, "
-                        "PME ~1 A grid, Langevin 300 K 1/ps, mixed, one pmx_md_run_many call"}
+                        f"PME ~1 A grid, Langevin 300 K 1/ps, {'mixed' if precision else 'FP64'}, one "
+                        f"pmx_md_run_{how} call"}
 
 
 # Reference build / oracle time per force evaluation of the 26,000-atom workload
@@ -520,7 +538,16 @@ def main():
     achieved = FLOP_PER_PAIR * mixed["npairs"] / (pair_ms * 1e-3) / 1e12
     cpu = None if a.no_cpu else cpu_baseline(s, a.cpu_budget)
     serve_out = serving(a.serve_requests, a.serve_slots, log=log) if a.serve_requests > 0 else None
-    rep_out = replicas(a.replicas, a.replica_steps, a.replica_lanes, log=log) if a.replicas > 0 else None
+    rep_out = None
+    if a.replicas > 0:
+        rep_out = replicas(a.replicas, a.replica_steps, a.replica_lanes, log=log)
+        r64 = replicas(a.replicas, a.replica_steps, a.replica_lanes, precision=0, log=log)
+        many = replicas(a.replicas, a.replica_steps, a.replica_lanes, how="many", log=log)
+        rep_out["fp64_aggregate_ns_per_day"] = r64["aggregate_ns_per_day"]
+        rep_out["mixed_over_fp64_batched"] = round(rep_out["aggregate_ns_per_day"] / r64["aggregate_ns_per_day"], 3)
+        rep_out["per_context_streams"] = {k: many[k] for k in ("lanes", "wall_s", "aggregate_ns_per_day")}
+        log(f"replicas: batched {rep_out['aggregate_ns_per_day']:.1f} ns/day (FP64 {r64['aggregate_ns_per_day']:.1f}), "
+            f"per-context streams {many['aggregate_ns_per_day']:.1f}")
     copies_out = copies(s, spec, a.copies, a.steps, a.replica_lanes, skin_m, log=log) if a.copies > 0 else None
     if copies_out is not None:
         copies_out["over_fp64_single"] = round(

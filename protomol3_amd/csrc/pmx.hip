@@ -2338,10 +2338,14 @@ int batch_tables(Batch &b) {
       args_begin[i] = {3ll * s->n, P<double>(s->xorig), P<double>(m.v), P<double>(s->fout), P<double>(m.xbak),
                        P<double>(m.vbak), P<double>(m.fbak), P<int>(s->overflow), P<unsigned>(m.dmax), kMaxChunk};
       args_half[i] = {s->n, kk, m.h, P<double>(s->fout), P<double>(m.minv), P<double>(m.v)};
+      // dynamic LDS of each launch: the largest of the systems it serves
       lds_z = std::max(lds_z, 2 * (size_t)s->zl * pm.nz * cx);
-      lds_y = std::max(lds_y, 2 * (size_t)s->tky * pm.ny * cx);
-      lds_x = std::max(lds_x, 2 * (size_t)s->tkx * pm.nx * cx);
-      lds_p = std::max(lds_p, 2 * (size_t)pm.nx * pm.ny * cx);
+      if (s->fft_plane) {
+        lds_p = std::max(lds_p, 2 * (size_t)pm.nx * pm.ny * cx);
+      } else {
+        lds_y = std::max(lds_y, 2 * (size_t)s->tky * pm.ny * cx);
+        lds_x = std::max(lds_x, 2 * (size_t)s->tkx * pm.nx * cx);
+      }
     }
     for (int i = 0; i < ns; ++i) nb[i] = (int)nblk(b.sys[i]->npad, 256);
     if ((rc = batch_table<OpKickDrift<R>>(b, b.kick, args_kick, nb, 0))) return rc;

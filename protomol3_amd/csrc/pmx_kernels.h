@@ -729,7 +729,7 @@ __device__ inline void list_append(const ListParams &q, int S, const float4 (&ic
 // eight i-clusters of S, and entries are appended in ascending cj.  Only
 // cj >= 8S is listed, so each cluster pair appears once.  The work per S is
 // bounded by the neighbourhood, independent of the system size.
-__device__ __forceinline__ void build_list_body(const int bid_, const int nbk_, ListParams q,
+__device__ __forceinline__ void build_list_body(const int bid_, const int nbk_, const ListParams &q,
                                                 const int *__restrict__ sc_col, const int *__restrict__ colsc,
                                                 const float4 *__restrict__ sctr, const float4 *__restrict__ shalf,
                                                 const float4 *__restrict__ ctr, const float4 *__restrict__ half,
@@ -2318,7 +2318,7 @@ __device__ Cx<T> *lds_fft(Cx<T> *a, Cx<T> *b, int nb, const FftPlan &pl, const C
 // z pass, real -> half complex, for the TY lines (i, j0 .. j0 + TY - 1):
 // the grid points are read once and cleared for the next spread; out: SA.
 template <class T, bool FIXED>
-__device__ __forceinline__ void fft_z_r2c_body(const int bid_, const int nbk_, PmeParams p, FftPlan pl, int TY,
+__device__ __forceinline__ void fft_z_r2c_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl, int TY,
                                                double *__restrict__ g64, unsigned long long *__restrict__ gfx,
                                                const Cx<T> *__restrict__ tw, Cx<T> *__restrict__ sa) {
   extern __shared__ unsigned char smem_raw[];
@@ -2356,7 +2356,7 @@ __global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int TY, double *__restrict_
 // y pass for the TI lines (i0 .. i0 + TI - 1, k): forward SA -> SB; inverse
 // (conj-in / conj-out around the forward transform) SB -> SA.
 template <class T>
-__device__ __forceinline__ void fft_y_body(const int bid_, const int nbk_, PmeParams p, FftPlan pl, int TI,
+__device__ __forceinline__ void fft_y_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl, int TI,
                                            bool inverse, const Cx<T> *__restrict__ tw, const Cx<T> *__restrict__ src,
                                            Cx<T> *__restrict__ dst) {
   extern __shared__ unsigned char smem_raw[];
@@ -2405,7 +2405,7 @@ __global__ void k_fft_y(PmeParams p, FftPlan pl, int TI, bool inverse, const Cx<
 // FFT, influence function multiply with the energy (Grid.h:237-341; theta in
 // the SB layout), inverse FFT.
 template <class T>
-__device__ __forceinline__ void fft_x_conv_body(const int bid_, const int nbk_, PmeParams p, FftPlan pl, int TJ,
+__device__ __forceinline__ void fft_x_conv_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl, int TJ,
                                                 const Cx<T> *__restrict__ tw, const T *__restrict__ theta,
                                                 Cx<T> *__restrict__ sb, double *__restrict__ epart) {
   extern __shared__ unsigned char smem_raw[];
@@ -2467,8 +2467,8 @@ constexpr int kFftPlanePoints = 1024;
 // written back to SA.  The conj-out of the inverse x transform and the
 // conj-in of the inverse y transform cancel, so only the last conj remains.
 template <class T>
-__device__ __forceinline__ void fft_plane_conv_body(const int bid_, const int nbk_, PmeParams p, FftPlan ply,
-                                                    FftPlan plx, const Cx<T> *__restrict__ twy,
+__device__ __forceinline__ void fft_plane_conv_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &ply,
+                                                    const FftPlan &plx, const Cx<T> *__restrict__ twy,
                                                     const Cx<T> *__restrict__ twx, const T *__restrict__ theta,
                                                     Cx<T> *__restrict__ sa, double *__restrict__ epart) {
   extern __shared__ unsigned char smem_raw[];
@@ -2539,7 +2539,7 @@ __global__ void __launch_bounds__(256) k_fft_plane_conv(PmeParams p, FftPlan ply
 // z pass, half complex -> real for the TY lines (i, j0 .. j0 + TY - 1) of SA
 // (Hermitian completion along z, inverse sign); out: the real potential.
 template <class T>
-__device__ __forceinline__ void fft_z_c2r_body(const int bid_, const int nbk_, PmeParams p, FftPlan pl, int TY,
+__device__ __forceinline__ void fft_z_c2r_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl, int TY,
                                                const Cx<T> *__restrict__ tw, const Cx<T> *__restrict__ sa,
                                                T *__restrict__ greal) {
   extern __shared__ unsigned char smem_raw[];

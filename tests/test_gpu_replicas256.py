@@ -8,7 +8,9 @@ form, Langevin 300 K, 1/ps).
 * Every system's 5-step Langevin trajectory run batched (pmx_md_run_many over
   four contexts, all 256 resident at once) against the same system run alone:
   bit-identical, since the mixed form accumulates in int64 fixed point and the
-  Langevin noise is a counter-based stream keyed by (seed, atom, step)."""
+  Langevin noise is a counter-based stream keyed by (seed, atom, step).
+* All 256 in one pmx_md_run_batch call against each in a batch of one:
+  bit-identical."""
 import os
 
 import numpy as np
@@ -99,3 +101,34 @@ def test_every_replica_batched_equals_alone(draw):
         for k in ea:
             assert ea[k] == em[k], (i, k)
     print(f"replicas: {NSYS} batched trajectories bit-identical to the runs alone", flush=True)
+
+
+def test_every_replica_in_batched_launches_equals_its_batch_of_one(draw):
+    """All 256 systems in one pmx_md_run_batch call (each kernel of a step
+    launched once for all of them) against each system in a batch of its
+    own: bit-identical, since a block reads only its own system's table entry
+    and each system's list rebuilds are decided from its own displacements."""
+    ctx = pmx.Context(0)
+    try:
+        ones = []
+        for i, b in enumerate(draw):
+            sy = _make(ctx, b, i)
+            try:
+                pmx.run_batch([sy], STEPS)
+                ones.append(sy.md_get_state())
+            finally:
+                sy.close()
+        sys_ = [_make(ctx, b, i) for i, b in enumerate(draw)]
+        try:
+            pmx.run_batch(sys_, STEPS)
+            batch = [sy.md_get_state() for sy in sys_]
+        finally:
+            for sy in sys_:
+                sy.close()
+    finally:
+        ctx.close()
+    for i, ((xa, va, ea), (xb, vb, eb)) in enumerate(zip(ones, batch)):
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb), i
+        for k in ea:
+            assert ea[k] == eb[k], (i, k)
+    print(f"replicas: {NSYS} trajectories in batched launches bit-identical to batches of one", flush=True)
