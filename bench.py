@@ -290,7 +290,7 @@ def time_pair(ctx, s, spec, x, skin, launch=None, reps: int = 200) -> float:
         sy.close()
 
 
-def serving(nreq: int, slots: int, seed: int = 7, pool: int = 256, log=print):
+def serving(nreq: int, slots: int, seed: int = 7, pool: int = 256, rate: float = 0.0, log=print):
     """Serving stream: requests of log-uniform size in [1,000, 100,000] atoms
     (water + bead-polymer solute at 0.0334 atoms/A^3, per-system seeds),
     half single force evaluations and half 100-step Langevin trajectories
@@ -325,7 +325,7 @@ def serving(nreq: int, slots: int, seed: int = 7, pool: int = 256, log=print):
             srv.wait(t)
         makespan = time.perf_counter() - t0
         capacity = npool / makespan
-        rate = 0.75 * capacity
+        rate = rate if rate > 0 else 0.75 * capacity
         gaps = rng.exponential(1.0 / rate, nreq)
         t0 = time.perf_counter()
         ts, due = [], t0
@@ -457,6 +457,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--serve-requests", type=int, default=64, help="requests in the serving stream (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="Poisson arrival rate in requests/s (0: 75%% of the measured capacity)")
     ap.add_argument("--serve-only", action="store_true",
                     help="only the serving stream (--serve-requests requests) and print it")
     ap.add_argument("--replicas", type=int, default=256,
@@ -480,7 +482,8 @@ def main():
     from protomol3_amd import pmx, tune
     if a.serve_only:
         log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
-        print(json.dumps({"serving": serving(a.serve_requests, a.serve_slots, log=log)}), flush=True)
+        print(json.dumps({"serving": serving(a.serve_requests, a.serve_slots, rate=a.serve_rate, log=log)}),
+              flush=True)
         return
     ctx = pmx.Context(0)
     if a.large:

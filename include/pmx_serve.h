@@ -29,6 +29,16 @@
  * Device blocks of finished systems are kept by the engine for reuse, so a
  * steady request stream does not allocate device memory.
  *
+ * Continuous batching: a slot that starts a one-shot trajectory request also
+ * starts the queued one-shot trajectory requests that can share its kernel
+ * launches (same precision, kinds of force, LJ type count, PME settings and
+ * step count) and fit in the budget, up to 64 requests and 200,000 atoms
+ * beyond its own, and runs them with
+ * pmx_md_run_batch (every kernel of a step launched once for all of them).
+ * Every one-shot trajectory runs that way, alone or batched, so its result
+ * does not depend on the requests it was batched with (a system that no batch
+ * takes runs with pmx_md_run).
+ *
  * All inputs are copied at submit; output pointers must stay valid until
  * pmx_server_wait returns for the ticket.
  */

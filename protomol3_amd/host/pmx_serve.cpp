@@ -9,6 +9,15 @@
 // queue, which the worker serves before the shared one.  Requests that
 // arrive while others run are admitted as soon as a slot and memory free up,
 // so work of different requests overlaps on the device.
+//
+// Continuous batching: a worker that takes a one-shot trajectory request also
+// takes the queued one-shot trajectory requests that can share its launches
+// (same precision, kinds of force, LJ types, PME settings and step count)
+// and fit in the budget, up to kMaxBatch requests and kMaxBatchAtoms atoms
+// beyond its own, and runs them with
+// pmx_md_run_batch: every kernel of a step is launched once for all of them.
+// Every one-shot trajectory runs through pmx_md_run_batch, alone or not, so
+// its result does not depend on what it was batched with.
 #include "../../include/pmx_serve.h"
 
 #include <algorithm>
@@ -127,29 +136,18 @@ struct pmx_server {
 
 namespace {
 
+// A one-shot force request (one-shot trajectories run in batches, below).
 int run_oneshot(pmx_ctx *ctx, Job &j) {
   pmx_sys *sys = nullptr;
   int rc = pmx_sys_create(ctx, &j.desc->sys, &sys);
   if (rc == PMX_OK) {
     const size_t n = (size_t)j.desc->sys.n;
-    if (j.kind == PMX_REQ_FORCES) {
-      std::vector<double> f(3 * n, 0.0);
-      double e[PMX_E_COUNT] = {0, 0};
-      rc = pmx_sys_evaluate(sys, j.x.data(), f.data(), e, 0);
-      if (rc == PMX_OK) {
-        if (j.out_a) std::memcpy(j.out_a, f.data(), 24 * n);
-        if (j.out_e) std::memcpy(j.out_e, e, sizeof(e));
-      }
-    } else {
-      double e[PMX_MD_E_COUNT] = {0, 0, 0, 0, 0};
-      std::vector<double> x(3 * n), v(3 * n);
-      if ((rc = pmx_md_init(sys, &j.desc->md)) == PMX_OK &&
-          (rc = pmx_md_set_state(sys, j.x.data(), j.v.data())) == PMX_OK && (rc = pmx_md_run(sys, j.nsteps)) == PMX_OK &&
-          (rc = pmx_md_get_state(sys, x.data(), v.data(), e)) == PMX_OK) {
-        if (j.out_a) std::memcpy(j.out_a, x.data(), 24 * n);
-        if (j.out_v) std::memcpy(j.out_v, v.data(), 24 * n);
-        if (j.out_e) std::memcpy(j.out_e, e, sizeof(e));
-      }
+    std::vector<double> f(3 * n, 0.0);
+    double e[PMX_E_COUNT] = {0, 0};
+    rc = pmx_sys_evaluate(sys, j.x.data(), f.data(), e, 0);
+    if (rc == PMX_OK) {
+      if (j.out_a) std::memcpy(j.out_a, f.data(), 24 * n);
+      if (j.out_e) std::memcpy(j.out_e, e, sizeof(e));
     }
   }
   if (sys) pmx_sys_destroy(sys);
@@ -200,11 +198,78 @@ bool admissible(const pmx_server *srv, const Job &j) {
   return srv->in_use == 0 || srv->in_use + j.bytes <= srv->budget;
 }
 
+constexpr size_t kMaxBatch = 64;            // one-shot trajectories run in one batch, at most
+constexpr long long kMaxBatchAtoms = 200000;  // atoms of the requests a batch adds to its first
+
+bool batch_head(const Job &j) { return j.op == OP_ONESHOT && j.kind == PMX_REQ_TRAJECTORY; }
+
+// j can join a batch headed by h (the engine takes the batch as one kernel
+// case: same precision, kinds of force, LJ type count and PME settings).
+bool joins(const Job &h, const Job &j) {
+  if (!batch_head(j) || j.nsteps != h.nsteps) return false;
+  const pmx_sys_desc &a = h.desc->sys, &b = j.desc->sys;
+  return a.precision == b.precision && a.coul_kind == b.coul_kind && a.lj_switch == b.lj_switch &&
+         a.coul_switch == b.coul_switch && a.ntypes == b.ntypes && a.pme_order == b.pme_order &&
+         a.pme_terms == b.pme_terms;
+}
+
+// One-shot trajectories in one pmx_md_run_batch call.  A batch the engine
+// does not take runs system by system: each in a batch of its own, or, for a
+// system no batch takes (no PME, say), with pmx_md_run.  Sets each job's
+// status.
+void run_trajectories(pmx_ctx *ctx, std::vector<std::shared_ptr<Job>> &jobs, std::string &err) {
+  const size_t nj = jobs.size();
+  std::vector<pmx_sys *> sys(nj, nullptr), live;
+  std::vector<size_t> at;
+  for (size_t i = 0; i < nj; ++i) {
+    Job &j = *jobs[i];
+    int rc = pmx_sys_create(ctx, &j.desc->sys, &sys[i]);
+    if (rc == PMX_OK && (rc = pmx_md_init(sys[i], &j.desc->md)) == PMX_OK)
+      rc = pmx_md_set_state(sys[i], j.x.data(), j.v.data());
+    j.status = rc;
+    if (rc == PMX_OK) {
+      live.push_back(sys[i]);
+      at.push_back(i);
+    } else {
+      err = pmx_last_error(ctx);
+    }
+  }
+  if (!live.empty()) {
+    const int rc = pmx_md_run_batch(live.data(), (int)live.size(), jobs[at[0]]->nsteps);
+    if (rc == PMX_ERR_UNSUPPORTED || rc == PMX_ERR_ARG) {
+      for (size_t k = 0; k < live.size(); ++k) {
+        int r1 = pmx_md_run_batch(&live[k], 1, jobs[at[k]]->nsteps);
+        if (r1 == PMX_ERR_UNSUPPORTED || r1 == PMX_ERR_ARG) r1 = pmx_md_run(live[k], jobs[at[k]]->nsteps);
+        jobs[at[k]]->status = r1;
+      }
+    } else {
+      for (size_t k : at) jobs[k]->status = rc;
+    }
+  }
+  for (size_t k = 0; k < live.size(); ++k) {
+    Job &j = *jobs[at[k]];
+    const size_t n = (size_t)j.desc->sys.n;
+    double e[PMX_MD_E_COUNT] = {0, 0, 0, 0, 0};
+    std::vector<double> x(3 * n), v(3 * n);
+    if (j.status == PMX_OK) j.status = pmx_md_get_state(live[k], x.data(), v.data(), e);
+    if (j.status == PMX_OK) {
+      if (j.out_a) std::memcpy(j.out_a, x.data(), 24 * n);
+      if (j.out_v) std::memcpy(j.out_v, v.data(), 24 * n);
+      if (j.out_e) std::memcpy(j.out_e, e, sizeof(e));
+    } else {
+      err = pmx_last_error(ctx);
+    }
+  }
+  for (pmx_sys *q : sys)
+    if (q) pmx_sys_destroy(q);
+}
+
 void worker(pmx_server *srv, int w) {
   pmx_ctx *ctx = nullptr;
   const int crc = pmx_ctx_create(srv->device, &ctx);
   for (;;) {
     std::shared_ptr<Job> job;
+    std::vector<std::shared_ptr<Job>> batch;  // one-shot trajectories run together
     {
       std::unique_lock<std::mutex> lk(srv->mu);
       srv->cv_work.wait(lk, [&] {
@@ -227,12 +292,58 @@ void worker(pmx_server *srv, int w) {
           for (auto &p : job->sess->pending) srv->priv[w].push_back(p);
           job->sess->pending.clear();
         }
+        if (batch_head(*job)) {
+          batch.push_back(job);
+          // a batch's requests finish together: the atoms added are capped
+          // so that small requests do not wait long behind large ones
+          long long added = 0;
+          for (auto it = srv->queue.begin(); it != srv->queue.end() && batch.size() < kMaxBatch;) {
+            if (joins(*job, **it) && admissible(srv, **it) && added + (*it)->desc->sys.n <= kMaxBatchAtoms) {
+              added += (*it)->desc->sys.n;
+              srv->in_use += (*it)->bytes;
+              srv->peak_bytes = std::max(srv->peak_bytes, srv->in_use);
+              batch.push_back(*it);
+              it = srv->queue.erase(it);
+            } else {
+              ++it;
+            }
+          }
+        }
       } else {
         break;  // stop requested and nothing left for this worker
       }
-      ++srv->running;
+      const auto now = Clock::now();
+      if (batch.empty()) {
+        ++srv->running;
+        job->t_start = now;
+      } else {
+        srv->running += (int)batch.size();
+        for (auto &b : batch) b->t_start = now;
+      }
       srv->peak_running = std::max(srv->peak_running, srv->running);
-      job->t_start = Clock::now();
+    }
+    if (!batch.empty()) {
+      std::string err = crc != PMX_OK ? std::string("no device context") : std::string();
+      if (crc == PMX_OK)
+        run_trajectories(ctx, batch, err);
+      else
+        for (auto &b : batch) b->status = crc;
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        const auto now = Clock::now();
+        for (auto &b : batch) {
+          srv->in_use -= b->bytes;
+          --srv->running;
+          b->t_done = now;
+          b->done = true;
+          if (b->status != PMX_OK) srv->err = err;
+        }
+        batch.clear();
+        job.reset();
+      }
+      srv->cv_done.notify_all();
+      srv->cv_work.notify_all();
+      continue;
     }
     int rc = crc;
     if (crc == PMX_OK) {

@@ -50,7 +50,7 @@ def test_gpu_requests_alone_and_in_the_mix_agree():
                 else:
                     sy.md_init(s, 1.0, skin=1.0, integrator=pmx.LANGEVIN, temperature=300.0, gamma=1.0, seed=9)
                     sy.md_set_state(s.pos, s.vel)
-                    sy.md_run(20)
+                    pmx.run_batch([sy], 20)  # one-shot trajectories run in batched launches
                     x, v, e = sy.md_get_state()
                     alone.append((x, v))
             finally:
@@ -176,10 +176,50 @@ def test_gpu_admission_keeps_resident_footprints_within_the_budget():
                 else:
                     sy.md_init(s, 1.0, skin=1.0, integrator=pmx.LANGEVIN, temperature=300.0, gamma=1.0, seed=9)
                     sy.md_set_state(s.pos, s.vel)
-                    sy.md_run(20)
+                    pmx.run_batch([sy], 20)
                     x, v, _ = sy.md_get_state()
                     assert np.array_equal(t.a, x) and np.array_equal(t.v, v)
             finally:
                 sy.close()
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_queued_trajectories_run_as_one_batch():
+    """One slot, busy with a force request while six trajectory requests
+    queue behind it: the slot then starts all six together (one batch of
+    launches), and each result equals the request run alone in a batch of
+    one; a request with another step count waits for its own batch."""
+    big = systems.replica_box(60000, 900)
+    trajs = []
+    for i, n in enumerate([1500, 800, 2400, 1100, 3000, 1900]):
+        s = systems.replica_box(n, 910 + i).with_velocities(300.0, seed=920 + i)
+        trajs.append((s, serve.request_spec(s)))
+    srv = serve.Server(0, slots=1)
+    try:
+        first = srv.submit_forces(big, serve.request_spec(big), pmx.MIXED)
+        ts = [srv.submit_trajectory(s, spec, pmx.MIXED, 30, skin=1.0, gamma=1.0, seed=9) for s, spec in trajs]
+        other = srv.submit_trajectory(*trajs[0], pmx.MIXED, 10, skin=1.0, gamma=1.0, seed=9)
+        for t in [first, *ts, other]:
+            srv.wait(t)
+    finally:
+        srv.close()
+    starts = {t.start_at for t in ts}
+    print(f"batch: starts {sorted(starts)} first done {first.done_at} other start {other.start_at}", flush=True)
+    assert len(starts) == 1 and min(starts) >= first.done_at
+    assert other.start_at >= max(t.done_at for t in ts)
+    ctx = pmx.Context(0)
+    try:
+        for (s, spec), t in zip(trajs, ts):
+            sy = pmx.System(ctx, s, spec, pmx.MIXED)
+            try:
+                sy.md_init(s, 1.0, skin=1.0, integrator=pmx.LANGEVIN, temperature=300.0, gamma=1.0, seed=9)
+                sy.md_set_state(s.pos, s.vel)
+                pmx.run_batch([sy], 30)
+                x, v, _ = sy.md_get_state()
+            finally:
+                sy.close()
+            assert np.array_equal(t.a, x) and np.array_equal(t.v, v)
     finally:
         ctx.close()
