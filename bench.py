@@ -297,8 +297,10 @@ def serving(nreq: int, slots: int, seed: int = 7, pool: int = 256, rate: float =
     (300 K, 1/ps, per-request noise seeds), mixed precision, LJ C2 7-9 A +
     PME ~1 A grid.  Up to `pool` distinct systems are built; request i uses
     system i mod pool.  First the first min(nreq, pool) requests at once
-    (capacity), then all nreq requests as Poisson arrivals at 75% of that
-    capacity; latency = submit to result, per request."""
+    (capacity), then all nreq requests as Poisson arrivals at `rate`
+    requests/s (0: 75% of that capacity); latency = submit to result, per
+    request."""
+    fixed_rate = rate
     from protomol3_amd import pmx, serve, systems
     rng = np.random.default_rng(seed)
     npool = min(nreq, pool)
@@ -350,7 +352,8 @@ def serving(nreq: int, slots: int, seed: int = 7, pool: int = 256, rate: float =
             "aggregate_ns_per_day": round(traj_steps * 1e-6 / wall * 86400.0, 3),
             "wall_s": round(wall, 2), "peak_device_bytes": int(peak_bytes), "peak_running": int(peak_running),
             "workload": "log-uniform 1k-100k atoms, 50% force evaluations / 50% 100-step Langevin 1 fs, mixed, "
-                        "Poisson arrivals at 75% of measured capacity"}
+                        + ("Poisson arrivals at 75% of measured capacity" if fixed_rate <= 0 else
+                           f"Poisson arrivals at {fixed_rate} requests/s")}
 
 
 def replicas(nsys: int, steps: int, lanes: int, seed: int = 11, skin: float = 1.0, graphs: bool = False,
