@@ -108,6 +108,15 @@ int pmx_sys_set_positions(pmx_sys *sys, const double *x_aos);
 int pmx_sys_compute(pmx_sys *sys, unsigned flags);
 int pmx_sys_get_forces(pmx_sys *sys, double *f_aos, double *energies);
 int pmx_sys_sync(pmx_sys *sys);
+/* No counterpart in the reference.  pmx_sys_compute of every system, each at
+ * the positions set with pmx_sys_set_positions, with every kernel of the
+ * force pass launched once for all of them (the batched launches of
+ * pmx_md_run_batch, with the same conditions on the systems; flags must be 0).
+ * Each system's forces and energies are those pmx_sys_compute gives it (the
+ * mixed form to the last bit); fetch them with pmx_sys_get_forces.  For many
+ * small systems at once (a request loop's force requests, ensemble force
+ * calls), whose passes alone are bound by launch latency. */
+int pmx_sys_compute_batch(pmx_sys *const *systems, int nsys, unsigned flags);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): pmx_sys_compute with no allocation, ever.  The force

@@ -2130,6 +2130,7 @@ struct Batch {
   int nsys = 0;
   BatchLaunch bin, scan, scatter, binrank, columns, split, list, excl, masks, rprefix, rcut, rtiles;
   BatchLaunch kick, spread, pair, zr2c, yfwd, xconv, yinv, plane, zc2r, gather, fixups, ukds, unsort, begin, half;
+  BatchLaunch energy;
   BatchBuf bsys, need, base, flags, rebuilds;
   BatchSys *bsys_h = nullptr;
   unsigned long long *base_h = nullptr;
@@ -2297,6 +2298,7 @@ int batch_tables(Batch &b) {
     std::vector<typename OpUnsortForces<MIXED>::Args> args_unsort(ns);
     std::vector<OpChunkBegin::Args> args_begin(ns);
     std::vector<OpHalfKick::Args> args_half(ns);
+    std::vector<OpReduceEnergy::Args> args_energy(ns);
     std::vector<int> nfix(ns);
     long long natoms = 0;
     size_t lds_z = 0, lds_y = 0, lds_x = 0, lds_p = 0;
@@ -2338,6 +2340,13 @@ int batch_tables(Batch &b) {
       args_begin[i] = {3ll * s->n, P<double>(s->xorig), P<double>(m.v), P<double>(s->fout), P<double>(m.xbak),
                        P<double>(m.vbak), P<double>(m.fbak), P<int>(s->overflow), P<unsigned>(m.dmax), kMaxChunk};
       args_half[i] = {s->n, kk, m.h, P<double>(s->fout), P<double>(m.minv), P<double>(m.v)};
+      const bool recip = s->pme_on && (s->d.pme_terms & PMX_PME_RECIP);
+      const bool corr = s->pme_on && (s->d.pme_terms & PMX_PME_CORR) && !s->etab.empty();
+      args_energy[i] = {s->ntiles_cap, P<int>(s->ntiles), P<double>(s->pair_e), s->mixed ? 1 : 0,
+                        recip ? s->recip_blocks : 0, corr ? s->corr_blocks : 0, P<double>(s->recip_e),
+                        P<double>(s->corr_e), P<long long>(s->npairs), P<unsigned long long>(s->hash),
+                        P<unsigned long long>(s->border_acc), P<unsigned long long>(s->border_acc) + 2,
+                        P<double>(s->eout), P<long long>(s->np_out), P<unsigned long long>(s->hash_out)};
       // dynamic LDS of each launch: the largest of the systems it serves
       lds_z = std::max(lds_z, 2 * (size_t)s->zl * pm.nz * cx);
       if (s->fft_plane) {
@@ -2390,6 +2399,8 @@ int batch_tables(Batch &b) {
     for (int i = 0; i < ns; ++i) nb[i] = (int)nblk(b.sys[i]->n, 256);
     if ((rc = batch_table<OpUnsortForces<MIXED>>(b, b.unsort, args_unsort, nb, 0))) return rc;
     if ((rc = batch_table<OpHalfKick>(b, b.half, args_half, nb, 0))) return rc;
+    for (int i = 0; i < ns; ++i) nb[i] = 1;
+    if ((rc = batch_table<OpReduceEnergy>(b, b.energy, args_energy, nb, 0))) return rc;
     for (int i = 0; i < ns; ++i) nb[i] = (int)nblk(std::max<long long>(3ll * b.sys[i]->n, kMaxChunk), 256);
     if ((rc = batch_table<OpChunkBegin>(b, b.begin, args_begin, nb, 0))) return rc;
   }
@@ -2464,6 +2475,77 @@ int batch_chunk(Batch &b, int K, bool opening) {
                      reinterpret_cast<int *>(b.flags.p));
   BHIPCHK(hipMemcpyAsync(b.flags_h, b.flags.p, 4 * (size_t)ns, hipMemcpyDeviceToHost, st));
   BHIPCHK(hipGetLastError());
+  return PMX_OK;
+}
+
+// One force pass of every system of the batch at its set positions (list
+// rebuilt for all), with the energies reduced: the work of compute() for
+// each.  Systems whose list outgrew its buffers are grown and computed again
+// alone.
+template <bool MIXED, int C, int LJ, int CS, bool BIGLJ, int ORDER>
+int batch_forces(Batch &b) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using T = R;
+  hipStream_t st = b.ctx->stream, st2 = b.ctx->stream2;
+  const int ns = b.nsys;
+  int rc = batch_tables<MIXED, C, LJ, CS, BIGLJ, ORDER>(b);
+  if (rc) return rc;
+  for (Sys *s : b.sys) {
+    if (!s->acc_clean || s->spread_ready) launch_zero(s);
+    s->spread_ready = false;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, P<int>(s->overflow), 0, 2ll);
+  }
+  std::vector<unsigned char> all(ns, 1);
+  BHIPCHK(hipMemcpyAsync(b.need.p, all.data(), ns, hipMemcpyHostToDevice, st));
+  BHIPCHK(hipMemcpyAsync(b.bsys.p, b.bsys_h, sizeof(BatchSys) * ns, hipMemcpyHostToDevice, st));
+  const BatchStep bk{0, 0, reinterpret_cast<unsigned char *>(b.need.p)};
+  batch_go<OpBin>(b.bin, st, bk);
+  batch_go<OpScanSingle>(b.scan, st, bk);
+  batch_go<OpScatter>(b.scatter, st, bk);
+  batch_go<OpBinrank>(b.binrank, st, bk);
+  batch_go<OpColumns>(b.columns, st, bk);
+  batch_go<OpSplitFill<R>>(b.split, st, bk);
+  batch_go<OpBuildList>(b.list, st, bk);
+  batch_go<OpMarkExclusions>(b.excl, st, bk);
+  batch_go<OpBuildMasks>(b.masks, st, bk);
+  batch_go<OpRunsPrefix>(b.rprefix, st, bk);
+  batch_go<OpRunsCut>(b.rcut, st, bk);
+  batch_go<OpRunsTiles>(b.rtiles, st, bk);
+  batch_go<OpSpreadPatch<MIXED, ORDER>>(b.spread, st, bk);
+  BHIPCHK(hipEventRecord(b.fork, st));
+  BHIPCHK(hipStreamWaitEvent(st2, b.fork, 0));
+  batch_go<PairOp<MIXED, C, LJ, CS, BIGLJ, ORDER>>(b.pair, st, bk);
+  batch_go<OpFftZr2c<T, MIXED>>(b.zr2c, st2, bk);
+  batch_go<OpFftPlane<T>>(b.plane, st2, bk);
+  batch_go<OpFftY<T>>(b.yfwd, st2, bk);
+  batch_go<OpFftXConv<T>>(b.xconv, st2, bk);
+  batch_go<OpFftY<T>>(b.yinv, st2, bk);
+  batch_go<OpFftZc2r<T>>(b.zc2r, st2, bk);
+  batch_go<OpGatherPatch<MIXED, ORDER>>(b.gather, st2, bk);
+  BHIPCHK(hipEventRecord(b.join, st2));
+  batch_go<OpFixups<MIXED, C, LJ, CS>>(b.fixups, st, bk);
+  BHIPCHK(hipStreamWaitEvent(st, b.join, 0));
+  batch_go<OpUnsortForces<MIXED>>(b.unsort, st, bk);
+  batch_go<OpReduceEnergy>(b.energy, st, bk);
+  hipLaunchKernelGGL(k_batch_flags, dim3(nblk(ns, 256)), dim3(256), 0, st, ns,
+                     reinterpret_cast<const BatchSys *>(b.bsys.p), reinterpret_cast<int *>(b.flags.p));
+  BHIPCHK(hipMemcpyAsync(b.flags_h, b.flags.p, 4 * (size_t)ns, hipMemcpyDeviceToHost, st));
+  BHIPCHK(hipGetLastError());
+  BHIPCHK(hipStreamSynchronize(st));
+  for (int i = 0; i < ns; ++i) {
+    Sys *s = b.sys[i];
+    s->list_valid = true;
+    s->prune_pending = false;
+    s->acc_clean = true;
+    for (int attempt = 0; b.flags_h[i] && attempt < 6; ++attempt) {
+      if ((rc = grow_and_retry(s)) || (rc = compute(s, 0))) return rc;
+      int ovf = 0;
+      if (hipMemcpyAsync(&ovf, s->overflow.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return fail(b.ctx, PMX_ERR_HIP, "overflow flag");
+      b.flags_h[i] = ovf;
+    }
+  }
   return PMX_OK;
 }
 
@@ -2618,15 +2700,21 @@ int pmx_md_run_many(pmx_sys *const *systems, int nsys, int nsteps) {
 extern "C++" {
 namespace {
 
+// nsteps < 0: one force pass (batch_forces), else nsteps MD steps.
+template <bool MIXED, int C, int LJ, int CS, bool BIG, int ORDER>
+int batch_exec(Batch &b, int nsteps) {
+  return nsteps < 0 ? batch_forces<MIXED, C, LJ, CS, BIG, ORDER>(b) : batch_run<MIXED, C, LJ, CS, BIG, ORDER>(b, nsteps);
+}
+
 template <bool MIXED, int CASE>
 int batch_run_case(Batch &b, int order, int nsteps) {
   constexpr int LJ = CASE == 0 ? SW_C2 : (CASE == 1 ? SW_NONE : SW_ANY);
   constexpr int CS = CASE == 0 ? SW_CUTOFF : SW_ANY;
   constexpr bool BIG = CASE == 3;
   switch (order) {
-    case 4: return batch_run<MIXED, COUL_PME, LJ, CS, BIG, 4>(b, nsteps);
-    case 6: return batch_run<MIXED, COUL_PME, LJ, CS, BIG, 6>(b, nsteps);
-    default: return batch_run<MIXED, COUL_PME, LJ, CS, BIG, 8>(b, nsteps);
+    case 4: return batch_exec<MIXED, COUL_PME, LJ, CS, BIG, 4>(b, nsteps);
+    case 6: return batch_exec<MIXED, COUL_PME, LJ, CS, BIG, 6>(b, nsteps);
+    default: return batch_exec<MIXED, COUL_PME, LJ, CS, BIG, 8>(b, nsteps);
   }
 }
 
@@ -2640,29 +2728,26 @@ int batch_run_form(Batch &b, int cs, int order, int nsteps) {
   }
 }
 
-}  // namespace
-}  // extern "C++"
-
-int pmx_md_run_batch(pmx_sys *const *systems, int nsys, int nsteps) {
-  if (!systems || nsys <= 0 || nsteps < 0) return PMX_ERR_ARG;
-  for (int i = 0; i < nsys; ++i)
-    if (!systems[i]) return PMX_ERR_ARG;
+// Checks a batch's systems, makes the batch's buffers and runs it
+// (nsteps < 0: one force pass).
+int batch_entry(pmx_sys *const *systems, int nsys, int nsteps) {
   Sys *s0 = &systems[0]->s;
   pmx_ctx *ctx = s0->ctx;
+  const bool md = nsteps >= 0;
   const int cs = batch_case(s0), order = s0->pme.order;
   std::set<const pmx_sys *> seen;
   for (int i = 0; i < nsys; ++i) {
     Sys *s = &systems[i]->s;
     if (!seen.insert(systems[i]).second) return fail(ctx, PMX_ERR_ARG, "a system is listed twice");
     if (s->ctx != ctx) return fail(ctx, PMX_ERR_ARG, "the systems of a batch share one context");
-    if (!s->md.on || !s->have_positions)
-      return fail(ctx, PMX_ERR_ARG, "pmx_md_init and pmx_md_set_state first (every system of the batch)");
+    if (md && !s->md.on) return fail(ctx, PMX_ERR_ARG, "pmx_md_init first (every system of the batch)");
+    if (!s->have_positions) return fail(ctx, PMX_ERR_ARG, "positions not set (every system of the batch)");
     if (s->mixed != s0->mixed || batch_case(s) != cs)
       return fail(ctx, PMX_ERR_ARG, "the systems of a batch share the precision and the kinds of force");
     if (!s->pme_on || !(s->d.pme_terms & PMX_PME_RECIP) || s->pme.order < 4 || s->pme.order != order)
-      return fail(ctx, PMX_ERR_UNSUPPORTED, "batched MD takes PME systems of one stencil order (4, 6 or 8)");
+      return fail(ctx, PMX_ERR_UNSUPPORTED, "batched launches take PME systems of one stencil order (4, 6 or 8)");
     if (s->nbins > kScanSingleMax)
-      return fail(ctx, PMX_ERR_UNSUPPORTED, "batched MD takes systems of at most 65,536 cell bins");
+      return fail(ctx, PMX_ERR_UNSUPPORTED, "batched launches take systems of at most 65,536 cell bins");
   }
   if (nsteps == 0) return PMX_OK;
   hipSetDevice(ctx->device);
@@ -2672,11 +2757,15 @@ int pmx_md_run_batch(pmx_sys *const *systems, int nsys, int nsteps) {
   for (int i = 0; i < nsys; ++i) {
     Sys *s = &systems[i]->s;
     b.sys.push_back(s);
-    if (int rc = md_ensure_forces(s)) return rc;
-    if (!s->acc_clean || s->spread_ready) {
-      s->spread_ready = false;
-      launch_zero(s);
-      s->acc_clean = true;
+    if (!s->cached)
+      if (int rc = setup(s)) return rc;
+    if (md) {
+      if (int rc = md_ensure_forces(s)) return rc;
+      if (!s->acc_clean || s->spread_ready) {
+        s->spread_ready = false;
+        launch_zero(s);
+        s->acc_clean = true;
+      }
     }
   }
   const int dev = ctx->device;
@@ -2692,13 +2781,28 @@ int pmx_md_run_batch(pmx_sys *const *systems, int nsys, int nsteps) {
         hipEventCreateWithFlags(&b.join, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess)
       return fail(ctx, PMX_ERR_HIP, "batch host buffers and events");
   }
-  // the systems' own streams are idle from here on; the batch runs on the
-  // context's two streams
-  for (Sys *s : b.sys) {
-    if (hipStreamSynchronize(s->ctx->stream) != hipSuccess || hipStreamSynchronize(s->ctx->stream2) != hipSuccess)
-      return fail(ctx, PMX_ERR_HIP, "stream synchronize");
-  }
+  // the batch runs on the context's two streams, idle from here on
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess || hipStreamSynchronize(ctx->stream2) != hipSuccess)
+    return fail(ctx, PMX_ERR_HIP, "stream synchronize");
   return s0->mixed ? batch_run_form<true>(b, cs, order, nsteps) : batch_run_form<false>(b, cs, order, nsteps);
+}
+
+}  // namespace
+}  // extern "C++"
+
+int pmx_md_run_batch(pmx_sys *const *systems, int nsys, int nsteps) {
+  if (!systems || nsys <= 0 || nsteps < 0) return PMX_ERR_ARG;
+  for (int i = 0; i < nsys; ++i)
+    if (!systems[i]) return PMX_ERR_ARG;
+  return batch_entry(systems, nsys, nsteps);
+}
+
+int pmx_sys_compute_batch(pmx_sys *const *systems, int nsys, unsigned flags) {
+  if (!systems || nsys <= 0) return PMX_ERR_ARG;
+  for (int i = 0; i < nsys; ++i)
+    if (!systems[i]) return PMX_ERR_ARG;
+  if (flags) return fail(systems[0]->s.ctx, PMX_ERR_UNSUPPORTED, "batched force passes take no flags");
+  return batch_entry(systems, nsys, -1);
 }
 
 int pmx_md_get_state(pmx_sys *sys, double *x, double *v, double *energies) {

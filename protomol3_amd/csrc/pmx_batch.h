@@ -1,7 +1,8 @@
 // Batched launches: one launch runs a kernel over the blocks of many systems.
 //
 // A batch is a set of systems of one context that share the kernel case (form,
-// force kinds, PME order).  Each kernel of the MD step gets, per batch, a
+// force kinds, PME order).  Each kernel of the MD step (and of a force pass)
+// gets, per batch, a
 // table with one argument block per system in device memory, the first block
 // of every system (first[s], first[nsys] = all blocks) and the system of every
 // block (b2s).  Block b then does the work of block b - first[s] of system
@@ -470,6 +471,28 @@ struct OpUnsortForces {
   static constexpr bool kGated = false;
   __device__ static void run(const Args &a, int bid, int nbk, const BatchStep &) {
     unsort_forces_body<MIXED>(bid, nbk, a.n, a.slot, a.facc, a.fout, a.tilectr, a.nborder);
+  }
+};
+
+// a force pass's energy partials into the system's energy slots (one block)
+struct OpReduceEnergy {
+  struct Args {
+    int ntiles_cap;
+    const int *ntiles;
+    const double *pair_e;
+    int pair_fixed, nrecip, ncorr;
+    const double *recip_e, *corr_e;
+    const long long *npairs;
+    const unsigned long long *hash, *border_e, *border_stat;
+    double *out;
+    long long *out_np;
+    unsigned long long *out_hash;
+  };
+  static constexpr int kBlock = 1024, kMinBlocks = 1;
+  static constexpr bool kGated = false;
+  __device__ static void run(const Args &a, int bid, int nbk, const BatchStep &) {
+    reduce_energy_body(bid, nbk, a.ntiles_cap, a.ntiles, a.pair_e, a.pair_fixed, a.nrecip, a.recip_e, a.ncorr,
+                       a.corr_e, a.npairs, a.hash, 0, a.border_e, a.border_stat, a.out, a.out_np, a.out_hash);
   }
 };
 

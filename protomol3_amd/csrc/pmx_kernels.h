@@ -2318,8 +2318,8 @@ __device__ Cx<T> *lds_fft(Cx<T> *a, Cx<T> *b, int nb, const FftPlan &pl, const C
 // z pass, real -> half complex, for the TY lines (i, j0 .. j0 + TY - 1):
 // the grid points are read once and cleared for the next spread; out: SA.
 template <class T, bool FIXED>
-__device__ __forceinline__ void fft_z_r2c_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl, int TY,
-                                               double *__restrict__ g64, unsigned long long *__restrict__ gfx,
+__device__ __forceinline__ void fft_z_r2c_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl,
+                                               int TY, double *__restrict__ g64, unsigned long long *__restrict__ gfx,
                                                const Cx<T> *__restrict__ tw, Cx<T> *__restrict__ sa) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
@@ -2356,9 +2356,9 @@ __global__ void k_fft_z_r2c(PmeParams p, FftPlan pl, int TY, double *__restrict_
 // y pass for the TI lines (i0 .. i0 + TI - 1, k): forward SA -> SB; inverse
 // (conj-in / conj-out around the forward transform) SB -> SA.
 template <class T>
-__device__ __forceinline__ void fft_y_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl, int TI,
-                                           bool inverse, const Cx<T> *__restrict__ tw, const Cx<T> *__restrict__ src,
-                                           Cx<T> *__restrict__ dst) {
+__device__ __forceinline__ void fft_y_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl,
+                                           int TI, bool inverse, const Cx<T> *__restrict__ tw,
+                                           const Cx<T> *__restrict__ src, Cx<T> *__restrict__ dst) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
   Cx<T> *b = a + TI * p.ny;
@@ -2405,8 +2405,8 @@ __global__ void k_fft_y(PmeParams p, FftPlan pl, int TI, bool inverse, const Cx<
 // FFT, influence function multiply with the energy (Grid.h:237-341; theta in
 // the SB layout), inverse FFT.
 template <class T>
-__device__ __forceinline__ void fft_x_conv_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl, int TJ,
-                                                const Cx<T> *__restrict__ tw, const T *__restrict__ theta,
+__device__ __forceinline__ void fft_x_conv_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl,
+                                                int TJ, const Cx<T> *__restrict__ tw, const T *__restrict__ theta,
                                                 Cx<T> *__restrict__ sb, double *__restrict__ epart) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
@@ -2467,10 +2467,11 @@ constexpr int kFftPlanePoints = 1024;
 // written back to SA.  The conj-out of the inverse x transform and the
 // conj-in of the inverse y transform cancel, so only the last conj remains.
 template <class T>
-__device__ __forceinline__ void fft_plane_conv_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &ply,
-                                                    const FftPlan &plx, const Cx<T> *__restrict__ twy,
-                                                    const Cx<T> *__restrict__ twx, const T *__restrict__ theta,
-                                                    Cx<T> *__restrict__ sa, double *__restrict__ epart) {
+__device__ __forceinline__ void fft_plane_conv_body(const int bid_, const int nbk_, const PmeParams &p,
+                                                    const FftPlan &ply, const FftPlan &plx,
+                                                    const Cx<T> *__restrict__ twy, const Cx<T> *__restrict__ twx,
+                                                    const T *__restrict__ theta, Cx<T> *__restrict__ sa,
+                                                    double *__restrict__ epart) {
   extern __shared__ unsigned char smem_raw[];
   const int nx = p.nx, ny = p.ny, nxy = nx * ny;
   Cx<T> *const a = reinterpret_cast<Cx<T> *>(smem_raw);
@@ -2539,8 +2540,8 @@ __global__ void __launch_bounds__(256) k_fft_plane_conv(PmeParams p, FftPlan ply
 // z pass, half complex -> real for the TY lines (i, j0 .. j0 + TY - 1) of SA
 // (Hermitian completion along z, inverse sign); out: the real potential.
 template <class T>
-__device__ __forceinline__ void fft_z_c2r_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl, int TY,
-                                               const Cx<T> *__restrict__ tw, const Cx<T> *__restrict__ sa,
+__device__ __forceinline__ void fft_z_c2r_body(const int bid_, const int nbk_, const PmeParams &p, const FftPlan &pl,
+                                               int TY, const Cx<T> *__restrict__ tw, const Cx<T> *__restrict__ sa,
                                                T *__restrict__ greal) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T> *a = reinterpret_cast<Cx<T> *>(smem_raw);
@@ -2864,13 +2865,16 @@ __global__ void k_unsort_forces(int n, const int *__restrict__ slot, typename Ac
 // Pair energies per tile are int64 fixed point (2^-32) when pair_fixed (the
 // mixed form: their sum is exact, whatever tiles the schedule made), FP64
 // otherwise.
-__global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p, const double *__restrict__ pair_e,
-                                int pair_fixed, int nrecip, const double *__restrict__ recip_e, int ncorr,
-                                const double *__restrict__ corr_e, const long long *__restrict__ npairs,
-                                const unsigned long long *__restrict__ hash, int stats,
-                                const unsigned long long *__restrict__ border_e,
-                                const unsigned long long *__restrict__ border_stat, double *__restrict__ out,
-                                long long *__restrict__ out_np, unsigned long long *__restrict__ out_hash) {
+__device__ __forceinline__ void reduce_energy_body(const int bid_, const int nbk_, int ntiles_cap,
+                                                   const int *__restrict__ ntiles_p, const double *__restrict__ pair_e,
+                                                   int pair_fixed, int nrecip, const double *__restrict__ recip_e,
+                                                   int ncorr, const double *__restrict__ corr_e,
+                                                   const long long *__restrict__ npairs,
+                                                   const unsigned long long *__restrict__ hash, int stats,
+                                                   const unsigned long long *__restrict__ border_e,
+                                                   const unsigned long long *__restrict__ border_stat,
+                                                   double *__restrict__ out, long long *__restrict__ out_np,
+                                                   unsigned long long *__restrict__ out_hash) {
   __shared__ double sd[4][1024 / kWave];
   __shared__ long long sn[1024 / kWave];
   __shared__ unsigned long long sh[1024 / kWave];
@@ -2935,6 +2939,17 @@ __global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p
     *out_np = N + (long long)border_stat[0];
     *out_hash = H + border_stat[1];
   }
+}
+
+__global__ void k_reduce_energy(int ntiles_cap, const int *__restrict__ ntiles_p, const double *__restrict__ pair_e,
+                                int pair_fixed, int nrecip, const double *__restrict__ recip_e, int ncorr,
+                                const double *__restrict__ corr_e, const long long *__restrict__ npairs,
+                                const unsigned long long *__restrict__ hash, int stats,
+                                const unsigned long long *__restrict__ border_e,
+                                const unsigned long long *__restrict__ border_stat, double *__restrict__ out,
+                                long long *__restrict__ out_np, unsigned long long *__restrict__ out_hash) {
+  reduce_energy_body(blockIdx.x, gridDim.x, ntiles_cap, ntiles_p, pair_e, pair_fixed, nrecip, recip_e, ncorr, corr_e,
+                     npairs, hash, stats, border_e, border_stat, out, out_np, out_hash);
 }
 
 // ---------------------------------------------------------------------------

@@ -67,7 +67,8 @@ EXPORTED = ["pmx_ctx_create", "pmx_ctx_destroy", "pmx_last_error", "pmx_sys_crea
             "pmx_sys_cells", "pmx_sys_stage_times", "pmx_sys_timing_begin", "pmx_sys_timing_end",
             "pmx_md_init", "pmx_md_set_state", "pmx_md_run", "pmx_md_get_state", "pmx_md_stats",
             "pmx_langevin_noise", "pmx_sys_device_bytes", "pmx_estimate_bytes", "pmx_md_run_many",
-            "pmx_sys_time_pair", "pmx_sys_set_tuning", "pmx_sys_compute_noalloc", "pmx_md_run_batch"]
+            "pmx_sys_time_pair", "pmx_sys_set_tuning", "pmx_sys_compute_noalloc", "pmx_md_run_batch",
+            "pmx_sys_compute_batch"]
 
 _lib = None
 
@@ -108,6 +109,7 @@ def load() -> C.CDLL:
     L.pmx_md_run.argtypes = [vp, C.c_int]
     L.pmx_md_run_many.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
     L.pmx_md_run_batch.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
+    L.pmx_sys_compute_batch.argtypes = [C.POINTER(vp), C.c_int, C.c_uint]
     L.pmx_sys_time_pair.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
     L.pmx_sys_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.pmx_md_get_state.argtypes = [vp, dp, dp, dp]
@@ -296,6 +298,21 @@ def run_batch(systems_: list, nsteps: int) -> None:
     ctx = systems_[0].ctx
     arr = (C.c_void_p * len(systems_))(*[sy.h for sy in systems_])
     ctx.check(ctx.lib.pmx_md_run_batch(arr, len(systems_), nsteps), "pmx_md_run_batch")
+
+
+def evaluate_batch(systems_: list, positions: list) -> list:
+    """pmx_sys_compute_batch: the forces and energies of every System at its
+    positions (an (n, 3) array each), every kernel of the force pass launched
+    once for all of them; returns [(forces, ecoul, elj)] like
+    System.evaluate."""
+    if not systems_:
+        return []
+    ctx = systems_[0].ctx
+    for sy, x in zip(systems_, positions):
+        sy.set_positions(x)
+    arr = (C.c_void_p * len(systems_))(*[sy.h for sy in systems_])
+    ctx.check(ctx.lib.pmx_sys_compute_batch(arr, len(systems_), 0), "pmx_sys_compute_batch")
+    return [sy.get_forces() for sy in systems_]
 
 
 def estimate_bytes(s: MolSystem, spec: ForceSpec, precision: int = FP64, trajectory: bool = False) -> int:

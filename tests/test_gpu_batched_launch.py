@@ -10,6 +10,8 @@ many systems of uneven size.
   steps (its FP32 tile sums follow the list).
 * The energies the batch leaves are those of a fresh evaluation at the final
   positions (the device-side list rebuilds kept every list valid).
+* pmx_sys_compute_batch gives every system the forces and energies that
+  pmx_sys_evaluate gives it.
 * Batches that mix precisions or force kinds, repeat a system or hold a
   non-PME system are refused."""
 import numpy as np
@@ -113,16 +115,51 @@ def test_batch_energies_are_those_of_the_final_positions(ctx):
         assert abs(e["lj"] - elj) <= 1e-9 * max(1.0, abs(elj))
 
 
+@pytest.mark.parametrize("precision", [pmx.MIXED, pmx.FP64])
+def test_force_batch_matches_each_system_alone(ctx, precision):
+    """pmx_sys_compute_batch: every system's forces and energies at its own
+    positions equal pmx_sys_evaluate's (mixed form to the last bit, FP64 to
+    the order of its double atomics), twice in a row on the same systems at
+    new positions (lists rebuilt, accumulators left clear)."""
+    reqs = _systems()
+    rng = np.random.default_rng(5)
+    sys_ = [pmx.System(ctx, s, spec, precision) for s, spec in reqs]
+    try:
+        for rnd in range(2):
+            xs = [s.pos + rng.normal(0.0, 0.02, s.pos.shape) * rnd for s, _ in reqs]
+            got = pmx.evaluate_batch(sys_, xs)
+            for i, ((s, spec), x, (f, ec, elj)) in enumerate(zip(reqs, xs, got)):
+                one = pmx.System(ctx, s, spec, precision)
+                try:
+                    f1, ec1, elj1 = one.evaluate(x)
+                finally:
+                    one.close()
+                df = np.abs(f - f1).max() / np.abs(f1).max()
+                print(f"round {rnd} system {i} n={s.n} prec={precision} df {df:.2e} dEc {ec - ec1:.2e} "
+                      f"dElj {elj - elj1:.2e}", flush=True)
+                if precision == pmx.MIXED:
+                    assert np.array_equal(f, f1) and ec == ec1 and elj == elj1
+                else:
+                    assert df <= 1e-12
+                    assert abs(ec - ec1) <= 1e-11 * abs(ec1) + 1e-9 and abs(elj - elj1) <= 1e-11 * abs(elj1) + 1e-9
+    finally:
+        for sy in sys_:
+            sy.close()
+
+
 def test_batch_refuses_mixed_batches(ctx):
     reqs = _systems()[:2]
     a = _make(ctx, *reqs[0], pmx.MIXED, 1)
     b = _make(ctx, *reqs[1], pmx.FP64, 2)
     s, _ = reqs[0]
     c = _make(ctx, s, pmx.ForceSpec(5.0, lj=(pmx.SW_C2, 7.0, 9.0), coul=(pmx.SW_C2, 7.0, 9.0)), pmx.MIXED, 3)
+    pos = {id(a): reqs[0][0].pos, id(b): reqs[1][0].pos, id(c): s.pos}
     try:
         for bad in ([a, b], [a, a], [a, c]):
             with pytest.raises(pmx.PmxError):
                 pmx.run_batch(bad, 5)
+            with pytest.raises(pmx.PmxError):
+                pmx.evaluate_batch(bad, [pos[id(sy)] for sy in bad])
         pmx.run_batch([a], 0)
     finally:
         for sy in (a, b, c):
