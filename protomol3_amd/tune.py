@@ -47,7 +47,7 @@ def _time_run(sy, x, v, steps: int, warmup: int = 20) -> float:
 
 
 def tune_skin(ctx, s, spec: pmx.ForceSpec, precision: int, x, v, timestep_fs: float = 1.0,
-              candidates=(0.5, 0.7, 0.9, 1.2, 1.5, 1.8), steps: int = 400, rounds: int = 2) -> tuple[float, dict]:
+              candidates=(0.5, 0.7, 0.9, 1.2, 1.5), steps: int = 400, rounds: int = 2) -> tuple[float, dict]:
     """Returns (best skin, {skin: ms per step}); each time is the best of
     `rounds` runs (the rebuild cadence makes a single short run noisy)."""
     times = {}
