@@ -164,3 +164,27 @@ def test_batch_refuses_mixed_batches(ctx):
     finally:
         for sy in (a, b, c):
             sy.close()
+
+
+def test_batch_order6_and_skin0_follow_md_run(ctx):
+    """PME stencil order 6 and a zero skin (a list build every step) in
+    batched launches: the FP64 trajectories equal pmx_md_run's to 1e-9."""
+    reqs = [(s, serve.request_spec(s, order=6)) for s, _ in _systems()[:4]]
+    out = {}
+    for how in ("batch", "md_run"):
+        sys_ = [_make(ctx, *reqs[i], pmx.FP64, 90 + i, skin=0.0) for i in range(len(reqs))]
+        try:
+            if how == "batch":
+                pmx.run_batch(sys_, 25)
+            else:
+                for sy in sys_:
+                    sy.md_run(25)
+            out[how] = [(sy.md_get_state(), sy.md_stats()) for sy in sys_]
+        finally:
+            for sy in sys_:
+                sy.close()
+    for i, (((xb, vb, _), sb), ((xr, vr, _), sr)) in enumerate(zip(out["batch"], out["md_run"])):
+        print(f"order 6 skin 0 system {i}: dx {np.abs(xb - xr).max():.2e} stats {sb} md_run {sr}", flush=True)
+        assert sb["rebuilds"] >= 25
+        assert np.abs(xb - xr).max() <= 1e-9 * max(1.0, np.abs(xr).max())
+        assert np.abs(vb - vr).max() <= 1e-9 * max(1.0, np.abs(vr).max())
