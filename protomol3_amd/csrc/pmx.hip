@@ -29,6 +29,11 @@ struct pmx_ctx {
   void *scratch = nullptr;  // device address of this device's copy of `scratch` (no-allocation force pass)
   hipStream_t stream = nullptr;   // main stream: sort, list, pair kernel, boundary pairs, finalize
   hipStream_t stream2 = nullptr;  // PME, exclusion correction and bonded terms, concurrent with the pair kernel
+  // batched launches: pinned host staging (grown as needed) and the fork/join
+  // events, kept from one batch to the next
+  void *batch_host = nullptr;
+  size_t batch_host_bytes = 0;
+  hipEvent_t batch_fork = nullptr, batch_join = nullptr;
   std::string err;
 };
 
@@ -1339,6 +1344,9 @@ int pmx_ctx_destroy(pmx_ctx *ctx) {
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   if (ctx->stream2) hipStreamDestroy(ctx->stream2);
+  if (ctx->batch_host) hipHostFree(ctx->batch_host);
+  if (ctx->batch_fork) hipEventDestroy(ctx->batch_fork);
+  if (ctx->batch_join) hipEventDestroy(ctx->batch_join);
   delete ctx;
   return PMX_OK;
 }
@@ -2132,18 +2140,11 @@ struct Batch {
   BatchLaunch kick, spread, pair, zr2c, yfwd, xconv, yinv, plane, zc2r, gather, fixups, ukds, unsort, begin, half;
   BatchLaunch energy;
   BatchBuf bsys, need, base, flags, rebuilds;
+  // slices of the context's pinned staging, and its events
   BatchSys *bsys_h = nullptr;
   unsigned long long *base_h = nullptr;
   int *flags_h = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
-  ~Batch() {
-    RtLock lk(g_rt_mu);
-    if (bsys_h) hipHostFree(bsys_h);
-    if (base_h) hipHostFree(base_h);
-    if (flags_h) hipHostFree(flags_h);
-    if (fork) hipEventDestroy(fork);
-    if (join) hipEventDestroy(join);
-  }
 };
 
 // Uploads the table of one batched launch (args[s], nblocks[s] blocks of
@@ -2774,13 +2775,26 @@ int batch_entry(pmx_sys *const *systems, int nsys, int nsteps) {
     return fail(ctx, PMX_ERR_OOM, "batch buffers");
   {
     RtLock lk(g_rt_mu);
-    if (hipHostMalloc((void **)&b.bsys_h, sizeof(BatchSys) * nsys, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&b.base_h, 8 * (size_t)nsys, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&b.flags_h, 4 * (size_t)nsys, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreateWithFlags(&b.fork, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess ||
-        hipEventCreateWithFlags(&b.join, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess)
-      return fail(ctx, PMX_ERR_HIP, "batch host buffers and events");
+    const size_t need = (sizeof(BatchSys) + 16) * (size_t)nsys;
+    if (ctx->batch_host_bytes < need) {
+      if (ctx->batch_host) hipHostFree(ctx->batch_host);
+      ctx->batch_host = nullptr;
+      ctx->batch_host_bytes = 0;
+      if (hipHostMalloc(&ctx->batch_host, need, hipHostMallocDefault) != hipSuccess)
+        return fail(ctx, PMX_ERR_HIP, "batch host buffers");
+      ctx->batch_host_bytes = need;
+    }
+    if ((!ctx->batch_fork &&
+         hipEventCreateWithFlags(&ctx->batch_fork, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) ||
+        (!ctx->batch_join &&
+         hipEventCreateWithFlags(&ctx->batch_join, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess))
+      return fail(ctx, PMX_ERR_HIP, "batch events");
   }
+  b.bsys_h = static_cast<BatchSys *>(ctx->batch_host);
+  b.base_h = reinterpret_cast<unsigned long long *>(b.bsys_h + nsys);
+  b.flags_h = reinterpret_cast<int *>(b.base_h + nsys);
+  b.fork = ctx->batch_fork;
+  b.join = ctx->batch_join;
   // the batch runs on the context's two streams, idle from here on
   if (hipStreamSynchronize(ctx->stream) != hipSuccess || hipStreamSynchronize(ctx->stream2) != hipSuccess)
     return fail(ctx, PMX_ERR_HIP, "stream synchronize");
