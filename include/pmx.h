@@ -112,8 +112,10 @@ int pmx_sys_sync(pmx_sys *sys);
  * the positions set with pmx_sys_set_positions, with every kernel of the
  * force pass launched once for all of them (the batched launches of
  * pmx_md_run_batch, with the same conditions on the systems; flags must be 0).
- * Each system's forces and energies are those pmx_sys_compute gives it (the
- * mixed form to the last bit); fetch them with pmx_sys_get_forces.  For many
+ * Each system's forces and energies are those it gets in a batch of its own
+ * (the mixed form to the last bit) and pmx_sys_compute's to FP32 rounding
+ * (a list built in batched launches is cut into coarser runs); fetch them
+ * with pmx_sys_get_forces.  For many
  * small systems at once (a request loop's force requests, ensemble force
  * calls), whose passes alone are bound by launch latency. */
 int pmx_sys_compute_batch(pmx_sys *const *systems, int nsys, unsigned flags);

@@ -2124,6 +2124,14 @@ struct BatchBuf {
   }
 };
 
+// Smallest run of a list built in batched launches, in list-cost units (about
+// 35 ns of wave time each): a system's runs share the batch's waves with
+// every other system's, so its list is cut into runs of real work rather than
+// into as many runs as the device has resident waves (the single-system cut,
+// which keeps a small system's pass short alone).  The runs, and so a mixed
+// result, are those of the same list in any batch.
+constexpr long long kBatchRunCost = 2048;
+
 // One batched launch: the argument table, each system's first block and the
 // system of each block.
 struct BatchLaunch {
@@ -2258,7 +2266,7 @@ int batch_tables(Batch &b) {
                        P<int>(s->permp), P<int>(s->eptr_d), P<int2>(s->elist_d), P<unsigned short>(s->masks)};
       int *const nruns = P<int>(s->ntiles) + 2;
       args_rprefix[i] = {s->nsuper, P<int>(s->count), P<int>(s->lcost), nruns, P<int>(s->tep), P<long long>(s->tcp),
-                         P<int>(s->tpos), P<int>(s->tposS)};
+                         P<int>(s->tpos), P<int>(s->tposS), kBatchRunCost};
       args_rcut[i] = {s->nsuper, s->cap, P<int>(s->count), P<int>(s->ecost), nruns, P<int>(s->tep),
                       P<long long>(s->tcp), P<int>(s->tpos), P<int>(s->tposS)};
       args_rtiles[i] = {P<int>(s->count), P<int>(s->tep), P<int>(s->tpos), P<int>(s->tposS), P<int>(s->wstart),

@@ -207,11 +207,12 @@ struct OpRunsPrefix {
     int *nruns, *ep;
     long long *cp;
     int *pos, *posS;
+    long long min_run_cost;
   };
   static constexpr int kBlock = 1024, kMinBlocks = 1;
   static constexpr bool kGated = true;
   __device__ static void run(const Args &a, int bid, int nbk, const BatchStep &) {
-    runs_prefix_body(bid, nbk, a.nsuper, a.count, a.cost, a.nruns, a.ep, a.cp, a.pos, a.posS);
+    runs_prefix_body(bid, nbk, a.nsuper, a.count, a.cost, a.nruns, a.ep, a.cp, a.pos, a.posS, a.min_run_cost);
   }
 };
 

@@ -1042,7 +1042,7 @@ __device__ __forceinline__ void runs_prefix_body(const int bid_, const int nbk_,
                                                  const int *__restrict__ count, const int *__restrict__ cost,
                                                  int *__restrict__ nruns_p, int *__restrict__ ep,
                                                  long long *__restrict__ cp, int *__restrict__ pos,
-                                                 int *__restrict__ posS) {
+                                                 int *__restrict__ posS, long long min_run_cost = 0) {
   __shared__ int s_e[1024];
   __shared__ long long s_c[1024];
   __shared__ int s_last[1024 / kWave];
@@ -1080,7 +1080,14 @@ __device__ __forceinline__ void runs_prefix_body(const int bid_, const int nbk_,
   for (int k = 0; k < 1024 / kWave; ++k) lastS = max(lastS, s_last[k]);
   const long long per = (long long)kMinRuns * kRunCost;
   const long long k = (ccarry + per - 1) / per;
-  const int nruns = kMinRuns * (int)(k < 1 ? 1 : (k > kMaxRuns / kMinRuns ? kMaxRuns / kMinRuns : k));
+  int nruns = kMinRuns * (int)(k < 1 ? 1 : (k > kMaxRuns / kMinRuns ? kMaxRuns / kMinRuns : k));
+  if (min_run_cost > 0) {
+    // runs of at least min_run_cost units, in whole waves' worth of 64 runs:
+    // a small list's runs would otherwise hold an entry or two each, and the
+    // wave would spend its time on the i-force flush at every tile end
+    const long long r = (ccarry + min_run_cost - 1) / min_run_cost;
+    nruns = (int)min((long long)nruns, max(64ll, (r + 63) / 64 * 64));
+  }
   if (tid == 0) {
     ep[nsuper] = ecarry;
     cp[nsuper] = ccarry;
