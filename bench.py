@@ -491,7 +491,7 @@ def main():
     ctx = pmx.Context(0)
     if a.large:
         log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
-        out = large(ctx, a.steps if a.steps != 400 else 1000, a.skin if a.skin >= 0 else 0.8, log=log)
+        out = large(ctx, a.steps if a.steps != 400 else 1000, a.skin if a.skin >= 0 else 1.2, log=log)
         print(json.dumps(out), flush=True)
         ctx.close()
         return
