@@ -1841,7 +1841,11 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
     const double *__restrict__ ljtab,
     const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ ljt,
     typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
-  pair_body<MIXED, COUL, LJSW, CSW, BIGLJ, MODE>(blockIdx.x, gridDim.x, pp, dynamic, tiles, ntiles_p, wstart,
+  // blocks are dealt to the 8 XCDs in turn: the blocks of one XCD take a
+  // contiguous range of runs (neighbouring superclusters share its L2)
+  const int nb = gridDim.x, bx = blockIdx.x;
+  const int bid = (nb & 7) == 0 ? (bx & 7) * (nb >> 3) + (bx >> 3) : bx;
+  pair_body<MIXED, COUL, LJSW, CSW, BIGLJ, MODE>(bid, gridDim.x, pp, dynamic, tiles, ntiles_p, wstart,
                                                  tile_counter, entries, entries_w, cap, masks, xs, sorg, ts, perm,
                                                  ljtab, ljt, facc, out);
 }
