@@ -31,10 +31,9 @@ template <class Op>
 __global__ void __launch_bounds__(Op::kBlock, Op::kMinBlocks) k_batch(const typename Op::Args *__restrict__ tab,
                                                                       const int *__restrict__ b2s,
                                                                       const int *__restrict__ first, BatchStep st) {
-  // blocks are dealt to the 8 XCDs in turn: the blocks of one XCD take a
-  // contiguous range of the table (whole systems share its L2)
-  const int nb = gridDim.x, bx = blockIdx.x;
-  const int b = (nb & 7) == 0 ? (bx & 7) * (nb >> 3) + (bx >> 3) : bx;
+  // the blocks of one XCD take a contiguous range of the table (whole
+  // systems share its L2)
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int s = __builtin_amdgcn_readfirstlane(b2s[b]);
   if (Op::kGated && !st.need[s]) return;
   const int f = first[s];

@@ -105,6 +105,17 @@ struct BinParams {
   double delta[3];    // origin - min
 };
 
+// Block number for work laid out in order: the hardware deals consecutive
+// blocks of a launch to the 8 XCDs in turn, so this renumbering gives the
+// blocks of one XCD one contiguous range of the work (block bx runs on XCD
+// bx mod 8; XCD x's ranks 0, 1, ... map to the x-th of 8 consecutive ranges,
+// the first nb mod 8 of them one block longer), and neighbouring work shares
+// that XCD's L2.  A bijection on [0, nb).
+__device__ inline int xcd_block(int bx, int nb) {
+  const int x = bx & 7, q = nb >> 3, r = nb & 7;
+  return x * q + min(x, r) + (bx >> 3);
+}
+
 __device__ __forceinline__ void bin_body(const int bid_, const int nbk_, int n, const double *__restrict__ xorig,
                                          BinParams p, uint32_t *__restrict__ key, double *__restrict__ xwrap,
                                          int *__restrict__ refcell, uint32_t *__restrict__ hist) {
@@ -1841,13 +1852,10 @@ __global__ void __launch_bounds__(kPairBlock, 4) k_pair(
     const double *__restrict__ ljtab,
     const typename Real4<typename std::conditional<MIXED, float, double>::type>::type *__restrict__ ljt,
     typename AccT<MIXED>::type *__restrict__ facc, PairOut out) {
-  // blocks are dealt to the 8 XCDs in turn: the blocks of one XCD take a
-  // contiguous range of runs (neighbouring superclusters share its L2)
-  const int nb = gridDim.x, bx = blockIdx.x;
-  const int bid = (nb & 7) == 0 ? (bx & 7) * (nb >> 3) + (bx >> 3) : bx;
-  pair_body<MIXED, COUL, LJSW, CSW, BIGLJ, MODE>(bid, gridDim.x, pp, dynamic, tiles, ntiles_p, wstart,
-                                                 tile_counter, entries, entries_w, cap, masks, xs, sorg, ts, perm,
-                                                 ljtab, ljt, facc, out);
+  // the blocks of one XCD take a contiguous range of runs
+  pair_body<MIXED, COUL, LJSW, CSW, BIGLJ, MODE>(xcd_block(blockIdx.x, gridDim.x), gridDim.x, pp, dynamic, tiles,
+                                                 ntiles_p, wstart, tile_counter, entries, entries_w, cap, masks, xs,
+                                                 sorg, ts, perm, ljtab, ljt, facc, out);
 }
 
 // Boundary pairs: exact FP64 membership on the original coordinates
@@ -2156,7 +2164,9 @@ __global__ void __launch_bounds__(256) k_spread_patch(int n, PmeParams p, const 
                                                       const int *__restrict__ perm, const double *__restrict__ q,
                                                       double *__restrict__ grid64,
                                                       unsigned long long *__restrict__ gridfx) {
-  spread_patch_body<MIXED, P>(blockIdx.x, gridDim.x, n, p, xorig, perm, q, grid64, gridfx);
+  // the blocks of one XCD take a contiguous range of slots (one region of
+  // the grid)
+  spread_patch_body<MIXED, P>(xcd_block(blockIdx.x, gridDim.x), gridDim.x, n, p, xorig, perm, q, grid64, gridfx);
 }
 
 // Charge spreading (Grid::anterpolateCharge, Grid.h:107-154).  The P^3
@@ -2785,7 +2795,7 @@ __global__ void __launch_bounds__(256) k_gather_patch(int n, PmeParams p, const 
                                                       const typename std::conditional<MIXED, float, double>::type
                                                           *__restrict__ greal,
                                                       typename AccT<MIXED>::type *__restrict__ facc) {
-  gather_patch_body<MIXED, P>(blockIdx.x, gridDim.x, n, p, xorig, perm, q, greal, facc);
+  gather_patch_body<MIXED, P>(xcd_block(blockIdx.x, gridDim.x), gridDim.x, n, p, xorig, perm, q, greal, facc);
 }
 
 // ---------------------------------------------------------------------------
@@ -3341,8 +3351,10 @@ __global__ void __launch_bounds__(256) k_unsort_kick_drift_spread(
     typename Real4<R>::type *__restrict__ xs, const double4 *__restrict__ xs64, const float4 *__restrict__ sorg,
     unsigned int *__restrict__ dmax, PmeParams pme, const double *__restrict__ q, double *__restrict__ grid64,
     unsigned long long *__restrict__ gridfx) {
-  unsort_kick_drift_spread_body<MIXED, R, P>(blockIdx.x, gridDim.x, n, kk, h, perm, facc, fout, tilectr, nborder, minv,
-                                             v, x, xbuild, xs, xs64, sorg, dmax, pme, q, grid64, gridfx);
+  // the blocks of one XCD take a contiguous range of slots
+  unsort_kick_drift_spread_body<MIXED, R, P>(xcd_block(blockIdx.x, gridDim.x), gridDim.x, n, kk, h, perm, facc, fout,
+                                             tilectr, nborder, minv, v, x, xbuild, xs, xs64, sorg, dmax, pme, q,
+                                             grid64, gridfx);
 }
 
 // Start of an MD chunk in one launch: the committed state (x, v, f: m3 =
