@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 FLOP_PER_PAIR = 69          # algorithmic FLOPs per in-cutoff pair (LJ C2 + PME real, counted once)
 # HBM bytes per k_pair launch (mixed form, skin 1.5 A): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
 # scales from a plain 256 MiB copy (profiles/k_pair_traffic_polymer26000.md)
-PAIR_TRAFFIC_BYTES = 44.35e6
+PAIR_TRAFFIC_BYTES = 41.65e6
 PAIR_TRAFFIC_SKIN = 1.5
 LAUNCH_KNOBS = {2: "fft_lines", 3: "pair_blocks_per_cu", 4: "pair_schedule", 6: "spread_on_second_stream",
                 7: "list_prune", 8: "fixups_on_main_stream", 9: "spread_fused_with_kick_drift", 10: "fft_planes"}
