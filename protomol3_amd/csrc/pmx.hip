@@ -753,6 +753,16 @@ int setup(Sys *s) {
 template <bool MIXED>
 FixupArgs<MIXED> fixup_args(Sys *s, bool stats, bool border_here, bool rest_here, int *nblocks);
 
+// Resident blocks per CU of a pair-kernel launch (at least 1, at most the
+// kernel's wave budget per CU); each caller keeps its kernel's value in a
+// function-local static, initialized once even under concurrent first calls.
+int pair_blocks_per_cu(const void *kernel) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kPairBlock, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  return std::min(per_cu, kMaxPairWavesPerCu / (kPairBlock / kWave));
+}
+
 template <bool MIXED, int C, int LJ, int CS, bool BIGLJ = false>
 void launch_pair_phase(Sys *s, bool stats, int phase) {
   hipStream_t st = phase == 2 ? s->ctx->stream2 : s->ctx->stream;
@@ -762,15 +772,8 @@ void launch_pair_phase(Sys *s, bool stats, int phase) {
   if (phase < 0) {
     // as many blocks as are resident at once (the schedules assign each
     // wave its first tile, or its whole run, statically)
-    static int per_cu = 0;
-    if (!per_cu) {
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair<MIXED, C, LJ, CS, BIGLJ, kPairPlain>,
-                                                       kPairBlock, 0) !=
-              hipSuccess ||
-          per_cu < 1)
-        per_cu = 1;
-      per_cu = std::min(per_cu, kMaxPairWavesPerCu / (kPairBlock / kWave));
-    }
+    static const int per_cu =
+        pair_blocks_per_cu(reinterpret_cast<const void *>(k_pair<MIXED, C, LJ, CS, BIGLJ, kPairPlain>));
     // knob > 0: blocks per CU; 0: as many as fit; < 0: that many fewer
     const int bpc = s->tune_pair_blocks > 0 ? std::min(s->tune_pair_blocks, per_cu)
                                             : std::max(1, per_cu + s->tune_pair_blocks);
@@ -2374,14 +2377,8 @@ int batch_tables(Batch &b) {
     // pair-kernel blocks in proportion to the atoms (a run's share of a
     // system's waves changes no result), about the device's resident blocks
     // in all
-    static int per_cu = 0;
-    if (!per_cu) {
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<PairOp<MIXED, C, LJ, CS, BIGLJ, ORDER>>,
-                                                       kPairBlock, 0) != hipSuccess ||
-          per_cu < 1)
-        per_cu = 1;
-      per_cu = std::min(per_cu, kMaxPairWavesPerCu / (kPairBlock / kWave));
-    }
+    static const int per_cu =
+        pair_blocks_per_cu(reinterpret_cast<const void *>(k_batch<PairOp<MIXED, C, LJ, CS, BIGLJ, ORDER>>));
     const double pair_blocks = (double)b.ctx->cus * per_cu;
     for (int i = 0; i < ns; ++i)
       nb[i] = std::max(1, (int)std::lround(pair_blocks * b.sys[i]->n / (double)natoms));
