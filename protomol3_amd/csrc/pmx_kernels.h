@@ -1381,6 +1381,10 @@ struct PairQ {
   int info;
 };
 constexpr int kSlotBits = 10;
+// r^2 given to an excluded candidate pair: beyond every cutoff, band and
+// list radius
+template <class R>
+constexpr R kFarR2 = (R)1e30;
 constexpr int kPairPlain = 0, kPairPrune = 1, kPairStats = 2;  // k_pair MODE
 
 // Charge and LJ type (clamped to 0 for padding) of a j-atom of the group.
@@ -1702,36 +1706,38 @@ __device__ __forceinline__ void pair_body(const int bid_, const int nbk_, PairPa
       R qel = 0, qec = 0;
       unsigned used = 0;  // bit g * 8 + c: the cluster pair has a member
       unsigned kept = 0;  // bit g * 8 + c: (prune pass) a candidate pair within the list radius
-#pragma unroll
-      for (int g = 0; g < kGroup; ++g) {
+      // Candidate scan of entry g, specialised on one cutoff for both kinds
+      // (SAME) and on the entry carrying an exclusion mask (MASKED), so that
+      // each membership flag is a single compare whose ballot is its lane
+      // mask.  An excluded candidate's r^2 becomes kFarR2, which fails every
+      // range and band test.  The flags are those of the plain form: with one
+      // cutoff, "outside the band and within range" is d < -guard.
+      auto scan = [&](auto same, auto masked, const int g) {
+        constexpr bool SAME = decltype(same)::value, MASKED = decltype(masked)::value;
         const int sj = cjv[g] * kClusterSize + tj;
         unsigned mrem = emv[g];
         while (mrem) {
           const int c = __builtin_ctz(mrem);
           mrem &= mrem - 1;
-          bool cand = true, modified = false;
-          if (mov[g] >= 0) {
-            cand = (mbv[g] >> c) & 1u;
-            modified = (mbv[g] >> (8 + c)) & 1u;
-          }
           const R4 xi = si[c * 8 + ti];
           R dx, dy, dz;
           pair_delta<R, R4>(xi, xj[g], E, shv[g] & kShiftPerPair, dx, dy, dz);
-          const R r2 = dx * dx + dy * dy + dz * dz;
+          R r2 = dx * dx + dy * dy + dz * dz;
+          if (MASKED && !(mbv[g] & (1u << c))) r2 = kFarR2<R>;
           // a pair within the guard band of a cutoff is left to the FP64
           // boundary pass; a member is in range of a cutoff and outside
           // every band
-          bool border = false, in;
-          if (same_cut) {
+          bool border, in;
+          if (SAME) {
             const R d = r2 - rmax2;
-            border = cand && fabs(d) <= guard;
-            in = cand && d < -guard;
+            border = fabs(d) <= guard;
+            in = d < -guard;
           } else {
-            border = (fabs(r2 - sl.rc2) <= guard || fabs(r2 - sc.rc2) <= guard) && cand;
-            in = cand && !border && r2 <= rmax2;
+            border = fmin(fabs(r2 - sl.rc2), fabs(r2 - sc.rc2)) <= guard;
+            in = (border ? kFarR2<R> : r2) <= rmax2;
           }
           const unsigned long long bm = __builtin_amdgcn_ballot_w64(in);
-          if (PRUNE && __builtin_amdgcn_ballot_w64(cand && r2 <= prune_r2)) kept |= 1u << (g * 8 + c);
+          if (PRUNE && __builtin_amdgcn_ballot_w64(r2 <= prune_r2)) kept |= 1u << (g * 8 + c);
           if (__builtin_amdgcn_ballot_w64(border) && border) {
             const int b = atomicAdd(out.nborder, 1);
             if (b < out.maxborder)
@@ -1747,21 +1753,32 @@ __device__ __forceinline__ void pair_body(const int bid_, const int nbk_, PairPa
           if (bm) {
             used |= 1u << (g * 8 + c);
             if (in) {
-              const unsigned below =
-                  __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
+              // queue position: qtail + members in lower lanes
+              const unsigned pos = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((unsigned)bm, (unsigned)qtail));
+              const int modified = MASKED ? (int)((mbv[g] >> (8 + c)) & 1u) : 0;
               PairQ<R> e;
               e.r2 = r2;
-              e.info = info_lane | (((g * 8 + c) << 6) | (c << 14) | (g << 20)) | ((int)modified << kSlotBits);
-              q[(qtail + (int)below) & (kQueue - 1)] = e;
+              e.info = info_lane | (((g * 8 + c) << 6) | (c << 14) | (g << 20)) | (modified << kSlotBits);
+              q[pos & (kQueue - 1)] = e;
             }
             qtail += __popcll(bm);
-            while (qtail - qhead >= kWave) {
-              if (same_cut)
-                eval_batch(std::true_type{}, kWave, qel, qec);
-              else
-                eval_batch(std::false_type{}, kWave, qel, qec);
-            }
+            while (qtail - qhead >= kWave) eval_batch(same, kWave, qel, qec);
           }
+        }
+      };
+#pragma unroll
+      for (int g = 0; g < kGroup; ++g) {
+        if (same_cut) {
+          if (mov[g] >= 0)
+            scan(std::true_type{}, std::true_type{}, g);
+          else
+            scan(std::true_type{}, std::false_type{}, g);
+        } else {
+          if (mov[g] >= 0)
+            scan(std::false_type{}, std::true_type{}, g);
+          else
+            scan(std::false_type{}, std::false_type{}, g);
         }
       }
       // prune pass: cluster pairs without a candidate pair inside the list
